@@ -1,0 +1,269 @@
+#!/usr/bin/env python3
+"""Throughput of the HIP descale path on one MI355X.
+
+Workload (one step): Debicubic(b=0, c=0.5) of 64 YUV420P16 frames
+1920x1080 -> 1280x720 (chroma 960x540 -> 640x360, limited range, padding 3),
+F32 internal, 16-bit output -- the horizontal pass (u16 rows -> F32
+intermediate) then the vertical pass (F32 columns -> u16) for each plane.
+Frames are synthetic: a seeded 1280x720 U[0,65535] image blurred by a
+Gaussian (sigma 1.5 px), upscaled by the forward bicubic operator, plus
+N(0, 64^2) noise, clamped to 16 bits; inputs already in HBM, rotating through
+more than 1 GiB of distinct buffers so the 256 MB Infinity Cache cannot serve
+them.
+
+Prints one JSON line: frames/s (whole job), the per-kernel roofline of the
+dominant kernel (HIP events on the launch stream over the timed region) and
+the reference CPU engine's scalar path (opt=1) on a bounded sample of the same
+workload on this host.
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "vapoursynth-descalemvc_amd"))
+
+METRIC = ("1080p→720p F32 frames/s + %HBM peak; mixed-size solves/s + p99 latency")
+HBM_PEAK_GBPS = 8000.0
+FRAMES = 64
+SRC_W, SRC_H, DST_W, DST_H = 1920, 1080, 1280, 720
+FRAME_BYTES_YUV16 = 2 * (SRC_W * SRC_H + 2 * (SRC_W // 2) * (SRC_H // 2)) \
+    + 2 * (DST_W * DST_H + 2 * (DST_W // 2) * (DST_H // 2))          # 8,985,600
+FRAME_BYTES_GRAY_F32 = 4 * (SRC_W * SRC_H + DST_W * DST_H)            # 11,980,800
+
+
+def plane_requests(dh, sw, sh, dw, dh_, chroma, kernel, b, c):
+    kw = dict(kernel=kernel, b=b, c=c, border=dh.SYMMETRIC, f64_mode=0)
+    if not chroma:
+        return (dh.AxisRequest(sw, dw, float(dw), 0.0, **kw),
+                dh.AxisRequest(sh, dh_, float(dh_), 0.0, **kw))
+    cw, ch = sw >> 1, sh >> 1
+    # reference src/vs_plugin.cpp:656-705 (4:2:0 chroma requests)
+    return (dh.AxisRequest(cw, dw >> 1, float(dw) * cw / sw, 0.25 - 0.25 * float(dw) / float(sw),
+                           **kw),
+            dh.AxisRequest(ch, dh_ >> 1, float(dh_) * ch / sh, 0.0, **kw))
+
+
+def synthetic_frames(torch, dh, count, seed, device):
+    """Config-2 frames as uint16 planes [count, H, W] (Y, U, V)."""
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed)
+    planes = []
+    for index, (sw, sh, dw, dh_) in enumerate([(SRC_W, SRC_H, DST_W, DST_H),
+                                               (SRC_W // 2, SRC_H // 2, DST_W // 2, DST_H // 2),
+                                               (SRC_W // 2, SRC_H // 2, DST_W // 2, DST_H // 2)]):
+        low = torch.rand((count, 1, dh_, dw), generator=gen, device=device) * 65535.0
+        radius = 5
+        t = torch.arange(-radius, radius + 1, device=device, dtype=torch.float32)
+        g = torch.exp(-0.5 * (t / 1.5) ** 2)
+        g = g / g.sum()
+        low = torch.nn.functional.conv2d(torch.nn.functional.pad(low, (radius, radius, 0, 0), mode="reflect"),
+                                         g.view(1, 1, 1, -1))
+        low = torch.nn.functional.conv2d(torch.nn.functional.pad(low, (0, 0, radius, radius), mode="reflect"),
+                                         g.view(1, 1, -1, 1))[:, 0]
+        hreq, vreq = plane_requests(dh, SRC_W, SRC_H, DST_W, DST_H, index > 0, dh.BICUBIC, 0.0, 0.5)
+        A_h = forward_operator(torch, dh.Plan(hreq), device)      # [sw, dw]
+        A_v = forward_operator(torch, dh.Plan(vreq), device)      # [sh, dh]
+        up = torch.matmul(torch.matmul(A_v, low), A_h.t().contiguous())
+        up = up + torch.randn(up.shape, generator=gen, device=device) * 64.0
+        planes.append(up.round().clamp(0, 65535).to(torch.int32))
+    return planes
+
+
+def forward_operator(torch, plan, device):
+    """Dense A (source x destination) from the plan's CSR of A^T."""
+    a = plan.arrays()
+    A = np.zeros((plan.source_size, plan.destination_size), np.float32)
+    off, idx, w = a["offsets"], a["indices"], a["weights"]
+    for i in range(plan.destination_size):
+        A[idx[off[i]:off[i + 1]], i] = w[off[i]:off[i + 1]]
+    return torch.from_numpy(A).to(device)
+
+
+def cpu_baseline(planes_host, seconds=10.0):
+    """Reference engine (oracle/_ref, built from the reference's own sources)
+    CpuExecutor scalar path (opt=1), one thread, whole YUV420P16 frames; falls
+    back to the oracle's C restatement when the reference build is absent."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from support import Oracle, Reference, make_request, reference_available
+    from workloads import chroma420_requests, integer_conversion, luma_requests
+    lreq = luma_requests(SRC_W, SRC_H, DST_W, DST_H, 1, b=0.0, c=0.5)
+    creq = chroma420_requests(SRC_W, SRC_H, DST_W, DST_H, 1, b=0.0, c=0.5)
+    convs = [integer_conversion(16, False, False), integer_conversion(16, True, False)]
+    if reference_available():
+        eng, kind = Reference(), "reference"
+        run = lambda h, v, x, conv: eng.inverse_2d_int(h, v, x, conv, path=Reference.CPU_SCALAR)  # noqa: E731
+    else:
+        eng, kind = Oracle(), "port"
+        run = lambda h, v, x, conv: eng.inverse_2d_int(h, v, x, conv)  # noqa: E731
+    plans = [(eng.build(lreq[0]), eng.build(lreq[1])), (eng.build(creq[0]), eng.build(creq[1]))]
+    frames = 0
+    start = time.perf_counter()
+    while True:
+        f = frames % planes_host[0].shape[0]
+        for p in range(3):
+            h, v = plans[0 if p == 0 else 1]
+            run(h, v, planes_host[p][f], convs[0 if p == 0 else 1])
+        frames += 1
+        elapsed = time.perf_counter() - start
+        if elapsed >= seconds or frames >= 400:
+            break
+    return {"value": round(frames / elapsed, 3), "unit": "frames/s", "cores": 1, "kind": kind,
+            "sample": f"{frames} YUV420P16 1920x1080->1280x720 Debicubic frames, CpuExecutor "
+                      f"scalar (opt=1) inverse_2d_u16 per plane, single thread, "
+                      f"{elapsed:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--solve", default="auto", choices=["auto", "exact", "chunked"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-gray", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              f"(the path shards by frame with no collectives; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import dsmvc_hip as dh
+    from dsmvc_hip import _lib
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    ex = dh.Executor(0, args.solve)
+    lib = _lib.load()
+    stream = torch.cuda.current_stream()
+    sptr = ctypes.c_void_p(stream.cuda_stream)
+
+    plans = []
+    for chroma in (False, True):
+        hreq, vreq = plane_requests(dh, SRC_W, SRC_H, DST_W, DST_H, chroma, dh.BICUBIC, 0.0, 0.5)
+        hp, vp = dh.Plan(hreq), dh.Plan(vreq)
+        ex.prepare(hp)
+        ex.prepare(vp)
+        plans.append((hp, vp))
+    convs = [dh.integer_conversion(16, False, False).to_c(),
+             dh.integer_conversion(16, True, False).to_c()]
+
+    unique = synthetic_frames(torch, dh, 8, 20260, dev)
+    batch_in_bytes = FRAMES * 2 * (SRC_W * SRC_H + 2 * (SRC_W // 2) * (SRC_H // 2))
+    nbuf = max(2, math.ceil((1 << 30) / batch_in_bytes))
+    inputs = []
+    for b in range(nbuf):
+        idx = torch.arange(FRAMES, device=dev) % unique[0].shape[0]
+        inputs.append([p.index_select(0, (idx + b) % unique[0].shape[0]).to(torch.uint16)
+                       .contiguous() for p in unique])
+    outputs = [[torch.empty((FRAMES, DST_H >> (p > 0), DST_W >> (p > 0)), dtype=torch.uint16,
+                            device=dev) for p in range(3)] for _ in range(nbuf)]
+    inter = [torch.empty((FRAMES, SRC_H >> (p > 0), DST_W >> (p > 0)), dtype=torch.float32,
+                         device=dev) for p in range(2)]
+
+    def step(s, events=None):
+        ins, outs = inputs[s % nbuf], outputs[s % nbuf]
+        for p in range(3):
+            hp, vp = plans[0 if p == 0 else 1]
+            x, y, t = ins[p], outs[p], inter[0 if p == 0 else 1]
+            conv = ctypes.byref(convs[0 if p == 0 else 1])
+            if events is not None and p == 0:
+                events[0].record(stream)
+            _lib.check(lib.dsmvc_hip_inverse_rows_batch(
+                ex._handle, hp.handle, 2, ctypes.c_void_p(x.data_ptr()), x.stride(1), x.stride(0),
+                ctypes.c_void_p(t.data_ptr()), t.stride(1), t.stride(0), x.shape[1], FRAMES, conv,
+                sptr, _lib.ASYNC))
+            if events is not None and p == 0:
+                events[1].record(stream)
+            _lib.check(lib.dsmvc_hip_inverse_columns_batch(
+                ex._handle, vp.handle, ctypes.c_void_p(t.data_ptr()), t.stride(1), t.stride(0), 2,
+                ctypes.c_void_p(y.data_ptr()), y.stride(1), y.stride(0), t.shape[2], FRAMES, conv,
+                sptr, _lib.ASYNC))
+            if events is not None and p == 0:
+                events[2].record(stream)
+
+    for s in range(args.warmup):
+        step(s)
+    torch.cuda.synchronize()
+    ex.synchronize()
+    events = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        step(s, events[s])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ex.synchronize()
+    h_ms = np.array([e[0].elapsed_time(e[1]) for e in events])
+    v_ms = np.array([e[1].elapsed_time(e[2]) for e in events])
+    fps = FRAMES * args.steps / elapsed
+
+    # Per-axis algorithmic bytes (source read once + destination written once),
+    # luma plane, one launch = 64 frames.
+    h_bytes = FRAMES * SRC_H * (2 * SRC_W + 4 * DST_W)
+    v_bytes = FRAMES * DST_W * (4 * SRC_H + 2 * DST_H)
+    h_avg, v_avg = float(np.mean(h_ms)) * 1e-3, float(np.mean(v_ms)) * 1e-3
+    if h_avg >= v_avg:
+        dom, dom_bytes, dom_s = "rows_chunked<u16> (luma horizontal pass, 64 frames)", h_bytes, h_avg
+    else:
+        dom, dom_bytes, dom_s = "columns_chunked<u16> (luma vertical pass, 64 frames)", v_bytes, v_avg
+    achieved = dom_bytes / dom_s / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                "kernel": dom, "algorithmic_bytes_per_launch": dom_bytes,
+                "avg_launch_ms": round(dom_s * 1e3, 4),
+                "luma_rows_ms": round(h_avg * 1e3, 4), "luma_columns_ms": round(v_avg * 1e3, 4)}
+    frame_gbps = FRAME_BYTES_YUV16 * fps / 1e9
+
+    extra = {"frame_hbm": {"algorithmic_bytes_per_frame": FRAME_BYTES_YUV16,
+                           "achieved_GBps": round(frame_gbps, 1),
+                           "frac_of_8TBps": round(frame_gbps / HBM_PEAK_GBPS, 4)},
+             "input_rotation_bytes": nbuf * batch_in_bytes, "solve": args.solve}
+
+    if not args.no_gray:
+        # GRAYS F32 Debicubic 1080p -> 720p through the 2D batch entry.
+        hp, vp = plans[0]
+        gin = [torch.rand((FRAMES, SRC_H, SRC_W), device=dev) for _ in range(nbuf)]
+        gout = torch.empty((FRAMES, DST_H, DST_W), device=dev)
+        for s in range(5):
+            ex.inverse_2d(hp, vp, gin[s % nbuf], out=gout, asynchronous=True)
+        torch.cuda.synchronize()
+        gsteps = max(20, args.steps // 4)
+        t0 = time.perf_counter()
+        for s in range(gsteps):
+            ex.inverse_2d(hp, vp, gin[s % nbuf], out=gout, asynchronous=True)
+        torch.cuda.synchronize()
+        gel = time.perf_counter() - t0
+        gfps = FRAMES * gsteps / gel
+        extra["grays_f32_debicubic_1080p_720p"] = {
+            "frames_per_s": round(gfps, 1),
+            "achieved_GBps": round(FRAME_BYTES_GRAY_F32 * gfps / 1e9, 1),
+            "frac_of_8TBps": round(FRAME_BYTES_GRAY_F32 * gfps / 1e9 / HBM_PEAK_GBPS, 4)}
+        del gin
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        host = [p[:4].cpu().numpy().astype(np.uint16) for p in unique]
+        cpu = cpu_baseline(host)
+
+    line = {"metric": METRIC, "value": round(fps, 1), "unit": "frames/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(elapsed / args.steps * 1e3, 4), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "Debicubic(b=0,c=0.5) YUV420P16 1920x1080->1280x720, "
+                                   "F32 internal, 16-bit output, padding=3, limited range",
+                       "frames_per_step": FRAMES, "parallelism": "single GPU"},
+            "roofline": roofline, "cpu_baseline": cpu, "extra": extra}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

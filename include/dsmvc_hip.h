@@ -1,0 +1,206 @@
+/*
+ * dsmvc_hip.h -- C ABI of the MI355X (gfx950) descale executor.
+ *
+ * The drop-in boundary for `backend="hip"`: plain C types, pointers and sizes,
+ * int status codes, no exceptions and no C++/torch types across it.  Every
+ * entry point names the reference interface it replaces (paths relative to
+ * the reference repository, AmusementClub/vapoursynth-descalemvc v0.1.4).
+ *
+ * Buffers passed to the inverse_* calls are DEVICE pointers on the
+ * executor's GPU.  Strides are in elements (samples), as in the reference's
+ * buffer contract (include/dsmvc/engine.hpp:231-235): a stride covers the
+ * logical width, only the logical width of each output row is written, and
+ * the last row needs no pitch padding.  Calls are synchronous -- they return
+ * after the output is written, like the reference's executors
+ * (src/cuda/cuda_executor.cpp:2968-2990) -- unless DSMVC_HIP_ASYNC is set, in
+ * which case work is queued on `stream` and dsmvc_hip_synchronize() reports
+ * any deferred Float64 non-finite failure.  `stream` is a hipStream_t (NULL:
+ * the null stream).
+ */
+#ifndef DSMVC_HIP_H
+#define DSMVC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  Messages: dsmvc_hip_last_error() (thread-local). */
+#define DSMVC_HIP_OK 0
+#define DSMVC_HIP_INVALID_ARGUMENT 1 /* reference: std::invalid_argument */
+#define DSMVC_HIP_RUNTIME_ERROR 2    /* reference: std::runtime_error   */
+#define DSMVC_HIP_UNAVAILABLE 3      /* no HIP device                   */
+
+/* Flags of the inverse_* calls. */
+#define DSMVC_HIP_ASYNC 1u
+
+/* Kernel kinds, border (padding) modes, f64mode values and backends keep the
+ * reference numbering (include/dsmvc/engine.hpp:16-58); backend 5 is hip. */
+enum { DSMVC_HIP_BILINEAR = 0, DSMVC_HIP_BICUBIC = 1, DSMVC_HIP_LANCZOS = 2,
+       DSMVC_HIP_SPLINE16 = 3, DSMVC_HIP_SPLINE36 = 4, DSMVC_HIP_SPLINE64 = 5 };
+enum { DSMVC_HIP_SOLVE_AUTO = 0, DSMVC_HIP_SOLVE_EXACT = 1, DSMVC_HIP_SOLVE_CHUNKED = 2 };
+enum { DSMVC_HIP_SAMPLE_F32 = 0, DSMVC_HIP_SAMPLE_U8 = 1, DSMVC_HIP_SAMPLE_U16 = 2 };
+
+/* dsmvc::AxisRequest with KernelSpec flattened (include/dsmvc/engine.hpp:60-78). */
+typedef struct dsmvc_hip_axis_request {
+    int32_t source_size;
+    int32_t destination_size;
+    double active_length;
+    double shift;
+    int32_t kernel;
+    int32_t taps;
+    double b;
+    double c;
+    double blur;
+    int32_t border;
+    int32_t f64_mode;
+} dsmvc_hip_axis_request;
+
+/* dsmvc::IntegerConversion (include/dsmvc/engine.hpp:167-173). */
+typedef struct dsmvc_hip_integer_conversion {
+    float input_offset;
+    float input_scale;
+    float output_scale;
+    float output_offset;
+    uint32_t output_maximum;
+} dsmvc_hip_integer_conversion;
+
+/* Scalar fields of dsmvc::AxisPlan (include/dsmvc/engine.hpp:82-107). */
+typedef struct dsmvc_hip_plan_info {
+    int32_t source_size;
+    int32_t destination_size;
+    int32_t support;
+    int32_t half_bandwidth;
+    double active_length;
+    double shift;
+    double normal_rcond;
+    double normal_inf_norm;
+    int32_t requires_float64;
+    int32_t nnz;
+} dsmvc_hip_plan_info;
+
+typedef struct dsmvc_hip_planner_stats {
+    uint64_t plan_hits;
+    uint64_t plan_builds;
+    uint64_t plan_entries;
+    uint64_t plan_resident_bytes;
+} dsmvc_hip_planner_stats;
+
+typedef struct dsmvc_hip_plan dsmvc_hip_plan;         /* immutable axis plan */
+typedef struct dsmvc_hip_executor dsmvc_hip_executor; /* one GPU            */
+
+const char *dsmvc_hip_last_error(void);
+
+/* ---- planner ------------------------------------------------------------ */
+/* Replaces dsmvc::get_or_build_axis_plan (include/dsmvc/engine.hpp:177-178,
+ * src/axis_plan.cpp:1367-1383): single-flight LRU keyed by the request's bits. */
+int dsmvc_hip_plan_build(const dsmvc_hip_axis_request *request, dsmvc_hip_plan **plan);
+void dsmvc_hip_plan_release(dsmvc_hip_plan *plan);
+/* AxisPlan scalar fields. */
+int dsmvc_hip_plan_info_get(const dsmvc_hip_plan *plan, dsmvc_hip_plan_info *info);
+/* Copies AxisPlan's arrays (engine.hpp:90-107) into caller buffers sized from
+ * the info: offsets n+1, indices/weights nnz, lower_ld/upper_l W*n,
+ * inverse_diagonal n; the F64 arrays (nnz, (W+1)*n, (W+1)*n, n) only when
+ * requires_float64.  NULL skips an array. */
+int dsmvc_hip_plan_arrays(const dsmvc_hip_plan *plan, uint32_t *offsets, int32_t *indices,
+                          float *weights, float *lower_ld, float *upper_l,
+                          float *inverse_diagonal, double *weights_f64,
+                          double *normal_bands_f64, double *ldlt_bands_f64,
+                          double *inverse_diagonal_f64);
+/* dsmvc::planner_cache_stats / clear_planner_caches (engine.hpp:179-180). */
+int dsmvc_hip_planner_cache_stats(dsmvc_hip_planner_stats *stats);
+void dsmvc_hip_clear_planner_caches(void);
+/* dsmvc::detail::padding_index (src/axis_plan.cpp:1024-1066). */
+int dsmvc_hip_padding_index(int64_t index, int32_t size, int32_t mode, int32_t *mapped);
+
+/* ---- backends ----------------------------------------------------------- */
+/* dsmvc::parse_backend / resolve_backend (src/backend.cpp:107-142): "hip" is
+ * accepted; resolving hip without a device fails, never falls back. */
+int dsmvc_hip_parse_backend(const char *name, int32_t *kind);
+int dsmvc_hip_resolve_backend(int32_t requested, int32_t *resolved);
+int dsmvc_hip_device_count(int32_t *count);
+
+/* ---- executor ----------------------------------------------------------- */
+/* dsmvc::Executor(BackendKind::hip) (engine.hpp:363-372, src/executor.cpp:39-59);
+ * device -1 = current device.  solve_mode: DSMVC_HIP_SOLVE_*. */
+int dsmvc_hip_executor_create(int32_t device, int32_t solve_mode, dsmvc_hip_executor **executor);
+void dsmvc_hip_executor_destroy(dsmvc_hip_executor *executor);
+/* Executor::prepare / seal (src/executor.cpp:313-349): uploads the plan image. */
+int dsmvc_hip_executor_prepare(dsmvc_hip_executor *executor, const dsmvc_hip_plan *plan);
+int dsmvc_hip_executor_seal(dsmvc_hip_executor *executor);
+/* Chunk counts of the partitioned solves (0 = automatic). */
+int dsmvc_hip_executor_tune(dsmvc_hip_executor *executor, int32_t rows_chunks,
+                            int32_t columns_chunks);
+
+/* Executor::inverse_rows (engine.hpp:468-472, src/executor.cpp:351-377). */
+int dsmvc_hip_inverse_rows(dsmvc_hip_executor *executor, const dsmvc_hip_plan *plan,
+                           const float *input, ptrdiff_t input_row_stride, float *output,
+                           ptrdiff_t output_row_stride, int32_t row_count, void *stream,
+                           uint32_t flags);
+/* Executor::inverse_columns (engine.hpp:474-478, src/executor.cpp:379-404). */
+int dsmvc_hip_inverse_columns(dsmvc_hip_executor *executor, const dsmvc_hip_plan *plan,
+                              const float *input, ptrdiff_t input_row_stride, float *output,
+                              ptrdiff_t output_row_stride, int32_t column_count, void *stream,
+                              uint32_t flags);
+/* Executor::inverse_2d (engine.hpp:480-484, src/executor.cpp:420-445). */
+int dsmvc_hip_inverse_2d(dsmvc_hip_executor *executor, const dsmvc_hip_plan *horizontal,
+                         const dsmvc_hip_plan *vertical, const float *input,
+                         ptrdiff_t input_row_stride, float *output, ptrdiff_t output_row_stride,
+                         void *stream, uint32_t flags);
+/* Executor::inverse_2d_u8 / inverse_2d_u16 (engine.hpp:486-498,
+ * src/executor.cpp:447-501); the streamed variants (503-557) compute the same
+ * values and map here too. */
+int dsmvc_hip_inverse_2d_u8(dsmvc_hip_executor *executor, const dsmvc_hip_plan *horizontal,
+                            const dsmvc_hip_plan *vertical, const uint8_t *input,
+                            ptrdiff_t input_row_stride, uint8_t *output,
+                            ptrdiff_t output_row_stride,
+                            const dsmvc_hip_integer_conversion *conversion, void *stream,
+                            uint32_t flags);
+int dsmvc_hip_inverse_2d_u16(dsmvc_hip_executor *executor, const dsmvc_hip_plan *horizontal,
+                             const dsmvc_hip_plan *vertical, const uint16_t *input,
+                             ptrdiff_t input_row_stride, uint16_t *output,
+                             ptrdiff_t output_row_stride,
+                             const dsmvc_hip_integer_conversion *conversion, void *stream,
+                             uint32_t flags);
+/* A batch of `frames` equally shaped planes in one submission: the reference
+ * runs one plane per call (src/vs_plugin.cpp:1636-1745); this entry keeps its
+ * arithmetic and adds a frame stride (elements).  sample_kind:
+ * DSMVC_HIP_SAMPLE_*; conversion is read for integer samples only. */
+int dsmvc_hip_inverse_2d_batch(dsmvc_hip_executor *executor, const dsmvc_hip_plan *horizontal,
+                               const dsmvc_hip_plan *vertical, int32_t sample_kind,
+                               const void *input, ptrdiff_t input_row_stride,
+                               ptrdiff_t input_frame_stride, void *output,
+                               ptrdiff_t output_row_stride, ptrdiff_t output_frame_stride,
+                               int32_t frames, const dsmvc_hip_integer_conversion *conversion,
+                               void *stream, uint32_t flags);
+/* The two halves of the 2D route as separate batched calls (Float32 plans):
+ * the reference's GPU pipeline runs the horizontal stage into an intermediate
+ * and the vertical stage out of it (src/cuda/cuda_executor.cpp:2866-2949).
+ * rows: `rows` rows of `input_kind` samples -> float32 output rows;
+ * columns: `columns` columns of float32 input -> `output_kind` samples.
+ * Integer kinds need `conversion` (input normalisation / output rounding). */
+int dsmvc_hip_inverse_rows_batch(dsmvc_hip_executor *executor, const dsmvc_hip_plan *plan,
+                                 int32_t input_kind, const void *input,
+                                 ptrdiff_t input_row_stride, ptrdiff_t input_frame_stride,
+                                 float *output, ptrdiff_t output_row_stride,
+                                 ptrdiff_t output_frame_stride, int32_t rows, int32_t frames,
+                                 const dsmvc_hip_integer_conversion *conversion, void *stream,
+                                 uint32_t flags);
+int dsmvc_hip_inverse_columns_batch(dsmvc_hip_executor *executor, const dsmvc_hip_plan *plan,
+                                    const float *input, ptrdiff_t input_row_stride,
+                                    ptrdiff_t input_frame_stride, int32_t output_kind,
+                                    void *output, ptrdiff_t output_row_stride,
+                                    ptrdiff_t output_frame_stride, int32_t columns,
+                                    int32_t frames, const dsmvc_hip_integer_conversion *conversion,
+                                    void *stream, uint32_t flags);
+
+/* Waits for `stream`; reports a deferred Float64 non-finite failure. */
+int dsmvc_hip_synchronize(dsmvc_hip_executor *executor, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DSMVC_HIP_H */

@@ -1,0 +1,367 @@
+// The C ABI (include/dsmvc_hip.h) over the C++ planner and HIP executor.
+// Exceptions never cross it: invalid_argument / out_of_range -> 1, other
+// exceptions -> 2, missing device -> 3; the message is kept per thread.
+#include "dsmvc_hip.h"
+
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+
+#include "engine.hpp"
+#include "executor.hpp"
+
+struct dsmvc_hip_plan {
+    std::shared_ptr<const dsmvc::AxisPlan> plan;
+};
+
+struct dsmvc_hip_executor {
+    std::unique_ptr<dsmvc::HipExecutor> executor;
+};
+
+namespace {
+
+thread_local std::string g_error;
+
+struct Unavailable : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+template <class F>
+int guarded(F &&f) noexcept {
+    try {
+        f();
+        g_error.clear();
+        return DSMVC_HIP_OK;
+    } catch (const Unavailable &e) {
+        g_error = e.what();
+        return DSMVC_HIP_UNAVAILABLE;
+    } catch (const std::invalid_argument &e) {
+        g_error = e.what();
+        return DSMVC_HIP_INVALID_ARGUMENT;
+    } catch (const std::out_of_range &e) {
+        g_error = e.what();
+        return DSMVC_HIP_INVALID_ARGUMENT;
+    } catch (const std::exception &e) {
+        g_error = e.what();
+        return DSMVC_HIP_RUNTIME_ERROR;
+    } catch (...) {
+        g_error = "dsmvc: unknown error";
+        return DSMVC_HIP_RUNTIME_ERROR;
+    }
+}
+
+void require(bool ok, const char *what) {
+    if (!ok) throw std::invalid_argument(what);
+}
+
+dsmvc::AxisRequest to_request(const dsmvc_hip_axis_request &r) {
+    dsmvc::AxisRequest q;
+    q.source_size = r.source_size;
+    q.destination_size = r.destination_size;
+    q.active_length = r.active_length;
+    q.shift = r.shift;
+    require(r.kernel >= 0 && r.kernel <= 5, "kernel is invalid");
+    q.kernel.kind = static_cast<dsmvc::KernelKind>(r.kernel);
+    q.kernel.taps = r.taps;
+    q.kernel.b = r.b;
+    q.kernel.c = r.c;
+    q.kernel.blur = r.blur;
+    require(r.border >= 0 && r.border <= 4, "padding mode is invalid");
+    q.border = static_cast<dsmvc::BorderMode>(r.border);
+    require(r.f64_mode >= 0 && r.f64_mode <= 2, "f64 mode is invalid");
+    q.f64_mode = static_cast<dsmvc::F64Mode>(r.f64_mode);
+    return q;
+}
+
+dsmvc::IntegerConversion to_conversion(const dsmvc_hip_integer_conversion &c) {
+    dsmvc::IntegerConversion r;
+    r.input_offset = c.input_offset;
+    r.input_scale = c.input_scale;
+    r.output_scale = c.output_scale;
+    r.output_offset = c.output_offset;
+    r.output_maximum = c.output_maximum;
+    return r;
+}
+
+dsmvc::HipExecutor &exec_of(dsmvc_hip_executor *e) {
+    require(e && e->executor, "dsmvc: executor is null");
+    return *e->executor;
+}
+
+const dsmvc::AxisPlan &plan_of(const dsmvc_hip_plan *p) {
+    require(p && p->plan, "dsmvc: plan is null");
+    return *p->plan;
+}
+
+hipStream_t stream_of(void *s) { return static_cast<hipStream_t>(s); }
+
+bool is_async(uint32_t flags) { return (flags & DSMVC_HIP_ASYNC) != 0U; }
+
+} // namespace
+
+extern "C" {
+
+const char *dsmvc_hip_last_error(void) { return g_error.c_str(); }
+
+int dsmvc_hip_plan_build(const dsmvc_hip_axis_request *request, dsmvc_hip_plan **plan) {
+    return guarded([&] {
+        require(request && plan, "dsmvc: null argument");
+        auto handle = std::make_unique<dsmvc_hip_plan>();
+        handle->plan = dsmvc::get_or_build_axis_plan(to_request(*request));
+        *plan = handle.release();
+    });
+}
+
+void dsmvc_hip_plan_release(dsmvc_hip_plan *plan) { delete plan; }
+
+int dsmvc_hip_plan_info_get(const dsmvc_hip_plan *plan, dsmvc_hip_plan_info *info) {
+    return guarded([&] {
+        const auto &p = plan_of(plan);
+        require(info, "dsmvc: null argument");
+        info->source_size = p.source_size;
+        info->destination_size = p.destination_size;
+        info->support = p.support;
+        info->half_bandwidth = p.half_bandwidth;
+        info->active_length = p.active_length;
+        info->shift = p.shift;
+        info->normal_rcond = p.normal_rcond;
+        info->normal_inf_norm = p.normal_inf_norm;
+        info->requires_float64 = p.requires_float64() ? 1 : 0;
+        info->nnz = static_cast<int32_t>(p.transpose_indices.size());
+    });
+}
+
+int dsmvc_hip_plan_arrays(const dsmvc_hip_plan *plan, uint32_t *offsets, int32_t *indices,
+                          float *weights, float *lower_ld, float *upper_l,
+                          float *inverse_diagonal, double *weights_f64,
+                          double *normal_bands_f64, double *ldlt_bands_f64,
+                          double *inverse_diagonal_f64) {
+    return guarded([&] {
+        const auto &p = plan_of(plan);
+        auto copy = [](auto *dst, const auto &src) {
+            if (dst && !src.empty()) std::memcpy(dst, src.data(), src.size() * sizeof(src[0]));
+        };
+        copy(offsets, p.transpose_offsets);
+        copy(indices, p.transpose_indices);
+        copy(weights, p.transpose_weights);
+        copy(lower_ld, p.lower_ld);
+        copy(upper_l, p.upper_l);
+        copy(inverse_diagonal, p.inverse_diagonal);
+        copy(weights_f64, p.transpose_weights_f64);
+        copy(normal_bands_f64, p.normal_bands_f64);
+        copy(ldlt_bands_f64, p.ldlt_bands_f64);
+        copy(inverse_diagonal_f64, p.inverse_diagonal_f64);
+    });
+}
+
+int dsmvc_hip_planner_cache_stats(dsmvc_hip_planner_stats *stats) {
+    return guarded([&] {
+        require(stats, "dsmvc: null argument");
+        const auto s = dsmvc::planner_cache_stats();
+        stats->plan_hits = s.plan_hits;
+        stats->plan_builds = s.plan_builds;
+        stats->plan_entries = s.plan_entries;
+        stats->plan_resident_bytes = s.plan_resident_bytes;
+    });
+}
+
+void dsmvc_hip_clear_planner_caches(void) { dsmvc::clear_planner_caches(); }
+
+int dsmvc_hip_padding_index(int64_t index, int32_t size, int32_t mode, int32_t *mapped) {
+    return guarded([&] {
+        require(mapped, "dsmvc: null argument");
+        require(mode >= 0 && mode <= 4, "padding mode is invalid");
+        *mapped = dsmvc::padding_index(index, size, static_cast<dsmvc::BorderMode>(mode));
+    });
+}
+
+int dsmvc_hip_parse_backend(const char *name, int32_t *kind) {
+    return guarded([&] {
+        require(name && kind, "dsmvc: null argument");
+        *kind = static_cast<int32_t>(dsmvc::parse_backend(name));
+    });
+}
+
+int dsmvc_hip_resolve_backend(int32_t requested, int32_t *resolved) {
+    return guarded([&] {
+        require(resolved && requested >= 0 && requested <= 5, "backend is invalid");
+        const auto kind = static_cast<dsmvc::BackendKind>(requested);
+        if (kind == dsmvc::BackendKind::hip && !dsmvc::hip_available())
+            throw Unavailable("backend hip is unavailable");
+        *resolved = static_cast<int32_t>(dsmvc::resolve_backend(kind));
+    });
+}
+
+int dsmvc_hip_device_count(int32_t *count) {
+    return guarded([&] {
+        require(count, "dsmvc: null argument");
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+        *count = n;
+    });
+}
+
+int dsmvc_hip_executor_create(int32_t device, int32_t solve_mode, dsmvc_hip_executor **executor) {
+    return guarded([&] {
+        require(executor && solve_mode >= 0 && solve_mode <= 2, "dsmvc: invalid executor arguments");
+        if (!dsmvc::hip_available()) throw Unavailable("dsmvc: backend hip is unavailable: no HIP device");
+        auto handle = std::make_unique<dsmvc_hip_executor>();
+        handle->executor = std::make_unique<dsmvc::HipExecutor>(
+            device, static_cast<dsmvc::SolveMode>(solve_mode));
+        *executor = handle.release();
+    });
+}
+
+void dsmvc_hip_executor_destroy(dsmvc_hip_executor *executor) { delete executor; }
+
+int dsmvc_hip_executor_prepare(dsmvc_hip_executor *executor, const dsmvc_hip_plan *plan) {
+    return guarded([&] {
+        require(plan && plan->plan, "dsmvc: plan is null");
+        exec_of(executor).prepare(plan->plan);
+    });
+}
+
+int dsmvc_hip_executor_seal(dsmvc_hip_executor *executor) {
+    return guarded([&] { exec_of(executor).seal(); });
+}
+
+int dsmvc_hip_executor_tune(dsmvc_hip_executor *executor, int32_t rows_chunks,
+                            int32_t columns_chunks) {
+    return guarded([&] {
+        require(rows_chunks >= 0 && rows_chunks <= 16 && columns_chunks >= 0 && columns_chunks <= 16,
+                "dsmvc: chunk counts must be 0..16");
+        exec_of(executor).set_tuning({rows_chunks, columns_chunks});
+    });
+}
+
+int dsmvc_hip_inverse_rows(dsmvc_hip_executor *executor, const dsmvc_hip_plan *plan,
+                           const float *input, ptrdiff_t input_row_stride, float *output,
+                           ptrdiff_t output_row_stride, int32_t row_count, void *stream,
+                           uint32_t flags) {
+    return guarded([&] {
+        exec_of(executor).inverse_rows(plan_of(plan), input, input_row_stride, output,
+                                       output_row_stride, row_count, stream_of(stream),
+                                       is_async(flags));
+    });
+}
+
+int dsmvc_hip_inverse_columns(dsmvc_hip_executor *executor, const dsmvc_hip_plan *plan,
+                              const float *input, ptrdiff_t input_row_stride, float *output,
+                              ptrdiff_t output_row_stride, int32_t column_count, void *stream,
+                              uint32_t flags) {
+    return guarded([&] {
+        exec_of(executor).inverse_columns(plan_of(plan), input, input_row_stride, output,
+                                          output_row_stride, column_count, stream_of(stream),
+                                          is_async(flags));
+    });
+}
+
+int dsmvc_hip_inverse_2d(dsmvc_hip_executor *executor, const dsmvc_hip_plan *horizontal,
+                         const dsmvc_hip_plan *vertical, const float *input,
+                         ptrdiff_t input_row_stride, float *output, ptrdiff_t output_row_stride,
+                         void *stream, uint32_t flags) {
+    return guarded([&] {
+        exec_of(executor).inverse_2d(plan_of(horizontal), plan_of(vertical), input,
+                                     input_row_stride, output, output_row_stride,
+                                     stream_of(stream), is_async(flags));
+    });
+}
+
+int dsmvc_hip_inverse_2d_u8(dsmvc_hip_executor *executor, const dsmvc_hip_plan *horizontal,
+                            const dsmvc_hip_plan *vertical, const uint8_t *input,
+                            ptrdiff_t input_row_stride, uint8_t *output,
+                            ptrdiff_t output_row_stride,
+                            const dsmvc_hip_integer_conversion *conversion, void *stream,
+                            uint32_t flags) {
+    return guarded([&] {
+        require(conversion, "dsmvc: conversion is null");
+        exec_of(executor).inverse_2d_u8(plan_of(horizontal), plan_of(vertical), input,
+                                        input_row_stride, output, output_row_stride,
+                                        to_conversion(*conversion), stream_of(stream),
+                                        is_async(flags));
+    });
+}
+
+int dsmvc_hip_inverse_2d_u16(dsmvc_hip_executor *executor, const dsmvc_hip_plan *horizontal,
+                             const dsmvc_hip_plan *vertical, const uint16_t *input,
+                             ptrdiff_t input_row_stride, uint16_t *output,
+                             ptrdiff_t output_row_stride,
+                             const dsmvc_hip_integer_conversion *conversion, void *stream,
+                             uint32_t flags) {
+    return guarded([&] {
+        require(conversion, "dsmvc: conversion is null");
+        exec_of(executor).inverse_2d_u16(plan_of(horizontal), plan_of(vertical), input,
+                                         input_row_stride, output, output_row_stride,
+                                         to_conversion(*conversion), stream_of(stream),
+                                         is_async(flags));
+    });
+}
+
+int dsmvc_hip_inverse_2d_batch(dsmvc_hip_executor *executor, const dsmvc_hip_plan *horizontal,
+                               const dsmvc_hip_plan *vertical, int32_t sample_kind,
+                               const void *input, ptrdiff_t input_row_stride,
+                               ptrdiff_t input_frame_stride, void *output,
+                               ptrdiff_t output_row_stride, ptrdiff_t output_frame_stride,
+                               int32_t frames, const dsmvc_hip_integer_conversion *conversion,
+                               void *stream, uint32_t flags) {
+    return guarded([&] {
+        require(sample_kind >= 0 && sample_kind <= 2, "dsmvc: sample kind is invalid");
+        dsmvc::IntegerConversion conv;
+        if (conversion) conv = to_conversion(*conversion);
+        const dsmvc::PlaneBatch batch{input, input_row_stride, input_frame_stride, output,
+                                      output_row_stride, output_frame_stride, frames};
+        exec_of(executor).inverse_2d_batch(plan_of(horizontal), plan_of(vertical),
+                                           static_cast<dsmvc::SampleKind>(sample_kind), batch,
+                                           conversion ? &conv : nullptr, stream_of(stream),
+                                           is_async(flags));
+    });
+}
+
+int dsmvc_hip_inverse_rows_batch(dsmvc_hip_executor *executor, const dsmvc_hip_plan *plan,
+                                 int32_t input_kind, const void *input,
+                                 ptrdiff_t input_row_stride, ptrdiff_t input_frame_stride,
+                                 float *output, ptrdiff_t output_row_stride,
+                                 ptrdiff_t output_frame_stride, int32_t rows, int32_t frames,
+                                 const dsmvc_hip_integer_conversion *conversion, void *stream,
+                                 uint32_t flags) {
+    return guarded([&] {
+        require(input_kind >= 0 && input_kind <= 2, "dsmvc: sample kind is invalid");
+        dsmvc::IntegerConversion conv;
+        if (conversion) conv = to_conversion(*conversion);
+        const dsmvc::PlaneBatch batch{input, input_row_stride, input_frame_stride, output,
+                                      output_row_stride, output_frame_stride, frames};
+        exec_of(executor).inverse_rows_batch(plan_of(plan),
+                                             static_cast<dsmvc::SampleKind>(input_kind), batch,
+                                             rows, conversion ? &conv : nullptr,
+                                             stream_of(stream), is_async(flags));
+    });
+}
+
+int dsmvc_hip_inverse_columns_batch(dsmvc_hip_executor *executor, const dsmvc_hip_plan *plan,
+                                    const float *input, ptrdiff_t input_row_stride,
+                                    ptrdiff_t input_frame_stride, int32_t output_kind,
+                                    void *output, ptrdiff_t output_row_stride,
+                                    ptrdiff_t output_frame_stride, int32_t columns,
+                                    int32_t frames, const dsmvc_hip_integer_conversion *conversion,
+                                    void *stream, uint32_t flags) {
+    return guarded([&] {
+        require(output_kind >= 0 && output_kind <= 2, "dsmvc: sample kind is invalid");
+        dsmvc::IntegerConversion conv;
+        if (conversion) conv = to_conversion(*conversion);
+        const dsmvc::PlaneBatch batch{input, input_row_stride, input_frame_stride, output,
+                                      output_row_stride, output_frame_stride, frames};
+        exec_of(executor).inverse_columns_batch(plan_of(plan),
+                                                static_cast<dsmvc::SampleKind>(output_kind),
+                                                batch, columns, conversion ? &conv : nullptr,
+                                                stream_of(stream), is_async(flags));
+    });
+}
+
+int dsmvc_hip_synchronize(dsmvc_hip_executor *executor, void *stream) {
+    return guarded([&] { exec_of(executor).synchronize(stream_of(stream)); });
+}
+
+} // extern "C"

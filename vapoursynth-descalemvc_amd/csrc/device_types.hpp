@@ -1,0 +1,65 @@
+// POD descriptors shared by the host executor and the gfx950 kernels.
+// Passed by value as kernel arguments; all pointers are device pointers.
+#pragma once
+
+#include <cstdint>
+
+namespace dsmvc::hip {
+
+// Row-record width of the packed substitution coefficients.  Row i holds
+//   [0 .. 6]   lld_d(i) = L(i, i-d) * D(i-d)   for d = 1..7 (zero past W)
+//   [7]        1 / (D_i + eps)
+//   [8 .. 14]  ul_d(i)  = L(i+d, i)           for d = 1..7 (zero past W)
+// so one 64-byte scalar load fetches everything a wavefront needs for row i.
+// Bands 8..15 (blurred wide kernels) use the same layout with 15 slots per
+// side: 32-float rows, the diagonal at [15], upper entries at [16 .. 30].
+constexpr int kMaxBand = 7;
+constexpr int kMaxWideBand = 15;
+
+constexpr int coef_stride(int w) { return w <= kMaxBand ? 16 : 32; }
+
+struct DevAxis {
+    std::int32_t n;      // destination length
+    std::int32_t m;      // source length
+    std::int32_t w;      // half bandwidth
+    std::int32_t nnz;
+    const std::uint32_t *off;   // [n + 1] CSR of A^T by destination element
+    const std::int32_t *idx;    // [nnz]   source index, ascending per row
+    const float *wt;            // [nnz]
+    const float *coef;          // [n * coef_stride(w)]
+    // Float64 route: weights and coefficient rows in double.  For plans that
+    // retain FP64 factors (zform = 1) the forward rows hold unit-L entries
+    // L(i, i-d) and the diagonal is applied after the recurrence (the
+    // recurrence runs on z = L^-1 b); otherwise they are the promoted F32 rows.
+    const double *wt64;
+    const double *coef64;
+    std::int32_t zform;
+    std::int32_t pad_;
+};
+
+// Partition of each vector into C chunks of L elements (the last may be
+// shorter) for the chunked substitution kernels, with the FP64-built W x W
+// carry transfer matrices of every chunk, stored F32:
+//   tf[q][r][c] = d y[b_q - 1 - r] / d y[a_q - 1 - c]   (forward)
+//   tb[q][r][c] = d x[a_q + r]     / d x[b_q + c]       (backward)
+// and the source window [win[2q], win[2q+1]] the chunk's CSR rows touch.
+struct DevChunks {
+    std::int32_t chunks;
+    std::int32_t length;
+    const float *tf;
+    const float *tb;
+    const std::int32_t *win;
+};
+
+struct DevConv {
+    float in_offset;
+    float in_scale;
+    float out_scale;
+    float out_offset;
+    float out_max;
+    std::int32_t pad_;
+};
+
+enum class Sample : std::int32_t { f32 = 0, u8 = 1, u16 = 2, f64 = 3 };
+
+} // namespace dsmvc::hip

@@ -1,0 +1,437 @@
+// HIP executor: route selection, scratch management and launches.
+#include "executor.hpp"
+
+#include <algorithm>
+#include <bit>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <stdexcept>
+#include <string>
+
+#include "kernels.hpp"
+
+namespace dsmvc {
+
+using hip::DevConv;
+using hip::Sample;
+
+namespace {
+
+void check(hipError_t e, const char *what) {
+    if (e != hipSuccess)
+        throw std::runtime_error(std::string("dsmvc: ") + what + ": " + hipGetErrorString(e));
+}
+
+std::size_t align256(std::size_t v) { return (v + 255U) & ~std::size_t{255}; }
+
+std::ptrdiff_t padded_width(std::int32_t n) { return (static_cast<std::ptrdiff_t>(n) + 63) & ~std::ptrdiff_t{63}; }
+
+Sample sample_of(SampleKind k) {
+    switch (k) {
+    case SampleKind::float32: return Sample::f32;
+    case SampleKind::u8: return Sample::u8;
+    case SampleKind::u16: return Sample::u16;
+    }
+    return Sample::f32;
+}
+
+DevConv make_conv(const IntegerConversion *c) {
+    DevConv d{};
+    if (c) {
+        d.in_offset = c->input_offset;
+        d.in_scale = c->input_scale;
+        d.out_scale = c->output_scale;
+        d.out_offset = c->output_offset;
+        d.out_max = static_cast<float>(c->output_maximum);
+    } else {
+        d.in_scale = 1.0F;
+        d.out_scale = 1.0F;
+        d.out_max = 0.0F;
+    }
+    return d;
+}
+
+bool same_plan(const AxisPlan &a, const AxisPlan &b) {
+    return a.source_size == b.source_size && a.destination_size == b.destination_size
+        && a.half_bandwidth == b.half_bandwidth
+        && a.transpose_indices.size() == b.transpose_indices.size()
+        && std::bit_cast<std::uint64_t>(a.normal_rcond) == std::bit_cast<std::uint64_t>(b.normal_rcond)
+        && std::bit_cast<std::uint64_t>(a.active_length) == std::bit_cast<std::uint64_t>(b.active_length)
+        && std::bit_cast<std::uint64_t>(a.shift) == std::bit_cast<std::uint64_t>(b.shift)
+        && a.requires_float64() == b.requires_float64()
+        && a.transpose_weights == b.transpose_weights && a.inverse_diagonal == b.inverse_diagonal;
+}
+
+void validate_conversion(const IntegerConversion &c, std::uint32_t sample_max, const char *what) {
+    if (!(c.input_scale > 0.0F) || !(c.output_scale > 0.0F) || !std::isfinite(c.input_offset)
+        || !std::isfinite(c.input_scale) || !std::isfinite(c.output_scale)
+        || !std::isfinite(c.output_offset) || c.output_maximum == 0U || c.output_maximum > sample_max)
+        throw std::invalid_argument(what);
+}
+
+} // namespace
+
+bool hip_available() noexcept {
+    int count = 0;
+    return hipGetDeviceCount(&count) == hipSuccess && count > 0;
+}
+
+HipExecutor::HipExecutor(int device, SolveMode mode) : device_(device), mode_(mode) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+        throw std::runtime_error("dsmvc: backend hip is unavailable: no HIP device");
+    if (device_ < 0) check(hipGetDevice(&device_), "hipGetDevice failed");
+    if (device_ >= count) throw std::invalid_argument("dsmvc: HIP device index out of range");
+    check(hipSetDevice(device_), "hipSetDevice failed");
+}
+
+HipExecutor::~HipExecutor() {
+    (void)hipSetDevice(device_);
+    for (auto &entry : scratch_) {
+        if (entry.first) (void)hipStreamSynchronize(entry.first);
+        else (void)hipDeviceSynchronize();
+        if (entry.second.buffer) (void)hipFree(entry.second.buffer);
+        if (entry.second.flag) (void)hipFree(entry.second.flag);
+    }
+    plans_.clear();
+}
+
+void HipExecutor::prepare(const std::shared_ptr<const AxisPlan> &plan) {
+    if (!plan || !plan->valid())
+        throw std::invalid_argument("dsmvc: cannot prepare an invalid HIP axis plan");
+    std::lock_guard lock(mutex_);
+    (void)device_plan(*plan);
+}
+
+void HipExecutor::seal() {
+    std::lock_guard lock(mutex_);
+    sealed_ = true;
+}
+
+hip::DevicePlan &HipExecutor::device_plan(const AxisPlan &plan) {
+    // Keyed by address; an entry whose plan no longer matches (the address
+    // was reused by another plan) is rebuilt.
+    auto it = plans_.find(&plan);
+    if (it != plans_.end() && same_plan(it->second->host(), plan)) return *it->second;
+    check(hipSetDevice(device_), "hipSetDevice failed");
+    auto image = std::make_unique<hip::DevicePlan>(plan, device_);
+    auto &ref = *image;
+    plans_[&plan] = std::move(image);
+    return ref;
+}
+
+HipExecutor::Scratch &HipExecutor::scratch(hipStream_t stream, std::size_t bytes) {
+    Scratch &s = scratch_[stream];
+    if (!s.flag) check(hipMalloc(reinterpret_cast<void **>(&s.flag), 256), "scratch allocation failed");
+    if (s.bytes < bytes) {
+        if (s.buffer) {
+            check(stream ? hipStreamSynchronize(stream) : hipDeviceSynchronize(), "stream synchronize failed");
+            check(hipFree(s.buffer), "scratch release failed");
+            s.buffer = nullptr;
+            s.bytes = 0;
+        }
+        const std::size_t grown = align256(std::max(bytes, s.bytes + s.bytes / 2));
+        check(hipMalloc(&s.buffer, grown), "scratch allocation failed");
+        s.bytes = grown;
+    }
+    return s;
+}
+
+void HipExecutor::finish(hipStream_t stream, Scratch &s, bool async, const char *what) {
+    check(hipGetLastError(), "kernel launch failed");
+    if (async) return;
+    check(stream ? hipStreamSynchronize(stream) : hipDeviceSynchronize(), "execution failed");
+    if (s.flag_armed) {
+        int flag = 0;
+        check(hipMemcpy(&flag, s.flag, sizeof flag, hipMemcpyDeviceToHost), "flag readback failed");
+        s.flag_armed = false;
+        if (flag) throw std::runtime_error(what);
+    }
+}
+
+void HipExecutor::synchronize(hipStream_t stream) {
+    std::lock_guard lock(mutex_);
+    check(hipSetDevice(device_), "hipSetDevice failed");
+    auto it = scratch_.find(stream);
+    if (it == scratch_.end()) {
+        check(stream ? hipStreamSynchronize(stream) : hipDeviceSynchronize(), "execution failed");
+        return;
+    }
+    finish(stream, it->second, false, "dsmvc: HIP Float64 execution produced NaN or infinity");
+}
+
+// One axis pass over `vectors` rows (rows = true) or columns of `frames`
+// planes.  Picks the chunked kernels when the mode and plan allow, else the
+// sequential solve, whose forward sweep goes to scratch at `fwd_offset`.
+void HipExecutor::axis_pass(const AxisPlan &plan, bool rows, const void *input, Sample in_kind,
+                            std::ptrdiff_t in_row, std::ptrdiff_t in_frame, void *output,
+                            Sample out_kind, std::ptrdiff_t out_row, std::ptrdiff_t out_frame,
+                            std::int32_t vectors, std::int32_t frames, const DevConv &conv,
+                            bool f64, Scratch &s, std::size_t fwd_offset, hipStream_t stream) {
+    hip::DevicePlan &dp = device_plan(plan);
+    if (!f64 && mode_ != SolveMode::exact) {
+        int count = rows ? tuning_.rows_chunks : tuning_.columns_chunks;
+        if (count <= 0) count = hip::choose_chunks(plan, rows);
+        const hip::DevChunks *ch = count > 0 ? dp.chunks(count, rows) : nullptr;
+        if (ch) {
+            hip::ChunkArgs a{};
+            a.axis = dp.axis();
+            a.chunks = *ch;
+            a.src = input;
+            a.s_row = in_row;
+            a.s_frame = in_frame;
+            a.dst = output;
+            a.d_row = out_row;
+            a.d_frame = out_frame;
+            a.nvec = vectors;
+            a.frames = frames;
+            a.conv = conv;
+            check(rows ? hip::launch_rows_chunked(a, in_kind, stream)
+                       : hip::launch_columns_chunked(a, out_kind, stream),
+                  "chunked solve launch failed");
+            return;
+        }
+        if (mode_ == SolveMode::chunked)
+            throw std::invalid_argument("dsmvc: plan cannot use the chunked HIP solve");
+    }
+    if (!hip::seq_band_supported(plan.half_bandwidth))
+        throw std::invalid_argument("dsmvc: half bandwidth exceeds the HIP executor's limit");
+    hip::SeqArgs a{};
+    a.axis = dp.axis();
+    a.src = input;
+    a.dst = output;
+    a.nvec = vectors;
+    a.frames = frames;
+    a.conv = conv;
+    a.flag = f64 ? s.flag : nullptr;
+    const std::ptrdiff_t n = plan.destination_size;
+    if (rows) {
+        a.s_vec = in_row; a.s_el = 1; a.s_frame = in_frame;
+        a.d_vec = out_row; a.d_el = 1; a.d_frame = out_frame;
+    } else {
+        a.s_vec = 1; a.s_el = in_row; a.s_frame = in_frame;
+        a.d_vec = 1; a.d_el = out_row; a.d_frame = out_frame;
+    }
+    // Forward sweep: vector-contiguous scratch, [frame][element][vector].
+    a.fwd = static_cast<unsigned char *>(s.buffer) + fwd_offset;
+    a.f_vec = 1;
+    a.f_el = vectors;
+    a.f_frame = static_cast<long>(vectors) * n;
+    check(hip::launch_seq(a, in_kind, out_kind, f64, stream), "sequential solve launch failed");
+}
+
+void HipExecutor::inverse_rows(const AxisPlan &plan, const float *input,
+                               std::ptrdiff_t input_row_stride, float *output,
+                               std::ptrdiff_t output_row_stride, std::int32_t row_count,
+                               hipStream_t stream, bool async) {
+    if (!plan.valid() || !input || !output || input_row_stride < plan.source_size
+        || output_row_stride < plan.destination_size || row_count < 0)
+        throw std::invalid_argument("dsmvc: invalid row executor arguments");
+    if (row_count == 0) return;
+    std::lock_guard lock(mutex_);
+    check(hipSetDevice(device_), "hipSetDevice failed");
+    const bool f64 = plan.requires_float64();
+    const std::size_t elem = f64 ? 8 : 4;
+    Scratch &s = scratch(stream, align256(static_cast<std::size_t>(row_count) * plan.destination_size * elem));
+    const DevConv conv = make_conv(nullptr);
+    if (f64) {
+        check(hipMemsetAsync(s.flag, 0, sizeof(int), stream), "flag reset failed");
+        s.flag_armed = true;
+        check(hip::launch_check_finite(input, Sample::f32, input_row_stride, 0, row_count,
+                                       plan.source_size, 1, s.flag, stream), "check launch failed");
+    }
+    axis_pass(plan, true, input, Sample::f32, input_row_stride, 0, output, Sample::f32,
+              output_row_stride, 0, row_count, 1, conv, f64, s, 0, stream);
+    finish(stream, s, async, "dsmvc: HIP Float64 row execution produced NaN or infinity");
+}
+
+void HipExecutor::inverse_columns(const AxisPlan &plan, const float *input,
+                                  std::ptrdiff_t input_row_stride, float *output,
+                                  std::ptrdiff_t output_row_stride, std::int32_t column_count,
+                                  hipStream_t stream, bool async) {
+    if (!plan.valid() || !input || !output || column_count < 0
+        || input_row_stride < column_count || output_row_stride < column_count)
+        throw std::invalid_argument("dsmvc: invalid column executor arguments");
+    if (column_count == 0) return;
+    std::lock_guard lock(mutex_);
+    check(hipSetDevice(device_), "hipSetDevice failed");
+    const bool f64 = plan.requires_float64();
+    const std::size_t elem = f64 ? 8 : 4;
+    Scratch &s = scratch(stream, align256(static_cast<std::size_t>(column_count) * plan.destination_size * elem));
+    const DevConv conv = make_conv(nullptr);
+    if (f64) {
+        check(hipMemsetAsync(s.flag, 0, sizeof(int), stream), "flag reset failed");
+        s.flag_armed = true;
+        check(hip::launch_check_finite(input, Sample::f32, input_row_stride, 0, plan.source_size,
+                                       column_count, 1, s.flag, stream), "check launch failed");
+    }
+    axis_pass(plan, false, input, Sample::f32, input_row_stride, 0, output, Sample::f32,
+              output_row_stride, 0, column_count, 1, conv, f64, s, 0, stream);
+    finish(stream, s, async, "dsmvc: HIP Float64 column execution produced NaN or infinity");
+}
+
+void HipExecutor::inverse_2d_batch(const AxisPlan &h, const AxisPlan &v, SampleKind kind,
+                                   const PlaneBatch &b, const IntegerConversion *conversion,
+                                   hipStream_t stream, bool async) {
+    const bool integer = kind != SampleKind::float32;
+    if (!h.valid() || !v.valid() || !b.input || !b.output || b.frames < 1
+        || b.input_row_stride < h.source_size || b.output_row_stride < h.destination_size
+        || v.source_size < 1 || v.destination_size < 1
+        || (b.frames > 1 && (b.input_frame_stride < b.input_row_stride * v.source_size
+                             || b.output_frame_stride < b.output_row_stride * v.destination_size)))
+        throw std::invalid_argument(integer ? "dsmvc: invalid integer 2D executor arguments"
+                                            : "dsmvc: invalid 2D executor arguments");
+    if (integer) {
+        if (!conversion) throw std::invalid_argument("dsmvc: integer 2D execution needs a conversion");
+        validate_conversion(*conversion, kind == SampleKind::u8 ? 255U : 65535U,
+                            "dsmvc: invalid integer 2D executor arguments");
+    }
+    std::lock_guard lock(mutex_);
+    check(hipSetDevice(device_), "hipSetDevice failed");
+    const bool f64 = h.requires_float64() || v.requires_float64();
+    const std::size_t elem = f64 ? 8 : 4;
+    const std::ptrdiff_t inter_row = padded_width(h.destination_size);
+    const std::ptrdiff_t inter_frame = inter_row * v.source_size;
+    const std::size_t inter_bytes = align256(static_cast<std::size_t>(inter_frame) * b.frames * elem);
+    // Sequential forward sweeps, the larger of the two passes.
+    const std::size_t fwd_bytes = align256(
+        std::max(static_cast<std::size_t>(v.source_size) * h.destination_size,
+                 static_cast<std::size_t>(h.destination_size) * v.destination_size)
+        * b.frames * elem);
+    Scratch &s = scratch(stream, inter_bytes + fwd_bytes);
+    const DevConv conv = make_conv(conversion);
+    const Sample in_kind = sample_of(kind);
+    if (f64) {
+        check(hipMemsetAsync(s.flag, 0, sizeof(int), stream), "flag reset failed");
+        s.flag_armed = true;
+        check(hip::launch_check_finite(b.input, in_kind, b.input_row_stride, b.input_frame_stride,
+                                       v.source_size, h.source_size, b.frames, s.flag, stream),
+              "check launch failed");
+    }
+    void *inter = s.buffer;
+    const Sample inter_kind = f64 ? Sample::f64 : Sample::f32;
+    axis_pass(h, true, b.input, in_kind, b.input_row_stride, b.input_frame_stride, inter,
+              inter_kind, inter_row, inter_frame, v.source_size, b.frames, conv, f64, s,
+              inter_bytes, stream);
+    axis_pass(v, false, inter, inter_kind, inter_row, inter_frame, b.output, in_kind,
+              b.output_row_stride, b.output_frame_stride, h.destination_size, b.frames, conv,
+              f64, s, inter_bytes, stream);
+    finish(stream, s, async, "dsmvc: HIP Float64 2D execution produced NaN or infinity");
+}
+
+void HipExecutor::inverse_rows_batch(const AxisPlan &plan, SampleKind in_kind,
+                                     const PlaneBatch &b, std::int32_t rows,
+                                     const IntegerConversion *conversion, hipStream_t stream,
+                                     bool async) {
+    if (!plan.valid() || !b.input || !b.output || b.frames < 1 || rows < 1
+        || b.input_row_stride < plan.source_size || b.output_row_stride < plan.destination_size
+        || (b.frames > 1 && (b.input_frame_stride < b.input_row_stride * rows
+                             || b.output_frame_stride < b.output_row_stride * rows)))
+        throw std::invalid_argument("dsmvc: invalid row executor arguments");
+    if (plan.requires_float64())
+        throw std::invalid_argument("dsmvc: batched single-axis passes take Float32 plans; "
+                                    "use the 2D entry for Float64 plans");
+    if (in_kind != SampleKind::float32) {
+        if (!conversion) throw std::invalid_argument("dsmvc: integer rows need a conversion");
+        validate_conversion(*conversion, in_kind == SampleKind::u8 ? 255U : 65535U,
+                            "dsmvc: invalid row executor arguments");
+    }
+    std::lock_guard lock(mutex_);
+    check(hipSetDevice(device_), "hipSetDevice failed");
+    Scratch &s = scratch(stream, align256(static_cast<std::size_t>(rows) * plan.destination_size
+                                          * b.frames * 4));
+    axis_pass(plan, true, b.input, sample_of(in_kind), b.input_row_stride, b.input_frame_stride,
+              b.output, Sample::f32, b.output_row_stride, b.output_frame_stride, rows, b.frames,
+              make_conv(conversion), false, s, 0, stream);
+    finish(stream, s, async, "dsmvc: HIP row execution failed");
+}
+
+void HipExecutor::inverse_columns_batch(const AxisPlan &plan, SampleKind out_kind,
+                                        const PlaneBatch &b, std::int32_t columns,
+                                        const IntegerConversion *conversion, hipStream_t stream,
+                                        bool async) {
+    if (!plan.valid() || !b.input || !b.output || b.frames < 1 || columns < 1
+        || b.input_row_stride < columns || b.output_row_stride < columns
+        || (b.frames > 1 && (b.input_frame_stride < b.input_row_stride * plan.source_size
+                             || b.output_frame_stride < b.output_row_stride * plan.destination_size)))
+        throw std::invalid_argument("dsmvc: invalid column executor arguments");
+    if (plan.requires_float64())
+        throw std::invalid_argument("dsmvc: batched single-axis passes take Float32 plans; "
+                                    "use the 2D entry for Float64 plans");
+    if (out_kind != SampleKind::float32) {
+        if (!conversion) throw std::invalid_argument("dsmvc: integer columns need a conversion");
+        validate_conversion(*conversion, out_kind == SampleKind::u8 ? 255U : 65535U,
+                            "dsmvc: invalid column executor arguments");
+    }
+    std::lock_guard lock(mutex_);
+    check(hipSetDevice(device_), "hipSetDevice failed");
+    Scratch &s = scratch(stream, align256(static_cast<std::size_t>(columns) * plan.destination_size
+                                          * b.frames * 4));
+    axis_pass(plan, false, b.input, Sample::f32, b.input_row_stride, b.input_frame_stride,
+              b.output, sample_of(out_kind), b.output_row_stride, b.output_frame_stride, columns,
+              b.frames, make_conv(conversion), false, s, 0, stream);
+    finish(stream, s, async, "dsmvc: HIP column execution failed");
+}
+
+void HipExecutor::inverse_2d(const AxisPlan &h, const AxisPlan &v, const float *input,
+                             std::ptrdiff_t is, float *output, std::ptrdiff_t os,
+                             hipStream_t stream, bool async) {
+    inverse_2d_batch(h, v, SampleKind::float32, {input, is, 0, output, os, 0, 1}, nullptr, stream, async);
+}
+
+void HipExecutor::inverse_2d_u8(const AxisPlan &h, const AxisPlan &v, const std::uint8_t *input,
+                                std::ptrdiff_t is, std::uint8_t *output, std::ptrdiff_t os,
+                                const IntegerConversion &c, hipStream_t stream, bool async) {
+    inverse_2d_batch(h, v, SampleKind::u8, {input, is, 0, output, os, 0, 1}, &c, stream, async);
+}
+
+void HipExecutor::inverse_2d_u16(const AxisPlan &h, const AxisPlan &v, const std::uint16_t *input,
+                                 std::ptrdiff_t is, std::uint16_t *output, std::ptrdiff_t os,
+                                 const IntegerConversion &c, hipStream_t stream, bool async) {
+    inverse_2d_batch(h, v, SampleKind::u16, {input, is, 0, output, os, 0, 1}, &c, stream, async);
+}
+
+// ---------------------------------------------------------------------------
+// Backend names (reference src/backend.cpp:107-154, plus "hip").
+// ---------------------------------------------------------------------------
+BackendKind parse_backend(std::string_view name) {
+    if (name.empty() || name == "auto" || name == "automatic") return BackendKind::automatic;
+    if (name == "cpu") return BackendKind::cpu;
+    if (name == "metal") return BackendKind::metal;
+    if (name == "vulkan") return BackendKind::vulkan;
+    if (name == "cuda") return BackendKind::cuda;
+    if (name == "hip") return BackendKind::hip;
+    throw std::invalid_argument("backend must be auto, cpu, metal, vulkan, cuda, or hip");
+}
+
+BackendKind resolve_backend(BackendKind requested) {
+    switch (requested) {
+    case BackendKind::automatic:
+    case BackendKind::cpu:
+        return BackendKind::cpu;
+    case BackendKind::hip:
+        if (!hip_available()) throw std::runtime_error("backend hip is unavailable");
+        return BackendKind::hip;
+    case BackendKind::metal:
+    case BackendKind::vulkan:
+    case BackendKind::cuda:
+        throw std::runtime_error(std::string("backend ") + backend_name(requested)
+                                 + " is unavailable");
+    }
+    throw std::invalid_argument("backend is invalid");
+}
+
+const char *backend_name(BackendKind kind) noexcept {
+    switch (kind) {
+    case BackendKind::automatic: return "auto";
+    case BackendKind::cpu: return "cpu";
+    case BackendKind::metal: return "metal";
+    case BackendKind::vulkan: return "vulkan";
+    case BackendKind::cuda: return "cuda";
+    case BackendKind::hip: return "hip";
+    }
+    return "unknown";
+}
+
+} // namespace dsmvc

@@ -1,0 +1,134 @@
+// HIP executor: applies axis plans to device-resident planes on one GPU.
+//
+// Public surface mirrors the reference's GPU executor and the backend-neutral
+// Executor (reference include/dsmvc/engine.hpp:363-504,
+// src/cuda/cuda_executor.hpp:11-74): prepare / seal, inverse_rows,
+// inverse_columns, inverse_2d, inverse_2d_u8, inverse_2d_u16, with strides in
+// elements, the output's logical width only written, std::invalid_argument
+// for bad arguments and std::runtime_error for device failures or non-finite
+// Float64 results.  Unlike the reference, buffers are device pointers (HBM
+// resident) and batches of frames are first class.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <mutex>
+#include <unordered_map>
+
+#include "device_plan.hpp"
+#include "device_types.hpp"
+#include "engine.hpp"
+
+namespace dsmvc {
+
+enum class SolveMode : std::uint8_t {
+    automatic = 0,   // chunked where the plan allows it, else sequential
+    exact = 1,       // sequential: bit-identical to the ordered CPU solver
+    chunked = 2,     // chunked only; throws when the plan cannot be chunked
+};
+
+enum class SampleKind : std::uint8_t { float32 = 0, u8 = 1, u16 = 2 };
+
+// A batch of `frames` planes: frame f's row r starts at base + f * frame_stride
+// + r * row_stride (elements).
+struct PlaneBatch {
+    const void *input = nullptr;
+    std::ptrdiff_t input_row_stride = 0;
+    std::ptrdiff_t input_frame_stride = 0;
+    void *output = nullptr;
+    std::ptrdiff_t output_row_stride = 0;
+    std::ptrdiff_t output_frame_stride = 0;
+    std::int32_t frames = 1;
+};
+
+// Per-launch tuning the executor chooses by itself unless overridden.
+struct LaunchTuning {
+    std::int32_t rows_chunks = 0;      // 0 = automatic
+    std::int32_t columns_chunks = 0;
+};
+
+class HipExecutor {
+public:
+    explicit HipExecutor(int device = -1, SolveMode mode = SolveMode::automatic);
+    ~HipExecutor();
+    HipExecutor(const HipExecutor &) = delete;
+    HipExecutor &operator=(const HipExecutor &) = delete;
+
+    [[nodiscard]] int device() const noexcept { return device_; }
+    [[nodiscard]] SolveMode mode() const noexcept { return mode_; }
+    [[nodiscard]] const char *name() const noexcept { return "hip"; }
+
+    void prepare(const std::shared_ptr<const AxisPlan> &plan);
+    void seal();
+    void set_tuning(const LaunchTuning &t) { tuning_ = t; }
+
+    // Float32 planes (device pointers).  Synchronous unless `async`.
+    void inverse_rows(const AxisPlan &plan, const float *input, std::ptrdiff_t input_row_stride,
+                      float *output, std::ptrdiff_t output_row_stride, std::int32_t row_count,
+                      hipStream_t stream = nullptr, bool async = false);
+    void inverse_columns(const AxisPlan &plan, const float *input,
+                         std::ptrdiff_t input_row_stride, float *output,
+                         std::ptrdiff_t output_row_stride, std::int32_t column_count,
+                         hipStream_t stream = nullptr, bool async = false);
+    void inverse_2d(const AxisPlan &horizontal, const AxisPlan &vertical, const float *input,
+                    std::ptrdiff_t input_row_stride, float *output,
+                    std::ptrdiff_t output_row_stride, hipStream_t stream = nullptr,
+                    bool async = false);
+    void inverse_2d_u8(const AxisPlan &horizontal, const AxisPlan &vertical,
+                       const std::uint8_t *input, std::ptrdiff_t input_row_stride,
+                       std::uint8_t *output, std::ptrdiff_t output_row_stride,
+                       const IntegerConversion &conversion, hipStream_t stream = nullptr,
+                       bool async = false);
+    void inverse_2d_u16(const AxisPlan &horizontal, const AxisPlan &vertical,
+                        const std::uint16_t *input, std::ptrdiff_t input_row_stride,
+                        std::uint16_t *output, std::ptrdiff_t output_row_stride,
+                        const IntegerConversion &conversion, hipStream_t stream = nullptr,
+                        bool async = false);
+
+    // The general 2D entry: a batch of frames of one sample kind.
+    void inverse_2d_batch(const AxisPlan &horizontal, const AxisPlan &vertical, SampleKind kind,
+                          const PlaneBatch &batch, const IntegerConversion *conversion,
+                          hipStream_t stream, bool async);
+
+    // One axis over a batch, for F32 plans: rows of `kind` samples into a
+    // float32 intermediate, or columns of a float32 intermediate into `kind`
+    // samples -- the two halves of inverse_2d_batch, callable on their own.
+    void inverse_rows_batch(const AxisPlan &plan, SampleKind in_kind, const PlaneBatch &batch,
+                            std::int32_t row_count, const IntegerConversion *conversion,
+                            hipStream_t stream, bool async);
+    void inverse_columns_batch(const AxisPlan &plan, SampleKind out_kind, const PlaneBatch &batch,
+                               std::int32_t column_count, const IntegerConversion *conversion,
+                               hipStream_t stream, bool async);
+
+    // Waits for `stream` and reports a deferred Float64 non-finite failure.
+    void synchronize(hipStream_t stream);
+
+private:
+    struct Scratch {
+        void *buffer = nullptr;
+        std::size_t bytes = 0;
+        int *flag = nullptr;
+        bool flag_armed = false;
+    };
+
+    dsmvc::hip::DevicePlan &device_plan(const AxisPlan &plan);
+    Scratch &scratch(hipStream_t stream, std::size_t bytes);
+    void finish(hipStream_t stream, Scratch &s, bool async, const char *what);
+    void axis_pass(const AxisPlan &plan, bool rows, const void *input, hip::Sample in_kind,
+                   std::ptrdiff_t in_row, std::ptrdiff_t in_frame, void *output,
+                   hip::Sample out_kind, std::ptrdiff_t out_row, std::ptrdiff_t out_frame,
+                   std::int32_t vectors, std::int32_t frames, const hip::DevConv &conv,
+                   bool f64, Scratch &s, std::size_t fwd_offset, hipStream_t stream);
+
+    int device_;
+    SolveMode mode_;
+    LaunchTuning tuning_{};
+    std::mutex mutex_;
+    std::unordered_map<const AxisPlan *, std::unique_ptr<dsmvc::hip::DevicePlan>> plans_;
+    std::unordered_map<hipStream_t, Scratch> scratch_;
+    bool sealed_ = false;
+};
+
+} // namespace dsmvc

@@ -1,0 +1,441 @@
+"""dsmvc_hip -- host side of the MI355X descale executor.
+
+Mirrors the reference plugin's operator surface (core.dsmvc.Descale,
+Debilinear, Debicubic, Delanczos, Despline16/36/64 with backend="hip";
+reference src/vs_plugin.cpp:1989-2068, dsmvc.py:116-150) for planes held as
+device tensors, and exposes the engine level (plans, executor) of
+include/dsmvc_hip.h.  Every compute call goes through libdsmvc_hip.so; there
+is no CPU path here -- without the library or a HIP device the calls raise.
+
+Tensors are torch tensors on a HIP device (torch is only the allocator and
+stream provider here).  A clip is a `Clip`: a list of planes, each
+[frames, height, width] (or [height, width]) of float32, uint8 or uint16.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import ASYNC, AxisRequestC, ConversionC, DsmvcError, PlanInfoC, check, load
+
+__all__ = [
+    "AxisRequest", "Plan", "Executor", "Clip", "IntegerConversion", "build_plan",
+    "integer_conversion", "Descale", "Debilinear", "Debicubic", "Delanczos", "Despline16",
+    "Despline36", "Despline64", "parse_backend", "resolve_backend", "device_count",
+    "padding_index", "planner_cache_stats", "clear_planner_caches", "DsmvcError",
+    "BILINEAR", "BICUBIC", "LANCZOS", "SPLINE16", "SPLINE36", "SPLINE64",
+    "ZERO", "REPEAT", "REFLECT101", "SYMMETRIC", "MIRROR",
+]
+
+# Reference numbering (include/dsmvc/engine.hpp:16-58).
+BILINEAR, BICUBIC, LANCZOS, SPLINE16, SPLINE36, SPLINE64 = range(6)
+ZERO, REPEAT, REFLECT101, SYMMETRIC, MIRROR = range(5)
+KERNELS = {"bilinear": BILINEAR, "bicubic": BICUBIC, "lanczos": LANCZOS,
+           "spline16": SPLINE16, "spline36": SPLINE36, "spline64": SPLINE64}
+BACKENDS = {0: "auto", 1: "cpu", 2: "metal", 3: "vulkan", 4: "cuda", 5: "hip"}
+SOLVE_MODES = {"auto": 0, "exact": 1, "chunked": 2}
+SAMPLE_KINDS = {"float32": 0, "uint8": 1, "uint16": 2}
+
+
+@dataclass(frozen=True)
+class AxisRequest:
+    """dsmvc::AxisRequest (reference include/dsmvc/engine.hpp:60-78)."""
+    source_size: int
+    destination_size: int
+    active_length: float
+    shift: float = 0.0
+    kernel: int = BICUBIC
+    taps: int = 3
+    b: float = 0.0
+    c: float = 0.5
+    blur: float = 1.0
+    border: int = SYMMETRIC
+    f64_mode: int = 0
+
+    def to_c(self) -> AxisRequestC:
+        return AxisRequestC(self.source_size, self.destination_size, float(self.active_length),
+                            float(self.shift), self.kernel, self.taps, float(self.b),
+                            float(self.c), float(self.blur), self.border, self.f64_mode)
+
+
+@dataclass(frozen=True)
+class IntegerConversion:
+    """dsmvc::IntegerConversion (reference include/dsmvc/engine.hpp:167-173)."""
+    input_offset: float
+    input_scale: float
+    output_scale: float
+    output_offset: float
+    output_maximum: int
+
+    def to_c(self) -> ConversionC:
+        return ConversionC(self.input_offset, self.input_scale, self.output_scale,
+                           self.output_offset, self.output_maximum)
+
+
+def integer_conversion(bits: int, chroma: bool, full_range: bool) -> IntegerConversion:
+    """Per-plane integer constants of a YUV clip (reference src/vs_plugin.cpp:761-781).
+    `full_range` follows the reference's frame_range(): _Range / _ColorRange == 1
+    means full (vs_plugin.cpp:750-759)."""
+    maximum = (1 << bits) - 1
+    offset, scale = 0, maximum
+    if not full_range:
+        depth = 1 << (bits - 8)
+        offset = (128 if chroma else 16) * depth
+        scale = (224 if chroma else 219) * depth
+    elif chroma:
+        offset = 1 << (bits - 1)
+    return IntegerConversion(float(offset), float(np.float32(1.0) / np.float32(scale)),
+                             float(scale), float(offset), maximum)
+
+
+class Plan:
+    """Immutable axis plan built by the library's FP64 planner (cached)."""
+
+    def __init__(self, request: AxisRequest):
+        lib = load()
+        handle = ctypes.c_void_p()
+        check(lib.dsmvc_hip_plan_build(ctypes.byref(request.to_c()), ctypes.byref(handle)))
+        self._handle = handle
+        self.request = request
+        info = PlanInfoC()
+        check(lib.dsmvc_hip_plan_info_get(handle, ctypes.byref(info)))
+        self.source_size = info.source_size
+        self.destination_size = info.destination_size
+        self.support = info.support
+        self.half_bandwidth = info.half_bandwidth
+        self.active_length = info.active_length
+        self.shift = info.shift
+        self.normal_rcond = info.normal_rcond
+        self.normal_inf_norm = info.normal_inf_norm
+        self.requires_float64 = bool(info.requires_float64)
+        self.nnz = info.nnz
+
+    @property
+    def handle(self):
+        return self._handle
+
+    def __del__(self):
+        try:
+            if self._handle:
+                load().dsmvc_hip_plan_release(self._handle)
+        except Exception:
+            pass
+
+    def arrays(self) -> dict:
+        """Copies of the AxisPlan arrays (reference include/dsmvc/engine.hpp:90-107)."""
+        n, w, nnz, f64 = self.destination_size, self.half_bandwidth, self.nnz, self.requires_float64
+        out = {
+            "offsets": np.zeros(n + 1, np.uint32), "indices": np.zeros(nnz, np.int32),
+            "weights": np.zeros(nnz, np.float32), "lower_ld": np.zeros(w * n, np.float32),
+            "upper_l": np.zeros(w * n, np.float32), "inverse_diagonal": np.zeros(n, np.float32),
+            "weights_f64": np.zeros(nnz if f64 else 0), "normal_bands_f64":
+                np.zeros((w + 1) * n if f64 else 0), "ldlt_bands_f64":
+                np.zeros((w + 1) * n if f64 else 0), "inverse_diagonal_f64":
+                np.zeros(n if f64 else 0),
+        }
+        ptrs = [ctypes.c_void_p(a.ctypes.data) if a.size else None for a in out.values()]
+        check(load().dsmvc_hip_plan_arrays(self._handle, *ptrs))
+        return out
+
+
+def build_plan(request: AxisRequest) -> Plan:
+    return Plan(request)
+
+
+def planner_cache_stats() -> dict:
+    s = _lib.PlannerStatsC()
+    check(load().dsmvc_hip_planner_cache_stats(ctypes.byref(s)))
+    return {"plan_hits": s.plan_hits, "plan_builds": s.plan_builds,
+            "plan_entries": s.plan_entries, "plan_resident_bytes": s.plan_resident_bytes}
+
+
+def clear_planner_caches():
+    load().dsmvc_hip_clear_planner_caches()
+
+
+def padding_index(index: int, size: int, mode: int) -> int:
+    out = ctypes.c_int32()
+    check(load().dsmvc_hip_padding_index(index, size, mode, ctypes.byref(out)))
+    return out.value
+
+
+def parse_backend(name: str) -> str:
+    kind = ctypes.c_int32()
+    check(load().dsmvc_hip_parse_backend(name.encode(), ctypes.byref(kind)))
+    return BACKENDS[kind.value]
+
+
+def resolve_backend(name: str) -> str:
+    """Explicit `hip` without a device raises; it never falls back
+    (reference src/backend.cpp:119-142)."""
+    lib = load()
+    kind = ctypes.c_int32()
+    check(lib.dsmvc_hip_parse_backend(name.encode(), ctypes.byref(kind)))
+    resolved = ctypes.c_int32()
+    check(lib.dsmvc_hip_resolve_backend(kind.value, ctypes.byref(resolved)))
+    return BACKENDS[resolved.value]
+
+
+def device_count() -> int:
+    out = ctypes.c_int32()
+    check(load().dsmvc_hip_device_count(ctypes.byref(out)))
+    return out.value
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream_handle(stream) -> ctypes.c_void_p:
+    if stream is None:
+        stream = _torch().cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def _dev_ptr(t) -> ctypes.c_void_p:
+    if not t.is_cuda:
+        raise ValueError("dsmvc: HIP executor buffers must be device tensors")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Executor:
+    """dsmvc::Executor(backend="hip") on one GPU (reference include/dsmvc/engine.hpp:363-504).
+
+    solve: "auto" (chunked partitioned solve where the plan allows, else
+    sequential), "exact" (sequential, bit-identical to the ordered CPU solver),
+    "chunked" (raise when not chunkable)."""
+
+    def __init__(self, device: int = -1, solve: str = "auto"):
+        lib = load()
+        handle = ctypes.c_void_p()
+        check(lib.dsmvc_hip_executor_create(device, SOLVE_MODES[solve], ctypes.byref(handle)))
+        self._handle = handle
+        self.solve = solve
+
+    def __del__(self):
+        try:
+            if self._handle:
+                load().dsmvc_hip_executor_destroy(self._handle)
+                self._handle = None
+        except Exception:
+            pass
+
+    def prepare(self, plan: Plan):
+        check(load().dsmvc_hip_executor_prepare(self._handle, plan.handle))
+
+    def seal(self):
+        check(load().dsmvc_hip_executor_seal(self._handle))
+
+    def tune(self, rows_chunks: int = 0, columns_chunks: int = 0):
+        check(load().dsmvc_hip_executor_tune(self._handle, rows_chunks, columns_chunks))
+
+    def synchronize(self, stream=None):
+        check(load().dsmvc_hip_synchronize(self._handle, _stream_handle(stream)))
+
+    # -- single axis, float32 ------------------------------------------------
+    def inverse_rows(self, plan: Plan, x, out=None, stream=None, asynchronous=False):
+        torch = _torch()
+        rows = x.shape[0]
+        if out is None:
+            out = torch.empty((rows, plan.destination_size), dtype=torch.float32, device=x.device)
+        check(load().dsmvc_hip_inverse_rows(
+            self._handle, plan.handle, _dev_ptr(x), x.stride(0), _dev_ptr(out), out.stride(0),
+            rows, _stream_handle(stream), ASYNC if asynchronous else 0))
+        return out
+
+    def inverse_columns(self, plan: Plan, x, out=None, stream=None, asynchronous=False):
+        torch = _torch()
+        cols = x.shape[1]
+        if out is None:
+            out = torch.empty((plan.destination_size, cols), dtype=torch.float32, device=x.device)
+        if x.shape[0] < plan.source_size:
+            raise ValueError("dsmvc: input has fewer rows than the plan's source size")
+        check(load().dsmvc_hip_inverse_columns(
+            self._handle, plan.handle, _dev_ptr(x), x.stride(0), _dev_ptr(out), out.stride(0),
+            cols, _stream_handle(stream), ASYNC if asynchronous else 0))
+        return out
+
+    # -- 2D ------------------------------------------------------------------
+    def inverse_2d(self, horizontal: Plan, vertical: Plan, x, out=None,
+                   conversion: Optional[IntegerConversion] = None, stream=None,
+                   asynchronous=False):
+        """x: [H, W] or [frames, H, W]; float32, uint8 or uint16 (integer needs a
+        conversion).  Returns [h, w] / [frames, h, w] of x's dtype."""
+        torch = _torch()
+        batched = x.dim() == 3
+        xb = x if batched else x.unsqueeze(0)
+        frames, rows, cols = xb.shape
+        if rows != vertical.source_size or cols < horizontal.source_size:
+            raise ValueError("dsmvc: input plane does not match the plans")
+        if xb.stride(2) != 1:
+            raise ValueError("dsmvc: rows must be contiguous")
+        kind = {torch.float32: 0, torch.uint8: 1, torch.uint16: 2}.get(xb.dtype)
+        if kind is None:
+            raise ValueError("dsmvc: samples must be float32, uint8 or uint16")
+        if out is None:
+            out = torch.empty((frames, vertical.destination_size, horizontal.destination_size),
+                              dtype=xb.dtype, device=xb.device)
+        ob = out if out.dim() == 3 else out.unsqueeze(0)
+        conv = ctypes.byref(conversion.to_c()) if conversion is not None else None
+        check(load().dsmvc_hip_inverse_2d_batch(
+            self._handle, horizontal.handle, vertical.handle, kind, _dev_ptr(xb), xb.stride(1),
+            xb.stride(0), _dev_ptr(ob), ob.stride(1), ob.stride(0), frames, conv,
+            _stream_handle(stream), ASYNC if asynchronous else 0))
+        return out if batched or out.dim() == 2 else out[0]
+
+
+# ---------------------------------------------------------------------------
+# Plugin-shaped operators
+# ---------------------------------------------------------------------------
+@dataclass
+class Clip:
+    """A clip: planes (luma first) of [frames, H, W] device tensors plus format.
+
+    color_family: "gray" or "yuv"; subsampling_w/h: log2 chroma subsampling;
+    bits: bits per integer sample (ignored for float32); full_range: the
+    _Range / _ColorRange frame property (True when it is 1)."""
+    planes: List
+    color_family: str = "gray"
+    subsampling_w: int = 0
+    subsampling_h: int = 0
+    bits: int = 0
+    full_range: bool = False
+
+    @property
+    def width(self):
+        return self.planes[0].shape[-1]
+
+    @property
+    def height(self):
+        return self.planes[0].shape[-2]
+
+
+_executors = {}
+
+
+def _executor(solve: str = "auto"):
+    if solve not in _executors:
+        _executors[solve] = Executor(-1, solve)
+    return _executors[solve]
+
+
+def Descale(src, width: int, height: int, kernel: str = "bicubic", taps: int = 3,
+            b: float = 0.0, c: float = 0.5, src_left: float = 0.0, src_top: float = 0.0,
+            src_width: Optional[float] = None, src_height: Optional[float] = None,
+            border_handling: Optional[int] = None, padding: Optional[int] = None,
+            f64mode: int = 0, blur: float = 1.0, force: bool = False,
+            force_h: Optional[bool] = None, force_v: Optional[bool] = None,
+            backend: str = "hip", solve: str = "auto"):
+    """core.dsmvc.Descale with backend="hip" (reference src/vs_plugin.cpp:362-509,
+    656-705, 1774-1987).  `src` is a Clip or a single float32 plane tensor."""
+    if resolve_backend(backend) != "hip":
+        raise ValueError("dsmvc: this build provides backend=\"hip\" only")
+    clip = src if isinstance(src, Clip) else Clip([src])
+    if kernel not in KERNELS:
+        raise ValueError("kernel is invalid")
+    if width < 1:
+        raise ValueError("width must be greater than zero")
+    if height < 8:
+        raise ValueError("height must be at least 8")
+    if padding is not None and border_handling is not None:
+        raise ValueError("specify either padding or border_handling, not both")
+    if padding is not None:
+        if padding not in (0, 1, 2, 3):
+            raise ValueError("padding must be 0, 1, 2, or 3")
+        border = padding
+    elif border_handling is not None:
+        border = {1: ZERO, 2: REPEAT}.get(border_handling, MIRROR)
+    else:
+        border = SYMMETRIC
+    if f64mode not in (0, 1, 2):
+        raise ValueError("f64mode must be 0, 1, or 2")
+    if not (blur > 0.0) or not math.isfinite(blur):
+        raise ValueError("blur must be finite and greater than zero")
+    if kernel == "lanczos" and taps < 1:
+        raise ValueError("taps must be greater than zero")
+    sw, sh = clip.width, clip.height
+    if clip.color_family == "yuv" and clip.subsampling_w | clip.subsampling_h:
+        if width % (1 << clip.subsampling_w) or height % (1 << clip.subsampling_h):
+            raise ValueError("output width is incompatible with subsampling")
+    src_width = float(width) if src_width is None else float(src_width)
+    src_height = float(height) if src_height is None else float(src_height)
+    force_h = force if force_h is None else force_h
+    force_v = force if force_v is None else force_v
+    process_h = width != sw or src_left != 0.0 or src_width != float(width) or force_h
+    process_v = height != sh or src_top != 0.0 or src_height != float(height) or force_v
+    if not process_h and not process_v:
+        return clip if isinstance(src, Clip) else src
+    kw = dict(kernel=KERNELS[kernel], taps=taps, b=b, c=c, blur=blur, border=border,
+              f64_mode=f64mode)
+    # Requests per plane kind (reference prepare_filter_requests, vs_plugin.cpp:656-705).
+    hreq = [AxisRequest(sw, width, src_width, src_left, **kw)]
+    vreq = [AxisRequest(sh, height, src_height, src_top, **kw)]
+    if len(clip.planes) > 1 and clip.subsampling_w > 0:
+        cw = sw >> clip.subsampling_w
+        hreq.append(AxisRequest(cw, width >> clip.subsampling_w,
+                                src_width * float(cw) / float(sw),
+                                0.25 - 0.25 * float(width) / float(sw) + src_left * float(cw) / float(sw),
+                                **kw))
+    if len(clip.planes) > 1 and clip.subsampling_h > 0:
+        ch = sh >> clip.subsampling_h
+        vreq.append(AxisRequest(ch, height >> clip.subsampling_h,
+                                src_height * float(ch) / float(sh),
+                                src_top * float(ch) / float(sh), **kw))
+    ex = _executor(solve)
+    out_planes = []
+    for index, plane in enumerate(clip.planes):
+        chroma = index > 0
+        hp = Plan(hreq[1 if chroma and len(hreq) > 1 else 0])
+        vp = Plan(vreq[1 if chroma and len(vreq) > 1 else 0])
+        integer = plane.dtype != _torch().float32
+        conv = None
+        if integer:
+            if not (process_h and process_v):
+                raise ValueError("dsmvc: the HIP executor descales integer clips on both axes "
+                                 "only; convert to float32 for single-axis descales")
+            conv = integer_conversion(clip.bits or (8 if plane.dtype == _torch().uint8 else 16),
+                                      chroma and clip.color_family == "yuv", clip.full_range)
+        if process_h and process_v:
+            out_planes.append(ex.inverse_2d(hp, vp, plane, conversion=conv))
+        elif process_h:
+            x = plane if plane.dim() == 3 else plane.unsqueeze(0)
+            res = [ex.inverse_rows(hp, f) for f in x]
+            out_planes.append(_torch().stack(res) if plane.dim() == 3 else res[0])
+        else:
+            x = plane if plane.dim() == 3 else plane.unsqueeze(0)
+            res = [ex.inverse_columns(vp, f) for f in x]
+            out_planes.append(_torch().stack(res) if plane.dim() == 3 else res[0])
+    if isinstance(src, Clip):
+        return Clip(out_planes, clip.color_family, clip.subsampling_w, clip.subsampling_h,
+                    clip.bits, clip.full_range)
+    return out_planes[0]
+
+
+def Debilinear(src, width, height, **kw):
+    return Descale(src, width, height, kernel="bilinear", **kw)
+
+
+def Debicubic(src, width, height, b=0.0, c=0.5, **kw):
+    return Descale(src, width, height, kernel="bicubic", b=b, c=c, **kw)
+
+
+def Delanczos(src, width, height, taps=3, **kw):
+    return Descale(src, width, height, kernel="lanczos", taps=taps, **kw)
+
+
+def Despline16(src, width, height, **kw):
+    return Descale(src, width, height, kernel="spline16", **kw)
+
+
+def Despline36(src, width, height, **kw):
+    return Descale(src, width, height, kernel="spline36", **kw)
+
+
+def Despline64(src, width, height, **kw):
+    return Descale(src, width, height, kernel="spline64", **kw)

@@ -1,0 +1,139 @@
+"""ctypes binding of libdsmvc_hip.so (the C ABI declared in include/dsmvc_hip.h).
+
+Loading never falls back to anything: a missing or unloadable library raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libdsmvc_hip.so"
+
+OK, INVALID_ARGUMENT, RUNTIME_ERROR, UNAVAILABLE = 0, 1, 2, 3
+ASYNC = 1
+
+# Every symbol include/dsmvc_hip.h declares.
+EXPORTS = (
+    "dsmvc_hip_last_error", "dsmvc_hip_plan_build", "dsmvc_hip_plan_release",
+    "dsmvc_hip_plan_info_get", "dsmvc_hip_plan_arrays", "dsmvc_hip_planner_cache_stats",
+    "dsmvc_hip_clear_planner_caches", "dsmvc_hip_padding_index", "dsmvc_hip_parse_backend",
+    "dsmvc_hip_resolve_backend", "dsmvc_hip_device_count", "dsmvc_hip_executor_create",
+    "dsmvc_hip_executor_destroy", "dsmvc_hip_executor_prepare", "dsmvc_hip_executor_seal",
+    "dsmvc_hip_executor_tune", "dsmvc_hip_inverse_rows", "dsmvc_hip_inverse_columns",
+    "dsmvc_hip_inverse_2d", "dsmvc_hip_inverse_2d_u8", "dsmvc_hip_inverse_2d_u16",
+    "dsmvc_hip_inverse_2d_batch", "dsmvc_hip_inverse_rows_batch",
+    "dsmvc_hip_inverse_columns_batch", "dsmvc_hip_synchronize",
+)
+
+
+class AxisRequestC(ctypes.Structure):
+    _fields_ = [
+        ("source_size", ctypes.c_int32),
+        ("destination_size", ctypes.c_int32),
+        ("active_length", ctypes.c_double),
+        ("shift", ctypes.c_double),
+        ("kernel", ctypes.c_int32),
+        ("taps", ctypes.c_int32),
+        ("b", ctypes.c_double),
+        ("c", ctypes.c_double),
+        ("blur", ctypes.c_double),
+        ("border", ctypes.c_int32),
+        ("f64_mode", ctypes.c_int32),
+    ]
+
+
+class ConversionC(ctypes.Structure):
+    _fields_ = [
+        ("input_offset", ctypes.c_float),
+        ("input_scale", ctypes.c_float),
+        ("output_scale", ctypes.c_float),
+        ("output_offset", ctypes.c_float),
+        ("output_maximum", ctypes.c_uint32),
+    ]
+
+
+class PlanInfoC(ctypes.Structure):
+    _fields_ = [
+        ("source_size", ctypes.c_int32),
+        ("destination_size", ctypes.c_int32),
+        ("support", ctypes.c_int32),
+        ("half_bandwidth", ctypes.c_int32),
+        ("active_length", ctypes.c_double),
+        ("shift", ctypes.c_double),
+        ("normal_rcond", ctypes.c_double),
+        ("normal_inf_norm", ctypes.c_double),
+        ("requires_float64", ctypes.c_int32),
+        ("nnz", ctypes.c_int32),
+    ]
+
+
+class PlannerStatsC(ctypes.Structure):
+    _fields_ = [
+        ("plan_hits", ctypes.c_uint64),
+        ("plan_builds", ctypes.c_uint64),
+        ("plan_entries", ctypes.c_uint64),
+        ("plan_resident_bytes", ctypes.c_uint64),
+    ]
+
+
+class DsmvcError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load():
+    """Loads libdsmvc_hip.so (built by `make -C vapoursynth-descalemvc_amd`)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() or "
+                          f"make -C vapoursynth-descalemvc_amd")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    vp, pd, i32, u32 = ctypes.c_void_p, ctypes.c_ssize_t, ctypes.c_int32, ctypes.c_uint32
+    lib.dsmvc_hip_last_error.restype = ctypes.c_char_p
+    lib.dsmvc_hip_plan_build.argtypes = [ctypes.POINTER(AxisRequestC), ctypes.POINTER(vp)]
+    lib.dsmvc_hip_plan_release.argtypes = [vp]
+    lib.dsmvc_hip_plan_release.restype = None
+    lib.dsmvc_hip_plan_info_get.argtypes = [vp, ctypes.POINTER(PlanInfoC)]
+    lib.dsmvc_hip_plan_arrays.argtypes = [vp] + [vp] * 10
+    lib.dsmvc_hip_planner_cache_stats.argtypes = [ctypes.POINTER(PlannerStatsC)]
+    lib.dsmvc_hip_clear_planner_caches.restype = None
+    lib.dsmvc_hip_padding_index.argtypes = [ctypes.c_int64, i32, i32, ctypes.POINTER(i32)]
+    lib.dsmvc_hip_parse_backend.argtypes = [ctypes.c_char_p, ctypes.POINTER(i32)]
+    lib.dsmvc_hip_resolve_backend.argtypes = [i32, ctypes.POINTER(i32)]
+    lib.dsmvc_hip_device_count.argtypes = [ctypes.POINTER(i32)]
+    lib.dsmvc_hip_executor_create.argtypes = [i32, i32, ctypes.POINTER(vp)]
+    lib.dsmvc_hip_executor_destroy.argtypes = [vp]
+    lib.dsmvc_hip_executor_destroy.restype = None
+    lib.dsmvc_hip_executor_prepare.argtypes = [vp, vp]
+    lib.dsmvc_hip_executor_seal.argtypes = [vp]
+    lib.dsmvc_hip_executor_tune.argtypes = [vp, i32, i32]
+    for name in ("dsmvc_hip_inverse_rows", "dsmvc_hip_inverse_columns"):
+        getattr(lib, name).argtypes = [vp, vp, vp, pd, vp, pd, i32, vp, u32]
+    lib.dsmvc_hip_inverse_2d.argtypes = [vp, vp, vp, vp, pd, vp, pd, vp, u32]
+    for name in ("dsmvc_hip_inverse_2d_u8", "dsmvc_hip_inverse_2d_u16"):
+        getattr(lib, name).argtypes = [vp, vp, vp, vp, pd, vp, pd, ctypes.POINTER(ConversionC),
+                                       vp, u32]
+    lib.dsmvc_hip_inverse_2d_batch.argtypes = [vp, vp, vp, i32, vp, pd, pd, vp, pd, pd, i32,
+                                               ctypes.POINTER(ConversionC), vp, u32]
+    lib.dsmvc_hip_inverse_rows_batch.argtypes = [vp, vp, i32, vp, pd, pd, vp, pd, pd, i32, i32,
+                                                 ctypes.POINTER(ConversionC), vp, u32]
+    lib.dsmvc_hip_inverse_columns_batch.argtypes = [vp, vp, vp, pd, pd, i32, vp, pd, pd, i32,
+                                                    i32, ctypes.POINTER(ConversionC), vp, u32]
+    lib.dsmvc_hip_synchronize.argtypes = [vp, vp]
+    _lib = lib
+    return lib
+
+
+def check(status: int):
+    """Maps a C-ABI status to the reference's exception kinds."""
+    if status == OK:
+        return
+    message = load().dsmvc_hip_last_error().decode()
+    if status == INVALID_ARGUMENT:
+        raise ValueError(message)
+    raise DsmvcError(message)
