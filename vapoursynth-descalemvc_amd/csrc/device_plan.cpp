@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <limits>
 #include <stdexcept>
 #include <string>
 
@@ -64,36 +65,105 @@ PackedPlan pack_plan(const AxisPlan &p) {
     return out;
 }
 
-bool build_chunk_plan(const AxisPlan &p, int chunks, ChunkPlan &out) {
+bool build_chunk_plan(const AxisPlan &p, int max_chunks, ChunkPlan &out) {
     const int n = p.destination_size, w = p.half_bandwidth;
-    if (!chunked_band_supported(w) || chunks < 1) return false;
-    const int length = (n + chunks - 1) / chunks;
-    if (length < w || length > chunked_max_length()) return false;
+    if (!chunked_band_supported(w) || max_chunks < 1 || max_chunks > kMaxChunks) return false;
+    const int length = chunked_length_for((n + max_chunks - 1) / max_chunks);
+    if (length == 0 || length < w) return false;
+    int taps = 0;
+    for (int i = 0; i < n; ++i)
+        taps = std::max(taps, static_cast<int>(p.transpose_offsets[static_cast<std::size_t>(i) + 1]
+                                               - p.transpose_offsets[static_cast<std::size_t>(i)]));
+    const int width = chunked_width_for(std::max(taps, 1));
+    if (width == 0) return false;
     const int count = (n + length - 1) / length;
+    const int rows = count * length;
+    const int blocks = length / kBlockRows;
     out.chunks = count;
     out.length = length;
+    out.width = width;
+    out.blocks = blocks;
+    const auto un = static_cast<std::size_t>(n);
+    const auto ur = static_cast<std::size_t>(rows);
+
+    // Coefficient rows, zero past n.
+    out.coef.assign(ur * 16, 0.0F);
+    for (std::size_t i = 0; i < un; ++i) {
+        float *row = out.coef.data() + i * 16;
+        row[kMaxBand] = p.inverse_diagonal[i];
+        for (int d = 1; d <= w; ++d) {
+            const std::size_t at = static_cast<std::size_t>(d - 1) * un + i;
+            row[d - 1] = p.lower_ld[at];
+            row[kMaxBand + d] = p.upper_l[at];
+        }
+    }
+
+    // Staging windows of 8-row blocks (horizontal kernel).
+    out.blk_lo.assign(static_cast<std::size_t>(count) * blocks, 0);
+    out.rows_feasible = true;
+    for (int q = 0; q < count; ++q) {
+        int previous = 0;
+        for (int b = 0; b < blocks; ++b) {
+            int lo = std::numeric_limits<int>::max(), hi = -1;
+            for (int t = 0; t < kBlockRows; ++t) {
+                const int i = q * length + b * kBlockRows + t;
+                if (i >= n) break;
+                const auto k0 = p.transpose_offsets[static_cast<std::size_t>(i)];
+                const auto k1 = p.transpose_offsets[static_cast<std::size_t>(i) + 1];
+                if (k1 == k0) continue;
+                lo = std::min(lo, p.transpose_indices[k0]);
+                hi = std::max(hi, p.transpose_indices[k1 - 1]);
+            }
+            if (hi < 0) lo = previous;
+            else if (hi - lo >= kStageCols) out.rows_feasible = false;
+            out.blk_lo[static_cast<std::size_t>(q) * blocks + b] = lo;
+            previous = lo;
+        }
+    }
+
+    // Taps packed to `width` per row, short rows padded at the front with
+    // weight 0 at an index inside the row's staging window.
+    out.ell_idx.assign(ur * width, 0);
+    out.ell_w.assign(ur * width, 0.0F);
+    for (int i = 0; i < rows; ++i) {
+        const int q = i / length, b = (i % length) / kBlockRows;
+        const int window = out.blk_lo[static_cast<std::size_t>(q) * blocks + b];
+        std::int32_t *idx = out.ell_idx.data() + static_cast<std::size_t>(i) * width;
+        float *wt = out.ell_w.data() + static_cast<std::size_t>(i) * width;
+        if (i >= n) {
+            std::fill(idx, idx + width, window);
+            continue;
+        }
+        const auto k0 = p.transpose_offsets[static_cast<std::size_t>(i)];
+        const auto k1 = p.transpose_offsets[static_cast<std::size_t>(i) + 1];
+        const int c = static_cast<int>(k1 - k0);
+        const int pad = width - c;
+        const std::int32_t fill = c > 0 ? p.transpose_indices[k0] : window;
+        for (int e = 0; e < pad; ++e) idx[e] = fill;
+        for (int e = 0; e < c; ++e) {
+            idx[pad + e] = p.transpose_indices[k0 + static_cast<std::uint32_t>(e)];
+            wt[pad + e] = p.transpose_weights[k0 + static_cast<std::uint32_t>(e)];
+        }
+    }
+
+    // Transfer matrices over the padded rows, in FP64.
     out.tf.assign(static_cast<std::size_t>(count) * w * w, 0.0F);
     out.tb.assign(static_cast<std::size_t>(count) * w * w, 0.0F);
-    out.win.assign(static_cast<std::size_t>(count) * 2, 0);
-    const auto un = static_cast<std::size_t>(n);
-    auto lld = [&](int d, int i) { return static_cast<double>(p.lower_ld[static_cast<std::size_t>(d - 1) * un + static_cast<std::size_t>(i)]); };
-    auto ul = [&](int d, int i) { return static_cast<double>(p.upper_l[static_cast<std::size_t>(d - 1) * un + static_cast<std::size_t>(i)]); };
     std::vector<double> ring(static_cast<std::size_t>(w));
     auto push = [&](double v) {
         for (int d = w - 1; d > 0; --d) ring[static_cast<std::size_t>(d)] = ring[static_cast<std::size_t>(d - 1)];
         ring[0] = v;
     };
-    out.ring_feasible = true;
-    const int span = rows_ring_span();
+    auto coef = [&](int i, int slot) { return static_cast<double>(out.coef[static_cast<std::size_t>(i) * 16 + slot]); };
     for (int q = 0; q < count; ++q) {
-        const int a = q * length, b = std::min(n, a + length);
+        const int a = q * length, b = a + length;
         for (int c = 0; c < w; ++c) {
             std::fill(ring.begin(), ring.end(), 0.0);
             ring[static_cast<std::size_t>(c)] = 1.0;
             for (int i = a; i < b; ++i) {
                 double v = 0.0;
-                for (int d = w; d >= 1; --d) v -= lld(d, i) * ring[static_cast<std::size_t>(d - 1)];
-                push(v * static_cast<double>(p.inverse_diagonal[static_cast<std::size_t>(i)]));
+                for (int d = w; d >= 1; --d) v -= coef(i, d - 1) * ring[static_cast<std::size_t>(d - 1)];
+                push(v * coef(i, kMaxBand));
             }
             for (int r = 0; r < w; ++r)
                 out.tf[(static_cast<std::size_t>(q) * w + r) * w + c] = static_cast<float>(ring[static_cast<std::size_t>(r)]);
@@ -101,49 +171,14 @@ bool build_chunk_plan(const AxisPlan &p, int chunks, ChunkPlan &out) {
             ring[static_cast<std::size_t>(c)] = 1.0;
             for (int i = b - 1; i >= a; --i) {
                 double v = 0.0;
-                for (int d = w; d >= 1; --d) v -= ul(d, i) * ring[static_cast<std::size_t>(d - 1)];
+                for (int d = w; d >= 1; --d) v -= coef(i, kMaxBand + d) * ring[static_cast<std::size_t>(d - 1)];
                 push(v);
             }
             for (int r = 0; r < w; ++r)
                 out.tb[(static_cast<std::size_t>(q) * w + r) * w + c] = static_cast<float>(ring[static_cast<std::size_t>(r)]);
         }
-        // Source window and the rows kernel's staging walk (a ring of two
-        // 16-column halves that only moves forward).
-        int lo = p.source_size, hi = 0;
-        for (int i = a; i < b; ++i) {
-            const auto k0 = p.transpose_offsets[static_cast<std::size_t>(i)];
-            const auto k1 = p.transpose_offsets[static_cast<std::size_t>(i) + 1];
-            if (k1 > k0) {
-                lo = std::min(lo, p.transpose_indices[k0]);
-                hi = std::max(hi, p.transpose_indices[k1 - 1]);
-            }
-        }
-        if (lo > hi) lo = hi = 0;
-        out.win[static_cast<std::size_t>(q) * 2] = lo;
-        out.win[static_cast<std::size_t>(q) * 2 + 1] = hi;
-        int base = lo & ~(span - 1);
-        for (int i = a; i < b; ++i) {
-            const auto k0 = p.transpose_offsets[static_cast<std::size_t>(i)];
-            const auto k1 = p.transpose_offsets[static_cast<std::size_t>(i) + 1];
-            if (k1 == k0) continue;
-            const int top = p.transpose_indices[k1 - 1];
-            while (top >= base + 2 * span) base += span;
-            if (p.transpose_indices[k0] < base) out.ring_feasible = false;
-        }
     }
     return true;
-}
-
-int choose_chunks(const AxisPlan &p, bool rows) {
-    const int n = p.destination_size, w = p.half_bandwidth;
-    if (!chunked_band_supported(w)) return 0;
-    for (const int c : {16, 12, 8}) {
-        const int length = (n + c - 1) / c;
-        if (length < std::max(w, 8) || length > chunked_max_length()) continue;
-        (void)rows;
-        return c;
-    }
-    return 0;
 }
 
 DevicePlan::DevicePlan(const AxisPlan &plan, int device)
@@ -192,21 +227,27 @@ DevicePlan::~DevicePlan() {
     if (memory_) (void)hipFree(memory_);
 }
 
-const DevChunks *DevicePlan::chunks(int count, bool rows) {
+const DevChunks *DevicePlan::chunks(int max_chunks, bool rows) {
     std::lock_guard lock(mutex_);
-    auto it = chunk_images_.find(count);
+    auto it = chunk_images_.find(max_chunks);
     if (it == chunk_images_.end()) {
         ChunkImage image;
         ChunkPlan cp;
-        if (build_chunk_plan(*plan_, count, cp)) {
-            const std::size_t mats = cp.tf.size() * 4;
-            const std::size_t o_tb = align256(mats);
-            const std::size_t o_win = align256(o_tb + mats);
-            const std::size_t total = align256(o_win + cp.win.size() * 4);
+        if (build_chunk_plan(*plan_, max_chunks, cp)) {
+            const std::size_t o_tf = 0;
+            const std::size_t o_tb = align256(o_tf + cp.tf.size() * 4);
+            const std::size_t o_idx = align256(o_tb + cp.tb.size() * 4);
+            const std::size_t o_w = align256(o_idx + cp.ell_idx.size() * 4);
+            const std::size_t o_coef = align256(o_w + cp.ell_w.size() * 4);
+            const std::size_t o_blk = align256(o_coef + cp.coef.size() * 4);
+            const std::size_t total = align256(o_blk + cp.blk_lo.size() * 4);
             std::vector<unsigned char> host(total, 0);
-            std::memcpy(host.data(), cp.tf.data(), mats);
-            std::memcpy(host.data() + o_tb, cp.tb.data(), mats);
-            std::memcpy(host.data() + o_win, cp.win.data(), cp.win.size() * 4);
+            std::memcpy(host.data() + o_tf, cp.tf.data(), cp.tf.size() * 4);
+            std::memcpy(host.data() + o_tb, cp.tb.data(), cp.tb.size() * 4);
+            std::memcpy(host.data() + o_idx, cp.ell_idx.data(), cp.ell_idx.size() * 4);
+            std::memcpy(host.data() + o_w, cp.ell_w.data(), cp.ell_w.size() * 4);
+            std::memcpy(host.data() + o_coef, cp.coef.data(), cp.coef.size() * 4);
+            std::memcpy(host.data() + o_blk, cp.blk_lo.data(), cp.blk_lo.size() * 4);
             int current = 0;
             check(hipGetDevice(&current), "hipGetDevice failed");
             check(hipSetDevice(device_), "hipSetDevice failed");
@@ -217,15 +258,20 @@ const DevChunks *DevicePlan::chunks(int count, bool rows) {
             auto *b = static_cast<unsigned char *>(image.memory);
             image.dev.chunks = cp.chunks;
             image.dev.length = cp.length;
-            image.dev.tf = reinterpret_cast<const float *>(b);
+            image.dev.ell_width = cp.width;
+            image.dev.blocks = cp.blocks;
+            image.dev.tf = reinterpret_cast<const float *>(b + o_tf);
             image.dev.tb = reinterpret_cast<const float *>(b + o_tb);
-            image.dev.win = reinterpret_cast<const std::int32_t *>(b + o_win);
+            image.dev.ell_idx = reinterpret_cast<const std::int32_t *>(b + o_idx);
+            image.dev.ell_w = reinterpret_cast<const float *>(b + o_w);
+            image.dev.coef = reinterpret_cast<const float *>(b + o_coef);
+            image.dev.blk_lo = reinterpret_cast<const std::int32_t *>(b + o_blk);
             image.ok = true;
-            image.ring_feasible = cp.ring_feasible;
+            image.rows_feasible = cp.rows_feasible;
         }
-        it = chunk_images_.emplace(count, image).first;
+        it = chunk_images_.emplace(max_chunks, image).first;
     }
-    if (!it->second.ok || (rows && !it->second.ring_feasible)) return nullptr;
+    if (!it->second.ok || (rows && !it->second.rows_feasible)) return nullptr;
     return &it->second.dev;
 }
 

@@ -27,20 +27,25 @@ struct PackedPlan {
 };
 PackedPlan pack_plan(const AxisPlan &plan);
 
-// A chunk partition of a plan for the chunked kernels, built on the host in
-// FP64 from the F32 coefficients (transfer matrices stored F32).
+// A chunk partition of a plan for the chunked kernels (layout in
+// device_types.hpp, DevChunks): padded ELL taps and coefficient rows, the
+// FP64-built transfer matrices (stored F32) and the horizontal kernel's
+// staging windows.
 struct ChunkPlan {
     std::int32_t chunks = 0;
     std::int32_t length = 0;
-    bool ring_feasible = false;      // rows_chunked can stage every CSR row
+    std::int32_t width = 0;          // packed taps per row
+    std::int32_t blocks = 0;         // staging blocks per chunk
+    bool rows_feasible = false;      // every 8-row block's taps fit 32 columns
     std::vector<float> tf, tb;       // [chunks][W][W]
-    std::vector<std::int32_t> win;   // [chunks][2]
+    std::vector<std::int32_t> ell_idx;
+    std::vector<float> ell_w;
+    std::vector<float> coef;         // [chunks * length][16]
+    std::vector<std::int32_t> blk_lo;
 };
-// Returns false when the plan cannot be chunked with `chunks` chunks.
-bool build_chunk_plan(const AxisPlan &plan, int chunks, ChunkPlan &out);
-// The chunk count the executor uses for an axis of length n, or 0 for the
-// sequential route.
-int choose_chunks(const AxisPlan &plan, bool rows);
+// Partition into at most `max_chunks` chunks; false when the plan cannot use
+// the chunked kernels (band, length or tap count out of range).
+bool build_chunk_plan(const AxisPlan &plan, int max_chunks, ChunkPlan &out);
 
 // One device allocation holding a plan's image (and its chunk partitions).
 class DevicePlan {
@@ -52,9 +57,10 @@ public:
 
     [[nodiscard]] const DevAxis &axis() const { return axis_; }
     [[nodiscard]] const AxisPlan &host() const { return *plan_; }
-    // Chunk partition on the device; nullptr when the plan is not chunkable
-    // with that count.  Built lazily, cached.
-    const DevChunks *chunks(int count, bool rows);
+    // Chunk partition (at most `max_chunks` chunks) on the device; nullptr when
+    // the plan cannot use the chunked kernels in that direction.  Built
+    // lazily, cached.
+    const DevChunks *chunks(int max_chunks, bool rows);
     [[nodiscard]] std::size_t device_bytes() const { return bytes_; }
 
 private:
@@ -62,7 +68,7 @@ private:
         DevChunks dev{};
         void *memory = nullptr;
         bool ok = false;
-        bool ring_feasible = false;
+        bool rows_feasible = false;
     };
     std::shared_ptr<const AxisPlan> keep_;
     const AxisPlan *plan_;

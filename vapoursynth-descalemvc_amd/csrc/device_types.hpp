@@ -37,19 +37,34 @@ struct DevAxis {
     std::int32_t pad_;
 };
 
-// Partition of each vector into C chunks of L elements (the last may be
-// shorter) for the chunked substitution kernels, with the FP64-built W x W
-// carry transfer matrices of every chunk, stored F32:
+// Partition of each vector into C chunks of exactly L elements for the chunked
+// substitution kernels.  The plan is padded to C * L rows with zero rows
+// (empty gather, zero coefficients): a zero row solves to 0 and no real row
+// refers to it, so padding changes no result.  Per padded row:
+//   ell_idx / ell_w  [C*L][K]  the CSR row packed to K taps; short rows are
+//                    padded at the FRONT with weight 0 (an FMA chain that starts
+//                    with 0 * y + 0 stays +0, so the sum is unchanged)
+//   coef             [C*L][16] the coefficient record (kMaxBand layout)
+// Per chunk q: W x W carry transfer matrices (FP64-built, stored F32)
 //   tf[q][r][c] = d y[b_q - 1 - r] / d y[a_q - 1 - c]   (forward)
 //   tb[q][r][c] = d x[a_q + r]     / d x[b_q + c]       (backward)
-// and the source window [win[2q], win[2q+1]] the chunk's CSR rows touch.
+// and, for the horizontal kernel, the first source column of each 8-row block
+// (blk_lo[q][b]); the block's taps all lie in [blk_lo, blk_lo + 32).
 struct DevChunks {
     std::int32_t chunks;
     std::int32_t length;
+    std::int32_t ell_width;
+    std::int32_t blocks;
     const float *tf;
     const float *tb;
-    const std::int32_t *win;
+    const std::int32_t *ell_idx;
+    const float *ell_w;
+    const float *coef;
+    const std::int32_t *blk_lo;
 };
+
+constexpr int kBlockRows = 8;      // destination rows per staging block (rows kernel)
+constexpr int kStageCols = 32;     // source columns staged per block
 
 struct DevConv {
     float in_offset;

@@ -172,8 +172,8 @@ void HipExecutor::axis_pass(const AxisPlan &plan, bool rows, const void *input, 
     hip::DevicePlan &dp = device_plan(plan);
     if (!f64 && mode_ != SolveMode::exact) {
         int count = rows ? tuning_.rows_chunks : tuning_.columns_chunks;
-        if (count <= 0) count = hip::choose_chunks(plan, rows);
-        const hip::DevChunks *ch = count > 0 ? dp.chunks(count, rows) : nullptr;
+        if (count <= 0) count = hip::kMaxChunks;
+        const hip::DevChunks *ch = dp.chunks(count, rows);
         if (ch) {
             hip::ChunkArgs a{};
             a.axis = dp.axis();
@@ -187,8 +187,7 @@ void HipExecutor::axis_pass(const AxisPlan &plan, bool rows, const void *input, 
             a.nvec = vectors;
             a.frames = frames;
             a.conv = conv;
-            check(rows ? hip::launch_rows_chunked(a, in_kind, stream)
-                       : hip::launch_columns_chunked(a, out_kind, stream),
+            check(hip::launch_chunked(a, rows, rows ? in_kind : out_kind, stream),
                   "chunked solve launch failed");
             return;
         }

@@ -40,13 +40,24 @@ struct ChunkArgs {
 };
 
 bool seq_band_supported(int w);
+
+// Chunked kernels: bands 1, 3, 5, 7; chunk lengths 32, 48, 64, 80; packed
+// tap counts 4, 8, 16; at most 16 chunks.
 bool chunked_band_supported(int w);
-int chunked_max_length();
-int rows_ring_span();   // widest CSR row (last - first source index) rows_chunked can stage
+int chunked_length_for(int needed);   // smallest supported length >= needed, or 0
+int chunked_width_for(int taps);      // smallest supported tap count >= taps, or 0
+constexpr int kMaxChunks = 16;
 
 hipError_t launch_seq(const SeqArgs &a, Sample in, Sample out, bool f64, hipStream_t s);
-hipError_t launch_rows_chunked(const ChunkArgs &a, Sample in, hipStream_t s);
-hipError_t launch_columns_chunked(const ChunkArgs &a, Sample out, hipStream_t s);
+// rows: horizontal pass, `kind` is the input sample; else vertical pass,
+// `kind` is the output sample.
+hipError_t launch_chunked(const ChunkArgs &a, bool rows, Sample kind, hipStream_t s);
+template <int W>
+hipError_t launch_chunked_band(const ChunkArgs &a, bool rows, Sample kind, hipStream_t s);
+extern template hipError_t launch_chunked_band<1>(const ChunkArgs &, bool, Sample, hipStream_t);
+extern template hipError_t launch_chunked_band<3>(const ChunkArgs &, bool, Sample, hipStream_t);
+extern template hipError_t launch_chunked_band<5>(const ChunkArgs &, bool, Sample, hipStream_t);
+extern template hipError_t launch_chunked_band<7>(const ChunkArgs &, bool, Sample, hipStream_t);
 hipError_t launch_check_finite(const void *src, Sample in, long s_row, long s_frame, int rows,
                                int cols, int frames, int *flag, hipStream_t s);
 

@@ -15,9 +15,7 @@
 namespace dsmvc::hip {
 namespace dev {
 
-constexpr int kRingSlots = 32;     // columns staged per wave in rows_chunked
-constexpr int kRingHalf = 16;
-constexpr int kRingPitch = 65;     // +1 float: conflict-free transposed access
+constexpr int kRingPitch = 65;     // +1 float: conflict-free transposed LDS access
 
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -26,6 +24,30 @@ __device__ __forceinline__ void wave_sync() {
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Raw buffer over a plane: loads take a per-lane byte offset and a scalar
+// (wave-uniform) byte offset, and read 0 past `bytes`.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void *base, long bytes) {
+    const int n = bytes > 0x7FFFFFF0L ? 0x7FFFFFF0 : static_cast<int>(bytes);
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, n, 0x00020000);
+}
+
+template <class T> struct BufferLoad;
+template <> struct BufferLoad<float> {
+    __device__ static float at(__amdgpu_buffer_rsrc_t r, int v, int s) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, v, s, 0));
+    }
+};
+template <> struct BufferLoad<std::uint16_t> {
+    __device__ static std::uint16_t at(__amdgpu_buffer_rsrc_t r, int v, int s) {
+        return __builtin_amdgcn_raw_buffer_load_b16(r, v, s, 0);
+    }
+};
+template <> struct BufferLoad<std::uint8_t> {
+    __device__ static std::uint8_t at(__amdgpu_buffer_rsrc_t r, int v, int s) {
+        return __builtin_amdgcn_raw_buffer_load_b8(r, v, s, 0);
+    }
+};
+
 // ---- sample conversion ----------------------------------------------------
 // Integer input is normalised as ((float)x - offset) * scale with two
 // roundings, output as clamp(v * scale + offset, 0, max) rounded to nearest
@@ -33,6 +55,7 @@ __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirs
 template <class In> struct Load;
 template <> struct Load<float> {
     __device__ static float f32(const float *p, const DevConv &) { return *p; }
+    __device__ static float norm(float x, const DevConv &) { return x; }
     __device__ static double f64(const float *p, const DevConv &) { return static_cast<double>(*p); }
 };
 template <> struct Load<double> {
@@ -42,6 +65,9 @@ template <> struct Load<std::uint8_t> {
     __device__ static float f32(const std::uint8_t *p, const DevConv &c) {
         return __fmul_rn(__fsub_rn(static_cast<float>(*p), c.in_offset), c.in_scale);
     }
+    __device__ static float norm(std::uint8_t x, const DevConv &c) {
+        return __fmul_rn(__fsub_rn(static_cast<float>(x), c.in_offset), c.in_scale);
+    }
     __device__ static double f64(const std::uint8_t *p, const DevConv &c) {
         return __dmul_rn(__dsub_rn(static_cast<double>(*p), static_cast<double>(c.in_offset)),
                          static_cast<double>(c.in_scale));
@@ -50,6 +76,9 @@ template <> struct Load<std::uint8_t> {
 template <> struct Load<std::uint16_t> {
     __device__ static float f32(const std::uint16_t *p, const DevConv &c) {
         return __fmul_rn(__fsub_rn(static_cast<float>(*p), c.in_offset), c.in_scale);
+    }
+    __device__ static float norm(std::uint16_t x, const DevConv &c) {
+        return __fmul_rn(__fsub_rn(static_cast<float>(x), c.in_offset), c.in_scale);
     }
     __device__ static double f64(const std::uint16_t *p, const DevConv &c) {
         return __dmul_rn(__dsub_rn(static_cast<double>(*p), static_cast<double>(c.in_offset)),

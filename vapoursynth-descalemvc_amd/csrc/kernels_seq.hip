@@ -152,6 +152,30 @@ hipError_t launch_seq_t(const SeqArgs &a, hipStream_t s) {
 
 bool seq_band_supported(int w) { return w >= 0 && w <= kMaxWideBand; }
 
+bool chunked_band_supported(int w) { return w == 1 || w == 3 || w == 5 || w == 7; }
+
+int chunked_length_for(int needed) {
+    for (const int l : {32, 48, 64, 80})
+        if (needed <= l) return l;
+    return 0;
+}
+
+int chunked_width_for(int taps) {
+    for (const int k : {4, 8, 16})
+        if (taps <= k) return k;
+    return 0;
+}
+
+hipError_t launch_chunked(const ChunkArgs &a, bool rows, Sample kind, hipStream_t s) {
+    switch (a.axis.w) {
+    case 1: return launch_chunked_band<1>(a, rows, kind, s);
+    case 3: return launch_chunked_band<3>(a, rows, kind, s);
+    case 5: return launch_chunked_band<5>(a, rows, kind, s);
+    case 7: return launch_chunked_band<7>(a, rows, kind, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t launch_seq(const SeqArgs &a, Sample in, Sample out, bool f64, hipStream_t s) {
     if (!f64) {
         if (out == Sample::f32) {
