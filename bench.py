@@ -129,6 +129,8 @@ def main():
     ap.add_argument("--solve", default="auto", choices=["auto", "exact", "chunked"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-gray", action="store_true")
+    ap.add_argument("--calibration-copy", action="store_true",
+                    help="run one 256 MiB device copy before the timed region (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -194,6 +196,14 @@ def main():
         step(s)
     torch.cuda.synchronize()
     ex.synchronize()
+    if args.calibration_copy:
+        # A plain 256 MiB device copy: the profile's FETCH_SIZE / WRITE_SIZE
+        # for this kernel calibrate the counters against known bytes.
+        a = torch.empty(64 << 20, dtype=torch.float32, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     events = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()

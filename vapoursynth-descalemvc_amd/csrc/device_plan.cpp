@@ -78,11 +78,9 @@ bool build_chunk_plan(const AxisPlan &p, int max_chunks, ChunkPlan &out) {
     if (width == 0) return false;
     const int count = (n + length - 1) / length;
     const int rows = count * length;
-    const int blocks = length / kBlockRows;
     out.chunks = count;
     out.length = length;
     out.width = width;
-    out.blocks = blocks;
     const auto un = static_cast<std::size_t>(n);
     const auto ur = static_cast<std::size_t>(rows);
 
@@ -98,27 +96,43 @@ bool build_chunk_plan(const AxisPlan &p, int max_chunks, ChunkPlan &out) {
         }
     }
 
-    // Staging windows of 8-row blocks (horizontal kernel).
-    out.blk_lo.assign(static_cast<std::size_t>(count) * blocks, 0);
-    out.rows_feasible = true;
-    for (int q = 0; q < count; ++q) {
-        int previous = 0;
-        for (int b = 0; b < blocks; ++b) {
-            int lo = std::numeric_limits<int>::max(), hi = -1;
-            for (int t = 0; t < kBlockRows; ++t) {
-                const int i = q * length + b * kBlockRows + t;
-                if (i >= n) break;
-                const auto k0 = p.transpose_offsets[static_cast<std::size_t>(i)];
-                const auto k1 = p.transpose_offsets[static_cast<std::size_t>(i) + 1];
-                if (k1 == k0) continue;
-                lo = std::min(lo, p.transpose_indices[k0]);
-                hi = std::max(hi, p.transpose_indices[k1 - 1]);
+    // Staging windows of the horizontal kernel's row blocks: 16 rows when
+    // every block's taps fit 32 columns, else 8.
+    auto staging = [&](int g) {
+        const int nb = length / g;
+        std::vector<std::int32_t> lo_table(static_cast<std::size_t>(count) * nb, 0);
+        bool fits = true;
+        for (int q = 0; q < count; ++q) {
+            int previous = 0;
+            for (int b = 0; b < nb; ++b) {
+                int lo = std::numeric_limits<int>::max(), hi = -1;
+                for (int t = 0; t < g; ++t) {
+                    const int i = q * length + b * g + t;
+                    if (i >= n) break;
+                    const auto k0 = p.transpose_offsets[static_cast<std::size_t>(i)];
+                    const auto k1 = p.transpose_offsets[static_cast<std::size_t>(i) + 1];
+                    if (k1 == k0) continue;
+                    lo = std::min(lo, p.transpose_indices[k0]);
+                    hi = std::max(hi, p.transpose_indices[k1 - 1]);
+                }
+                if (hi < 0) lo = previous;
+                else if (hi - lo >= kStageCols) fits = false;
+                lo_table[static_cast<std::size_t>(q) * nb + b] = lo;
+                previous = lo;
             }
-            if (hi < 0) lo = previous;
-            else if (hi - lo >= kStageCols) out.rows_feasible = false;
-            out.blk_lo[static_cast<std::size_t>(q) * blocks + b] = lo;
-            previous = lo;
         }
+        if (fits) {
+            out.block_rows = g;
+            out.blocks = nb;
+            out.blk_lo = std::move(lo_table);
+        }
+        return fits;
+    };
+    out.rows_feasible = staging(16) || staging(kBlockRowsMin);
+    if (!out.rows_feasible) {
+        out.block_rows = kBlockRowsMin;
+        out.blocks = length / kBlockRowsMin;
+        out.blk_lo.assign(static_cast<std::size_t>(count) * out.blocks, 0);
     }
 
     // Taps packed to `width` per row, short rows padded at the front with
@@ -126,8 +140,8 @@ bool build_chunk_plan(const AxisPlan &p, int max_chunks, ChunkPlan &out) {
     out.ell_idx.assign(ur * width, 0);
     out.ell_w.assign(ur * width, 0.0F);
     for (int i = 0; i < rows; ++i) {
-        const int q = i / length, b = (i % length) / kBlockRows;
-        const int window = out.blk_lo[static_cast<std::size_t>(q) * blocks + b];
+        const int q = i / length, b = (i % length) / out.block_rows;
+        const int window = out.blk_lo[static_cast<std::size_t>(q) * out.blocks + b];
         std::int32_t *idx = out.ell_idx.data() + static_cast<std::size_t>(i) * width;
         float *wt = out.ell_w.data() + static_cast<std::size_t>(i) * width;
         if (i >= n) {
@@ -260,6 +274,7 @@ const DevChunks *DevicePlan::chunks(int max_chunks, bool rows) {
             image.dev.length = cp.length;
             image.dev.ell_width = cp.width;
             image.dev.blocks = cp.blocks;
+            image.dev.block_rows = cp.block_rows;
             image.dev.tf = reinterpret_cast<const float *>(b + o_tf);
             image.dev.tb = reinterpret_cast<const float *>(b + o_tb);
             image.dev.ell_idx = reinterpret_cast<const std::int32_t *>(b + o_idx);

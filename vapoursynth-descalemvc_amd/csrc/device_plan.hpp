@@ -36,6 +36,7 @@ struct ChunkPlan {
     std::int32_t length = 0;
     std::int32_t width = 0;          // packed taps per row
     std::int32_t blocks = 0;         // staging blocks per chunk
+    std::int32_t block_rows = 8;     // destination rows per staging block
     bool rows_feasible = false;      // every 8-row block's taps fit 32 columns
     std::vector<float> tf, tb;       // [chunks][W][W]
     std::vector<std::int32_t> ell_idx;

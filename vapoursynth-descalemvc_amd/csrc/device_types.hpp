@@ -48,13 +48,16 @@ struct DevAxis {
 // Per chunk q: W x W carry transfer matrices (FP64-built, stored F32)
 //   tf[q][r][c] = d y[b_q - 1 - r] / d y[a_q - 1 - c]   (forward)
 //   tb[q][r][c] = d x[a_q + r]     / d x[b_q + c]       (backward)
-// and, for the horizontal kernel, the first source column of each 8-row block
-// (blk_lo[q][b]); the block's taps all lie in [blk_lo, blk_lo + 32).
+// and, for the horizontal kernel, the first source column of each block of
+// block_rows destination rows (blk_lo[q][b]); the block's taps all lie in
+// [blk_lo, blk_lo + 32).  Blocks are 16 rows when every window fits, else 8.
 struct DevChunks {
     std::int32_t chunks;
     std::int32_t length;
     std::int32_t ell_width;
     std::int32_t blocks;
+    std::int32_t block_rows;
+    std::int32_t pad_;
     const float *tf;
     const float *tb;
     const std::int32_t *ell_idx;
@@ -63,7 +66,7 @@ struct DevChunks {
     const std::int32_t *blk_lo;
 };
 
-constexpr int kBlockRows = 8;      // destination rows per staging block (rows kernel)
+constexpr int kBlockRowsMin = 8;   // destination rows per staging block (rows kernel)
 constexpr int kStageCols = 32;     // source columns staged per block
 
 struct DevConv {

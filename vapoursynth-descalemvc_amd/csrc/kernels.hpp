@@ -42,7 +42,7 @@ struct ChunkArgs {
 bool seq_band_supported(int w);
 
 // Chunked kernels: bands 1, 3, 5, 7; chunk lengths 32, 48, 64, 80; packed
-// tap counts 4, 8, 16; at most 16 chunks.
+// tap counts 8, 16; at most 16 chunks.
 bool chunked_band_supported(int w);
 int chunked_length_for(int needed);   // smallest supported length >= needed, or 0
 int chunked_width_for(int taps);      // smallest supported tap count >= taps, or 0

@@ -19,8 +19,9 @@
 //
 // Vertical pass (columns_chunked): lanes are 64 adjacent columns; taps read
 // source rows directly (buffer loads at a scalar row offset).
-// Horizontal pass (rows_chunked): lanes are 64 rows; every 8 destination rows
-// the wave stages the 32 source columns their taps need into its LDS ring,
+// Horizontal pass (rows_chunked): lanes are 64 rows; every G (16, or 8 for
+// wide windows) destination rows the wave stages the 32 source columns their
+// taps need into its LDS ring,
 // transposed (pitch 65, conflict-free), and the results leave through the
 // same ring transposed back, so global traffic stays row-contiguous.
 #pragma once
@@ -102,7 +103,7 @@ columns_chunked(DevAxis ax, DevChunks ch, const float *__restrict__ src, long s_
         if (a + k < n && live) Store<Out>::f32(o + static_cast<long>(a + k) * d_row, y[k], cv);
 }
 
-template <int W, int L, int K, class In>
+template <int W, int L, int K, int G, class In>
 __global__ void __launch_bounds__(1024)
 rows_chunked(DevAxis ax, DevChunks ch, const In *__restrict__ src, long s_row, long s_frame,
              float *__restrict__ dst, long d_row, long d_frame, int nrows, DevConv cv) {
@@ -119,11 +120,11 @@ rows_chunked(DevAxis ax, DevChunks ch, const In *__restrict__ src, long s_row, l
     const int srow = lane >> 5;
     const std::int32_t *ei = ch.ell_idx + static_cast<long>(a) * K;
     const float *ew = ch.ell_w + static_cast<long>(a) * K;
-    const std::int32_t *blk = ch.blk_lo + j * (L / kBlockRows);
+    const std::int32_t *blk = ch.blk_lo + j * (L / G);
 
     float y[L];
 #pragma unroll
-    for (int b = 0; b < L / kBlockRows; ++b) {
+    for (int b = 0; b < L / G; ++b) {
         const int lo = blk[b];
         const int column = min(lo + scol, m1);
         wave_sync();
@@ -136,8 +137,8 @@ rows_chunked(DevAxis ax, DevChunks ch, const In *__restrict__ src, long s_row, l
         }
         wave_sync();
 #pragma unroll
-        for (int t = 0; t < kBlockRows; ++t) {
-            const int k = b * kBlockRows + t;
+        for (int t = 0; t < G; ++t) {
+            const int k = b * G + t;
             float v = 0.0F;
 #pragma unroll
             for (int e = 0; e < K; ++e)
@@ -189,25 +190,30 @@ hipError_t launch_cols_wlk(const ChunkArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
-template <int W, int L, int K, class In>
-hipError_t launch_rows_wlk(const ChunkArgs &a, hipStream_t s) {
+template <int W, int L, int K, int G, class In>
+hipError_t launch_rows_wlkg(const ChunkArgs &a, hipStream_t s) {
     const dim3 grid((a.nvec + 63) / 64, a.frames);
     const std::size_t shm = static_cast<std::size_t>(a.chunks.chunks) * kWaveRing * sizeof(float);
     // Up to 16 staging rings (133 KB) exceed the default dynamic-LDS limit.
     static const hipError_t attr = hipFuncSetAttribute(
-        reinterpret_cast<const void *>(&rows_chunked<W, L, K, In>),
+        reinterpret_cast<const void *>(&rows_chunked<W, L, K, G, In>),
         hipFuncAttributeMaxDynamicSharedMemorySize, 16 * kWaveRing * sizeof(float));
     if (attr != hipSuccess) return attr;
-    rows_chunked<W, L, K, In><<<grid, 64 * a.chunks.chunks, shm, s>>>(
+    rows_chunked<W, L, K, G, In><<<grid, 64 * a.chunks.chunks, shm, s>>>(
         a.axis, a.chunks, static_cast<const In *>(a.src), a.s_row, a.s_frame,
         static_cast<float *>(a.dst), a.d_row, a.d_frame, a.nvec, a.conv);
     return hipGetLastError();
 }
 
+template <int W, int L, int K, class In>
+hipError_t launch_rows_wlk(const ChunkArgs &a, hipStream_t s) {
+    return a.chunks.block_rows == 16 ? launch_rows_wlkg<W, L, K, 16, In>(a, s)
+                                     : launch_rows_wlkg<W, L, K, 8, In>(a, s);
+}
+
 template <int W, int L, class T, bool Rows>
 hipError_t launch_wl(const ChunkArgs &a, hipStream_t s) {
     switch (a.chunks.ell_width) {
-    case 4: return Rows ? launch_rows_wlk<W, L, 4, T>(a, s) : launch_cols_wlk<W, L, 4, T>(a, s);
     case 8: return Rows ? launch_rows_wlk<W, L, 8, T>(a, s) : launch_cols_wlk<W, L, 8, T>(a, s);
     case 16: return Rows ? launch_rows_wlk<W, L, 16, T>(a, s) : launch_cols_wlk<W, L, 16, T>(a, s);
     default: return hipErrorInvalidValue;

@@ -161,7 +161,7 @@ int chunked_length_for(int needed) {
 }
 
 int chunked_width_for(int taps) {
-    for (const int k : {4, 8, 16})
+    for (const int k : {8, 16})
         if (taps <= k) return k;
     return 0;
 }
