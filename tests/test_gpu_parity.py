@@ -214,6 +214,35 @@ def test_batch_equals_single_frames(executors):
             assert torch.equal(batched[f], single)
 
 
+def test_autotuned_partition_stable_and_close(oracle):
+    """A batch that fills the GPU triggers the partition autotuner (chroma
+    planes of the 1080p -> 720p Debicubic request, 64 frames).  Bars: the
+    tuned executor repeats its result bit for bit; every partition the tuner
+    can pick stays within 4e-6 * max(1, |x|) of the ordered oracle."""
+    reqs = config_plan_requests("config2")
+    h, v = reqs[2], reqs[3]
+    hp, vp = dh.Plan(_hreq(h)), dh.Plan(_hreq(v))
+    x = torch.rand(64, 540, 960, device=DEV)
+    tuned = dh.Executor(0, "auto")
+    first = tuned.inverse_2d(hp, vp, x)
+    second = tuned.inverse_2d(hp, vp, x)
+    assert torch.equal(first, second)
+    frames = x[[0, 63]].cpu().numpy()
+    oh, ov = oracle.build(h), oracle.build(v)
+    results = {"tuned": first[[0, 63]].cpu().numpy()}
+    for count in (16, 8, 6):
+        pinned = dh.Executor(0, "auto")
+        pinned.tune(count, count)
+        results[f"c{count}"] = pinned.inverse_2d(hp, vp, x[[0, 63]]).cpu().numpy()
+    for f in range(2):
+        expected = oracle.inverse_2d(oh, ov, frames[f], ordered=True)
+        scale = max(1.0, float(np.abs(expected).max()))
+        for name, got in results.items():
+            err = float(np.abs(got[f] - expected).max())
+            print(f"{name} frame {f}: max-abs {err:.3g}", flush=True)
+            assert err <= 4e-6 * scale
+
+
 def test_plugin_descale_yuv420(executors, oracle):
     """dsmvc_hip.Debicubic on a YUV420P16 clip equals the per-plane oracle
     within one code (chroma uses the shifted chroma plans)."""

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <bit>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <stdexcept>
@@ -24,6 +25,16 @@ void check(hipError_t e, const char *what) {
 }
 
 std::size_t align256(std::size_t v) { return (v + 255U) & ~std::size_t{255}; }
+
+std::ptrdiff_t sample_size(hip::Sample kind) {
+    switch (kind) {
+    case hip::Sample::u8: return 1;
+    case hip::Sample::u16: return 2;
+    case hip::Sample::f64: return 8;
+    case hip::Sample::f32: break;
+    }
+    return 4;
+}
 
 std::ptrdiff_t padded_width(std::int32_t n) { return (static_cast<std::ptrdiff_t>(n) + 63) & ~std::ptrdiff_t{63}; }
 
@@ -84,6 +95,10 @@ HipExecutor::HipExecutor(int device, SolveMode mode) : device_(device), mode_(mo
     if (device_ < 0) check(hipGetDevice(&device_), "hipGetDevice failed");
     if (device_ >= count) throw std::invalid_argument("dsmvc: HIP device index out of range");
     check(hipSetDevice(device_), "hipSetDevice failed");
+    hipDeviceProp_t prop{};
+    if (hipGetDeviceProperties(&prop, device_) == hipSuccess && prop.multiProcessorCount > 0)
+        compute_units_ = prop.multiProcessorCount;
+    if (const char *env = std::getenv("DSMVC_HIP_AUTOTUNE")) autotune_ = std::string(env) != "0";
 }
 
 HipExecutor::~HipExecutor() {
@@ -114,6 +129,10 @@ hip::DevicePlan &HipExecutor::device_plan(const AxisPlan &plan) {
     // was reused by another plan) is rebuilt.
     auto it = plans_.find(&plan);
     if (it != plans_.end() && same_plan(it->second->host(), plan)) return *it->second;
+    if (it != plans_.end()) {
+        const hip::DevicePlan *stale = it->second.get();
+        std::erase_if(tuned_, [stale](const auto &entry) { return entry.first.plan == stale; });
+    }
     check(hipSetDevice(device_), "hipSetDevice failed");
     auto image = std::make_unique<hip::DevicePlan>(plan, device_);
     auto &ref = *image;
@@ -161,6 +180,66 @@ void HipExecutor::synchronize(hipStream_t stream) {
     finish(stream, it->second, false, "dsmvc: HIP Float64 execution produced NaN or infinity");
 }
 
+// Picks the chunk count for a chunked pass: the executor's tuned choice, a
+// fresh measurement when the launch fills the GPU (every feasible count timed
+// with HIP events on the call's own buffers, whose output the caller's launch
+// then rewrites), or the largest partition.  0 = not chunkable.
+int HipExecutor::choose_partition(hip::DevicePlan &dp, bool rows, const hip::ChunkArgs &base,
+                                  hip::Sample kind, hipStream_t stream) {
+    const TuneKey key{&dp, rows, static_cast<int>(kind)};
+    if (auto it = tuned_.find(key); it != tuned_.end()) return it->second;
+    std::vector<int> candidates;
+    std::vector<std::pair<int, int>> shapes;
+    for (const int c : {hip::kMaxChunks, 12, 10, 8, 6, 4}) {
+        const hip::DevChunks *ch = dp.chunks(c, rows);
+        if (!ch) continue;
+        const std::pair<int, int> shape{ch->chunks, ch->length};
+        if (std::find(shapes.begin(), shapes.end(), shape) != shapes.end()) continue;
+        shapes.push_back(shape);
+        candidates.push_back(c);
+    }
+    if (candidates.empty()) return 0;
+    const long workgroups = static_cast<long>((base.nvec + 63) / 64) * base.frames;
+    if (!autotune_ || candidates.size() == 1) {
+        tuned_[key] = candidates.front();
+        return candidates.front();
+    }
+    if (workgroups < 2L * compute_units_) return candidates.front();
+    hipEvent_t start = nullptr, stop = nullptr;
+    check(hipEventCreate(&start), "event creation failed");
+    check(hipEventCreate(&stop), "event creation failed");
+    std::vector<float> best_ms(candidates.size(), std::numeric_limits<float>::max());
+    for (std::size_t i = 0; i < candidates.size(); ++i) {
+        hip::ChunkArgs a = base;
+        a.chunks = *dp.chunks(candidates[i], rows);
+        check(hip::launch_chunked(a, rows, kind, stream), "chunked solve launch failed");
+        for (int rep = 0; rep < 2; ++rep) {
+            check(hipEventRecord(start, stream), "event record failed");
+            check(hip::launch_chunked(a, rows, kind, stream), "chunked solve launch failed");
+            check(hipEventRecord(stop, stream), "event record failed");
+            check(hipEventSynchronize(stop), "event synchronize failed");
+            float ms = 0.0F;
+            check(hipEventElapsedTime(&ms, start, stop), "event timing failed");
+            best_ms[i] = std::min(best_ms[i], ms);
+        }
+    }
+    (void)hipEventDestroy(start);
+    (void)hipEventDestroy(stop);
+    std::size_t best = 0;
+    for (std::size_t i = 1; i < candidates.size(); ++i)
+        if (best_ms[i] < 0.9F * best_ms[0] && best_ms[i] < best_ms[best]) best = i;
+    tuned_[key] = candidates[best];
+    return candidates[best];
+}
+
+int HipExecutor::tuned_chunks(const AxisPlan &plan, bool rows, int kind) {
+    std::lock_guard lock(mutex_);
+    auto it = plans_.find(&plan);
+    if (it == plans_.end() || !same_plan(it->second->host(), plan)) return -1;
+    auto t = tuned_.find(TuneKey{it->second.get(), rows, kind});
+    return t == tuned_.end() ? 0 : t->second;
+}
+
 // One axis pass over `vectors` rows (rows = true) or columns of `frames`
 // planes.  Picks the chunked kernels when the mode and plan allow, else the
 // sequential solve, whose forward sweep goes to scratch at `fwd_offset`.
@@ -169,26 +248,42 @@ void HipExecutor::axis_pass(const AxisPlan &plan, bool rows, const void *input, 
                             Sample out_kind, std::ptrdiff_t out_row, std::ptrdiff_t out_frame,
                             std::int32_t vectors, std::int32_t frames, const DevConv &conv,
                             bool f64, Scratch &s, std::size_t fwd_offset, hipStream_t stream) {
+    {
+        // The kernels read other vectors' samples after writing their own
+        // outputs, so the two ranges must be disjoint (the 2D entries pass
+        // through an intermediate buffer and always are).
+        const std::ptrdiff_t m = plan.source_size, n = plan.destination_size;
+        const std::ptrdiff_t in_span = (frames - 1) * in_frame
+            + (rows ? (vectors - 1) * in_row + m : (m - 1) * in_row + vectors);
+        const std::ptrdiff_t out_span = (frames - 1) * out_frame
+            + (rows ? (vectors - 1) * out_row + n : (n - 1) * out_row + vectors);
+        const auto *a0 = static_cast<const unsigned char *>(input);
+        const auto *b0 = static_cast<const unsigned char *>(output);
+        const auto *a1 = a0 + in_span * sample_size(in_kind);
+        const auto *b1 = b0 + out_span * sample_size(out_kind);
+        if (a0 < b1 && b0 < a1)
+            throw std::invalid_argument("dsmvc: input and output planes must not overlap");
+    }
     hip::DevicePlan &dp = device_plan(plan);
     if (!f64 && mode_ != SolveMode::exact) {
+        hip::ChunkArgs a{};
+        a.axis = dp.axis();
+        a.src = input;
+        a.s_row = in_row;
+        a.s_frame = in_frame;
+        a.dst = output;
+        a.d_row = out_row;
+        a.d_frame = out_frame;
+        a.nvec = vectors;
+        a.frames = frames;
+        a.conv = conv;
+        const hip::Sample kind = rows ? in_kind : out_kind;
         int count = rows ? tuning_.rows_chunks : tuning_.columns_chunks;
-        if (count <= 0) count = hip::kMaxChunks;
-        const hip::DevChunks *ch = dp.chunks(count, rows);
+        if (count <= 0) count = choose_partition(dp, rows, a, kind, stream);
+        const hip::DevChunks *ch = count > 0 ? dp.chunks(count, rows) : nullptr;
         if (ch) {
-            hip::ChunkArgs a{};
-            a.axis = dp.axis();
             a.chunks = *ch;
-            a.src = input;
-            a.s_row = in_row;
-            a.s_frame = in_frame;
-            a.dst = output;
-            a.d_row = out_row;
-            a.d_frame = out_frame;
-            a.nvec = vectors;
-            a.frames = frames;
-            a.conv = conv;
-            check(hip::launch_chunked(a, rows, rows ? in_kind : out_kind, stream),
-                  "chunked solve launch failed");
+            check(hip::launch_chunked(a, rows, kind, stream), "chunked solve launch failed");
             return;
         }
         if (mode_ == SolveMode::chunked)

@@ -22,6 +22,9 @@
 #include "engine.hpp"
 
 namespace dsmvc {
+namespace hip {
+struct ChunkArgs;
+}
 
 enum class SolveMode : std::uint8_t {
     automatic = 0,   // chunked where the plan allows it, else sequential
@@ -45,7 +48,7 @@ struct PlaneBatch {
 
 // Per-launch tuning the executor chooses by itself unless overridden.
 struct LaunchTuning {
-    std::int32_t rows_chunks = 0;      // 0 = automatic
+    std::int32_t rows_chunks = 0;      // 0 = automatic (autotuned when enabled)
     std::int32_t columns_chunks = 0;
 };
 
@@ -63,6 +66,18 @@ public:
     void prepare(const std::shared_ptr<const AxisPlan> &plan);
     void seal();
     void set_tuning(const LaunchTuning &t) { tuning_ = t; }
+    // Autotuning of the chunked kernels' partition (on by default; the
+    // environment variable DSMVC_HIP_AUTOTUNE=0 turns it off).  The first
+    // chunked pass of a (plan, direction, sample kind) that fills the GPU
+    // (at least two workgroups per compute unit) times every feasible chunk
+    // count once on the call's own buffers and keeps the fastest; a smaller
+    // count must win by 10% to replace the largest.  Passes before that use
+    // the largest partition.  The choice is fixed for the executor's lifetime,
+    // so every later result of that key is bit-reproducible within it; the
+    // partition affects the last bits of Float32 results (carry rounding).
+    void set_autotune(bool enabled) { autotune_ = enabled; }
+    // The tuned chunk count of a key, 0 before tuning, -1 for an unknown plan.
+    [[nodiscard]] int tuned_chunks(const AxisPlan &plan, bool rows, int kind);
 
     // Float32 planes (device pointers).  Synchronous unless `async`.
     void inverse_rows(const AxisPlan &plan, const float *input, std::ptrdiff_t input_row_stride,
@@ -122,9 +137,30 @@ private:
                    std::int32_t vectors, std::int32_t frames, const hip::DevConv &conv,
                    bool f64, Scratch &s, std::size_t fwd_offset, hipStream_t stream);
 
+    int choose_partition(hip::DevicePlan &dp, bool rows, const hip::ChunkArgs &base,
+                         hip::Sample kind, hipStream_t stream);
+
+    struct TuneKey {
+        const hip::DevicePlan *plan;
+        bool rows;
+        int kind;
+        bool operator==(const TuneKey &o) const {
+            return plan == o.plan && rows == o.rows && kind == o.kind;
+        }
+    };
+    struct TuneKeyHash {
+        std::size_t operator()(const TuneKey &k) const noexcept {
+            return std::hash<const void *>()(k.plan) ^ (static_cast<std::size_t>(k.rows) << 1)
+                ^ (static_cast<std::size_t>(k.kind) << 2);
+        }
+    };
+
     int device_;
     SolveMode mode_;
     LaunchTuning tuning_{};
+    bool autotune_ = true;
+    int compute_units_ = 256;
+    std::unordered_map<TuneKey, int, TuneKeyHash> tuned_;
     std::mutex mutex_;
     std::unordered_map<const AxisPlan *, std::unique_ptr<dsmvc::hip::DevicePlan>> plans_;
     std::unordered_map<hipStream_t, Scratch> scratch_;
