@@ -121,6 +121,159 @@ def cpu_baseline(planes_host, seconds=10.0):
                       f"{elapsed:.1f} s"}
 
 
+def batched_scan(torch, dh, dev, frame, repeats):
+    """Mixed-size solves: one 1920x1080 F32 frame descaled (Mitchell, b = c =
+    1/3) to every height 500..1000 with width round(16h/9) -- 501 plan pairs,
+    1002 axis solves per pass.  The multi-plan batch (one launch per
+    direction and tap width over device job tables, replayed from a HIP
+    graph) against one inverse_2d call per plan pair, both asynchronous on
+    one stream and timed with HIP events."""
+    heights = list(range(500, 1001))
+    kw = dict(kernel=dh.BICUBIC, b=1.0 / 3.0, c=1.0 / 3.0, border=dh.SYMMETRIC, f64_mode=0)
+    jobs, pair_bytes = [], 0
+    t0 = time.perf_counter()
+    for h in heights:
+        w = int(math.floor(h * 16.0 / 9.0 + 0.5))
+        hp = dh.Plan(dh.AxisRequest(SRC_W, w, float(w), 0.0, **kw))
+        vp = dh.Plan(dh.AxisRequest(SRC_H, h, float(h), 0.0, **kw))
+        jobs.append((hp, vp, frame, torch.empty((h, w), device=dev)))
+        pair_bytes += 4 * (SRC_W * SRC_H + w * h)
+    plan_s = time.perf_counter() - t0
+    ex = dh.Executor(0, "auto")
+    t0 = time.perf_counter()
+    batch = dh.Batch(ex, jobs, graph=True)
+    setup_s = time.perf_counter() - t0
+    info = batch.info()
+    reference = [torch.empty_like(j[3]) for j in jobs]
+    stream = torch.cuda.current_stream()
+
+    def per_plan(outs):
+        for (hp, vp, x, _), out in zip(jobs, outs):
+            ex.inverse_2d(hp, vp, x, out=out, asynchronous=True)
+
+    batch.run()
+    per_plan(reference)
+    torch.cuda.synchronize()
+    diff = max(float((j[3] - r).abs().max()) for j, r in zip(jobs, reference))
+
+    def timed(fn):
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        for _ in range(repeats):
+            fn()
+        stop.record(stream)
+        stop.synchronize()
+        return start.elapsed_time(stop) / 1e3 / repeats
+
+    batch_s = timed(lambda: batch.run(stream, asynchronous=True))
+    plain_s = timed(lambda: per_plan(reference))
+    solves = 2 * len(heights)
+    return {
+        "workload": "Mitchell 1920x1080 F32 -> h x round(16h/9), h = 500..1000 (501 plan pairs)",
+        "axis_solves_per_pass": solves,
+        "batched": {"solves_per_s": round(solves / batch_s, 1),
+                    "pairs_per_s": round(len(heights) / batch_s, 1),
+                    "ms_per_pass": round(batch_s * 1e3, 3),
+                    "achieved_GBps": round(pair_bytes / batch_s / 1e9, 1),
+                    "frac_of_8TBps": round(pair_bytes / batch_s / 1e9 / HBM_PEAK_GBPS, 4),
+                    "launches_per_pass": info["launches"], "graph": bool(info["graph"]),
+                    "items_per_pass": info["items"]},
+        "one_plan_per_launch": {"solves_per_s": round(solves / plain_s, 1),
+                                "ms_per_pass": round(plain_s * 1e3, 3)},
+        "speedup": round(plain_s / batch_s, 2),
+        "max_abs_diff_batched_vs_per_plan": diff,
+        "algorithmic_bytes_per_pass": pair_bytes,
+        "host_plan_build_s": round(plan_s, 3), "batch_setup_s": round(setup_s, 3),
+        "repeats": repeats,
+    }
+
+
+def served_stream(torch, dh, dev, seconds, rate=400.0, seed=5):
+    """Mixed-size serving: a seeded Poisson request stream (mean `rate`
+    requests/s for `seconds`) of GRAYS / YUV420PS frames from sources
+    3840x2160, 1920x1080 and 1280x720, target height uniform in
+    [0.5, 0.95] x source height (16:9 width, even sizes), kernel drawn from
+    {bilinear, bicubic(0, 0.5), lanczos3, spline36} and f64mode from {0, 1, 2},
+    through the continuous-batching request loop.  Latency is submit ->
+    completion observed by the loop; a client thread submits on schedule."""
+    import threading
+    rng = np.random.default_rng(seed)
+    sizes = [(3840, 2160), (1920, 1080), (1280, 720)]
+    kernels = [(dh.BILINEAR, 3, 0.0, 0.5), (dh.BICUBIC, 3, 0.0, 0.5), (dh.LANCZOS, 3, 0.0, 0.5),
+               (dh.SPLINE36, 3, 0.0, 0.5)]
+    # Source frames: smoothed seeded noise in [0, 1], two frames per size.
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    pool = {}
+    for (w, h) in sizes:
+        frames = []
+        for _ in range(2):
+            planes = []
+            for (pw, ph) in [(w, h), (w // 2, h // 2), (w // 2, h // 2)]:
+                x = torch.rand((1, 1, ph, pw), generator=gen, device=dev)
+                x = torch.nn.functional.avg_pool2d(x, 3, stride=1, padding=1,
+                                                   count_include_pad=False)[0, 0]
+                planes.append(x.contiguous())
+            frames.append(planes)
+        pool[(w, h)] = frames
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=int(rate * seconds * 1.5) + 16))
+    arrivals = arrivals[arrivals < seconds]
+    reqs = []
+    for k, t in enumerate(arrivals):
+        w, h = sizes[rng.integers(3)]
+        dh_ = int(h * rng.uniform(0.5, 0.95)) & ~1
+        dw = int(math.floor(dh_ * 16.0 / 9.0 + 0.5))
+        dw += dw & 1
+        yuv = bool(rng.integers(2))
+        src = pool[(w, h)][k % 2]
+        ins = src if yuv else src[:1]
+        outs = [torch.empty((dh_, dw), device=dev)]
+        if yuv:
+            outs += [torch.empty((dh_ // 2, dw // 2), device=dev) for _ in range(2)]
+        reqs.append((float(t), ins, outs, kernels[rng.integers(4)], int(rng.integers(3))))
+    torch.cuda.synchronize()
+    ex = dh.Executor(0, "auto")
+    server = dh.Server(ex, memory_budget=8 << 30, slots=4)
+    ids = [None] * len(reqs)
+    lag = np.zeros(len(reqs))
+    start = time.perf_counter()
+
+    def client():
+        for k, (t, ins, outs, (kernel, taps, b, c), mode) in enumerate(reqs):
+            delay = t - (time.perf_counter() - start)
+            if delay > 0:
+                time.sleep(delay)
+            lag[k] = (time.perf_counter() - start) - t
+            ids[k] = server.submit(ins, outs, kernel=kernel, taps=taps, b=b, c=c, f64_mode=mode)
+
+    thread = threading.Thread(target=client)
+    thread.start()
+    thread.join()
+    lat = []
+    for rid in ids:
+        t0, t1 = server.wait(rid)
+        lat.append(t1 - t0)
+    elapsed = time.perf_counter() - start
+    stats = server.stats()
+    server.close()
+    lat = np.array(lat) * 1e3
+    planes = sum(len(r[1]) for r in reqs)
+    return {
+        "workload": "Poisson %.0f req/s, GRAYS/YUV420PS from 3840x2160/1920x1080/1280x720, "
+                    "h ~ U[0.5,0.95]H, 4 kernels, f64mode 0/1/2" % rate,
+        "requests": len(reqs), "seconds": round(elapsed, 3),
+        "requests_per_s": round(len(reqs) / elapsed, 1),
+        "solves_per_s": round(2 * planes / elapsed, 1),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                       "p99": round(float(np.percentile(lat, 99)), 3),
+                       "max": round(float(lat.max()), 3)},
+        "client_submit_lag_ms_p99": round(float(np.percentile(lag, 99)) * 1e3, 3),
+        "rounds": stats["rounds"], "largest_round": stats["largest_round"],
+        "table_planes": stats["table_planes"], "fallback_planes": stats["fallback_planes"],
+        "memory_high_water_bytes": stats["memory_high_water"],
+    }
+
+
 TRAFFIC_FILE = ROOT / "profiles" / "yuv420p16_debicubic_1080p_720p_traffic.json"
 
 
@@ -142,9 +295,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--solve", default="auto", choices=["auto", "exact", "chunked"])
+    ap.add_argument("--solve", default="auto", choices=["auto", "exact", "chunked", "windowed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-gray", action="store_true")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the mixed-size multi-plan scan (501 plan pairs)")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the served Poisson request stream")
+    ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--calibration-copy", action="store_true",
                     help="run one 256 MiB device copy before the timed region (counter calibration)")
     args = ap.parse_args()
@@ -276,6 +434,13 @@ def main():
             "achieved_GBps": round(FRAME_BYTES_GRAY_F32 * gfps / 1e9, 1),
             "frac_of_8TBps": round(FRAME_BYTES_GRAY_F32 * gfps / 1e9 / HBM_PEAK_GBPS, 4)}
         del gin
+
+    if not args.no_batch:
+        frame = (unique[0][0].to(torch.float32) / 65535.0).contiguous()
+        extra["mixed_size_scan"] = batched_scan(torch, dh, dev, frame, repeats=5)
+
+    if not args.no_serve:
+        extra["served_stream"] = served_stream(torch, dh, dev, seconds=args.serve_seconds)
 
     cpu = None
     if not args.no_cpu_baseline:

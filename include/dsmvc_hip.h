@@ -40,7 +40,8 @@ extern "C" {
  * reference numbering (include/dsmvc/engine.hpp:16-58); backend 5 is hip. */
 enum { DSMVC_HIP_BILINEAR = 0, DSMVC_HIP_BICUBIC = 1, DSMVC_HIP_LANCZOS = 2,
        DSMVC_HIP_SPLINE16 = 3, DSMVC_HIP_SPLINE36 = 4, DSMVC_HIP_SPLINE64 = 5 };
-enum { DSMVC_HIP_SOLVE_AUTO = 0, DSMVC_HIP_SOLVE_EXACT = 1, DSMVC_HIP_SOLVE_CHUNKED = 2 };
+enum { DSMVC_HIP_SOLVE_AUTO = 0, DSMVC_HIP_SOLVE_EXACT = 1, DSMVC_HIP_SOLVE_CHUNKED = 2,
+       DSMVC_HIP_SOLVE_WINDOWED = 3 };
 enum { DSMVC_HIP_SAMPLE_F32 = 0, DSMVC_HIP_SAMPLE_U8 = 1, DSMVC_HIP_SAMPLE_U16 = 2 };
 
 /* dsmvc::AxisRequest with KernelSpec flattened (include/dsmvc/engine.hpp:60-78). */
@@ -81,6 +82,19 @@ typedef struct dsmvc_hip_plan_info {
     int32_t nnz;
 } dsmvc_hip_plan_info;
 
+/* Windowed partition of a plan (see dsmvc_hip_plan_window_info). */
+typedef struct dsmvc_hip_window_info {
+    int32_t feasible;        /* 1: the windowed kernels can solve this plan */
+    int32_t rows_feasible;   /* 1: also in the horizontal pass (staging fits) */
+    int32_t band;            /* band class the kernels run: 1, 3, 5 or 7 */
+    int32_t chunks;
+    int32_t length;          /* destination rows per chunk (L) */
+    int32_t warmup;          /* warm-up rows on each side (P) */
+    int32_t width;           /* packed taps per row: 8 or 16 */
+    int32_t reserved;
+    double residual;         /* worst influence of a chunk's zero start, relative */
+} dsmvc_hip_window_info;
+
 typedef struct dsmvc_hip_planner_stats {
     uint64_t plan_hits;
     uint64_t plan_builds;
@@ -110,6 +124,9 @@ int dsmvc_hip_plan_arrays(const dsmvc_hip_plan *plan, uint32_t *offsets, int32_t
                           double *normal_bands_f64, double *ldlt_bands_f64,
                           double *inverse_diagonal_f64);
 /* dsmvc::planner_cache_stats / clear_planner_caches (engine.hpp:179-180). */
+/* No counterpart in the reference: reports how the windowed (independent
+ * chunk) kernels would partition this plan, computed on the host. */
+int dsmvc_hip_plan_window_info(const dsmvc_hip_plan *plan, dsmvc_hip_window_info *info);
 int dsmvc_hip_planner_cache_stats(dsmvc_hip_planner_stats *stats);
 void dsmvc_hip_clear_planner_caches(void);
 /* dsmvc::detail::padding_index (src/axis_plan.cpp:1024-1066). */
@@ -130,7 +147,9 @@ void dsmvc_hip_executor_destroy(dsmvc_hip_executor *executor);
 /* Executor::prepare / seal (src/executor.cpp:313-349): uploads the plan image. */
 int dsmvc_hip_executor_prepare(dsmvc_hip_executor *executor, const dsmvc_hip_plan *plan);
 int dsmvc_hip_executor_seal(dsmvc_hip_executor *executor);
-/* Chunk counts of the partitioned solves (0 = automatic). */
+/* Route of the parallel F32 solves per direction: a chunk count for the
+ * carry-chain chunked kernels, -1 for the windowed kernels, 0 = automatic
+ * (autotuned).  A route the plan cannot use falls back to automatic. */
 int dsmvc_hip_executor_tune(dsmvc_hip_executor *executor, int32_t rows_chunks,
                             int32_t columns_chunks);
 
@@ -197,6 +216,115 @@ int dsmvc_hip_inverse_columns_batch(dsmvc_hip_executor *executor, const dsmvc_hi
                                     void *stream, uint32_t flags);
 
 /* Waits for `stream`; reports a deferred Float64 non-finite failure. */
+/* ---- multi-plan batches --------------------------------------------------
+ * No single counterpart in the reference, whose executors solve one plane per
+ * call (src/vs_plugin.cpp:1636-1745); its closest analogue is the Metal
+ * route's axis-job descriptor table, at most 48 jobs per dispatch
+ * (src/metal/metal_routes.metal:25-35, 363-408).  A batch holds many 2D F32
+ * jobs of uneven geometry; each run solves all of them with one launch per
+ * (direction, band class, tap width) over device-resident job tables, and
+ * solves jobs those kernels cannot take (Float64 route, bands above 7) one by
+ * one after them.  Buffers are fixed at creation; a run rewrites the outputs.
+ * Job outputs must not overlap any job's input or output.  The executor must
+ * outlive the batch. */
+typedef struct dsmvc_hip_batch_job {
+    const dsmvc_hip_plan *horizontal;
+    const dsmvc_hip_plan *vertical;
+    const float *input;              /* device, F32 */
+    ptrdiff_t input_row_stride;      /* samples */
+    ptrdiff_t input_frame_stride;    /* samples; read when frames > 1 */
+    float *output;                   /* device, F32 */
+    ptrdiff_t output_row_stride;
+    ptrdiff_t output_frame_stride;
+    int32_t frames;
+    int32_t reserved;
+} dsmvc_hip_batch_job;
+
+typedef struct dsmvc_hip_batch_info {
+    int32_t jobs;
+    int32_t table_jobs;              /* solved by the table launches */
+    int32_t fallback_jobs;           /* solved one by one */
+    int32_t launches;                /* table launches per run */
+    int64_t items;                   /* wavefront work items per run */
+    int64_t intermediate_bytes;
+    int32_t graph;                   /* 1: runs replay a captured HIP graph */
+    int32_t reserved;
+} dsmvc_hip_batch_info;
+
+typedef struct dsmvc_hip_batch dsmvc_hip_batch;
+
+/* flags for dsmvc_hip_batch_create: capture the table launches into a HIP
+ * graph at creation and replay it on every run. */
+#define DSMVC_HIP_BATCH_GRAPH 2u
+
+int dsmvc_hip_batch_create(dsmvc_hip_executor *executor, const dsmvc_hip_batch_job *jobs,
+                           int32_t count, uint32_t flags, dsmvc_hip_batch **batch);
+/* flags: DSMVC_HIP_ASYNC to return once the run is queued on `stream`. */
+int dsmvc_hip_batch_run(dsmvc_hip_batch *batch, void *stream, uint32_t flags);
+int dsmvc_hip_batch_info_get(const dsmvc_hip_batch *batch, dsmvc_hip_batch_info *info);
+void dsmvc_hip_batch_destroy(dsmvc_hip_batch *batch);
+
+/* ---- request loop (continuous batching) ------------------------------------
+ * Counterpart of serving concurrent getFrame calls through the reference's
+ * executors: fmParallel filters (src/vs_plugin.cpp:1955-1957) whose CUDA
+ * calls are bounded by execution slots (src/cuda/cuda_executor.cpp:1102-1157,
+ * 1462-1503) with a per-executor plan cache (1936-2020).  Requests of any
+ * geometry are planned on submit (plans cached by kernel, geometry, padding
+ * and precision), admitted when a slot and device memory for their
+ * intermediates are free, and solved together with everything else admitted
+ * in the same round; rounds on different slots overlap.  A request's result
+ * is the same alone and in any mix.  Planes are F32 device buffers that must
+ * stay valid until the request finishes. */
+typedef struct dsmvc_hip_serve_request {
+    int32_t planes;                  /* 1 (gray) or 3 (YUV) */
+    int32_t subsampling_w;           /* log2 chroma subsampling, YUV only */
+    int32_t subsampling_h;
+    int32_t src_width, src_height;
+    int32_t dst_width, dst_height;
+    int32_t kernel;                  /* DSMVC_HIP_BILINEAR ... */
+    int32_t taps;
+    int32_t border;                  /* padding mode 0..4 */
+    int32_t f64_mode;                /* 0 automatic, 1 float32 only, 2 float64 */
+    int32_t reserved;
+    double b, c, blur;
+    const float *src[3];
+    ptrdiff_t src_stride[3];         /* samples */
+    float *dst[3];
+    ptrdiff_t dst_stride[3];
+} dsmvc_hip_serve_request;
+
+typedef struct dsmvc_hip_server_config {
+    int64_t memory_budget;           /* device bytes for in-flight intermediates */
+    int32_t slots;                   /* rounds in flight, 1..16 */
+    int32_t max_round_requests;
+} dsmvc_hip_server_config;
+
+typedef struct dsmvc_hip_server_stats {
+    uint64_t submitted, completed, failed, rounds;
+    uint64_t table_planes, fallback_planes, largest_round;
+    int64_t memory_high_water;
+} dsmvc_hip_server_stats;
+
+typedef struct dsmvc_hip_server dsmvc_hip_server;
+
+int dsmvc_hip_server_create(dsmvc_hip_executor *executor, const dsmvc_hip_server_config *config,
+                            dsmvc_hip_server **server);
+/* Plans and queues a request (thread-safe); *id identifies it. */
+int dsmvc_hip_server_submit(dsmvc_hip_server *server, const dsmvc_hip_serve_request *request,
+                            uint64_t *id);
+/* Waits for a request; returns its status (DSMVC_HIP_OK or the failure's
+ * code).  Times are seconds since the server started: submission and the
+ * moment completion was observed (either pointer may be NULL). */
+int dsmvc_hip_server_wait(dsmvc_hip_server *server, uint64_t id, double *submitted,
+                          double *completed);
+/* *state: 0 pending, 1 done, -1 failed. */
+int dsmvc_hip_server_poll(dsmvc_hip_server *server, uint64_t id, int32_t *state);
+/* Forgets a finished request. */
+int dsmvc_hip_server_release(dsmvc_hip_server *server, uint64_t id);
+int dsmvc_hip_server_stats_get(const dsmvc_hip_server *server, dsmvc_hip_server_stats *stats);
+/* Finishes every accepted request, then stops. */
+void dsmvc_hip_server_destroy(dsmvc_hip_server *server);
+
 int dsmvc_hip_synchronize(dsmvc_hip_executor *executor, void *stream);
 
 #ifdef __cplusplus

@@ -1,0 +1,110 @@
+"""GPU tests of multi-plan batches (include/dsmvc_hip.h, dsmvc_hip_batch_*).
+
+Bars:
+  * every table-solved job within 4e-6 * max(1, |x|) of the ordered oracle's
+    2D solve (the same tolerance as any parallel F32 route);
+  * a batch equals itself across runs bit for bit, with or without the
+    captured HIP graph;
+  * jobs the table kernels cannot take (Float64 route) are solved one by one
+    and equal the executor's own result for that job bit for bit.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+import dsmvc_hip as dh  # noqa: E402
+from support import BICUBIC, F64_ALWAYS, make_request  # noqa: E402
+from workloads import width_for_height  # noqa: E402
+
+DEV = torch.device("cuda:0")
+
+
+def _plan(req):
+    return dh.Plan(dh.AxisRequest(req.source_size, req.destination_size, req.active_length,
+                                  req.shift, req.kernel, req.taps, req.b, req.c, req.blur,
+                                  req.border, req.f64_mode))
+
+
+@pytest.fixture(scope="module")
+def executor():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return dh.Executor(0, "auto")
+
+
+def _mitchell_jobs(heights, x):
+    jobs = []
+    for h in heights:
+        w = width_for_height(h)
+        hr = make_request(1920, w, kernel=BICUBIC, b=1 / 3, c=1 / 3)
+        vr = make_request(1080, h, kernel=BICUBIC, b=1 / 3, c=1 / 3)
+        out = torch.full((h, w), float("nan"), device=DEV)
+        jobs.append((hr, vr, _plan(hr), _plan(vr), out))
+    return jobs
+
+
+def test_batch_of_uneven_plans_matches_oracle(executor, oracle):
+    rng = np.random.default_rng(4)
+    frame = rng.uniform(0, 1, (1080, 1920)).astype(np.float32)
+    x = torch.from_numpy(frame).to(DEV)
+    heights = [500, 537, 600, 719, 720, 811, 900, 999, 1000]
+    jobs = _mitchell_jobs(heights, x)
+    batch = dh.Batch(executor, [(hp, vp, x, out) for _, _, hp, vp, out in jobs])
+    info = batch.info()
+    print(info, flush=True)
+    assert info["jobs"] == len(heights) and info["table_jobs"] == len(heights)
+    assert info["fallback_jobs"] == 0 and info["launches"] <= 4
+    batch.run()
+    for (hr, vr, _, _, out), h in zip(jobs, heights):
+        got = out.cpu().numpy()
+        assert np.isfinite(got).all()
+        expected = oracle.inverse_2d(oracle.build(hr), oracle.build(vr), frame, ordered=True)
+        scale = max(1.0, float(np.abs(expected).max()))
+        err = float(np.abs(got - expected).max())
+        print(f"height {h}: max-abs {err:.3g}", flush=True)
+        assert err <= 4e-6 * scale
+
+
+def test_batch_runs_repeat_bit_for_bit_with_and_without_graph(executor):
+    rng = np.random.default_rng(5)
+    x = torch.from_numpy(rng.uniform(0, 1, (1080, 1920)).astype(np.float32)).to(DEV)
+    heights = list(range(500, 1001, 50))
+    plain = _mitchell_jobs(heights, x)
+    graphed = _mitchell_jobs(heights, x)
+    a = dh.Batch(executor, [(hp, vp, x, out) for _, _, hp, vp, out in plain])
+    b = dh.Batch(executor, [(hp, vp, x, out) for _, _, hp, vp, out in graphed], graph=True)
+    assert b.info()["graph"] == 1
+    a.run()
+    first = [out.clone() for *_, out in plain]
+    for replay in range(3):
+        for *_, out in graphed:
+            out.fill_(float("nan"))
+        b.run()
+        torch.cuda.synchronize()
+        for (*_, out), ref in zip(graphed, first):
+            assert torch.equal(out, ref), replay
+        # Eager work and a synchronisation between replays.
+        a.run()
+        for (*_, out), ref in zip(plain, first):
+            assert torch.equal(out, ref)
+
+
+def test_batch_solves_float64_jobs_one_by_one(executor):
+    rng = np.random.default_rng(6)
+    x = torch.from_numpy(rng.uniform(0, 1, (2, 540, 960)).astype(np.float32)).to(DEV)
+    hr = make_request(960, 640, kernel=BICUBIC, f64_mode=F64_ALWAYS)
+    vr = make_request(540, 360, kernel=BICUBIC, f64_mode=F64_ALWAYS)
+    hp, vp = _plan(hr), _plan(vr)
+    fr = make_request(960, 700, kernel=BICUBIC)
+    gr = make_request(540, 400, kernel=BICUBIC)
+    fp, gp = _plan(fr), _plan(gr)
+    out64 = torch.empty((2, 360, 640), device=DEV)
+    out32 = torch.empty((2, 400, 700), device=DEV)
+    batch = dh.Batch(executor, [(hp, vp, x, out64), (fp, gp, x, out32)])
+    info = batch.info()
+    assert info["fallback_jobs"] == 1 and info["table_jobs"] == 1
+    batch.run()
+    expected = executor.inverse_2d(hp, vp, x)
+    assert torch.equal(out64, expected)

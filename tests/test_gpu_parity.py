@@ -33,7 +33,7 @@ def _hreq(r):
 def executors():
     if not torch.cuda.is_available():
         pytest.skip("no HIP device")
-    return {m: dh.Executor(0, m) for m in ("exact", "auto")}
+    return {m: dh.Executor(0, m) for m in ("exact", "auto", "windowed")}
 
 
 def _identical(a, b):
@@ -101,6 +101,82 @@ def test_auto_rows_and_columns_close(executors, oracle, name, req):
     xc = np.ascontiguousarray(x.T)
     gotc = executors["auto"].inverse_columns(plan, torch.from_numpy(xc).to(DEV)).cpu().numpy()
     assert float(np.abs(gotc - expected.T).max()) <= 4e-6 * scale
+
+
+@pytest.mark.parametrize("name,req", AXIS_CASES, ids=[c[0] for c in AXIS_CASES])
+def test_windowed_rows_and_columns_close(executors, oracle, name, req):
+    """Windowed (independent-chunk) route: |HIP - ordered oracle| <= 4e-6 * max(1, |x|_inf).
+    Plans the windowed kernels cannot take must be refused, not solved."""
+    print(f"case {name}", flush=True)
+    plan = dh.Plan(_hreq(req))
+    info = plan.window_info()
+    m, n = req.source_size, req.destination_size
+    rng = np.random.default_rng(m * 5 + n)
+    x = rng.uniform(0, 1, (130, m)).astype(np.float32)
+    xt = torch.from_numpy(x).to(DEV)
+    if not info["feasible"]:
+        with pytest.raises(ValueError, match="windowed"):
+            executors["windowed"].inverse_columns(plan, xt.T.contiguous())
+        return
+    op = oracle.build(req)
+    expected = np.stack([oracle.axis_f32_ordered(op, row) for row in x])
+    scale = max(1.0, float(np.abs(expected).max()))
+    if info["rows_feasible"]:
+        got = executors["windowed"].inverse_rows(plan, xt).cpu().numpy()
+        assert float(np.abs(got - expected).max()) <= 4e-6 * scale
+    gotc = executors["windowed"].inverse_columns(plan, xt.T.contiguous()).cpu().numpy()
+    assert float(np.abs(gotc - expected.T).max()) <= 4e-6 * scale
+
+
+LONG_CASES = [
+    ("bicubic-3840-2000", make_request(3840, 2000, kernel=BICUBIC, f64_mode=F64_NEVER)),
+    ("mitchell-1920-1778", make_request(1920, 1778, kernel=BICUBIC, b=1 / 3, c=1 / 3,
+                                        f64_mode=F64_NEVER)),
+    ("spline36-2160-1500", make_request(2160, 1500, kernel=SPLINE36, f64_mode=F64_NEVER)),
+]
+
+
+@pytest.mark.parametrize("name,req", LONG_CASES, ids=[c[0] for c in LONG_CASES])
+def test_long_vectors_auto_close(executors, oracle, name, req):
+    """Vectors longer than the carry-chain kernels take (n > 1280) run on the
+    windowed kernels in auto mode: within 4e-6 * max(1, |x|) of the ordered
+    oracle, rows and columns, with enough vectors to fill the GPU."""
+    print(f"case {name}", flush=True)
+    plan = dh.Plan(_hreq(req))
+    assert plan.window_info()["feasible"] == 1
+    m, n = req.source_size, req.destination_size
+    rng = np.random.default_rng(n)
+    x = rng.uniform(0, 1, (1024, m)).astype(np.float32)
+    xt = torch.from_numpy(x).to(DEV)
+    got = executors["auto"].inverse_rows(plan, xt).cpu().numpy()
+    gotc = executors["auto"].inverse_columns(plan, xt.T.contiguous()).cpu().numpy()
+    op = oracle.build(req)
+    for r in (0, 1, 511, 1023):
+        expected = oracle.axis_f32_ordered(op, x[r])
+        scale = max(1.0, float(np.abs(expected).max()))
+        assert float(np.abs(got[r] - expected).max()) <= 4e-6 * scale
+        assert float(np.abs(gotc[:, r] - expected).max()) <= 4e-6 * scale
+
+
+def test_windowed_yuv420p16_within_one_code(executors, oracle):
+    """Config 2 geometry on the windowed kernels: <= 1 code vs the oracle's
+    scalar integer route."""
+    reqs = config_plan_requests("config2")
+    rng = np.random.default_rng(22)
+    for plane, (h, v, sw, sh) in enumerate([(reqs[0], reqs[1], 1920, 1080),
+                                            (reqs[2], reqs[3], 960, 540)]):
+        x = rng.integers(0, 65536, (2, sh, sw)).astype(np.uint16)
+        conv = integer_conversion(16, chroma=plane > 0, full_range=False)
+        dconv = dh.IntegerConversion(conv.input_offset, conv.input_scale, conv.output_scale,
+                                     conv.output_offset, conv.output_maximum)
+        xt = torch.from_numpy(x.astype(np.int32)).to(DEV).to(torch.uint16)
+        got = executors["windowed"].inverse_2d(dh.Plan(_hreq(h)), dh.Plan(_hreq(v)), xt,
+                                               conversion=dconv).to(torch.int32).cpu().numpy()
+        for f in range(2):
+            expected = oracle.inverse_2d_int(oracle.build(h), oracle.build(v), x[f], conv)
+            diff = int(np.abs(got[f] - expected.astype(np.int32)).max())
+            print(f"plane {plane} frame {f}: max code diff {diff}", flush=True)
+            assert diff <= 1
 
 
 @pytest.mark.parametrize("kernel", [BILINEAR, BICUBIC, LANCZOS, SPLINE36, SPLINE64])

@@ -1,0 +1,137 @@
+"""GPU tests of the continuous-batching request loop (dsmvc_hip_server_*).
+
+Bars:
+  * every request's result is bit-identical alone (one request in flight)
+    and in a mix submitted concurrently from several threads;
+  * F32-route planes are within 4e-6 * max(1, |x|) of the ordered oracle's
+    2D solve; Float64-route planes equal the executor's own Float64 result;
+  * a request larger than the whole memory budget is refused at submit;
+    with a budget that holds only a few requests at once, every request
+    still completes (admission waits for memory to free up).
+"""
+import threading
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+import dsmvc_hip as dh  # noqa: E402
+from support import BICUBIC, BILINEAR, LANCZOS, SPLINE36, make_request  # noqa: E402
+
+DEV = torch.device("cuda:0")
+
+KERNELS = [(BILINEAR, 3, 0.0, 0.5), (BICUBIC, 3, 0.0, 0.5), (LANCZOS, 3, 0.0, 0.5),
+           (SPLINE36, 3, 0.0, 0.5)]
+
+
+@pytest.fixture(scope="module")
+def executor():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return dh.Executor(0, "auto")
+
+
+def _requests(seed, count):
+    """Mixed geometry: gray and 4:2:0 YUV, three source sizes, random target
+    height, all kernels, all f64 modes."""
+    rng = np.random.default_rng(seed)
+    sources = [(1280, 720), (960, 540), (640, 360)]
+    out = []
+    for k in range(count):
+        sw, sh = sources[rng.integers(len(sources))]
+        dh_ = int(sh * rng.uniform(0.5, 0.95)) & ~1
+        dw = int(np.floor(dh_ * 16 / 9 + 0.5))
+        dw += dw & 1
+        yuv = bool(rng.integers(2))
+        kernel = KERNELS[rng.integers(len(KERNELS))]
+        mode = int(rng.integers(3))
+        planes = [(sw, sh, dw, dh_)]
+        if yuv:
+            planes += [(sw // 2, sh // 2, dw // 2, dh_ // 2)] * 2
+        inputs = [torch.from_numpy(rng.uniform(0, 1, (h, w)).astype(np.float32)).to(DEV)
+                  for (w, h, _, _) in planes]
+        out.append({"planes": planes, "inputs": inputs, "kernel": kernel, "mode": mode})
+    return out
+
+
+def _submit(server, req):
+    outputs = [torch.full((h, w), float("nan"), device=DEV) for (_, _, w, h) in req["planes"]]
+    kernel, taps, b, c = req["kernel"]
+    rid = server.submit(req["inputs"], outputs, kernel=kernel, taps=taps, b=b, c=c,
+                        f64_mode=req["mode"])
+    return rid, outputs
+
+
+def test_results_same_alone_and_in_the_mix(executor):
+    reqs = _requests(11, 24)
+    server = dh.Server(executor, memory_budget=512 << 20, slots=3)
+    alone = []
+    for req in reqs:
+        rid, outs = _submit(server, req)
+        server.wait(rid)
+        alone.append(outs)
+    mixed = [None] * len(reqs)
+    ids = [None] * len(reqs)
+
+    def client(first):
+        for k in range(first, len(reqs), 3):
+            ids[k], mixed[k] = _submit(server, reqs[k])
+
+    threads = [threading.Thread(target=client, args=(t,)) for t in range(3)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    for rid in ids:
+        server.wait(rid)
+    stats = server.stats()
+    print(stats, flush=True)
+    assert stats["completed"] == 2 * len(reqs) and stats["failed"] == 0
+    for k, (a, m) in enumerate(zip(alone, mixed)):
+        for pa, pm in zip(a, m):
+            assert torch.equal(pa, pm), k
+    server.close()
+
+
+def test_served_results_match_oracle_and_float64_route(executor, oracle):
+    reqs = _requests(12, 12)
+    server = dh.Server(executor, memory_budget=512 << 20, slots=2)
+    submitted = [_submit(server, r) for r in reqs]
+    for (rid, outs), req in zip(submitted, reqs):
+        server.wait(rid)
+        kernel, taps, b, c = req["kernel"]
+        (sw, sh, dw, dh_) = req["planes"][0]
+        hr = make_request(sw, dw, kernel=kernel, taps=taps, b=b, c=c, f64_mode=req["mode"])
+        vr = make_request(sh, dh_, kernel=kernel, taps=taps, b=b, c=c, f64_mode=req["mode"])
+        hp = dh.Plan(dh.AxisRequest(sw, dw, float(dw), 0.0, kernel, taps, b, c, 1.0,
+                                    dh.SYMMETRIC, req["mode"]))
+        vp = dh.Plan(dh.AxisRequest(sh, dh_, float(dh_), 0.0, kernel, taps, b, c, 1.0,
+                                    dh.SYMMETRIC, req["mode"]))
+        got = outs[0].cpu().numpy()
+        if hp.requires_float64 or vp.requires_float64:
+            expected = executor.inverse_2d(hp, vp, req["inputs"][0]).cpu().numpy()
+            assert np.array_equal(got, expected)
+        else:
+            x = req["inputs"][0].cpu().numpy()
+            expected = oracle.inverse_2d(oracle.build(hr), oracle.build(vr), x, ordered=True)
+            scale = max(1.0, float(np.abs(expected).max()))
+            assert float(np.abs(got - expected).max()) <= 4e-6 * scale
+    server.close()
+
+
+def test_admission_waits_for_memory(executor):
+    reqs = [r for r in _requests(13, 16) if r["mode"] != 2]
+    big = max(4 * ((w + 63) // 64 * 64) * h for r in reqs for (w, h, dw, _) in r["planes"])
+    # Room for about two requests at a time.
+    server = dh.Server(executor, memory_budget=3 * big, slots=4)
+    submitted = [_submit(server, r) for r in reqs]
+    for rid, outs in submitted:
+        server.wait(rid)
+        assert all(bool(torch.isfinite(o).all()) for o in outs)
+    assert server.stats()["memory_high_water"] <= 3 * big
+    with pytest.raises(ValueError, match="budget"):
+        tiny = dh.Server(executor, memory_budget=4096, slots=1)
+        _submit(tiny, reqs[0])
+    server.close()

@@ -1,0 +1,233 @@
+// Multi-plan batches over windowed job tables (see batch.hpp).
+#include "batch.hpp"
+
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+
+#include "device_plan.hpp"
+#include "executor.hpp"
+#include "kernels.hpp"
+
+namespace dsmvc {
+namespace {
+
+void check(hipError_t e, const char *what) {
+    if (e != hipSuccess)
+        throw std::runtime_error(std::string("dsmvc: ") + what + ": " + hipGetErrorString(e));
+}
+
+std::size_t align256(std::size_t v) { return (v + 255U) & ~std::size_t{255}; }
+
+std::ptrdiff_t pitch_of(std::int32_t n) { return (static_cast<std::ptrdiff_t>(n) + 63) & ~std::ptrdiff_t{63}; }
+
+long items_of(const hip::WinJob &j) { return static_cast<long>(j.frames) * j.plan.chunks * j.groups; }
+
+hip::DevConv identity_conv() {
+    hip::DevConv c{};
+    c.in_scale = 1.0F;
+    c.out_scale = 1.0F;
+    return c;
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// WindowTables
+// ---------------------------------------------------------------------------
+WindowTables::~WindowTables() {
+    if (device_jobs_) {
+        (void)hipSetDevice(device_);
+        (void)hipFree(device_jobs_);
+    }
+}
+
+void WindowTables::clear() { groups_.clear(); }
+
+void WindowTables::add(const hip::WinJob &job, int band, bool rows) {
+    const int width = job.plan.ell_width;
+    auto it = std::find_if(groups_.begin(), groups_.end(), [&](const Group &g) {
+        return g.rows == rows && g.band == band && g.width == width;
+    });
+    if (it == groups_.end()) {
+        groups_.push_back(Group{band, width, rows, {}, 0, 0});
+        it = groups_.end() - 1;
+    }
+    hip::WinJob j = job;
+    j.first_item = it->items;
+    it->items += items_of(j);
+    it->jobs.push_back(j);
+}
+
+int WindowTables::upload(hipStream_t stream) {
+    // Horizontal groups first: every vertical pass reads an intermediate
+    // some horizontal group writes.
+    std::stable_partition(groups_.begin(), groups_.end(), [](const Group &g) { return g.rows; });
+    staging_.clear();
+    for (Group &g : groups_) {
+        g.offset = staging_.size();
+        staging_.insert(staging_.end(), g.jobs.begin(), g.jobs.end());
+    }
+    check(hipSetDevice(device_), "hipSetDevice failed");
+    if (staging_.size() > capacity_) {
+        if (device_jobs_) {
+            // A previous launch may still read the old table.
+            check(stream ? hipStreamSynchronize(stream) : hipDeviceSynchronize(), "table wait failed");
+            check(hipFree(device_jobs_), "job table release failed");
+            device_jobs_ = nullptr;
+        }
+        const std::size_t want = std::max<std::size_t>(staging_.size(), capacity_ * 2);
+        check(hipMalloc(reinterpret_cast<void **>(&device_jobs_), want * sizeof(hip::WinJob)),
+              "job table allocation failed");
+        capacity_ = want;
+    }
+    if (!staging_.empty()) {
+        const std::size_t bytes = staging_.size() * sizeof(hip::WinJob);
+        if (stream)
+            check(hipMemcpyAsync(device_jobs_, staging_.data(), bytes, hipMemcpyHostToDevice, stream),
+                  "job table upload failed");
+        else
+            check(hipMemcpy(device_jobs_, staging_.data(), bytes, hipMemcpyHostToDevice),
+                  "job table upload failed");
+    }
+    return static_cast<int>(groups_.size());
+}
+
+void WindowTables::launch(hipStream_t stream) const {
+    for (const Group &g : groups_) {
+        if (g.items == 0) continue;
+        check(hip::launch_window_table(device_jobs_ + g.offset, static_cast<int>(g.jobs.size()),
+                                       g.items, g.band, g.width, g.rows, stream),
+              "table launch failed");
+    }
+}
+
+std::int64_t WindowTables::items() const {
+    std::int64_t total = 0;
+    for (const Group &g : groups_) total += g.items;
+    return total;
+}
+
+// ---------------------------------------------------------------------------
+// HipBatch
+// ---------------------------------------------------------------------------
+HipBatch::HipBatch(HipExecutor &executor, std::vector<BatchJob> jobs, bool graph)
+    : executor_(executor), jobs_(std::move(jobs)), tables_(executor.device()) {
+    for (const BatchJob &j : jobs_) {
+        const AxisPlan *h = j.horizontal.get(), *v = j.vertical.get();
+        if (!h || !v || !h->valid() || !v->valid() || !j.input || !j.output || j.frames < 1
+            || j.input_row_stride < h->source_size || j.output_row_stride < h->destination_size
+            || (j.frames > 1 && (j.input_frame_stride < j.input_row_stride * v->source_size
+                                 || j.output_frame_stride < j.output_row_stride * v->destination_size)))
+            throw std::invalid_argument("dsmvc: invalid batch job");
+    }
+    check(hipSetDevice(executor_.device()), "hipSetDevice failed");
+    const bool parallel = executor_.mode() == SolveMode::automatic
+                          || executor_.mode() == SolveMode::windowed;
+    // Intermediate layout of the table jobs: [frame][source row][pitch(n_h)].
+    std::vector<std::size_t> offset(jobs_.size(), 0);
+    std::vector<const hip::DevWindow *> wh(jobs_.size(), nullptr), wv(jobs_.size(), nullptr);
+    std::size_t total = 0;
+    for (std::size_t k = 0; k < jobs_.size(); ++k) {
+        const BatchJob &j = jobs_[k];
+        const AxisPlan &h = *j.horizontal, &v = *j.vertical;
+        if (parallel && !h.requires_float64() && !v.requires_float64()) {
+            wh[k] = executor_.plan_image(h).window(true);
+            wv[k] = executor_.plan_image(v).window(false);
+        }
+        if (!wh[k] || !wv[k]) {
+            if (executor_.mode() == SolveMode::windowed)
+                throw std::invalid_argument("dsmvc: batch job cannot use the windowed HIP solve");
+            fallback_.push_back(k);
+            continue;
+        }
+        offset[k] = total;
+        total += align256(static_cast<std::size_t>(pitch_of(h.destination_size)) * v.source_size
+                          * static_cast<std::size_t>(j.frames) * sizeof(float));
+    }
+    if (total) check(hipMalloc(&intermediate_, total), "batch intermediate allocation failed");
+    for (std::size_t k = 0; k < jobs_.size(); ++k) {
+        if (!wh[k]) continue;
+        const BatchJob &j = jobs_[k];
+        const AxisPlan &h = *j.horizontal, &v = *j.vertical;
+        const std::ptrdiff_t row = pitch_of(h.destination_size);
+        const std::ptrdiff_t frame = row * v.source_size;
+        float *inter = reinterpret_cast<float *>(static_cast<unsigned char *>(intermediate_) + offset[k]);
+        hip::WinJob a{};
+        a.plan = *wh[k];
+        a.src = j.input;
+        a.s_row = j.input_row_stride;
+        a.s_frame = j.input_frame_stride;
+        a.dst = inter;
+        a.d_row = row;
+        a.d_frame = frame;
+        a.nvec = v.source_size;
+        a.frames = j.frames;
+        a.groups = (a.nvec + 63) / 64;
+        a.conv = identity_conv();
+        tables_.add(a, hip::window_band_class(h.half_bandwidth), true);
+        hip::WinJob b{};
+        b.plan = *wv[k];
+        b.src = inter;
+        b.s_row = row;
+        b.s_frame = frame;
+        b.dst = j.output;
+        b.d_row = j.output_row_stride;
+        b.d_frame = j.output_frame_stride;
+        b.nvec = h.destination_size;
+        b.frames = j.frames;
+        b.groups = (b.nvec + 63) / 64;
+        b.conv = identity_conv();
+        tables_.add(b, hip::window_band_class(v.half_bandwidth), false);
+        ++stats_.table_jobs;
+    }
+    stats_.jobs = static_cast<std::int32_t>(jobs_.size());
+    stats_.fallback_jobs = static_cast<std::int32_t>(fallback_.size());
+    stats_.launches = tables_.upload(nullptr);
+    stats_.items = tables_.items();
+    stats_.intermediate_bytes = static_cast<std::int64_t>(total);
+    if (graph && stats_.launches > 0) {
+        hipStream_t capture = nullptr;
+        check(hipStreamCreateWithFlags(&capture, hipStreamNonBlocking), "stream creation failed");
+        hipError_t e = hipStreamBeginCapture(capture, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            try {
+                tables_.launch(capture);
+            } catch (...) {
+                (void)hipStreamEndCapture(capture, &graph_);
+                (void)hipStreamDestroy(capture);
+                throw;
+            }
+            e = hipStreamEndCapture(capture, &graph_);
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0);
+        (void)hipStreamDestroy(capture);
+        check(e, "batch graph capture failed");
+        stats_.graph = true;
+    }
+}
+
+HipBatch::~HipBatch() {
+    (void)hipSetDevice(executor_.device());
+    (void)hipDeviceSynchronize();
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    if (intermediate_) (void)hipFree(intermediate_);
+}
+
+void HipBatch::run(hipStream_t stream, bool async) {
+    check(hipSetDevice(executor_.device()), "hipSetDevice failed");
+    if (graph_exec_) check(hipGraphLaunch(graph_exec_, stream), "batch graph launch failed");
+    else tables_.launch(stream);
+    for (const std::size_t k : fallback_) {
+        const BatchJob &j = jobs_[k];
+        executor_.inverse_2d_batch(*j.horizontal, *j.vertical, SampleKind::float32,
+                                   PlaneBatch{j.input, j.input_row_stride, j.input_frame_stride,
+                                              j.output, j.output_row_stride, j.output_frame_stride,
+                                              j.frames},
+                                   nullptr, stream, true);
+    }
+    if (!async) executor_.synchronize(stream);
+}
+
+} // namespace dsmvc

@@ -1,0 +1,111 @@
+// Multi-plan batches: many 2D descale jobs of uneven geometry (different
+// plans, sizes and buffers) solved by a few launches over device-resident job
+// tables.
+//
+// The reference has no multi-plan launch: its executors solve one plane per
+// call (src/vs_plugin.cpp:1636-1745), and its Metal route batches at most 48
+// axis jobs per dispatch from a descriptor table (metal_routes.metal:25-35,
+// 363-408).  Here every job's horizontal pass goes into one table launch per
+// (band class, tap width) group, then every vertical pass likewise, on the
+// windowed kernels.  Jobs those kernels cannot take (Float64 route, bands
+// above 7, slowly decaying plans) are solved one by one through the
+// executor after the table launches.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+#include "device_types.hpp"
+#include "engine.hpp"
+
+namespace dsmvc {
+
+class HipExecutor;
+
+// One 2D job: `frames` F32 planes of the horizontal plan's source width and
+// the vertical plan's source height, into planes of the destination size.
+struct BatchJob {
+    std::shared_ptr<const AxisPlan> horizontal;
+    std::shared_ptr<const AxisPlan> vertical;
+    const float *input = nullptr;
+    std::ptrdiff_t input_row_stride = 0;
+    std::ptrdiff_t input_frame_stride = 0;
+    float *output = nullptr;
+    std::ptrdiff_t output_row_stride = 0;
+    std::ptrdiff_t output_frame_stride = 0;
+    std::int32_t frames = 1;
+};
+
+struct BatchStats {
+    std::int32_t jobs = 0;
+    std::int32_t table_jobs = 0;       // solved by the table launches
+    std::int32_t fallback_jobs = 0;    // solved one by one
+    std::int32_t launches = 0;         // table launches per run
+    std::int64_t items = 0;            // wavefront items per run
+    std::int64_t intermediate_bytes = 0;
+    bool graph = false;                // table launches replayed from a HIP graph
+};
+
+// Groups windowed axis jobs by (direction, band class, tap width) into one
+// device buffer of job tables and launches them: all horizontal groups
+// first, then all vertical groups, on one stream.  Reusable: `build` lays
+// out a new set of jobs (growing the device buffer), `launch` enqueues.
+class WindowTables {
+public:
+    explicit WindowTables(int device) : device_(device) {}
+    ~WindowTables();
+    WindowTables(const WindowTables &) = delete;
+    WindowTables &operator=(const WindowTables &) = delete;
+
+    void clear();
+    void add(const hip::WinJob &job, int band, bool rows);
+    // Uploads the tables (stream-ordered when `stream` is given, else
+    // synchronously) and returns the number of launches `launch` makes.
+    int upload(hipStream_t stream);
+    void launch(hipStream_t stream) const;
+    [[nodiscard]] std::int64_t items() const;
+
+private:
+    struct Group {
+        int band = 0, width = 0;
+        bool rows = false;
+        std::vector<hip::WinJob> jobs;
+        std::size_t offset = 0;     // first job's index in the device buffer
+        long items = 0;
+    };
+    int device_;
+    std::vector<Group> groups_;
+    hip::WinJob *device_jobs_ = nullptr;
+    std::size_t capacity_ = 0;
+    std::vector<hip::WinJob> staging_;
+};
+
+class HipBatch {
+public:
+    // Validates and uploads everything a run needs.  `graph`: capture the
+    // table launches into a HIP graph once and replay it on every run.
+    // The executor must outlive the batch.
+    HipBatch(HipExecutor &executor, std::vector<BatchJob> jobs, bool graph);
+    ~HipBatch();
+    HipBatch(const HipBatch &) = delete;
+    HipBatch &operator=(const HipBatch &) = delete;
+
+    // Enqueues one run on `stream`; waits for it unless `async`.
+    void run(hipStream_t stream, bool async);
+    [[nodiscard]] const BatchStats &stats() const { return stats_; }
+
+private:
+    HipExecutor &executor_;
+    std::vector<BatchJob> jobs_;
+    std::vector<std::size_t> fallback_;
+    WindowTables tables_;
+    void *intermediate_ = nullptr;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t graph_exec_ = nullptr;
+    BatchStats stats_{};
+};
+
+} // namespace dsmvc

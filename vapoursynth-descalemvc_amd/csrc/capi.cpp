@@ -11,7 +11,10 @@
 #include <string>
 
 #include "engine.hpp"
+#include "batch.hpp"
+#include "device_plan.hpp"
 #include "executor.hpp"
+#include "server.hpp"
 
 struct dsmvc_hip_plan {
     std::shared_ptr<const dsmvc::AxisPlan> plan;
@@ -19,6 +22,14 @@ struct dsmvc_hip_plan {
 
 struct dsmvc_hip_executor {
     std::unique_ptr<dsmvc::HipExecutor> executor;
+};
+
+struct dsmvc_hip_batch {
+    std::unique_ptr<dsmvc::HipBatch> batch;
+};
+
+struct dsmvc_hip_server {
+    std::unique_ptr<dsmvc::HipServer> server;
 };
 
 namespace {
@@ -134,6 +145,24 @@ int dsmvc_hip_plan_info_get(const dsmvc_hip_plan *plan, dsmvc_hip_plan_info *inf
     });
 }
 
+int dsmvc_hip_plan_window_info(const dsmvc_hip_plan *plan, dsmvc_hip_window_info *info) {
+    return guarded([&] {
+        const auto &p = plan_of(plan);
+        require(info, "dsmvc: null argument");
+        dsmvc::hip::WindowPlan wp;
+        const bool ok = dsmvc::hip::build_window_plan(p, wp);
+        *info = dsmvc_hip_window_info{};
+        info->feasible = ok ? 1 : 0;
+        info->rows_feasible = ok && wp.rows_feasible ? 1 : 0;
+        info->band = wp.band;
+        info->chunks = wp.chunks;
+        info->length = wp.length;
+        info->warmup = wp.warmup;
+        info->width = wp.width;
+        info->residual = wp.residual;
+    });
+}
+
 int dsmvc_hip_plan_arrays(const dsmvc_hip_plan *plan, uint32_t *offsets, int32_t *indices,
                           float *weights, float *lower_ld, float *upper_l,
                           float *inverse_diagonal, double *weights_f64,
@@ -206,7 +235,7 @@ int dsmvc_hip_device_count(int32_t *count) {
 
 int dsmvc_hip_executor_create(int32_t device, int32_t solve_mode, dsmvc_hip_executor **executor) {
     return guarded([&] {
-        require(executor && solve_mode >= 0 && solve_mode <= 2, "dsmvc: invalid executor arguments");
+        require(executor && solve_mode >= 0 && solve_mode <= 3, "dsmvc: invalid executor arguments");
         if (!dsmvc::hip_available()) throw Unavailable("dsmvc: backend hip is unavailable: no HIP device");
         auto handle = std::make_unique<dsmvc_hip_executor>();
         handle->executor = std::make_unique<dsmvc::HipExecutor>(
@@ -231,8 +260,8 @@ int dsmvc_hip_executor_seal(dsmvc_hip_executor *executor) {
 int dsmvc_hip_executor_tune(dsmvc_hip_executor *executor, int32_t rows_chunks,
                             int32_t columns_chunks) {
     return guarded([&] {
-        require(rows_chunks >= 0 && rows_chunks <= 16 && columns_chunks >= 0 && columns_chunks <= 16,
-                "dsmvc: chunk counts must be 0..16");
+        require(rows_chunks >= -1 && rows_chunks <= 16 && columns_chunks >= -1 && columns_chunks <= 16,
+                "dsmvc: routes must be -1 (windowed), 0 (automatic) or a chunk count up to 16");
         exec_of(executor).set_tuning({rows_chunks, columns_chunks});
     });
 }
@@ -359,6 +388,154 @@ int dsmvc_hip_inverse_columns_batch(dsmvc_hip_executor *executor, const dsmvc_hi
                                                 stream_of(stream), is_async(flags));
     });
 }
+
+int dsmvc_hip_batch_create(dsmvc_hip_executor *executor, const dsmvc_hip_batch_job *jobs,
+                           int32_t count, uint32_t flags, dsmvc_hip_batch **batch) {
+    return guarded([&] {
+        require(batch && (jobs || count == 0) && count >= 0, "dsmvc: invalid batch arguments");
+        require((flags & ~DSMVC_HIP_BATCH_GRAPH) == 0, "dsmvc: unknown batch flags");
+        auto &ex = exec_of(executor);
+        std::vector<dsmvc::BatchJob> list;
+        list.reserve(static_cast<std::size_t>(count));
+        for (int32_t k = 0; k < count; ++k) {
+            const dsmvc_hip_batch_job &j = jobs[k];
+            require(j.horizontal && j.vertical, "dsmvc: invalid batch job");
+            dsmvc::BatchJob b;
+            b.horizontal = j.horizontal->plan;
+            b.vertical = j.vertical->plan;
+            b.input = j.input;
+            b.input_row_stride = j.input_row_stride;
+            b.input_frame_stride = j.input_frame_stride;
+            b.output = j.output;
+            b.output_row_stride = j.output_row_stride;
+            b.output_frame_stride = j.output_frame_stride;
+            b.frames = j.frames;
+            list.push_back(std::move(b));
+        }
+        auto handle = std::make_unique<dsmvc_hip_batch>();
+        handle->batch = std::make_unique<dsmvc::HipBatch>(ex, std::move(list),
+                                                          (flags & DSMVC_HIP_BATCH_GRAPH) != 0);
+        *batch = handle.release();
+    });
+}
+
+int dsmvc_hip_batch_run(dsmvc_hip_batch *batch, void *stream, uint32_t flags) {
+    return guarded([&] {
+        require(batch && batch->batch, "dsmvc: null batch");
+        batch->batch->run(static_cast<hipStream_t>(stream), (flags & DSMVC_HIP_ASYNC) != 0);
+    });
+}
+
+int dsmvc_hip_batch_info_get(const dsmvc_hip_batch *batch, dsmvc_hip_batch_info *info) {
+    return guarded([&] {
+        require(batch && batch->batch && info, "dsmvc: null argument");
+        const dsmvc::BatchStats &s = batch->batch->stats();
+        *info = dsmvc_hip_batch_info{};
+        info->jobs = s.jobs;
+        info->table_jobs = s.table_jobs;
+        info->fallback_jobs = s.fallback_jobs;
+        info->launches = s.launches;
+        info->items = s.items;
+        info->intermediate_bytes = s.intermediate_bytes;
+        info->graph = s.graph ? 1 : 0;
+    });
+}
+
+void dsmvc_hip_batch_destroy(dsmvc_hip_batch *batch) { delete batch; }
+
+int dsmvc_hip_server_create(dsmvc_hip_executor *executor, const dsmvc_hip_server_config *config,
+                            dsmvc_hip_server **server) {
+    return guarded([&] {
+        require(config && server && config->memory_budget > 0, "dsmvc: invalid server arguments");
+        dsmvc::ServerConfig c;
+        c.memory_budget = static_cast<std::size_t>(config->memory_budget);
+        c.slots = config->slots;
+        c.max_round_requests = config->max_round_requests;
+        auto handle = std::make_unique<dsmvc_hip_server>();
+        handle->server = std::make_unique<dsmvc::HipServer>(exec_of(executor), c);
+        *server = handle.release();
+    });
+}
+
+int dsmvc_hip_server_submit(dsmvc_hip_server *server, const dsmvc_hip_serve_request *request,
+                            uint64_t *id) {
+    return guarded([&] {
+        require(server && server->server && request && id, "dsmvc: null argument");
+        const dsmvc_hip_serve_request &r = *request;
+        require(r.kernel >= 0 && r.kernel <= 5, "kernel is invalid");
+        require(r.border >= 0 && r.border <= 4, "padding mode is invalid");
+        require(r.f64_mode >= 0 && r.f64_mode <= 2, "f64 mode is invalid");
+        dsmvc::ServeRequest q;
+        q.planes = r.planes;
+        q.subsampling_w = r.subsampling_w;
+        q.subsampling_h = r.subsampling_h;
+        q.src_width = r.src_width;
+        q.src_height = r.src_height;
+        q.dst_width = r.dst_width;
+        q.dst_height = r.dst_height;
+        q.kernel.kind = static_cast<dsmvc::KernelKind>(r.kernel);
+        q.kernel.taps = r.taps;
+        q.kernel.b = r.b;
+        q.kernel.c = r.c;
+        q.kernel.blur = r.blur;
+        q.border = static_cast<dsmvc::BorderMode>(r.border);
+        q.f64_mode = static_cast<dsmvc::F64Mode>(r.f64_mode);
+        for (int p = 0; p < 3; ++p) {
+            q.src[p] = r.src[p];
+            q.src_stride[p] = r.src_stride[p];
+            q.dst[p] = r.dst[p];
+            q.dst_stride[p] = r.dst_stride[p];
+        }
+        *id = server->server->submit(q);
+    });
+}
+
+int dsmvc_hip_server_wait(dsmvc_hip_server *server, uint64_t id, double *submitted,
+                          double *completed) {
+    std::string failure;
+    dsmvc::RequestState state = dsmvc::RequestState::pending;
+    const int status = guarded([&] {
+        require(server && server->server, "dsmvc: null server");
+        state = server->server->wait(id, &failure, submitted, completed);
+    });
+    if (status != DSMVC_HIP_OK) return status;
+    if (state == dsmvc::RequestState::failed) {
+        g_error = failure;
+        return DSMVC_HIP_RUNTIME_ERROR;
+    }
+    return DSMVC_HIP_OK;
+}
+
+int dsmvc_hip_server_poll(dsmvc_hip_server *server, uint64_t id, int32_t *state) {
+    return guarded([&] {
+        require(server && server->server && state, "dsmvc: null argument");
+        *state = static_cast<int32_t>(server->server->poll(id));
+    });
+}
+
+int dsmvc_hip_server_release(dsmvc_hip_server *server, uint64_t id) {
+    return guarded([&] {
+        require(server && server->server, "dsmvc: null server");
+        server->server->release(id);
+    });
+}
+
+int dsmvc_hip_server_stats_get(const dsmvc_hip_server *server, dsmvc_hip_server_stats *stats) {
+    return guarded([&] {
+        require(server && server->server && stats, "dsmvc: null argument");
+        const dsmvc::ServerStats s = server->server->stats();
+        stats->submitted = s.submitted;
+        stats->completed = s.completed;
+        stats->failed = s.failed;
+        stats->rounds = s.rounds;
+        stats->table_planes = s.table_planes;
+        stats->fallback_planes = s.fallback_planes;
+        stats->largest_round = s.largest_round;
+        stats->memory_high_water = static_cast<int64_t>(s.memory_high_water);
+    });
+}
+
+void dsmvc_hip_server_destroy(dsmvc_hip_server *server) { delete server; }
 
 int dsmvc_hip_synchronize(dsmvc_hip_executor *executor, void *stream) {
     return guarded([&] { exec_of(executor).synchronize(stream_of(stream)); });

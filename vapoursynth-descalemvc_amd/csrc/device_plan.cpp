@@ -2,6 +2,7 @@
 #include "device_plan.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <limits>
 #include <stdexcept>
@@ -195,6 +196,136 @@ bool build_chunk_plan(const AxisPlan &p, int max_chunks, ChunkPlan &out) {
     return true;
 }
 
+bool build_window_plan(const AxisPlan &p, WindowPlan &out) {
+    const int n = p.destination_size, w = p.half_bandwidth;
+    const int band = window_band_class(w);
+    if (band == 0 || n < 1) return false;
+    int taps = 0;
+    for (int i = 0; i < n; ++i)
+        taps = std::max(taps, static_cast<int>(p.transpose_offsets[static_cast<std::size_t>(i) + 1]
+                                               - p.transpose_offsets[static_cast<std::size_t>(i)]));
+    const int width = chunked_width_for(std::max(taps, 1));
+    if (width == 0) return false;
+    const int length = window_length(band), warm = window_warmup(band);
+    const int chunks = (n + length - 1) / length;
+    const int rows = chunks * length + 2 * warm;
+    out.band = band;
+    out.chunks = chunks;
+    out.length = length;
+    out.warmup = warm;
+    out.width = width;
+    const auto un = static_cast<std::size_t>(n);
+    const auto ur = static_cast<std::size_t>(rows);
+    auto real = [&](int r) { return r - warm; };   // plan row of padded row r
+
+    out.coef.assign(ur * 16, 0.0F);
+    for (int r = 0; r < rows; ++r) {
+        const int i = real(r);
+        if (i < 0 || i >= n) continue;
+        float *row = out.coef.data() + static_cast<std::size_t>(r) * 16;
+        row[kMaxBand] = p.inverse_diagonal[static_cast<std::size_t>(i)];
+        for (int d = 1; d <= w; ++d) {
+            const std::size_t at = static_cast<std::size_t>(d - 1) * un + static_cast<std::size_t>(i);
+            row[d - 1] = p.lower_ld[at];
+            row[kMaxBand + d] = p.upper_l[at];
+        }
+    }
+
+    // Staging windows of 8-row blocks; blocks of zero rows reuse a
+    // neighbouring window (their taps carry weight 0).
+    const int nb = rows / kWindowBlock;
+    out.blk_lo.assign(static_cast<std::size_t>(nb), 0);
+    out.rows_feasible = true;
+    int previous = 0;
+    for (int b = 0; b < nb; ++b) {
+        int lo = std::numeric_limits<int>::max(), hi = -1;
+        for (int t = 0; t < kWindowBlock; ++t) {
+            const int i = real(b * kWindowBlock + t);
+            if (i < 0 || i >= n) continue;
+            const auto k0 = p.transpose_offsets[static_cast<std::size_t>(i)];
+            const auto k1 = p.transpose_offsets[static_cast<std::size_t>(i) + 1];
+            if (k1 == k0) continue;
+            lo = std::min(lo, p.transpose_indices[k0]);
+            hi = std::max(hi, p.transpose_indices[k1 - 1]);
+        }
+        if (hi < 0) lo = previous;
+        else if (hi - lo >= kStageCols) out.rows_feasible = false;
+        out.blk_lo[static_cast<std::size_t>(b)] = lo;
+        previous = lo;
+    }
+
+    out.ell_idx.assign(ur * width, 0);
+    out.ell_w.assign(ur * width, 0.0F);
+    for (int r = 0; r < rows; ++r) {
+        const int i = real(r);
+        const int window = out.blk_lo[static_cast<std::size_t>(r / kWindowBlock)];
+        std::int32_t *idx = out.ell_idx.data() + static_cast<std::size_t>(r) * width;
+        float *wt = out.ell_w.data() + static_cast<std::size_t>(r) * width;
+        if (i < 0 || i >= n) {
+            std::fill(idx, idx + width, window);
+            continue;
+        }
+        const auto k0 = p.transpose_offsets[static_cast<std::size_t>(i)];
+        const auto k1 = p.transpose_offsets[static_cast<std::size_t>(i) + 1];
+        const int c = static_cast<int>(k1 - k0);
+        const int pad = width - c;
+        const std::int32_t fill = c > 0 ? p.transpose_indices[k0] : window;
+        for (int e = 0; e < pad; ++e) idx[e] = fill;
+        for (int e = 0; e < c; ++e) {
+            idx[pad + e] = p.transpose_indices[k0 + static_cast<std::uint32_t>(e)];
+            wt[pad + e] = p.transpose_weights[k0 + static_cast<std::uint32_t>(e)];
+        }
+    }
+
+    // Warm-up check, in FP64 on the F32 coefficients the kernels use: from
+    // each unit starting state P rows before a chunk boundary (forward) or
+    // after it (backward), the state must stay below the tolerance over the
+    // whole chunk.  Boundaries next to the plan's ends start in zero rows and
+    // are exact.
+    auto coef = [&](int r, int slot) {
+        return static_cast<double>(out.coef[static_cast<std::size_t>(r) * 16 + static_cast<std::size_t>(slot)]);
+    };
+    std::vector<double> ring(static_cast<std::size_t>(band));
+    auto push = [&](double v) {
+        for (int d = band - 1; d > 0; --d) ring[static_cast<std::size_t>(d)] = ring[static_cast<std::size_t>(d - 1)];
+        ring[0] = v;
+    };
+    auto peak = [&] {
+        double m = 0.0;
+        for (double v : ring) m = std::max(m, std::fabs(v));
+        return m;
+    };
+    double worst = 0.0;
+    for (int q = 1; q < chunks; ++q) {
+        const int a = q * length + warm;      // padded row of the chunk's first row
+        for (int c = 0; c < band; ++c) {
+            std::fill(ring.begin(), ring.end(), 0.0);
+            ring[static_cast<std::size_t>(c)] = 1.0;
+            for (int r = a - warm; r < a + length; ++r) {
+                double v = 0.0;
+                for (int d = band; d >= 1; --d) v -= coef(r, d - 1) * ring[static_cast<std::size_t>(d - 1)];
+                push(v * coef(r, kMaxBand));
+                if (r >= a - 1) worst = std::max(worst, peak());
+            }
+        }
+    }
+    for (int q = 0; q + 1 < chunks; ++q) {
+        const int b = (q + 1) * length + warm;   // padded row just past the chunk
+        for (int c = 0; c < band; ++c) {
+            std::fill(ring.begin(), ring.end(), 0.0);
+            ring[static_cast<std::size_t>(c)] = 1.0;
+            for (int r = b + warm - 1; r >= b - length; --r) {
+                double v = 0.0;
+                for (int d = band; d >= 1; --d) v -= coef(r, kMaxBand + d) * ring[static_cast<std::size_t>(d - 1)];
+                push(v);
+                if (r <= b) worst = std::max(worst, peak());
+            }
+        }
+    }
+    out.residual = worst;
+    return worst <= kWindowTolerance;
+}
+
 DevicePlan::DevicePlan(const AxisPlan &plan, int device)
     : keep_(std::make_shared<const AxisPlan>(plan)), plan_(keep_.get()), device_(device) {
     const PackedPlan packed = pack_plan(*plan_);
@@ -238,6 +369,7 @@ DevicePlan::DevicePlan(const AxisPlan &plan, int device)
 DevicePlan::~DevicePlan() {
     for (auto &entry : chunk_images_)
         if (entry.second.memory) (void)hipFree(entry.second.memory);
+    if (window_image_.memory) (void)hipFree(window_image_.memory);
     if (memory_) (void)hipFree(memory_);
 }
 
@@ -288,6 +420,49 @@ const DevChunks *DevicePlan::chunks(int max_chunks, bool rows) {
     }
     if (!it->second.ok || (rows && !it->second.rows_feasible)) return nullptr;
     return &it->second.dev;
+}
+
+const DevWindow *DevicePlan::window(bool rows) {
+    std::lock_guard lock(mutex_);
+    WindowImage &image = window_image_;
+    if (!image.built) {
+        image.built = true;
+        WindowPlan wp;
+        if (build_window_plan(*plan_, wp)) {
+            const std::size_t o_coef = 0;
+            const std::size_t o_idx = align256(o_coef + wp.coef.size() * 4);
+            const std::size_t o_w = align256(o_idx + wp.ell_idx.size() * 4);
+            const std::size_t o_blk = align256(o_w + wp.ell_w.size() * 4);
+            const std::size_t total = align256(o_blk + wp.blk_lo.size() * 4);
+            std::vector<unsigned char> host(total, 0);
+            std::memcpy(host.data() + o_coef, wp.coef.data(), wp.coef.size() * 4);
+            std::memcpy(host.data() + o_idx, wp.ell_idx.data(), wp.ell_idx.size() * 4);
+            std::memcpy(host.data() + o_w, wp.ell_w.data(), wp.ell_w.size() * 4);
+            std::memcpy(host.data() + o_blk, wp.blk_lo.data(), wp.blk_lo.size() * 4);
+            int current = 0;
+            check(hipGetDevice(&current), "hipGetDevice failed");
+            check(hipSetDevice(device_), "hipSetDevice failed");
+            check(hipMalloc(&image.memory, total), "window table allocation failed");
+            check(hipMemcpy(image.memory, host.data(), total, hipMemcpyHostToDevice),
+                  "window table upload failed");
+            (void)hipSetDevice(current);
+            auto *b = static_cast<unsigned char *>(image.memory);
+            image.dev.n = plan_->destination_size;
+            image.dev.m = plan_->source_size;
+            image.dev.length = wp.length;
+            image.dev.warmup = wp.warmup;
+            image.dev.chunks = wp.chunks;
+            image.dev.ell_width = wp.width;
+            image.dev.coef = reinterpret_cast<const float *>(b + o_coef);
+            image.dev.ell_idx = reinterpret_cast<const std::int32_t *>(b + o_idx);
+            image.dev.ell_w = reinterpret_cast<const float *>(b + o_w);
+            image.dev.blk_lo = reinterpret_cast<const std::int32_t *>(b + o_blk);
+            image.ok = true;
+            image.rows_feasible = wp.rows_feasible;
+        }
+    }
+    if (!image.ok || (rows && !image.rows_feasible)) return nullptr;
+    return &image.dev;
 }
 
 } // namespace dsmvc::hip

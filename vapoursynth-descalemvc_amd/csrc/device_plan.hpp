@@ -48,6 +48,29 @@ struct ChunkPlan {
 // the chunked kernels (band, length or tap count out of range).
 bool build_chunk_plan(const AxisPlan &plan, int max_chunks, ChunkPlan &out);
 
+// A windowed partition (layout in device_types.hpp, DevWindow): padded
+// coefficient rows and ELL taps with P zero rows on each side, the staging
+// windows of the horizontal kernel, and the warm-up check.
+struct WindowPlan {
+    std::int32_t band = 0;           // band class the kernels run (1, 3, 5, 7)
+    std::int32_t chunks = 0;
+    std::int32_t length = 0;
+    std::int32_t warmup = 0;
+    std::int32_t width = 0;          // packed taps per row
+    bool rows_feasible = false;      // every 8-row block's taps fit 32 columns
+    double residual = 0.0;           // worst relative influence of a chunk's zero start
+    std::vector<float> coef;         // [rows][16]
+    std::vector<std::int32_t> ell_idx;
+    std::vector<float> ell_w;
+    std::vector<std::int32_t> blk_lo; // [rows / 8]
+};
+// Largest influence (relative) of a wrong starting state allowed at a
+// windowed chunk's first forward / last backward row: 2^-28.
+constexpr double kWindowTolerance = 1.0 / 268435456.0;
+// False when the plan cannot use the windowed kernels: band above 7, taps
+// above 16, or a warm-up longer than the band class's P would be needed.
+bool build_window_plan(const AxisPlan &plan, WindowPlan &out);
+
 // One device allocation holding a plan's image (and its chunk partitions).
 class DevicePlan {
 public:
@@ -62,6 +85,9 @@ public:
     // the plan cannot use the chunked kernels in that direction.  Built
     // lazily, cached.
     const DevChunks *chunks(int max_chunks, bool rows);
+    // Windowed partition on the device; nullptr when the plan cannot use the
+    // windowed kernels in that direction.  Built lazily, cached.
+    const DevWindow *window(bool rows);
     [[nodiscard]] std::size_t device_bytes() const { return bytes_; }
 
 private:
@@ -79,6 +105,14 @@ private:
     DevAxis axis_{};
     std::mutex mutex_;
     std::map<int, ChunkImage> chunk_images_;
+    struct WindowImage {
+        DevWindow dev{};
+        void *memory = nullptr;
+        bool built = false;
+        bool ok = false;
+        bool rows_feasible = false;
+    };
+    WindowImage window_image_;
 };
 
 } // namespace dsmvc::hip

@@ -69,6 +69,34 @@ struct DevChunks {
 constexpr int kBlockRowsMin = 8;   // destination rows per staging block (rows kernel)
 constexpr int kStageCols = 32;     // source columns staged per block
 
+// Windowed partition for the independent-chunk kernels.  Chunk q owns the
+// destination rows [qL, qL + L); its wavefront runs the forward recurrence
+// over [qL - P, qL + L + P) from a zero state and the backward recurrence
+// over [qL, qL + L + P) from a zero state.  P (the warm-up) is chosen per
+// plan from its FP64 factors so that the influence of the wrong starting
+// state has decayed below 2^-28 of the carried values by the chunk's first
+// (forward) and last (backward) row.  Rows are stored padded: padded row
+// r <-> plan row r - P, with zero rows before 0 and from n on, so every chunk
+// visits exactly L + 2P padded rows starting at padded row qL.
+//   coef     [rows][16]  coefficient records (kMaxBand layout)
+//   ell_idx / ell_w [rows][K] taps, front-padded with weight 0
+//   blk_lo   [rows / 8]  horizontal kernel: first staged source column of
+//                        each 8-row block (its taps lie in [lo, lo + 32))
+struct DevWindow {
+    std::int32_t n;
+    std::int32_t m;
+    std::int32_t length;      // L
+    std::int32_t warmup;      // P
+    std::int32_t chunks;
+    std::int32_t ell_width;   // K
+    const float *coef;
+    const std::int32_t *ell_idx;
+    const float *ell_w;
+    const std::int32_t *blk_lo;
+};
+
+constexpr int kWindowBlock = 8;    // padded rows per staging block (rows kernel)
+
 struct DevConv {
     float in_offset;
     float in_scale;
@@ -79,5 +107,28 @@ struct DevConv {
 };
 
 enum class Sample : std::int32_t { f32 = 0, u8 = 1, u16 = 2, f64 = 3 };
+
+// One axis pass of one plan over `frames` planes, as a unit of a
+// (possibly multi-plan) windowed launch.  Items are (frame, chunk, group of
+// 64 vectors), the vector group varying fastest.  Rows pass: vectors are
+// rows, `src` holds samples of the launch's kind, `dst` F32.  Columns pass:
+// vectors are columns, `src` F32, `dst` samples of the launch's kind.
+struct WinJob {
+    DevWindow plan;
+    const void *src;
+    long s_row, s_frame;
+    void *dst;
+    long d_row, d_frame;
+    std::int32_t nvec;
+    std::int32_t frames;
+    std::int32_t groups;      // ceil(nvec / 64)
+    std::int32_t pad_;
+    long first_item;          // items of the jobs before this one in its table
+    DevConv conv;
+};
+
+// Windowed-kernel shape per band class: chunk length L and warm-up P.
+constexpr int window_length(int w) { return w <= 3 ? 64 : (w <= 5 ? 48 : 32); }
+constexpr int window_warmup(int w) { return w <= 3 ? 24 : (w <= 5 ? 32 : 48); }
 
 } // namespace dsmvc::hip

@@ -140,6 +140,13 @@ hip::DevicePlan &HipExecutor::device_plan(const AxisPlan &plan) {
     return ref;
 }
 
+hip::DevicePlan &HipExecutor::plan_image(const AxisPlan &plan) {
+    if (!plan.valid()) throw std::invalid_argument("dsmvc: invalid HIP axis plan");
+    std::lock_guard lock(mutex_);
+    check(hipSetDevice(device_), "hipSetDevice failed");
+    return device_plan(plan);
+}
+
 HipExecutor::Scratch &HipExecutor::scratch(hipStream_t stream, std::size_t bytes) {
     Scratch &s = scratch_[stream];
     if (!s.flag) check(hipMalloc(reinterpret_cast<void **>(&s.flag), 256), "scratch allocation failed");
@@ -180,24 +187,57 @@ void HipExecutor::synchronize(hipStream_t stream) {
     finish(stream, it->second, false, "dsmvc: HIP Float64 execution produced NaN or infinity");
 }
 
-// Picks the chunk count for a chunked pass: the executor's tuned choice, a
-// fresh measurement when the launch fills the GPU (every feasible count timed
-// with HIP events on the call's own buffers, whose output the caller's launch
-// then rewrites), or the largest partition.  0 = not chunkable.
+// Launches one F32 pass on a route: a chunk count (> 0) for the carry-chain
+// chunked kernels, or kWindowRoute for the windowed kernels.
+hipError_t HipExecutor::launch_route(hip::DevicePlan &dp, int route, bool rows,
+                                     const hip::ChunkArgs &a, hip::Sample kind, hipStream_t stream) {
+    if (route == kWindowRoute) {
+        const hip::DevWindow *win = dp.window(rows);
+        if (!win) return hipErrorInvalidValue;
+        hip::WinJob job{};
+        job.plan = *win;
+        job.src = a.src;
+        job.s_row = a.s_row;
+        job.s_frame = a.s_frame;
+        job.dst = a.dst;
+        job.d_row = a.d_row;
+        job.d_frame = a.d_frame;
+        job.nvec = a.nvec;
+        job.frames = a.frames;
+        job.groups = (a.nvec + 63) / 64;
+        job.first_item = 0;
+        job.conv = a.conv;
+        return hip::launch_window(job, hip::window_band_class(dp.axis().w), rows, kind, stream);
+    }
+    const hip::DevChunks *ch = dp.chunks(route, rows);
+    if (!ch) return hipErrorInvalidValue;
+    hip::ChunkArgs c = a;
+    c.chunks = *ch;
+    return hip::launch_chunked(c, rows, kind, stream);
+}
+
+// Picks the route of an F32 pass: the executor's tuned choice, a fresh
+// measurement when the launch fills the GPU (every feasible route timed with
+// HIP events on the call's own buffers, whose output the caller's launch then
+// rewrites), or the default: the largest chunked partition, else the
+// windowed kernels.  0 = no parallel route (sequential solve).
 int HipExecutor::choose_partition(hip::DevicePlan &dp, bool rows, const hip::ChunkArgs &base,
                                   hip::Sample kind, hipStream_t stream) {
     const TuneKey key{&dp, rows, static_cast<int>(kind)};
     if (auto it = tuned_.find(key); it != tuned_.end()) return it->second;
     std::vector<int> candidates;
-    std::vector<std::pair<int, int>> shapes;
-    for (const int c : {hip::kMaxChunks, 12, 10, 8, 6, 4}) {
-        const hip::DevChunks *ch = dp.chunks(c, rows);
-        if (!ch) continue;
-        const std::pair<int, int> shape{ch->chunks, ch->length};
-        if (std::find(shapes.begin(), shapes.end(), shape) != shapes.end()) continue;
-        shapes.push_back(shape);
-        candidates.push_back(c);
+    if (mode_ != SolveMode::windowed) {
+        std::vector<std::pair<int, int>> shapes;
+        for (const int c : {hip::kMaxChunks, 12, 10, 8, 6, 4}) {
+            const hip::DevChunks *ch = dp.chunks(c, rows);
+            if (!ch) continue;
+            const std::pair<int, int> shape{ch->chunks, ch->length};
+            if (std::find(shapes.begin(), shapes.end(), shape) != shapes.end()) continue;
+            shapes.push_back(shape);
+            candidates.push_back(c);
+        }
     }
+    if (mode_ != SolveMode::chunked && dp.window(rows)) candidates.push_back(kWindowRoute);
     if (candidates.empty()) return 0;
     const long workgroups = static_cast<long>((base.nvec + 63) / 64) * base.frames;
     if (!autotune_ || candidates.size() == 1) {
@@ -210,12 +250,10 @@ int HipExecutor::choose_partition(hip::DevicePlan &dp, bool rows, const hip::Chu
     check(hipEventCreate(&stop), "event creation failed");
     std::vector<float> best_ms(candidates.size(), std::numeric_limits<float>::max());
     for (std::size_t i = 0; i < candidates.size(); ++i) {
-        hip::ChunkArgs a = base;
-        a.chunks = *dp.chunks(candidates[i], rows);
-        check(hip::launch_chunked(a, rows, kind, stream), "chunked solve launch failed");
+        check(launch_route(dp, candidates[i], rows, base, kind, stream), "solve launch failed");
         for (int rep = 0; rep < 2; ++rep) {
             check(hipEventRecord(start, stream), "event record failed");
-            check(hip::launch_chunked(a, rows, kind, stream), "chunked solve launch failed");
+            check(launch_route(dp, candidates[i], rows, base, kind, stream), "solve launch failed");
             check(hipEventRecord(stop, stream), "event record failed");
             check(hipEventSynchronize(stop), "event synchronize failed");
             float ms = 0.0F;
@@ -278,16 +316,19 @@ void HipExecutor::axis_pass(const AxisPlan &plan, bool rows, const void *input, 
         a.frames = frames;
         a.conv = conv;
         const hip::Sample kind = rows ? in_kind : out_kind;
-        int count = rows ? tuning_.rows_chunks : tuning_.columns_chunks;
-        if (count <= 0) count = choose_partition(dp, rows, a, kind, stream);
-        const hip::DevChunks *ch = count > 0 ? dp.chunks(count, rows) : nullptr;
-        if (ch) {
-            a.chunks = *ch;
-            check(hip::launch_chunked(a, rows, kind, stream), "chunked solve launch failed");
+        int route = rows ? tuning_.rows_chunks : tuning_.columns_chunks;
+        if ((route > 0 && !dp.chunks(route, rows)) || (route == kWindowRoute && !dp.window(rows))
+            || route < kWindowRoute)
+            route = 0;
+        if (route <= 0) route = choose_partition(dp, rows, a, kind, stream);
+        if (route != 0) {
+            check(launch_route(dp, route, rows, a, kind, stream), "parallel solve launch failed");
             return;
         }
         if (mode_ == SolveMode::chunked)
             throw std::invalid_argument("dsmvc: plan cannot use the chunked HIP solve");
+        if (mode_ == SolveMode::windowed)
+            throw std::invalid_argument("dsmvc: plan cannot use the windowed HIP solve");
     }
     if (!hip::seq_band_supported(plan.half_bandwidth))
         throw std::invalid_argument("dsmvc: half bandwidth exceeds the HIP executor's limit");

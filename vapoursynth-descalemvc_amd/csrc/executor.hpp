@@ -27,9 +27,10 @@ struct ChunkArgs;
 }
 
 enum class SolveMode : std::uint8_t {
-    automatic = 0,   // chunked where the plan allows it, else sequential
+    automatic = 0,   // chunked or windowed where the plan allows it (autotuned), else sequential
     exact = 1,       // sequential: bit-identical to the ordered CPU solver
-    chunked = 2,     // chunked only; throws when the plan cannot be chunked
+    chunked = 2,     // carry-chain chunked kernels only; throws when the plan cannot use them
+    windowed = 3,    // windowed kernels only; throws when the plan cannot use them
 };
 
 enum class SampleKind : std::uint8_t { float32 = 0, u8 = 1, u16 = 2 };
@@ -47,8 +48,10 @@ struct PlaneBatch {
 };
 
 // Per-launch tuning the executor chooses by itself unless overridden.
+// Per direction: a chunk count (carry-chain kernels), -1 (windowed kernels)
+// or 0 (automatic: autotuned when enabled).
 struct LaunchTuning {
-    std::int32_t rows_chunks = 0;      // 0 = automatic (autotuned when enabled)
+    std::int32_t rows_chunks = 0;
     std::int32_t columns_chunks = 0;
 };
 
@@ -76,7 +79,8 @@ public:
     // so every later result of that key is bit-reproducible within it; the
     // partition affects the last bits of Float32 results (carry rounding).
     void set_autotune(bool enabled) { autotune_ = enabled; }
-    // The tuned chunk count of a key, 0 before tuning, -1 for an unknown plan.
+    // The tuned route of a key: a chunk count, kWindowRoute, 0 before tuning,
+    // -1 for an unknown plan.
     [[nodiscard]] int tuned_chunks(const AxisPlan &plan, bool rows, int kind);
 
     // Float32 planes (device pointers).  Synchronous unless `async`.
@@ -120,6 +124,11 @@ public:
     // Waits for `stream` and reports a deferred Float64 non-finite failure.
     void synchronize(hipStream_t stream);
 
+    // The device image of a plan (uploaded on first use), for callers that
+    // launch the kernels themselves (multi-plan batches, the request loop).
+    // It lives as long as the executor and the plan object.
+    hip::DevicePlan &plan_image(const AxisPlan &plan);
+
 private:
     struct Scratch {
         void *buffer = nullptr;
@@ -137,8 +146,11 @@ private:
                    std::int32_t vectors, std::int32_t frames, const hip::DevConv &conv,
                    bool f64, Scratch &s, std::size_t fwd_offset, hipStream_t stream);
 
+    static constexpr int kWindowRoute = -1;
     int choose_partition(hip::DevicePlan &dp, bool rows, const hip::ChunkArgs &base,
                          hip::Sample kind, hipStream_t stream);
+    hipError_t launch_route(hip::DevicePlan &dp, int route, bool rows, const hip::ChunkArgs &a,
+                            hip::Sample kind, hipStream_t stream);
 
     struct TuneKey {
         const hip::DevicePlan *plan;

@@ -58,6 +58,37 @@ extern template hipError_t launch_chunked_band<1>(const ChunkArgs &, bool, Sampl
 extern template hipError_t launch_chunked_band<3>(const ChunkArgs &, bool, Sample, hipStream_t);
 extern template hipError_t launch_chunked_band<5>(const ChunkArgs &, bool, Sample, hipStream_t);
 extern template hipError_t launch_chunked_band<7>(const ChunkArgs &, bool, Sample, hipStream_t);
+// Windowed (independent-chunk) kernels.  The band class is 1, 3, 5 or 7
+// (smaller bands run in the next class with zero coefficients); the tap
+// width 8 or 16.  `launch_window` runs one job; `launch_window_table` runs
+// `count` jobs from a device-resident table (`items` items in all) in one
+// launch, all jobs sharing the band class, tap width and direction, with F32
+// samples.
+int window_band_class(int w);   // 1, 3, 5, 7, or 0 when unsupported
+constexpr int kWindowWaves = 4;   // wavefronts (one item each) per workgroup
+hipError_t launch_window(const WinJob &job, int band, bool rows, Sample kind, hipStream_t s);
+hipError_t launch_window_table(const WinJob *jobs, int count, long items, int band, int width,
+                               bool rows, hipStream_t s);
+template <int W, int K>
+hipError_t launch_window_band(const WinJob &job, bool rows, Sample kind, hipStream_t s);
+template <int W, int K>
+hipError_t launch_window_table_band(const WinJob *jobs, int count, long items, bool rows,
+                                    hipStream_t s);
+#define DSMVC_WINDOW_EXTERN(W, K)                                                                 \
+    extern template hipError_t launch_window_band<W, K>(const WinJob &, bool, Sample,             \
+                                                        hipStream_t);                             \
+    extern template hipError_t launch_window_table_band<W, K>(const WinJob *, int, long, bool,    \
+                                                              hipStream_t);
+DSMVC_WINDOW_EXTERN(1, 8)
+DSMVC_WINDOW_EXTERN(1, 16)
+DSMVC_WINDOW_EXTERN(3, 8)
+DSMVC_WINDOW_EXTERN(3, 16)
+DSMVC_WINDOW_EXTERN(5, 8)
+DSMVC_WINDOW_EXTERN(5, 16)
+DSMVC_WINDOW_EXTERN(7, 8)
+DSMVC_WINDOW_EXTERN(7, 16)
+#undef DSMVC_WINDOW_EXTERN
+
 hipError_t launch_check_finite(const void *src, Sample in, long s_row, long s_frame, int rows,
                                int cols, int frames, int *flag, hipStream_t s);
 

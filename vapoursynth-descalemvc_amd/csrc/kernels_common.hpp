@@ -24,6 +24,21 @@ __device__ __forceinline__ void wave_sync() {
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Tables that no kernel writes while it runs, read through the constant
+// address space: a wave-uniform read is then a scalar load even after the
+// kernel's own global stores.
+template <class T> using ConstPtr = const __attribute__((address_space(4))) T *;
+template <class T> __device__ __forceinline__ ConstPtr<T> constant(const T *p) {
+    return (ConstPtr<T>)(p);   // address-space cast: flat to constant
+}
+// Device-memory pointers read from a table (not kernel arguments) are flat
+// pointers to the compiler; casting them to the global address space lets
+// wave-uniform reads through them become scalar loads.
+template <class T> using GlobalPtr = const __attribute__((address_space(1))) T *;
+template <class T> __device__ __forceinline__ GlobalPtr<T> global(const T *p) {
+    return (GlobalPtr<T>)(p);   // address-space cast: flat to global
+}
+
 // Raw buffer over a plane: loads take a per-lane byte offset and a scalar
 // (wave-uniform) byte offset, and read 0 past `bytes`.
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void *base, long bytes) {
@@ -142,8 +157,8 @@ __device__ __forceinline__ void push(T (&r)[W], T v) {
 // Forward + diagonal for row i.  r[d-1] holds y_{i-d}.  Terms with i-d < 0
 // meet zero coefficients and a zero ring, which leaves the FMA chain's value
 // unchanged, so applying all W terms equals the reference's truncated loop.
-template <int W>
-__device__ __forceinline__ float forward_step(const float *__restrict__ c, float rhs, float (&r)[W]) {
+template <int W, class Coef>
+__device__ __forceinline__ float forward_step_at(Coef c, float rhs, float (&r)[W]) {
     float v = rhs;
 #pragma unroll
     for (int d = W; d >= 1; --d) v = __builtin_fmaf(-c[d - 1], r[d - 1], v);
@@ -154,8 +169,8 @@ __device__ __forceinline__ float forward_step(const float *__restrict__ c, float
 
 // Backward for row i.  r[d-1] holds x_{i+d}.  W == 3 sums d ascending, other
 // bands descending, as the reference solver does.
-template <int W>
-__device__ __forceinline__ float backward_step(const float *__restrict__ c, float y, float (&r)[W]) {
+template <int W, class Coef>
+__device__ __forceinline__ float backward_step_at(Coef c, float y, float (&r)[W]) {
     float v = y;
     if constexpr (W == 3) {
 #pragma unroll
@@ -166,6 +181,31 @@ __device__ __forceinline__ float backward_step(const float *__restrict__ c, floa
     }
     push(r, v);
     return v;
+}
+
+template <int W>
+__device__ __forceinline__ float forward_step(const float *__restrict__ c, float rhs, float (&r)[W]) {
+    return forward_step_at<W>(c, rhs, r);
+}
+template <int W>
+__device__ __forceinline__ float forward_step(ConstPtr<float> c, float rhs, float (&r)[W]) {
+    return forward_step_at<W>(c, rhs, r);
+}
+template <int W>
+__device__ __forceinline__ float backward_step(const float *__restrict__ c, float y, float (&r)[W]) {
+    return backward_step_at<W>(c, y, r);
+}
+template <int W>
+__device__ __forceinline__ float backward_step(ConstPtr<float> c, float y, float (&r)[W]) {
+    return backward_step_at<W>(c, y, r);
+}
+template <int W>
+__device__ __forceinline__ float forward_step(GlobalPtr<float> __restrict__ c, float rhs, float (&r)[W]) {
+    return forward_step_at<W>(c, rhs, r);
+}
+template <int W>
+__device__ __forceinline__ float backward_step(GlobalPtr<float> __restrict__ c, float y, float (&r)[W]) {
+    return backward_step_at<W>(c, y, r);
 }
 
 // ---- chunk carry chain --------------------------------------------------------

@@ -1,0 +1,298 @@
+// Windowed (independent-chunk) axis solves for gfx950, templated on the band
+// class W, the tap width K and the sample type.  The chunk length L and the
+// warm-up P are fixed per band class (window_length / window_warmup).
+// Included by kernels_window_w{1,3,5,7}_k{8,16}.hip, one (band class, tap
+// width) pair each, so the build compiles them in parallel.
+//
+// Every wavefront solves one item: 64 vectors (lanes) x one chunk of L
+// destination rows of one plan.  It runs the forward recurrence over the
+// L + 2P padded rows [qL - P, qL + L + P) from a zero state, keeps the last
+// L + P values in registers, runs the backward recurrence down from the end
+// of the window from a zero state, and stores the chunk's L rows.  Chunks
+// never exchange carries, so a workgroup's wavefronts are independent (no
+// workgroup barrier) and one launch can hold items of many plans.  The four
+// wavefronts of a workgroup take consecutive items, which are consecutive
+// vector groups of the same chunk, so they read the same plan rows.
+//
+// Vertical pass (columns_window): lanes are 64 adjacent columns; taps are
+// buffer loads of whole source rows at a scalar row offset.
+// Horizontal pass (rows_window): lanes are 64 rows; each 8-row block stages
+// the 32 source columns its taps need through the wavefront's LDS ring
+// (transposed, pitch 65) and results leave through the same ring.
+#pragma once
+
+#include "kernels_common.hpp"
+
+namespace dsmvc::hip {
+namespace dev {
+namespace {
+
+constexpr int kWinRing = kStageCols * kRingPitch;   // floats of LDS per wavefront
+
+struct ItemPos {
+    int frame, chunk, group;
+};
+
+__device__ __forceinline__ ItemPos decode_item(const WinJob &jb, int local) {
+    const int per_frame = jb.plan.chunks * jb.groups;
+    ItemPos p;
+    p.frame = local / per_frame;
+    const int r = local - p.frame * per_frame;
+    p.chunk = r / jb.groups;
+    p.group = r - p.chunk * jb.groups;
+    return p;
+}
+
+template <int W, int K, class In>
+__device__ __forceinline__ void rows_item(const WinJob &jb, const ItemPos &at, float *ring,
+                                          int lane) {
+    constexpr int L = window_length(W);
+    constexpr int P = window_warmup(W);
+    constexpr int NB = (L + 2 * P) / kWindowBlock;
+    const DevWindow &pl = jb.plan;
+    const int nrows = jb.nvec;
+    const int row0 = at.group * 64;
+    const int r0 = at.chunk * L;
+    const In *src = static_cast<const In *>(jb.src) + static_cast<long>(at.frame) * jb.s_frame;
+    const auto rs = plane_rsrc(src, static_cast<long>(nrows) * jb.s_row * static_cast<long>(sizeof(In)));
+    const int m1 = pl.m - 1;
+    const int scol = lane & (kStageCols - 1);
+    const int srow = lane >> 5;
+    const auto ei = global(pl.ell_idx + static_cast<long>(r0) * K);
+    const auto ew = global(pl.ell_w + static_cast<long>(r0) * K);
+    const auto cf = global(pl.coef + static_cast<long>(r0) * Band<W>::kStride);
+    const auto blk = global(pl.blk_lo + r0 / kWindowBlock);
+    const DevConv cv = jb.conv;
+
+    float st[W];
+#pragma unroll
+    for (int d = 0; d < W; ++d) st[d] = 0.0F;
+    float y[L + P];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int lo = blk[b];
+        const int column = min(lo + scol, m1);
+        wave_sync();
+#pragma unroll
+        for (int t = 0; t < 32; ++t) {
+            const int rr = srow + 2 * t;
+            const int row = min(row0 + rr, nrows - 1);
+            const int voff = (row * static_cast<int>(jb.s_row) + column) * static_cast<int>(sizeof(In));
+            ring[scol * kRingPitch + rr] = Load<In>::norm(BufferLoad<In>::at(rs, voff, 0), cv);
+        }
+        wave_sync();
+#pragma unroll
+        for (int t = 0; t < kWindowBlock; ++t) {
+            const int k = b * kWindowBlock + t;
+            float v = 0.0F;
+#pragma unroll
+            for (int e = 0; e < K; ++e)
+                v = __builtin_fmaf(ew[k * K + e], ring[(ei[k * K + e] - lo) * kRingPitch + lane], v);
+            const float yv = forward_step<W>(cf + k * Band<W>::kStride, v, st);
+            if (k >= P) y[k - P] = yv;
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < W; ++d) st[d] = 0.0F;
+#pragma unroll
+    for (int k = L + P - 1; k >= 0; --k)
+        y[k] = backward_step<W>(cf + (k + P) * Band<W>::kStride, y[k], st);
+
+    // Results leave through the ring, 32 destination columns at a time:
+    // lanes 0-31 write row t, lanes 32-63 row t + 1 (128-byte runs).
+    float *out = static_cast<float *>(jb.dst) + static_cast<long>(at.frame) * jb.d_frame + r0;
+    const int oc = lane & 31;
+    const int orow = lane >> 5;
+    const int n = pl.n;
+#pragma unroll
+    for (int kb = 0; kb < L; kb += 32) {
+        wave_sync();
+#pragma unroll
+        for (int c = 0; c < 32; ++c)
+            if (kb + c < L) ring[c * kRingPitch + lane] = y[kb + c];
+        wave_sync();
+        const bool col_ok = kb + oc < L && r0 + kb + oc < n;
+#pragma unroll 4
+        for (int t = 0; t < 64; t += 2) {
+            const int rr = t + orow;
+            const int row = row0 + rr;
+            if (col_ok && row < nrows)
+                out[static_cast<long>(row) * jb.d_row + kb + oc] = ring[oc * kRingPitch + rr];
+        }
+    }
+}
+
+template <int W, int K, class Out>
+__device__ __forceinline__ void columns_item(const WinJob &jb, const ItemPos &at, int lane) {
+    constexpr int L = window_length(W);
+    constexpr int P = window_warmup(W);
+    constexpr int R = L + 2 * P;
+    const DevWindow &pl = jb.plan;
+    const int ncols = jb.nvec;
+    const int col = at.group * 64 + lane;
+    const bool live = col < ncols;
+    const int cc = live ? col : ncols - 1;
+    const int r0 = at.chunk * L;
+    const float *src = static_cast<const float *>(jb.src) + static_cast<long>(at.frame) * jb.s_frame;
+    const auto rs = plane_rsrc(src, static_cast<long>(pl.m) * jb.s_row * 4);
+    const int voff = cc * 4;
+    const int row_bytes = static_cast<int>(jb.s_row * 4);
+    const auto ei = global(pl.ell_idx + static_cast<long>(r0) * K);
+    const auto ew = global(pl.ell_w + static_cast<long>(r0) * K);
+    const auto cf = global(pl.coef + static_cast<long>(r0) * Band<W>::kStride);
+
+    float st[W];
+#pragma unroll
+    for (int d = 0; d < W; ++d) st[d] = 0.0F;
+    float y[L + P];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        float v = 0.0F;
+#pragma unroll
+        for (int e = 0; e < K; ++e)
+            v = __builtin_fmaf(ew[k * K + e], BufferLoad<float>::at(rs, voff, ei[k * K + e] * row_bytes), v);
+        const float yv = forward_step<W>(cf + k * Band<W>::kStride, v, st);
+        if (k >= P) y[k - P] = yv;
+    }
+#pragma unroll
+    for (int d = 0; d < W; ++d) st[d] = 0.0F;
+#pragma unroll
+    for (int k = L + P - 1; k >= 0; --k)
+        y[k] = backward_step<W>(cf + (k + P) * Band<W>::kStride, y[k], st);
+
+    Out *o = static_cast<Out *>(jb.dst) + static_cast<long>(at.frame) * jb.d_frame + cc;
+    const int n = pl.n;
+    const DevConv cv = jb.conv;
+#pragma unroll
+    for (int k = 0; k < L; ++k)
+        if (r0 + k < n && live) Store<Out>::f32(o + static_cast<long>(r0 + k) * jb.d_row, y[k], cv);
+}
+
+// One job: item = blockIdx.x * kWindowWaves + wave.
+template <int W, int K, class In>
+__global__ void __launch_bounds__(64 * kWindowWaves) rows_window(WinJob jb, int items) {
+    __shared__ float lds[kWindowWaves * kWinRing];
+    const int lane = threadIdx.x & 63;
+    const int wave = uniform(threadIdx.x >> 6);
+    const int item = static_cast<int>(blockIdx.x) * kWindowWaves + wave;
+    if (item >= items) return;
+    rows_item<W, K, In>(jb, decode_item(jb, item), lds + wave * kWinRing, lane);
+}
+
+template <int W, int K, class Out>
+__global__ void __launch_bounds__(64 * kWindowWaves) columns_window(WinJob jb, int items) {
+    const int lane = threadIdx.x & 63;
+    const int wave = uniform(threadIdx.x >> 6);
+    const int item = static_cast<int>(blockIdx.x) * kWindowWaves + wave;
+    if (item >= items) return;
+    columns_item<W, K, Out>(jb, decode_item(jb, item), lane);
+}
+
+// A job record, read with scalar loads.
+__device__ __forceinline__ WinJob load_job(const WinJob *p) {
+    static_assert(sizeof(WinJob) % 4 == 0, "job record is whole words");
+    struct Words {
+        std::uint32_t v[sizeof(WinJob) / 4];
+    } out;
+    const auto words = constant(reinterpret_cast<const std::uint32_t *>(p));
+#pragma unroll
+    for (unsigned i = 0; i < sizeof(WinJob) / 4; ++i) out.v[i] = words[i];
+    return __builtin_bit_cast(WinJob, out);
+}
+
+__device__ __forceinline__ int find_job(const WinJob *__restrict__ jobs, int count, long item) {
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (constant(jobs)[mid].first_item <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    return uniform(lo);
+}
+
+// Many jobs in one launch: the grid covers every item of every job in the
+// table; wave w of workgroup b takes item b * kWindowWaves + w and finds its
+// job by a binary search over the table.  Each wave solves one item and
+// exits, so the hardware dispatcher balances plans of uneven size.
+template <int W, int K, class In>
+__global__ void __launch_bounds__(64 * kWindowWaves)
+rows_window_table(const WinJob *__restrict__ jobs, int count, long items) {
+    __shared__ float lds[kWindowWaves * kWinRing];
+    const int lane = threadIdx.x & 63;
+    const int wave = uniform(threadIdx.x >> 6);
+    const long item = static_cast<long>(blockIdx.x) * kWindowWaves + wave;
+    if (item >= items) return;
+    const WinJob jb = load_job(jobs + find_job(jobs, count, item));
+    rows_item<W, K, In>(jb, decode_item(jb, static_cast<int>(item - jb.first_item)),
+                        lds + wave * kWinRing, lane);
+}
+
+template <int W, int K, class Out>
+__global__ void __launch_bounds__(64 * kWindowWaves)
+columns_window_table(const WinJob *__restrict__ jobs, int count, long items) {
+    const int lane = threadIdx.x & 63;
+    const int wave = uniform(threadIdx.x >> 6);
+    const long item = static_cast<long>(blockIdx.x) * kWindowWaves + wave;
+    if (item >= items) return;
+    const WinJob jb = load_job(jobs + find_job(jobs, count, item));
+    columns_item<W, K, Out>(jb, decode_item(jb, static_cast<int>(item - jb.first_item)), lane);
+}
+
+inline long job_items(const WinJob &jb) {
+    return static_cast<long>(jb.frames) * jb.plan.chunks * jb.groups;
+}
+
+template <int W, int K, class T>
+hipError_t launch_one(const WinJob &jb, bool rows, hipStream_t s) {
+    const long items = job_items(jb);
+    if (items <= 0) return hipSuccess;
+    if (items > 0x7FFFFFFFL) return hipErrorInvalidValue;
+    const dim3 grid(static_cast<unsigned>((items + kWindowWaves - 1) / kWindowWaves));
+    if (rows)
+        rows_window<W, K, T><<<grid, 64 * kWindowWaves, 0, s>>>(jb, static_cast<int>(items));
+    else
+        columns_window<W, K, T><<<grid, 64 * kWindowWaves, 0, s>>>(jb, static_cast<int>(items));
+    return hipGetLastError();
+}
+
+template <int W, int K, class T>
+hipError_t launch_table(const WinJob *jobs, int count, long items, bool rows, hipStream_t s) {
+    if (items <= 0 || count <= 0) return hipSuccess;
+    const long needed = (items + kWindowWaves - 1) / kWindowWaves;
+    if (needed > 0x7FFFFFFFL) return hipErrorInvalidValue;
+    const dim3 grid(static_cast<unsigned>(needed));
+    if (rows)
+        rows_window_table<W, K, T><<<grid, 64 * kWindowWaves, 0, s>>>(jobs, count, items);
+    else
+        columns_window_table<W, K, T><<<grid, 64 * kWindowWaves, 0, s>>>(jobs, count, items);
+    return hipGetLastError();
+}
+
+// Horizontal pass: the kind is the input sample (output F32).  Vertical
+// pass: the kind is the output sample (input F32).
+template <int W, int K>
+hipError_t dispatch_one(const WinJob &jb, bool rows, Sample kind, hipStream_t s) {
+    switch (kind) {
+    case Sample::f32: return launch_one<W, K, float>(jb, rows, s);
+    case Sample::u8: return launch_one<W, K, std::uint8_t>(jb, rows, s);
+    case Sample::u16: return launch_one<W, K, std::uint16_t>(jb, rows, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+} // namespace
+} // namespace dev
+
+template <int W, int K>
+hipError_t launch_window_band(const WinJob &job, bool rows, Sample kind, hipStream_t s) {
+    return dev::dispatch_one<W, K>(job, rows, kind, s);
+}
+
+// Multi-job launches run F32 samples only (the batched and served formats).
+template <int W, int K>
+hipError_t launch_window_table_band(const WinJob *jobs, int count, long items, bool rows,
+                                    hipStream_t s) {
+    return dev::launch_table<W, K, float>(jobs, count, items, rows, s);
+}
+
+} // namespace dsmvc::hip

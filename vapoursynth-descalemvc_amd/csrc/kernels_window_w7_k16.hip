@@ -1,0 +1,7 @@
+// Windowed axis kernels for band class 7, 16 taps per row (see kernels_window.hpp).
+#include "kernels_window.hpp"
+
+namespace dsmvc::hip {
+template hipError_t launch_window_band<7, 16>(const WinJob &, bool, Sample, hipStream_t);
+template hipError_t launch_window_table_band<7, 16>(const WinJob *, int, long, bool, hipStream_t);
+} // namespace dsmvc::hip

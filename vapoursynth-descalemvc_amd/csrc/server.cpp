@@ -1,0 +1,401 @@
+// Continuous-batching request loop (see server.hpp).
+#include "server.hpp"
+
+#include <algorithm>
+#include <stdexcept>
+
+#include "batch.hpp"
+#include "device_plan.hpp"
+#include "executor.hpp"
+#include "kernels.hpp"
+
+namespace dsmvc {
+namespace {
+
+void check(hipError_t e, const char *what) {
+    if (e != hipSuccess)
+        throw std::runtime_error(std::string("dsmvc: ") + what + ": " + hipGetErrorString(e));
+}
+
+std::size_t align256(std::size_t v) { return (v + 255U) & ~std::size_t{255}; }
+
+std::ptrdiff_t pitch_of(std::int32_t n) { return (static_cast<std::ptrdiff_t>(n) + 63) & ~std::ptrdiff_t{63}; }
+
+hip::DevConv identity_conv() {
+    hip::DevConv c{};
+    c.in_scale = 1.0F;
+    c.out_scale = 1.0F;
+    return c;
+}
+
+void require(bool ok, const char *what) {
+    if (!ok) throw std::invalid_argument(what);
+}
+
+} // namespace
+
+struct HipServer::Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    std::unique_ptr<WindowTables> tables;
+    std::vector<std::shared_ptr<Entry>> round;
+    bool fallback = false;   // the round solved planes through the executor
+    bool busy = false;
+};
+
+HipServer::HipServer(HipExecutor &executor, const ServerConfig &config)
+    : executor_(executor), config_(config), epoch_(std::chrono::steady_clock::now()) {
+    require(config_.memory_budget >= 256 && config_.slots >= 1 && config_.slots <= 16
+                && config_.max_round_requests >= 1,
+            "dsmvc: invalid server configuration");
+    config_.memory_budget &= ~std::size_t{255};
+    check(hipSetDevice(executor_.device()), "hipSetDevice failed");
+    check(hipMalloc(&arena_, config_.memory_budget), "server memory allocation failed");
+    free_[0] = config_.memory_budget;
+    for (int k = 0; k < config_.slots; ++k) {
+        auto slot = std::make_unique<Slot>();
+        check(hipStreamCreateWithFlags(&slot->stream, hipStreamNonBlocking), "stream creation failed");
+        check(hipEventCreateWithFlags(&slot->done, hipEventDisableTiming), "event creation failed");
+        slot->tables = std::make_unique<WindowTables>(executor_.device());
+        slots_.push_back(std::move(slot));
+    }
+    worker_ = std::thread([this] { loop(); });
+}
+
+HipServer::~HipServer() {
+    {
+        std::lock_guard lock(mutex_);
+        stop_ = true;
+    }
+    work_cv_.notify_all();
+    if (worker_.joinable()) worker_.join();
+    (void)hipSetDevice(executor_.device());
+    for (auto &slot : slots_) {
+        if (slot->stream) (void)hipStreamSynchronize(slot->stream);
+        slot->tables.reset();
+        if (slot->done) (void)hipEventDestroy(slot->done);
+        if (slot->stream) (void)hipStreamDestroy(slot->stream);
+    }
+    if (arena_) (void)hipFree(arena_);
+}
+
+double HipServer::now() const {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - epoch_).count();
+}
+
+std::uint64_t HipServer::submit(const ServeRequest &r) {
+    const double t0 = now();
+    require(r.planes == 1 || r.planes == 3, "dsmvc: a request has 1 or 3 planes");
+    require(r.src_width > 0 && r.src_height > 0 && r.dst_width > 0 && r.dst_height > 0,
+            "dsmvc: invalid request geometry");
+    const int ssw = r.planes == 3 ? r.subsampling_w : 0, ssh = r.planes == 3 ? r.subsampling_h : 0;
+    require(ssw >= 0 && ssw <= 2 && ssh >= 0 && ssh <= 2, "dsmvc: invalid chroma subsampling");
+    require((r.dst_width % (1 << ssw)) == 0 && (r.dst_height % (1 << ssh)) == 0,
+            "dsmvc: output width is incompatible with subsampling");
+
+    // Requests per plane kind (reference prepare_filter_requests,
+    // src/vs_plugin.cpp:656-705, with src_left = src_top = 0 and the
+    // active region equal to the output size).
+    AxisRequest base;
+    base.kernel = r.kernel;
+    base.border = r.border;
+    base.f64_mode = r.f64_mode;
+    auto axis = [&](int source, int destination, double active, double shift) {
+        AxisRequest q = base;
+        q.source_size = source;
+        q.destination_size = destination;
+        q.active_length = active;
+        q.shift = shift;
+        return get_or_build_axis_plan(q);
+    };
+    auto entry = std::make_shared<Entry>();
+    entry->submitted = t0;
+    const bool parallel = executor_.mode() == SolveMode::automatic
+                          || executor_.mode() == SolveMode::windowed;
+    std::shared_ptr<const AxisPlan> luma_h, luma_v, chroma_h, chroma_v;
+    for (int p = 0; p < r.planes; ++p) {
+        const bool chroma = p > 0;
+        const int sw = chroma ? r.src_width >> ssw : r.src_width;
+        const int sh = chroma ? r.src_height >> ssh : r.src_height;
+        const int dw = chroma ? r.dst_width >> ssw : r.dst_width;
+        const int dh = chroma ? r.dst_height >> ssh : r.dst_height;
+        require(sw > 0 && sh > 0 && r.src[p] && r.dst[p] && r.src_stride[p] >= sw
+                    && r.dst_stride[p] >= dw,
+                "dsmvc: invalid request plane");
+        if (!chroma) {
+            if (!luma_h) {
+                luma_h = axis(sw, dw, dw, 0.0);
+                luma_v = axis(sh, dh, dh, 0.0);
+            }
+        } else if (!chroma_h) {
+            const double cw = sw, fw = r.src_width;
+            chroma_h = ssw ? axis(sw, dw, double(r.dst_width) * cw / fw,
+                                  0.25 - 0.25 * double(r.dst_width) / fw)
+                           : axis(sw, dw, dw, 0.0);
+            chroma_v = ssh ? axis(sh, dh, double(r.dst_height) * double(sh) / double(r.src_height), 0.0)
+                           : axis(sh, dh, dh, 0.0);
+        }
+        Plane plane;
+        plane.h = chroma ? chroma_h : luma_h;
+        plane.v = chroma ? chroma_v : luma_v;
+        plane.src = r.src[p];
+        plane.src_stride = r.src_stride[p];
+        plane.dst = r.dst[p];
+        plane.dst_stride = r.dst_stride[p];
+        // Device images are uploaded here, on the caller's thread, so the
+        // loop only builds tables and launches.
+        hip::DevicePlan &ih = executor_.plan_image(*plane.h);
+        hip::DevicePlan &iv = executor_.plan_image(*plane.v);
+        plane.table = parallel && !plane.h->requires_float64() && !plane.v->requires_float64()
+                      && ih.window(true) && iv.window(false);
+        if (plane.table)
+            plane.inter_bytes = align256(static_cast<std::size_t>(pitch_of(dw)) * sh * sizeof(float));
+        entry->bytes += plane.inter_bytes;
+        entry->planes.push_back(std::move(plane));
+    }
+    require(entry->bytes <= config_.memory_budget,
+            "dsmvc: request needs more device memory than the server's budget");
+    std::uint64_t id;
+    {
+        std::lock_guard lock(mutex_);
+        require(!stop_, "dsmvc: server is stopping");
+        id = next_id_++;
+        entry->id = id;
+        entries_[id] = entry;
+        queue_.push_back(entry);
+        ++stats_.submitted;
+    }
+    work_cv_.notify_one();
+    return id;
+}
+
+RequestState HipServer::wait(std::uint64_t id, std::string *error, double *submitted,
+                             double *completed) {
+    std::unique_lock lock(mutex_);
+    auto it = entries_.find(id);
+    if (it == entries_.end()) throw std::invalid_argument("dsmvc: unknown request id");
+    std::shared_ptr<Entry> e = it->second;
+    done_cv_.wait(lock, [&] { return e->state != RequestState::pending; });
+    if (error) *error = e->error;
+    if (submitted) *submitted = e->submitted;
+    if (completed) *completed = e->completed;
+    return e->state;
+}
+
+RequestState HipServer::poll(std::uint64_t id) {
+    std::lock_guard lock(mutex_);
+    auto it = entries_.find(id);
+    if (it == entries_.end()) throw std::invalid_argument("dsmvc: unknown request id");
+    return it->second->state;
+}
+
+void HipServer::release(std::uint64_t id) {
+    std::lock_guard lock(mutex_);
+    auto it = entries_.find(id);
+    if (it != entries_.end() && it->second->state != RequestState::pending) entries_.erase(it);
+}
+
+ServerStats HipServer::stats() const {
+    std::lock_guard lock(mutex_);
+    return stats_;
+}
+
+bool HipServer::arena_alloc(std::size_t bytes, std::size_t &offset) {
+    if (bytes == 0) {
+        offset = 0;
+        return true;
+    }
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+        if (it->second < bytes) continue;
+        offset = it->first;
+        const std::size_t rest = it->second - bytes;
+        free_.erase(it);
+        if (rest) free_[offset + bytes] = rest;
+        in_use_ += bytes;
+        stats_.memory_high_water = std::max(stats_.memory_high_water, in_use_);
+        return true;
+    }
+    return false;
+}
+
+void HipServer::arena_free(std::size_t offset, std::size_t bytes) {
+    if (bytes == 0) return;
+    in_use_ -= bytes;
+    auto it = free_.emplace(offset, bytes).first;
+    auto next = std::next(it);
+    if (next != free_.end() && it->first + it->second == next->first) {
+        it->second += next->second;
+        free_.erase(next);
+    }
+    if (it != free_.begin()) {
+        auto prev = std::prev(it);
+        if (prev->first + prev->second == it->first) {
+            prev->second += it->second;
+            free_.erase(it);
+        }
+    }
+}
+
+// Takes every queued request that fits (in arrival order, stopping at the
+// first that does not) and launches them as one round on `slot`.
+bool HipServer::admit_round(Slot &slot, std::unique_lock<std::mutex> &lock) {
+    std::vector<std::shared_ptr<Entry>> round;
+    while (!queue_.empty() && static_cast<int>(round.size()) < config_.max_round_requests) {
+        const std::shared_ptr<Entry> &e = queue_.front();
+        if (!arena_alloc(e->bytes, e->offset)) break;
+        round.push_back(e);
+        queue_.pop_front();
+    }
+    if (round.empty()) return false;
+    slot.busy = true;
+    slot.round = round;
+    ++stats_.rounds;
+    stats_.largest_round = std::max<std::uint64_t>(stats_.largest_round, round.size());
+    lock.unlock();
+    bool fallback = false;
+    std::string failure;
+    std::uint64_t table_planes = 0, fallback_planes = 0;
+    try {
+        check(hipSetDevice(executor_.device()), "hipSetDevice failed");
+        WindowTables &tables = *slot.tables;
+        tables.clear();
+        for (const auto &e : round) {
+            std::size_t at = e->offset;
+            for (const Plane &p : e->planes) {
+                if (!p.table) continue;
+                const AxisPlan &h = *p.h, &v = *p.v;
+                const std::ptrdiff_t row = pitch_of(h.destination_size);
+                float *inter = reinterpret_cast<float *>(static_cast<unsigned char *>(arena_) + at);
+                at += p.inter_bytes;
+                hip::WinJob a{};
+                a.plan = *executor_.plan_image(h).window(true);
+                a.src = p.src;
+                a.s_row = p.src_stride;
+                a.dst = inter;
+                a.d_row = row;
+                a.nvec = v.source_size;
+                a.frames = 1;
+                a.groups = (a.nvec + 63) / 64;
+                a.conv = identity_conv();
+                tables.add(a, hip::window_band_class(h.half_bandwidth), true);
+                hip::WinJob b{};
+                b.plan = *executor_.plan_image(v).window(false);
+                b.src = inter;
+                b.s_row = row;
+                b.dst = p.dst;
+                b.d_row = p.dst_stride;
+                b.nvec = h.destination_size;
+                b.frames = 1;
+                b.groups = (b.nvec + 63) / 64;
+                b.conv = identity_conv();
+                tables.add(b, hip::window_band_class(v.half_bandwidth), false);
+                ++table_planes;
+            }
+        }
+        tables.upload(slot.stream);
+        tables.launch(slot.stream);
+        for (const auto &e : round)
+            for (const Plane &p : e->planes) {
+                if (p.table) continue;
+                executor_.inverse_2d_batch(*p.h, *p.v, SampleKind::float32,
+                                           PlaneBatch{p.src, p.src_stride, 0, p.dst, p.dst_stride, 0, 1},
+                                           nullptr, slot.stream, true);
+                fallback = true;
+                ++fallback_planes;
+            }
+        check(hipEventRecord(slot.done, slot.stream), "event record failed");
+    } catch (const std::exception &ex) {
+        failure = ex.what();
+    }
+    lock.lock();
+    slot.fallback = fallback;
+    stats_.table_planes += table_planes;
+    stats_.fallback_planes += fallback_planes;
+    if (!failure.empty()) {
+        // Nothing of this round can be trusted: wait for whatever was queued,
+        // then fail its requests.
+        lock.unlock();
+        (void)hipStreamSynchronize(slot.stream);
+        lock.lock();
+        const double t = now();
+        for (const auto &e : round) {
+            arena_free(e->offset, e->bytes);
+            e->state = RequestState::failed;
+            e->error = failure;
+            e->completed = t;
+            ++stats_.failed;
+        }
+        slot.busy = false;
+        slot.round.clear();
+        done_cv_.notify_all();
+    }
+    return true;
+}
+
+void HipServer::finish_round(Slot &slot, std::unique_lock<std::mutex> &lock, std::string device_error) {
+    std::string failure;
+    if (slot.fallback && device_error.empty()) {
+        lock.unlock();
+        try {
+            executor_.synchronize(slot.stream);   // reports a Float64 non-finite result
+        } catch (const std::exception &ex) {
+            failure = ex.what();
+        }
+        lock.lock();
+    }
+    const double t = now();
+    for (const auto &e : slot.round) {
+        arena_free(e->offset, e->bytes);
+        bool uses_fallback = false;
+        for (const Plane &p : e->planes) uses_fallback |= !p.table;
+        if (!device_error.empty() || (!failure.empty() && uses_fallback)) {
+            e->state = RequestState::failed;
+            e->error = device_error.empty() ? failure : device_error;
+            ++stats_.failed;
+        } else {
+            e->state = RequestState::done;
+            ++stats_.completed;
+        }
+        e->completed = t;
+    }
+    slot.round.clear();
+    slot.busy = false;
+    slot.fallback = false;
+    done_cv_.notify_all();
+}
+
+void HipServer::loop() {
+    std::unique_lock lock(mutex_);
+    for (;;) {
+        bool progressed = false, busy = false;
+        for (auto &slot : slots_) {
+            if (!slot->busy) continue;
+            const hipError_t q = hipEventQuery(slot->done);
+            if (q == hipErrorNotReady) {
+                busy = true;
+                continue;
+            }
+            finish_round(*slot, lock, q == hipSuccess ? std::string()
+                                                      : std::string("dsmvc: round failed: ")
+                                                            + hipGetErrorString(q));
+            progressed = true;
+        }
+        for (auto &slot : slots_) {
+            if (queue_.empty()) break;
+            if (slot->busy) continue;
+            if (!admit_round(*slot, lock)) break;
+            progressed = true;
+            busy = true;
+        }
+        if (stop_ && queue_.empty() && !busy) {
+            bool any = false;
+            for (auto &slot : slots_) any |= slot->busy;
+            if (!any) break;
+        }
+        if (progressed) continue;
+        if (busy) work_cv_.wait_for(lock, std::chrono::microseconds(20));
+        else work_cv_.wait(lock, [&] { return stop_ || !queue_.empty(); });
+    }
+}
+
+} // namespace dsmvc

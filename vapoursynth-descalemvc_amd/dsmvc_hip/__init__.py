@@ -38,7 +38,7 @@ ZERO, REPEAT, REFLECT101, SYMMETRIC, MIRROR = range(5)
 KERNELS = {"bilinear": BILINEAR, "bicubic": BICUBIC, "lanczos": LANCZOS,
            "spline16": SPLINE16, "spline36": SPLINE36, "spline64": SPLINE64}
 BACKENDS = {0: "auto", 1: "cpu", 2: "metal", 3: "vulkan", 4: "cuda", 5: "hip"}
-SOLVE_MODES = {"auto": 0, "exact": 1, "chunked": 2}
+SOLVE_MODES = {"auto": 0, "exact": 1, "chunked": 2, "windowed": 3}
 SAMPLE_KINDS = {"float32": 0, "uint8": 1, "uint16": 2}
 
 
@@ -125,6 +125,13 @@ class Plan:
                 load().dsmvc_hip_plan_release(self._handle)
         except Exception:
             pass
+
+    def window_info(self) -> dict:
+        """How the windowed (independent-chunk) kernels partition this plan."""
+        info = _lib.WindowInfoC()
+        check(load().dsmvc_hip_plan_window_info(self._handle, ctypes.byref(info)))
+        return {name: getattr(info, name) for name, _ in _lib.WindowInfoC._fields_
+                if name != "reserved"}
 
     def arrays(self) -> dict:
         """Copies of the AxisPlan arrays (reference include/dsmvc/engine.hpp:90-107)."""
@@ -290,9 +297,131 @@ class Executor:
         return out if batched or out.dim() == 2 else out[0]
 
 
+class Batch:
+    """Many 2D F32 descale jobs of uneven geometry solved together: one launch
+    per (direction, band class, tap width) over device-resident job tables
+    (include/dsmvc_hip.h, dsmvc_hip_batch_*).
+
+    jobs: (horizontal plan, vertical plan, input, output) with input a
+    [frames,] H x W float32 device tensor and output [frames,] h x w.
+    graph: replay the table launches from a captured HIP graph."""
+
+    def __init__(self, executor: "Executor", jobs, graph: bool = False):
+        lib = load()
+        self._executor = executor
+        self._keep = []
+        table = (_lib.BatchJobC * max(1, len(jobs)))()
+        for k, (hp, vp, x, out) in enumerate(jobs):
+            frames = x.shape[0] if x.dim() == 3 else 1
+            if out.shape[-2:] != (vp.destination_size, hp.destination_size) or \
+                    x.shape[-2] < vp.source_size or x.shape[-1] < hp.source_size:
+                raise ValueError("dsmvc: batch job shapes do not match its plans")
+            table[k] = _lib.BatchJobC(
+                hp.handle, vp.handle, _dev_ptr(x), x.stride(-2),
+                x.stride(0) if x.dim() == 3 else 0, _dev_ptr(out), out.stride(-2),
+                out.stride(0) if out.dim() == 3 else 0, frames, 0)
+            self._keep += [hp, vp, x, out]
+        handle = ctypes.c_void_p()
+        check(lib.dsmvc_hip_batch_create(executor._handle, table, len(jobs),
+                                         _lib.BATCH_GRAPH if graph else 0, ctypes.byref(handle)))
+        self._handle = handle
+
+    def __del__(self):
+        handle = getattr(self, "_handle", None)
+        if handle:
+            load().dsmvc_hip_batch_destroy(handle)
+            self._handle = None
+
+    def run(self, stream=None, asynchronous=False):
+        check(load().dsmvc_hip_batch_run(self._handle, _stream_handle(stream),
+                                         ASYNC if asynchronous else 0))
+
+    def info(self) -> dict:
+        info = _lib.BatchInfoC()
+        check(load().dsmvc_hip_batch_info_get(self._handle, ctypes.byref(info)))
+        return {name: getattr(info, name) for name, _ in _lib.BatchInfoC._fields_
+                if name != "reserved"}
+
+
+class Server:
+    """Continuous-batching request loop (include/dsmvc_hip.h,
+    dsmvc_hip_server_*): requests of any geometry are planned on submit,
+    admitted as slots and device memory free up, and solved together with
+    whatever else is admitted in the same round.  Planes are float32 device
+    tensors that must stay alive until the request finishes (the Server keeps
+    references until `wait`)."""
+
+    def __init__(self, executor: "Executor", memory_budget: int = 1 << 30, slots: int = 4,
+                 max_round_requests: int = 256):
+        self._executor = executor
+        cfg = _lib.ServerConfigC(memory_budget, slots, max_round_requests)
+        handle = ctypes.c_void_p()
+        check(load().dsmvc_hip_server_create(executor._handle, ctypes.byref(cfg),
+                                             ctypes.byref(handle)))
+        self._handle = handle
+        self._keep = {}
+
+    def close(self):
+        handle = getattr(self, "_handle", None)
+        if handle:
+            load().dsmvc_hip_server_destroy(handle)
+            self._handle = None
+            self._keep.clear()
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, planes_in, planes_out, kernel: int = BICUBIC, taps: int = 3, b: float = 0.0,
+               c: float = 0.5, blur: float = 1.0, border: int = SYMMETRIC, f64_mode: int = 0,
+               subsampling_w: int = 1, subsampling_h: int = 1) -> int:
+        """planes_in / planes_out: 1 (gray) or 3 (YUV) 2D float32 tensors."""
+        n = len(planes_in)
+        if n not in (1, 3) or len(planes_out) != n:
+            raise ValueError("dsmvc: a request has 1 or 3 planes")
+        r = _lib.ServeRequestC()
+        r.planes = n
+        r.subsampling_w = subsampling_w if n == 3 else 0
+        r.subsampling_h = subsampling_h if n == 3 else 0
+        r.src_height, r.src_width = planes_in[0].shape
+        r.dst_height, r.dst_width = planes_out[0].shape
+        r.kernel, r.taps, r.border, r.f64_mode = kernel, taps, border, f64_mode
+        r.b, r.c, r.blur = b, c, blur
+        for k in range(n):
+            if planes_in[k].dtype != _torch().float32 or planes_out[k].dtype != _torch().float32:
+                raise ValueError("dsmvc: served planes are float32")
+            r.src[k] = _dev_ptr(planes_in[k]).value
+            r.src_stride[k] = planes_in[k].stride(0)
+            r.dst[k] = _dev_ptr(planes_out[k]).value
+            r.dst_stride[k] = planes_out[k].stride(0)
+        rid = ctypes.c_uint64()
+        check(load().dsmvc_hip_server_submit(self._handle, ctypes.byref(r), ctypes.byref(rid)))
+        self._keep[rid.value] = (planes_in, planes_out)
+        return rid.value
+
+    def wait(self, rid: int):
+        """Blocks until the request finishes; returns (submitted, completed)
+        in seconds since the server started.  Raises on a failed request."""
+        t0, t1 = ctypes.c_double(), ctypes.c_double()
+        lib = load()
+        status = lib.dsmvc_hip_server_wait(self._handle, rid, ctypes.byref(t0), ctypes.byref(t1))
+        self._keep.pop(rid, None)
+        if status != _lib.OK:
+            message = lib.dsmvc_hip_last_error().decode()
+            lib.dsmvc_hip_server_release(self._handle, rid)
+            raise (ValueError if status == _lib.INVALID_ARGUMENT else DsmvcError)(message)
+        check(lib.dsmvc_hip_server_release(self._handle, rid))
+        return t0.value, t1.value
+
+    def stats(self) -> dict:
+        s = _lib.ServerStatsC()
+        check(load().dsmvc_hip_server_stats_get(self._handle, ctypes.byref(s)))
+        return {name: getattr(s, name) for name, _ in _lib.ServerStatsC._fields_}
+
+
 # ---------------------------------------------------------------------------
 # Plugin-shaped operators
 # ---------------------------------------------------------------------------
+
 @dataclass
 class Clip:
     """A clip: planes (luma first) of [frames, H, W] device tensors plus format.

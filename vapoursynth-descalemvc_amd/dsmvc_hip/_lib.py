@@ -16,15 +16,21 @@ ASYNC = 1
 # Every symbol include/dsmvc_hip.h declares.
 EXPORTS = (
     "dsmvc_hip_last_error", "dsmvc_hip_plan_build", "dsmvc_hip_plan_release",
-    "dsmvc_hip_plan_info_get", "dsmvc_hip_plan_arrays", "dsmvc_hip_planner_cache_stats",
+    "dsmvc_hip_plan_info_get", "dsmvc_hip_plan_arrays", "dsmvc_hip_plan_window_info",
+    "dsmvc_hip_planner_cache_stats",
     "dsmvc_hip_clear_planner_caches", "dsmvc_hip_padding_index", "dsmvc_hip_parse_backend",
     "dsmvc_hip_resolve_backend", "dsmvc_hip_device_count", "dsmvc_hip_executor_create",
     "dsmvc_hip_executor_destroy", "dsmvc_hip_executor_prepare", "dsmvc_hip_executor_seal",
     "dsmvc_hip_executor_tune", "dsmvc_hip_inverse_rows", "dsmvc_hip_inverse_columns",
     "dsmvc_hip_inverse_2d", "dsmvc_hip_inverse_2d_u8", "dsmvc_hip_inverse_2d_u16",
     "dsmvc_hip_inverse_2d_batch", "dsmvc_hip_inverse_rows_batch",
-    "dsmvc_hip_inverse_columns_batch", "dsmvc_hip_synchronize",
+    "dsmvc_hip_inverse_columns_batch", "dsmvc_hip_batch_create", "dsmvc_hip_batch_run",
+    "dsmvc_hip_batch_info_get", "dsmvc_hip_batch_destroy", "dsmvc_hip_server_create",
+    "dsmvc_hip_server_submit", "dsmvc_hip_server_wait", "dsmvc_hip_server_poll",
+    "dsmvc_hip_server_release", "dsmvc_hip_server_stats_get", "dsmvc_hip_server_destroy",
+    "dsmvc_hip_synchronize",
 )
+BATCH_GRAPH = 2
 
 
 class AxisRequestC(ctypes.Structure):
@@ -50,6 +56,93 @@ class ConversionC(ctypes.Structure):
         ("output_scale", ctypes.c_float),
         ("output_offset", ctypes.c_float),
         ("output_maximum", ctypes.c_uint32),
+    ]
+
+
+class WindowInfoC(ctypes.Structure):
+    _fields_ = [
+        ("feasible", ctypes.c_int32),
+        ("rows_feasible", ctypes.c_int32),
+        ("band", ctypes.c_int32),
+        ("chunks", ctypes.c_int32),
+        ("length", ctypes.c_int32),
+        ("warmup", ctypes.c_int32),
+        ("width", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("residual", ctypes.c_double),
+    ]
+
+
+class BatchJobC(ctypes.Structure):
+    _fields_ = [
+        ("horizontal", ctypes.c_void_p),
+        ("vertical", ctypes.c_void_p),
+        ("input", ctypes.c_void_p),
+        ("input_row_stride", ctypes.c_ssize_t),
+        ("input_frame_stride", ctypes.c_ssize_t),
+        ("output", ctypes.c_void_p),
+        ("output_row_stride", ctypes.c_ssize_t),
+        ("output_frame_stride", ctypes.c_ssize_t),
+        ("frames", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class BatchInfoC(ctypes.Structure):
+    _fields_ = [
+        ("jobs", ctypes.c_int32),
+        ("table_jobs", ctypes.c_int32),
+        ("fallback_jobs", ctypes.c_int32),
+        ("launches", ctypes.c_int32),
+        ("items", ctypes.c_int64),
+        ("intermediate_bytes", ctypes.c_int64),
+        ("graph", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class ServeRequestC(ctypes.Structure):
+    _fields_ = [
+        ("planes", ctypes.c_int32),
+        ("subsampling_w", ctypes.c_int32),
+        ("subsampling_h", ctypes.c_int32),
+        ("src_width", ctypes.c_int32),
+        ("src_height", ctypes.c_int32),
+        ("dst_width", ctypes.c_int32),
+        ("dst_height", ctypes.c_int32),
+        ("kernel", ctypes.c_int32),
+        ("taps", ctypes.c_int32),
+        ("border", ctypes.c_int32),
+        ("f64_mode", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("b", ctypes.c_double),
+        ("c", ctypes.c_double),
+        ("blur", ctypes.c_double),
+        ("src", ctypes.c_void_p * 3),
+        ("src_stride", ctypes.c_ssize_t * 3),
+        ("dst", ctypes.c_void_p * 3),
+        ("dst_stride", ctypes.c_ssize_t * 3),
+    ]
+
+
+class ServerConfigC(ctypes.Structure):
+    _fields_ = [
+        ("memory_budget", ctypes.c_int64),
+        ("slots", ctypes.c_int32),
+        ("max_round_requests", ctypes.c_int32),
+    ]
+
+
+class ServerStatsC(ctypes.Structure):
+    _fields_ = [
+        ("submitted", ctypes.c_uint64),
+        ("completed", ctypes.c_uint64),
+        ("failed", ctypes.c_uint64),
+        ("rounds", ctypes.c_uint64),
+        ("table_planes", ctypes.c_uint64),
+        ("fallback_planes", ctypes.c_uint64),
+        ("largest_round", ctypes.c_uint64),
+        ("memory_high_water", ctypes.c_int64),
     ]
 
 
@@ -100,6 +193,7 @@ def load():
     lib.dsmvc_hip_plan_release.restype = None
     lib.dsmvc_hip_plan_info_get.argtypes = [vp, ctypes.POINTER(PlanInfoC)]
     lib.dsmvc_hip_plan_arrays.argtypes = [vp] + [vp] * 10
+    lib.dsmvc_hip_plan_window_info.argtypes = [vp, ctypes.POINTER(WindowInfoC)]
     lib.dsmvc_hip_planner_cache_stats.argtypes = [ctypes.POINTER(PlannerStatsC)]
     lib.dsmvc_hip_clear_planner_caches.restype = None
     lib.dsmvc_hip_padding_index.argtypes = [ctypes.c_int64, i32, i32, ctypes.POINTER(i32)]
@@ -124,6 +218,22 @@ def load():
                                                  ctypes.POINTER(ConversionC), vp, u32]
     lib.dsmvc_hip_inverse_columns_batch.argtypes = [vp, vp, vp, pd, pd, i32, vp, pd, pd, i32,
                                                     i32, ctypes.POINTER(ConversionC), vp, u32]
+    lib.dsmvc_hip_batch_create.argtypes = [vp, ctypes.POINTER(BatchJobC), i32, u32,
+                                           ctypes.POINTER(vp)]
+    lib.dsmvc_hip_batch_run.argtypes = [vp, vp, u32]
+    lib.dsmvc_hip_batch_info_get.argtypes = [vp, ctypes.POINTER(BatchInfoC)]
+    lib.dsmvc_hip_batch_destroy.argtypes = [vp]
+    lib.dsmvc_hip_batch_destroy.restype = None
+    lib.dsmvc_hip_server_create.argtypes = [vp, ctypes.POINTER(ServerConfigC), ctypes.POINTER(vp)]
+    lib.dsmvc_hip_server_submit.argtypes = [vp, ctypes.POINTER(ServeRequestC),
+                                            ctypes.POINTER(ctypes.c_uint64)]
+    lib.dsmvc_hip_server_wait.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_double)]
+    lib.dsmvc_hip_server_poll.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(i32)]
+    lib.dsmvc_hip_server_release.argtypes = [vp, ctypes.c_uint64]
+    lib.dsmvc_hip_server_stats_get.argtypes = [vp, ctypes.POINTER(ServerStatsC)]
+    lib.dsmvc_hip_server_destroy.argtypes = [vp]
+    lib.dsmvc_hip_server_destroy.restype = None
     lib.dsmvc_hip_synchronize.argtypes = [vp, vp]
     _lib = lib
     return lib
