@@ -41,7 +41,7 @@ extern "C" {
 enum { DSMVC_HIP_BILINEAR = 0, DSMVC_HIP_BICUBIC = 1, DSMVC_HIP_LANCZOS = 2,
        DSMVC_HIP_SPLINE16 = 3, DSMVC_HIP_SPLINE36 = 4, DSMVC_HIP_SPLINE64 = 5 };
 enum { DSMVC_HIP_SOLVE_AUTO = 0, DSMVC_HIP_SOLVE_EXACT = 1, DSMVC_HIP_SOLVE_CHUNKED = 2,
-       DSMVC_HIP_SOLVE_WINDOWED = 3 };
+       DSMVC_HIP_SOLVE_WINDOWED = 3, DSMVC_HIP_SOLVE_REFINED = 4 };
 enum { DSMVC_HIP_SAMPLE_F32 = 0, DSMVC_HIP_SAMPLE_U8 = 1, DSMVC_HIP_SAMPLE_U16 = 2 };
 
 /* dsmvc::AxisRequest with KernelSpec flattened (include/dsmvc/engine.hpp:60-78). */
@@ -147,6 +147,12 @@ void dsmvc_hip_executor_destroy(dsmvc_hip_executor *executor);
 /* Executor::prepare / seal (src/executor.cpp:313-349): uploads the plan image. */
 int dsmvc_hip_executor_prepare(dsmvc_hip_executor *executor, const dsmvc_hip_plan *plan);
 int dsmvc_hip_executor_seal(dsmvc_hip_executor *executor);
+/* No counterpart in the reference (which solves such plans in double only,
+ * src/axis_plan.cpp:1124-1248): in DSMVC_HIP_SOLVE_REFINED mode, plans that
+ * retain FP64 data are solved in F32 and refined `steps` times (1..3) with
+ * the residual A^T y - G x formed in compensated F32 (two-product, two-sum)
+ * from the FP64 weights and normal matrix.  Float32 planes only. */
+int dsmvc_hip_executor_set_refinement(dsmvc_hip_executor *executor, int32_t steps);
 /* Route of the parallel F32 solves per direction: a chunk count for the
  * carry-chain chunked kernels, -1 for the windowed kernels, 0 = automatic
  * (autotuned).  A route the plan cannot use falls back to automatic. */

@@ -55,7 +55,8 @@ def test_batch_of_uneven_plans_matches_oracle(executor, oracle):
     info = batch.info()
     print(info, flush=True)
     assert info["jobs"] == len(heights) and info["table_jobs"] == len(heights)
-    assert info["fallback_jobs"] == 0 and info["launches"] <= 4
+    # One launch per (direction, tap width, staging block size) group.
+    assert info["fallback_jobs"] == 0 and info["launches"] <= 6
     batch.run()
     for (hr, vr, _, _, out), h in zip(jobs, heights):
         got = out.cpu().numpy()

@@ -1,0 +1,84 @@
+"""Precision on ill-conditioned plans: FP32, FP32 with refinement, and the
+Float64 route, each scored against the FP64 oracle.
+
+Workload: Delanczos taps=4 and Despline64 with blur 0.9 on GRAYS F32
+1920x1080 -> 1904x1071 (near-identity, condition ~50) and -> 960x540; frames
+of seeded uniform [0, 1] noise plus a sum of 8 random 2-D sinusoids.  The
+oracle is the C restatement's Float64 route on forced-FP64 plans (reference
+src/axis_plan.cpp:1124-1248 with f64mode=2).
+
+Bars (max-abs over the frame):
+  * Float64 route: <= 2 F32 ULPs of the output scale (it rounds once to F32);
+  * FP32 + one refinement step: <= 2e-6, and no worse than plain FP32;
+  * FP32 + two steps: no worse than one step (within one F32 ULP of scale).
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+import dsmvc_hip as dh  # noqa: E402
+from support import F64_ALWAYS, F64_NEVER, LANCZOS, SPLINE64, make_request  # noqa: E402
+
+DEV = torch.device("cuda:0")
+
+CASES = [
+    ("lanczos4-1904x1071", LANCZOS, 4, 1904, 1071),
+    ("spline64-1904x1071", SPLINE64, 3, 1904, 1071),
+    ("lanczos4-960x540", LANCZOS, 4, 960, 540),
+    ("spline64-960x540", SPLINE64, 3, 960, 540),
+]
+
+
+def frame_with_sinusoids(seed, w=1920, h=1080):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0.0, 1.0, (h, w))
+    yy, xx = np.mgrid[0:h, 0:w]
+    for _ in range(8):
+        amp = rng.uniform(0.05, 0.25)
+        fx, fy = rng.uniform(0.002, 0.2, 2)
+        phase = rng.uniform(0, 2 * np.pi)
+        x += amp * np.sin(2 * np.pi * (fx * xx + fy * yy) + phase)
+    return x.astype(np.float32)
+
+
+def _plan(w_src, w_dst, kernel, taps, mode):
+    return dh.Plan(dh.AxisRequest(w_src, w_dst, float(w_dst), 0.0, kernel, taps, 0.0, 0.5, 0.9,
+                                  dh.SYMMETRIC, mode))
+
+
+@pytest.fixture(scope="module")
+def executors():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    refined1 = dh.Executor(0, "refined")
+    refined2 = dh.Executor(0, "refined")
+    refined2.set_refinement(2)
+    return {"auto": dh.Executor(0, "auto"), "refined1": refined1, "refined2": refined2}
+
+
+@pytest.mark.parametrize("name,kernel,taps,dw,dh_", CASES, ids=[c[0] for c in CASES])
+def test_refinement_against_fp64_oracle(executors, oracle, name, kernel, taps, dw, dh_):
+    frame = frame_with_sinusoids(len(name))
+    x = torch.from_numpy(frame).to(DEV)
+    h32, v32 = _plan(1920, dw, kernel, taps, F64_NEVER), _plan(1080, dh_, kernel, taps, F64_NEVER)
+    h64, v64 = _plan(1920, dw, kernel, taps, F64_ALWAYS), _plan(1080, dh_, kernel, taps, F64_ALWAYS)
+    assert h64.requires_float64 and v64.requires_float64
+    plain = executors["auto"].inverse_2d(h32, v32, x).cpu().numpy()
+    f64 = executors["auto"].inverse_2d(h64, v64, x).cpu().numpy()
+    ref1 = executors["refined1"].inverse_2d(h64, v64, x).cpu().numpy()
+    ref2 = executors["refined2"].inverse_2d(h64, v64, x).cpu().numpy()
+    oh = oracle.build(make_request(1920, dw, kernel=kernel, taps=taps, blur=0.9, f64_mode=F64_ALWAYS))
+    ov = oracle.build(make_request(1080, dh_, kernel=kernel, taps=taps, blur=0.9, f64_mode=F64_ALWAYS))
+    exact = oracle.inverse_2d_f64_exact(oh, ov, frame.astype(np.float64))
+    scale = float(np.abs(exact).max())
+    ulp = float(np.spacing(np.float32(scale)))
+    err = {k: float(np.abs(v.astype(np.float64) - exact).max())
+           for k, v in (("fp32", plain), ("fp32+refine1", ref1), ("fp32+refine2", ref2), ("f64", f64))}
+    print(f"{name}: scale {scale:.3f} rcond h {h64.normal_rcond:.3g} v {v64.normal_rcond:.3g} "
+          + " ".join(f"{k} {v:.3g}" for k, v in err.items()), flush=True)
+    assert err["f64"] <= 2 * ulp
+    assert err["fp32+refine1"] <= 2e-6
+    assert err["fp32+refine1"] <= err["fp32"] + ulp
+    assert err["fp32+refine2"] <= err["fp32+refine1"] + ulp

@@ -46,11 +46,12 @@ void WindowTables::clear() { groups_.clear(); }
 
 void WindowTables::add(const hip::WinJob &job, int band, bool rows) {
     const int width = job.plan.ell_width;
+    const bool block16 = job.plan.block16 != 0;
     auto it = std::find_if(groups_.begin(), groups_.end(), [&](const Group &g) {
-        return g.rows == rows && g.band == band && g.width == width;
+        return g.rows == rows && g.band == band && g.width == width && g.block16 == block16;
     });
     if (it == groups_.end()) {
-        groups_.push_back(Group{band, width, rows, {}, 0, 0});
+        groups_.push_back(Group{band, width, rows, block16, {}, 0, 0});
         it = groups_.end() - 1;
     }
     hip::WinJob j = job;
@@ -97,7 +98,7 @@ void WindowTables::launch(hipStream_t stream) const {
     for (const Group &g : groups_) {
         if (g.items == 0) continue;
         check(hip::launch_window_table(device_jobs_ + g.offset, static_cast<int>(g.jobs.size()),
-                                       g.items, g.band, g.width, g.rows, stream),
+                                       g.items, g.band, g.width, g.rows, g.block16, stream),
               "table launch failed");
     }
 }

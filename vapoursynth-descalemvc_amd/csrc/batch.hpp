@@ -49,7 +49,8 @@ struct BatchStats {
     bool graph = false;                // table launches replayed from a HIP graph
 };
 
-// Groups windowed axis jobs by (direction, band class, tap width) into one
+// Groups windowed axis jobs by (direction, band class, tap width, staging
+// block size) into one
 // device buffer of job tables and launches them: all horizontal groups
 // first, then all vertical groups, on one stream.  Reusable: `build` lays
 // out a new set of jobs (growing the device buffer), `launch` enqueues.
@@ -72,6 +73,7 @@ private:
     struct Group {
         int band = 0, width = 0;
         bool rows = false;
+        bool block16 = false;       // 16-row staging blocks
         std::vector<hip::WinJob> jobs;
         std::size_t offset = 0;     // first job's index in the device buffer
         long items = 0;

@@ -235,7 +235,7 @@ int dsmvc_hip_device_count(int32_t *count) {
 
 int dsmvc_hip_executor_create(int32_t device, int32_t solve_mode, dsmvc_hip_executor **executor) {
     return guarded([&] {
-        require(executor && solve_mode >= 0 && solve_mode <= 3, "dsmvc: invalid executor arguments");
+        require(executor && solve_mode >= 0 && solve_mode <= 4, "dsmvc: invalid executor arguments");
         if (!dsmvc::hip_available()) throw Unavailable("dsmvc: backend hip is unavailable: no HIP device");
         auto handle = std::make_unique<dsmvc_hip_executor>();
         handle->executor = std::make_unique<dsmvc::HipExecutor>(
@@ -255,6 +255,10 @@ int dsmvc_hip_executor_prepare(dsmvc_hip_executor *executor, const dsmvc_hip_pla
 
 int dsmvc_hip_executor_seal(dsmvc_hip_executor *executor) {
     return guarded([&] { exec_of(executor).seal(); });
+}
+
+int dsmvc_hip_executor_set_refinement(dsmvc_hip_executor *executor, int32_t steps) {
+    return guarded([&] { exec_of(executor).set_refinement(steps); });
 }
 
 int dsmvc_hip_executor_tune(dsmvc_hip_executor *executor, int32_t rows_chunks,

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <stdexcept>
@@ -231,27 +232,63 @@ bool build_window_plan(const AxisPlan &p, WindowPlan &out) {
         }
     }
 
-    // Staging windows of 8-row blocks; blocks of zero rows reuse a
+    // Staging windows of 8- and 16-row blocks; blocks of zero rows reuse a
     // neighbouring window (their taps carry weight 0).
-    const int nb = rows / kWindowBlock;
-    out.blk_lo.assign(static_cast<std::size_t>(nb), 0);
-    out.rows_feasible = true;
-    int previous = 0;
-    for (int b = 0; b < nb; ++b) {
-        int lo = std::numeric_limits<int>::max(), hi = -1;
-        for (int t = 0; t < kWindowBlock; ++t) {
-            const int i = real(b * kWindowBlock + t);
-            if (i < 0 || i >= n) continue;
+    auto staging = [&](int g, std::vector<std::int32_t> &table) {
+        const int nb = rows / g;
+        table.assign(static_cast<std::size_t>(nb), 0);
+        bool fits = true;
+        int previous = 0;
+        for (int b = 0; b < nb; ++b) {
+            int lo = std::numeric_limits<int>::max(), hi = -1;
+            for (int t = 0; t < g; ++t) {
+                const int i = real(b * g + t);
+                if (i < 0 || i >= n) continue;
+                const auto k0 = p.transpose_offsets[static_cast<std::size_t>(i)];
+                const auto k1 = p.transpose_offsets[static_cast<std::size_t>(i) + 1];
+                if (k1 == k0) continue;
+                lo = std::min(lo, p.transpose_indices[k0]);
+                hi = std::max(hi, p.transpose_indices[k1 - 1]);
+            }
+            if (hi < 0) lo = previous;
+            else if (hi - lo >= kStageCols) fits = false;
+            table[static_cast<std::size_t>(b)] = lo;
+            previous = lo;
+        }
+        return fits;
+    };
+    out.rows_feasible = staging(kWindowBlock, out.blk_lo);
+    out.block16 = staging(2 * kWindowBlock, out.blk_lo16);
+
+    // Contiguous tap runs for the horizontal kernel: row i reads columns
+    // [s, s + K) of its staging window, s = max(lo8, last - K + 1), with its
+    // CSR weights at their columns and 0 elsewhere.  The run must hold the
+    // whole row and lie inside the 8-row window (and the 16-row window when
+    // those are used); otherwise the horizontal pass is not windowed.
+    out.tap_start.assign(ur, 0);
+    out.tap_w.assign(ur * width, 0.0F);
+    for (int r = 0; r < rows; ++r) {
+        const int i = real(r);
+        const int lo8 = out.blk_lo[static_cast<std::size_t>(r / kWindowBlock)];
+        const int lo16 = out.blk_lo16[static_cast<std::size_t>(r / (2 * kWindowBlock))];
+        int start = lo8;
+        if (i >= 0 && i < n) {
             const auto k0 = p.transpose_offsets[static_cast<std::size_t>(i)];
             const auto k1 = p.transpose_offsets[static_cast<std::size_t>(i) + 1];
-            if (k1 == k0) continue;
-            lo = std::min(lo, p.transpose_indices[k0]);
-            hi = std::max(hi, p.transpose_indices[k1 - 1]);
+            if (k1 > k0) {
+                const int first = p.transpose_indices[k0], last = p.transpose_indices[k1 - 1];
+                start = std::max(lo8, last - width + 1);
+                if (last - first + 1 > width) out.rows_feasible = false;
+                float *wt = out.tap_w.data() + static_cast<std::size_t>(r) * width;
+                for (auto q = k0; q < k1; ++q) {
+                    const int e = p.transpose_indices[q] - start;
+                    if (e >= 0 && e < width) wt[e] = p.transpose_weights[q];
+                }
+            }
         }
-        if (hi < 0) lo = previous;
-        else if (hi - lo >= kStageCols) out.rows_feasible = false;
-        out.blk_lo[static_cast<std::size_t>(b)] = lo;
-        previous = lo;
+        if (start + width > lo8 + kStageCols) out.rows_feasible = false;
+        if (start < lo16 || start + width > lo16 + kStageCols) out.block16 = false;
+        out.tap_start[static_cast<std::size_t>(r)] = start;
     }
 
     out.ell_idx.assign(ur * width, 0);
@@ -323,7 +360,8 @@ bool build_window_plan(const AxisPlan &p, WindowPlan &out) {
         }
     }
     out.residual = worst;
-    return worst <= kWindowTolerance;
+    // Both passes stage 32 source samples per block, so the runs must fit.
+    return out.rows_feasible && worst <= kWindowTolerance;
 }
 
 DevicePlan::DevicePlan(const AxisPlan &plan, int device)
@@ -370,6 +408,7 @@ DevicePlan::~DevicePlan() {
     for (auto &entry : chunk_images_)
         if (entry.second.memory) (void)hipFree(entry.second.memory);
     if (window_image_.memory) (void)hipFree(window_image_.memory);
+    if (refine_memory_) (void)hipFree(refine_memory_);
     if (memory_) (void)hipFree(memory_);
 }
 
@@ -433,12 +472,18 @@ const DevWindow *DevicePlan::window(bool rows) {
             const std::size_t o_idx = align256(o_coef + wp.coef.size() * 4);
             const std::size_t o_w = align256(o_idx + wp.ell_idx.size() * 4);
             const std::size_t o_blk = align256(o_w + wp.ell_w.size() * 4);
-            const std::size_t total = align256(o_blk + wp.blk_lo.size() * 4);
+            const std::size_t o_blk16 = align256(o_blk + wp.blk_lo.size() * 4);
+            const std::size_t o_ts = align256(o_blk16 + wp.blk_lo16.size() * 4);
+            const std::size_t o_tw = align256(o_ts + wp.tap_start.size() * 4);
+            const std::size_t total = align256(o_tw + wp.tap_w.size() * 4);
             std::vector<unsigned char> host(total, 0);
             std::memcpy(host.data() + o_coef, wp.coef.data(), wp.coef.size() * 4);
             std::memcpy(host.data() + o_idx, wp.ell_idx.data(), wp.ell_idx.size() * 4);
             std::memcpy(host.data() + o_w, wp.ell_w.data(), wp.ell_w.size() * 4);
             std::memcpy(host.data() + o_blk, wp.blk_lo.data(), wp.blk_lo.size() * 4);
+            std::memcpy(host.data() + o_blk16, wp.blk_lo16.data(), wp.blk_lo16.size() * 4);
+            std::memcpy(host.data() + o_ts, wp.tap_start.data(), wp.tap_start.size() * 4);
+            std::memcpy(host.data() + o_tw, wp.tap_w.data(), wp.tap_w.size() * 4);
             int current = 0;
             check(hipGetDevice(&current), "hipGetDevice failed");
             check(hipSetDevice(device_), "hipSetDevice failed");
@@ -457,12 +502,70 @@ const DevWindow *DevicePlan::window(bool rows) {
             image.dev.ell_idx = reinterpret_cast<const std::int32_t *>(b + o_idx);
             image.dev.ell_w = reinterpret_cast<const float *>(b + o_w);
             image.dev.blk_lo = reinterpret_cast<const std::int32_t *>(b + o_blk);
+            image.dev.blk_lo16 = reinterpret_cast<const std::int32_t *>(b + o_blk16);
+            image.dev.block16 = wp.block16 ? 1 : 0;
+            image.dev.tap_start = reinterpret_cast<const std::int32_t *>(b + o_ts);
+            image.dev.tap_w = reinterpret_cast<const float *>(b + o_tw);
             image.ok = true;
             image.rows_feasible = wp.rows_feasible;
         }
     }
     if (!image.ok || (rows && !image.rows_feasible)) return nullptr;
     return &image.dev;
+}
+
+const DevRefine *DevicePlan::refine() {
+    std::lock_guard lock(mutex_);
+    if (!refine_built_) {
+        refine_built_ = true;
+        const AxisPlan &p = *plan_;
+        if (!p.requires_float64()) return nullptr;
+        const int n = p.destination_size, w = p.half_bandwidth, span = 2 * w + 1;
+        const std::size_t nnz = p.transpose_weights_f64.size();
+        const auto un = static_cast<std::size_t>(n);
+        std::vector<float> w_hi(nnz), w_lo(nnz), g_hi(un * span, 0.0F), g_lo(un * span, 0.0F);
+        auto split = [](double v, float &hi, float &lo) {
+            hi = static_cast<float>(v);
+            lo = static_cast<float>(v - static_cast<double>(hi));
+        };
+        for (std::size_t q = 0; q < nnz; ++q) split(p.transpose_weights_f64[q], w_hi[q], w_lo[q]);
+        // normal_bands_f64[d * n + i] = G(i, i + d) (reference layout).
+        for (int i = 0; i < n; ++i)
+            for (int d = -w; d <= w; ++d) {
+                const int j = i + d;
+                if (j < 0 || j >= n) continue;
+                const int lo_index = std::min(i, j);
+                const double g = p.normal_bands_f64[static_cast<std::size_t>(std::abs(d)) * un
+                                                    + static_cast<std::size_t>(lo_index)];
+                const std::size_t at = static_cast<std::size_t>(i) * span + static_cast<std::size_t>(d + w);
+                split(g, g_hi[at], g_lo[at]);
+            }
+        const std::size_t o_wh = 0;
+        const std::size_t o_wl = align256(o_wh + std::max<std::size_t>(nnz, 1) * 4);
+        const std::size_t o_gh = align256(o_wl + std::max<std::size_t>(nnz, 1) * 4);
+        const std::size_t o_gl = align256(o_gh + g_hi.size() * 4);
+        const std::size_t total = align256(o_gl + g_lo.size() * 4);
+        std::vector<unsigned char> host(total, 0);
+        std::memcpy(host.data() + o_wh, w_hi.data(), nnz * 4);
+        std::memcpy(host.data() + o_wl, w_lo.data(), nnz * 4);
+        std::memcpy(host.data() + o_gh, g_hi.data(), g_hi.size() * 4);
+        std::memcpy(host.data() + o_gl, g_lo.data(), g_lo.size() * 4);
+        int current = 0;
+        check(hipGetDevice(&current), "hipGetDevice failed");
+        check(hipSetDevice(device_), "hipSetDevice failed");
+        check(hipMalloc(&refine_memory_, total), "refinement table allocation failed");
+        check(hipMemcpy(refine_memory_, host.data(), total, hipMemcpyHostToDevice),
+              "refinement table upload failed");
+        (void)hipSetDevice(current);
+        auto *b = static_cast<unsigned char *>(refine_memory_);
+        refine_.n = n;
+        refine_.w = w;
+        refine_.w_hi = reinterpret_cast<const float *>(b + o_wh);
+        refine_.w_lo = reinterpret_cast<const float *>(b + o_wl);
+        refine_.g_hi = reinterpret_cast<const float *>(b + o_gh);
+        refine_.g_lo = reinterpret_cast<const float *>(b + o_gl);
+    }
+    return refine_memory_ ? &refine_ : nullptr;
 }
 
 } // namespace dsmvc::hip

@@ -63,12 +63,17 @@ struct WindowPlan {
     std::vector<std::int32_t> ell_idx;
     std::vector<float> ell_w;
     std::vector<std::int32_t> blk_lo; // [rows / 8]
+    std::vector<std::int32_t> blk_lo16; // [rows / 16]
+    bool block16 = false;            // every 16-row block's taps fit 32 columns
+    std::vector<std::int32_t> tap_start; // [rows] contiguous tap runs (horizontal kernel)
+    std::vector<float> tap_w;            // [rows][K]
 };
 // Largest influence (relative) of a wrong starting state allowed at a
 // windowed chunk's first forward / last backward row: 2^-28.
 constexpr double kWindowTolerance = 1.0 / 268435456.0;
-// False when the plan cannot use the windowed kernels: band above 7, taps
-// above 16, or a warm-up longer than the band class's P would be needed.
+// False when the plan cannot use the windowed kernels: band above 7, a tap
+// run longer than 16 or outside its 32-sample staging window, or a warm-up
+// longer than the band class's P would be needed.
 bool build_window_plan(const AxisPlan &plan, WindowPlan &out);
 
 // One device allocation holding a plan's image (and its chunk partitions).
@@ -88,6 +93,9 @@ public:
     // Windowed partition on the device; nullptr when the plan cannot use the
     // windowed kernels in that direction.  Built lazily, cached.
     const DevWindow *window(bool rows);
+    // Refinement tables (FP64 weights and normal matrix split into F32
+    // pairs); nullptr for a plan without FP64 data.  Built lazily, cached.
+    const DevRefine *refine();
     [[nodiscard]] std::size_t device_bytes() const { return bytes_; }
 
 private:
@@ -113,6 +121,9 @@ private:
         bool rows_feasible = false;
     };
     WindowImage window_image_;
+    DevRefine refine_{};
+    void *refine_memory_ = nullptr;
+    bool refine_built_ = false;
 };
 
 } // namespace dsmvc::hip

@@ -82,6 +82,12 @@ constexpr int kStageCols = 32;     // source columns staged per block
 //   ell_idx / ell_w [rows][K] taps, front-padded with weight 0
 //   blk_lo   [rows / 8]  horizontal kernel: first staged source column of
 //                        each 8-row block (its taps lie in [lo, lo + 32))
+//   blk_lo16 [rows / 16] the same for 16-row blocks, valid when block16 = 1
+//                        (every 16-row block's taps fit 32 columns)
+//   tap_start / tap_w    horizontal kernel: each row's taps as one run of K
+//                        consecutive source columns [start, start + K) with
+//                        weight 0 where the row has no tap, inside its
+//                        staging window (both block sizes)
 struct DevWindow {
     std::int32_t n;
     std::int32_t m;
@@ -89,10 +95,15 @@ struct DevWindow {
     std::int32_t warmup;      // P
     std::int32_t chunks;
     std::int32_t ell_width;   // K
+    std::int32_t block16;
+    std::int32_t pad_;
     const float *coef;
     const std::int32_t *ell_idx;
     const float *ell_w;
     const std::int32_t *blk_lo;
+    const std::int32_t *blk_lo16;
+    const std::int32_t *tap_start;   // [rows]
+    const float *tap_w;              // [rows][K]
 };
 
 constexpr int kWindowBlock = 8;    // padded rows per staging block (rows kernel)
@@ -125,6 +136,19 @@ struct WinJob {
     std::int32_t pad_;
     long first_item;          // items of the jobs before this one in its table
     DevConv conv;
+};
+
+// Refinement tables of a plan that retains FP64 data: the FP64 weights of
+// A^T (same CSR as DevAxis) and the FP64 normal matrix G = A^T A as rows of
+// 2w+1 entries G(i, i+d), d = -w..w (zero outside the matrix), each split
+// into a high F32 part and the F32 rounding of the remainder.
+struct DevRefine {
+    std::int32_t n;
+    std::int32_t w;
+    const float *w_hi;
+    const float *w_lo;
+    const float *g_hi;     // [n][2w+1]
+    const float *g_lo;
 };
 
 // Windowed-kernel shape per band class: chunk length L and warm-up P.

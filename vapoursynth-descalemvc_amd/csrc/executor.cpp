@@ -280,8 +280,44 @@ int HipExecutor::tuned_chunks(const AxisPlan &plan, bool rows, int kind) {
 
 // One axis pass over `vectors` rows (rows = true) or columns of `frames`
 // planes.  Picks the chunked kernels when the mode and plan allow, else the
-// sequential solve, whose forward sweep goes to scratch at `fwd_offset`.
+// sequential solve, whose forward sweep goes to scratch at `fwd_offset`.  In
+// refined mode, a plan that retains FP64 data is solved on the F32 route and
+// then refined in place (the scratch at `fwd_offset` holds the residuals).
 void HipExecutor::axis_pass(const AxisPlan &plan, bool rows, const void *input, Sample in_kind,
+                            std::ptrdiff_t in_row, std::ptrdiff_t in_frame, void *output,
+                            Sample out_kind, std::ptrdiff_t out_row, std::ptrdiff_t out_frame,
+                            std::int32_t vectors, std::int32_t frames, const DevConv &conv,
+                            bool f64, Scratch &s, std::size_t fwd_offset, hipStream_t stream) {
+    solve_pass(plan, rows, input, in_kind, in_row, in_frame, output, out_kind, out_row, out_frame,
+               vectors, frames, conv, f64, s, fwd_offset, stream);
+    if (f64 || mode_ != SolveMode::refined || !plan.requires_float64()) return;
+    if (in_kind != Sample::f32 || out_kind != Sample::f32)
+        throw std::invalid_argument("dsmvc: refined solves need Float32 planes");
+    hip::DevicePlan &dp = device_plan(plan);
+    const hip::DevRefine *rt = dp.refine();
+    if (!rt) throw std::runtime_error("dsmvc: refinement tables are missing");
+    hip::RefineArgs a{};
+    a.axis = dp.axis();
+    a.refine = *rt;
+    a.y = static_cast<const float *>(input);
+    a.x = static_cast<float *>(output);
+    if (rows) {
+        a.y_vec = in_row; a.y_el = 1;
+        a.x_vec = out_row; a.x_el = 1;
+    } else {
+        a.y_vec = 1; a.y_el = in_row;
+        a.x_vec = 1; a.x_el = out_row;
+    }
+    a.y_frame = in_frame;
+    a.x_frame = out_frame;
+    a.scratch = reinterpret_cast<float *>(static_cast<unsigned char *>(s.buffer) + fwd_offset);
+    a.nvec = vectors;
+    a.frames = frames;
+    for (int step = 0; step < refine_steps_; ++step)
+        check(hip::launch_refine_step(a, stream), "refinement launch failed");
+}
+
+void HipExecutor::solve_pass(const AxisPlan &plan, bool rows, const void *input, Sample in_kind,
                             std::ptrdiff_t in_row, std::ptrdiff_t in_frame, void *output,
                             Sample out_kind, std::ptrdiff_t out_row, std::ptrdiff_t out_frame,
                             std::int32_t vectors, std::int32_t frames, const DevConv &conv,
@@ -366,7 +402,7 @@ void HipExecutor::inverse_rows(const AxisPlan &plan, const float *input,
     if (row_count == 0) return;
     std::lock_guard lock(mutex_);
     check(hipSetDevice(device_), "hipSetDevice failed");
-    const bool f64 = plan.requires_float64();
+    const bool f64 = float64_route(plan);
     const std::size_t elem = f64 ? 8 : 4;
     Scratch &s = scratch(stream, align256(static_cast<std::size_t>(row_count) * plan.destination_size * elem));
     const DevConv conv = make_conv(nullptr);
@@ -391,7 +427,7 @@ void HipExecutor::inverse_columns(const AxisPlan &plan, const float *input,
     if (column_count == 0) return;
     std::lock_guard lock(mutex_);
     check(hipSetDevice(device_), "hipSetDevice failed");
-    const bool f64 = plan.requires_float64();
+    const bool f64 = float64_route(plan);
     const std::size_t elem = f64 ? 8 : 4;
     Scratch &s = scratch(stream, align256(static_cast<std::size_t>(column_count) * plan.destination_size * elem));
     const DevConv conv = make_conv(nullptr);
@@ -424,7 +460,7 @@ void HipExecutor::inverse_2d_batch(const AxisPlan &h, const AxisPlan &v, SampleK
     }
     std::lock_guard lock(mutex_);
     check(hipSetDevice(device_), "hipSetDevice failed");
-    const bool f64 = h.requires_float64() || v.requires_float64();
+    const bool f64 = float64_route(h) || float64_route(v);
     const std::size_t elem = f64 ? 8 : 4;
     const std::ptrdiff_t inter_row = padded_width(h.destination_size);
     const std::ptrdiff_t inter_frame = inter_row * v.source_size;
@@ -464,7 +500,7 @@ void HipExecutor::inverse_rows_batch(const AxisPlan &plan, SampleKind in_kind,
         || (b.frames > 1 && (b.input_frame_stride < b.input_row_stride * rows
                              || b.output_frame_stride < b.output_row_stride * rows)))
         throw std::invalid_argument("dsmvc: invalid row executor arguments");
-    if (plan.requires_float64())
+    if (float64_route(plan))
         throw std::invalid_argument("dsmvc: batched single-axis passes take Float32 plans; "
                                     "use the 2D entry for Float64 plans");
     if (in_kind != SampleKind::float32) {
@@ -491,7 +527,7 @@ void HipExecutor::inverse_columns_batch(const AxisPlan &plan, SampleKind out_kin
         || (b.frames > 1 && (b.input_frame_stride < b.input_row_stride * plan.source_size
                              || b.output_frame_stride < b.output_row_stride * plan.destination_size)))
         throw std::invalid_argument("dsmvc: invalid column executor arguments");
-    if (plan.requires_float64())
+    if (float64_route(plan))
         throw std::invalid_argument("dsmvc: batched single-axis passes take Float32 plans; "
                                     "use the 2D entry for Float64 plans");
     if (out_kind != SampleKind::float32) {

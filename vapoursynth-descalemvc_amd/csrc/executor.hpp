@@ -31,6 +31,8 @@ enum class SolveMode : std::uint8_t {
     exact = 1,       // sequential: bit-identical to the ordered CPU solver
     chunked = 2,     // carry-chain chunked kernels only; throws when the plan cannot use them
     windowed = 3,    // windowed kernels only; throws when the plan cannot use them
+    refined = 4,     // as automatic, but plans that retain FP64 data take the F32 route
+                     // plus refinement steps instead of the Float64 route
 };
 
 enum class SampleKind : std::uint8_t { float32 = 0, u8 = 1, u16 = 2 };
@@ -79,6 +81,11 @@ public:
     // so every later result of that key is bit-reproducible within it; the
     // partition affects the last bits of Float32 results (carry rounding).
     void set_autotune(bool enabled) { autotune_ = enabled; }
+    // Refinement steps per axis in refined mode (1..3).
+    void set_refinement(int steps) {
+        if (steps < 1 || steps > 3) throw std::invalid_argument("dsmvc: refinement steps must be 1..3");
+        refine_steps_ = steps;
+    }
     // The tuned route of a key: a chunk count, kWindowRoute, 0 before tuning,
     // -1 for an unknown plan.
     [[nodiscard]] int tuned_chunks(const AxisPlan &plan, bool rows, int kind);
@@ -140,6 +147,14 @@ private:
     dsmvc::hip::DevicePlan &device_plan(const AxisPlan &plan);
     Scratch &scratch(hipStream_t stream, std::size_t bytes);
     void finish(hipStream_t stream, Scratch &s, bool async, const char *what);
+    bool float64_route(const AxisPlan &plan) const {
+        return plan.requires_float64() && mode_ != SolveMode::refined;
+    }
+    void solve_pass(const AxisPlan &plan, bool rows, const void *input, hip::Sample in_kind,
+                    std::ptrdiff_t in_row, std::ptrdiff_t in_frame, void *output,
+                    hip::Sample out_kind, std::ptrdiff_t out_row, std::ptrdiff_t out_frame,
+                    std::int32_t vectors, std::int32_t frames, const hip::DevConv &conv, bool f64,
+                    Scratch &s, std::size_t fwd_offset, hipStream_t stream);
     void axis_pass(const AxisPlan &plan, bool rows, const void *input, hip::Sample in_kind,
                    std::ptrdiff_t in_row, std::ptrdiff_t in_frame, void *output,
                    hip::Sample out_kind, std::ptrdiff_t out_row, std::ptrdiff_t out_frame,
@@ -171,6 +186,7 @@ private:
     SolveMode mode_;
     LaunchTuning tuning_{};
     bool autotune_ = true;
+    int refine_steps_ = 1;
     int compute_units_ = 256;
     std::unordered_map<TuneKey, int, TuneKeyHash> tuned_;
     std::mutex mutex_;

@@ -62,23 +62,23 @@ extern template hipError_t launch_chunked_band<7>(const ChunkArgs &, bool, Sampl
 // (smaller bands run in the next class with zero coefficients); the tap
 // width 8 or 16.  `launch_window` runs one job; `launch_window_table` runs
 // `count` jobs from a device-resident table (`items` items in all) in one
-// launch, all jobs sharing the band class, tap width and direction, with F32
-// samples.
+// launch, all jobs sharing the band class, tap width, direction and (rows)
+// staging block size, with F32 samples.
 int window_band_class(int w);   // 1, 3, 5, 7, or 0 when unsupported
 constexpr int kWindowWaves = 4;   // wavefronts (one item each) per workgroup
 hipError_t launch_window(const WinJob &job, int band, bool rows, Sample kind, hipStream_t s);
 hipError_t launch_window_table(const WinJob *jobs, int count, long items, int band, int width,
-                               bool rows, hipStream_t s);
+                               bool rows, bool block16, hipStream_t s);
 template <int W, int K>
 hipError_t launch_window_band(const WinJob &job, bool rows, Sample kind, hipStream_t s);
 template <int W, int K>
 hipError_t launch_window_table_band(const WinJob *jobs, int count, long items, bool rows,
-                                    hipStream_t s);
+                                    bool block16, hipStream_t s);
 #define DSMVC_WINDOW_EXTERN(W, K)                                                                 \
     extern template hipError_t launch_window_band<W, K>(const WinJob &, bool, Sample,             \
                                                         hipStream_t);                             \
     extern template hipError_t launch_window_table_band<W, K>(const WinJob *, int, long, bool,    \
-                                                              hipStream_t);
+                                                              bool, hipStream_t);
 DSMVC_WINDOW_EXTERN(1, 8)
 DSMVC_WINDOW_EXTERN(1, 16)
 DSMVC_WINDOW_EXTERN(3, 8)
@@ -88,6 +88,27 @@ DSMVC_WINDOW_EXTERN(5, 16)
 DSMVC_WINDOW_EXTERN(7, 8)
 DSMVC_WINDOW_EXTERN(7, 16)
 #undef DSMVC_WINDOW_EXTERN
+
+// Refinement of an F32 axis solve x (in place): `steps` times,
+//   r = A^T y - G x   in compensated F32 (two-product and two-sum, with the
+//                     FP64 weights and normal matrix split into F32 pairs),
+//   solve G d = r     with the plan's F32 factors,
+//   x += d.
+// y: source vectors (element e of vector v of frame f at
+// y + f*y_frame + v*y_vec + e*y_el), x likewise; `scratch` holds
+// nvec * n * frames floats.
+struct RefineArgs {
+    DevAxis axis;
+    DevRefine refine;
+    const float *y;
+    long y_vec, y_el, y_frame;
+    float *x;
+    long x_vec, x_el, x_frame;
+    float *scratch;
+    int nvec;
+    int frames;
+};
+hipError_t launch_refine_step(const RefineArgs &a, hipStream_t s);
 
 hipError_t launch_check_finite(const void *src, Sample in, long s_row, long s_frame, int rows,
                                int cols, int frames, int *flag, hipStream_t s);

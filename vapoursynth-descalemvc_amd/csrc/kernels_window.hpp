@@ -14,12 +14,16 @@
 // wavefronts of a workgroup take consecutive items, which are consecutive
 // vector groups of the same chunk, so they read the same plan rows.
 //
-// Vertical pass (columns_window): lanes are 64 adjacent columns; taps are
-// buffer loads of whole source rows at a scalar row offset.
-// Horizontal pass (rows_window): lanes are 64 rows; each 8-row block stages
-// the 32 source columns its taps need through the wavefront's LDS ring
-// (transposed, pitch 65) and results leave through the same ring.
+// Vertical pass (columns_window): lanes are 64 adjacent columns; each block
+// of G rows stages the 32 source rows its taps need (256-byte runs) into the
+// wavefront's LDS ring, the next block's loads in flight meanwhile.
+// Horizontal pass (rows_window): lanes are 64 rows; each block of G (16 when
+// the plan's windows allow it, else 8) rows stages the 32 source columns its
+// taps need through the wavefront's LDS ring (transposed, pitch 65), the next
+// block's loads in flight meanwhile, and results leave through the same ring.
 #pragma once
+
+#include <type_traits>
 
 #include "kernels_common.hpp"
 
@@ -27,7 +31,11 @@ namespace dsmvc::hip {
 namespace dev {
 namespace {
 
-constexpr int kWinRing = kStageCols * kRingPitch;   // floats of LDS per wavefront
+// Per-wavefront LDS ring of the horizontal kernel: 64 rows (one per lane) of
+// 32 staged source columns, row pitch 33 floats so that lanes reading the
+// same column, and a row's 32 writers, hit 32 different banks.
+constexpr int kRowPitch = kStageCols + 1;
+constexpr int kWinRing = 64 * kRowPitch;
 
 struct ItemPos {
     int frame, chunk, group;
@@ -43,12 +51,13 @@ __device__ __forceinline__ ItemPos decode_item(const WinJob &jb, int local) {
     return p;
 }
 
-template <int W, int K, class In>
+template <int W, int K, int G, class In>
 __device__ __forceinline__ void rows_item(const WinJob &jb, const ItemPos &at, float *ring,
                                           int lane) {
     constexpr int L = window_length(W);
     constexpr int P = window_warmup(W);
-    constexpr int NB = (L + 2 * P) / kWindowBlock;
+    constexpr int NB = (L + 2 * P) / G;
+    static_assert((L + 2 * P) % G == 0 && L % G == 0, "blocks tile the window");
     const DevWindow &pl = jb.plan;
     const int nrows = jb.nvec;
     const int row0 = at.group * 64;
@@ -58,11 +67,30 @@ __device__ __forceinline__ void rows_item(const WinJob &jb, const ItemPos &at, f
     const int m1 = pl.m - 1;
     const int scol = lane & (kStageCols - 1);
     const int srow = lane >> 5;
-    const auto ei = global(pl.ell_idx + static_cast<long>(r0) * K);
-    const auto ew = global(pl.ell_w + static_cast<long>(r0) * K);
-    const auto cf = global(pl.coef + static_cast<long>(r0) * Band<W>::kStride);
-    const auto blk = global(pl.blk_lo + r0 / kWindowBlock);
+    // Plan tables through the constant address space: the loads stay scalar
+    // across the inline-asm pins below (which count as memory side effects).
+    const auto ts = constant(pl.tap_start + r0);
+    const auto tw = constant(pl.tap_w + static_cast<long>(r0) * K);
+    const auto cf = constant(pl.coef + static_cast<long>(r0) * Band<W>::kStride);
+    const auto blk = constant((G == 16 ? pl.blk_lo16 : pl.blk_lo) + r0 / G);
     const DevConv cv = jb.conv;
+    const float *mine = ring + lane * kRowPitch;   // this lane's row of the ring
+
+    // The next block's 32 source columns are loaded into registers while the
+    // current block is solved, so the loads' latency overlaps the solve.
+    // Held as 32-bit words so they can be pinned in registers below.
+    using Word = std::conditional_t<std::is_same_v<In, float>, float, std::uint32_t>;
+    Word staged[32];
+    auto fetch = [&](int b) {
+        const int column = min(blk[b] + scol, m1);
+#pragma unroll
+        for (int t = 0; t < 32; ++t) {
+            const int row = min(row0 + srow + 2 * t, nrows - 1);
+            staged[t] = static_cast<Word>(BufferLoad<In>::at(
+                rs, (row * static_cast<int>(jb.s_row) + column) * static_cast<int>(sizeof(In)), 0));
+        }
+    };
+    fetch(0);
 
     float st[W];
 #pragma unroll
@@ -71,23 +99,29 @@ __device__ __forceinline__ void rows_item(const WinJob &jb, const ItemPos &at, f
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
         const int lo = blk[b];
-        const int column = min(lo + scol, m1);
         wave_sync();
 #pragma unroll
         for (int t = 0; t < 32; ++t) {
-            const int rr = srow + 2 * t;
-            const int row = min(row0 + rr, nrows - 1);
-            const int voff = (row * static_cast<int>(jb.s_row) + column) * static_cast<int>(sizeof(In));
-            ring[scol * kRingPitch + rr] = Load<In>::norm(BufferLoad<In>::at(rs, voff, 0), cv);
+            // Pinned here, so the conversion (and the wait for the load)
+            // stays at this point instead of moving up to the fetch.
+            asm volatile("" : "+v"(staged[t]));
+            ring[(srow + 2 * t) * kRowPitch + scol] = Load<In>::norm(static_cast<In>(staged[t]), cv);
         }
         wave_sync();
+        if (b + 1 < NB) fetch(b + 1);
 #pragma unroll
-        for (int t = 0; t < kWindowBlock; ++t) {
-            const int k = b * kWindowBlock + t;
+        for (int t = 0; t < G; ++t) {
+            const int k = b * G + t;
+            // The row's K taps are consecutive columns of this lane's ring
+            // row: one address, K immediate offsets, all reads in flight
+            // before the FMA chain (ascending column = CSR order).
+            const float *at = mine + (ts[k] - lo);
+            float tap[K];
+#pragma unroll
+            for (int e = 0; e < K; ++e) tap[e] = at[e];
             float v = 0.0F;
 #pragma unroll
-            for (int e = 0; e < K; ++e)
-                v = __builtin_fmaf(ew[k * K + e], ring[(ei[k * K + e] - lo) * kRingPitch + lane], v);
+            for (int e = 0; e < K; ++e) v = __builtin_fmaf(tw[k * K + e], tap[e], v);
             const float yv = forward_step<W>(cf + k * Band<W>::kStride, v, st);
             if (k >= P) y[k - P] = yv;
         }
@@ -109,7 +143,7 @@ __device__ __forceinline__ void rows_item(const WinJob &jb, const ItemPos &at, f
         wave_sync();
 #pragma unroll
         for (int c = 0; c < 32; ++c)
-            if (kb + c < L) ring[c * kRingPitch + lane] = y[kb + c];
+            if (kb + c < L) ring[lane * kRowPitch + c] = y[kb + c];
         wave_sync();
         const bool col_ok = kb + oc < L && r0 + kb + oc < n;
 #pragma unroll 4
@@ -117,16 +151,19 @@ __device__ __forceinline__ void rows_item(const WinJob &jb, const ItemPos &at, f
             const int rr = t + orow;
             const int row = row0 + rr;
             if (col_ok && row < nrows)
-                out[static_cast<long>(row) * jb.d_row + kb + oc] = ring[oc * kRingPitch + rr];
+                out[static_cast<long>(row) * jb.d_row + kb + oc] = ring[rr * kRowPitch + oc];
         }
     }
 }
 
-template <int W, int K, class Out>
-__device__ __forceinline__ void columns_item(const WinJob &jb, const ItemPos &at, int lane) {
+template <int W, int K, int G, class Out>
+__device__ __forceinline__ void columns_item(const WinJob &jb, const ItemPos &at, float *ring,
+                                             int lane) {
     constexpr int L = window_length(W);
     constexpr int P = window_warmup(W);
-    constexpr int R = L + 2 * P;
+    constexpr int NB = (L + 2 * P) / G;
+    static_assert((L + 2 * P) % G == 0 && L % G == 0, "blocks tile the window");
+    static_assert(kStageCols * 64 <= kWinRing, "ring holds a block");
     const DevWindow &pl = jb.plan;
     const int ncols = jb.nvec;
     const int col = at.group * 64 + lane;
@@ -137,22 +174,54 @@ __device__ __forceinline__ void columns_item(const WinJob &jb, const ItemPos &at
     const auto rs = plane_rsrc(src, static_cast<long>(pl.m) * jb.s_row * 4);
     const int voff = cc * 4;
     const int row_bytes = static_cast<int>(jb.s_row * 4);
-    const auto ei = global(pl.ell_idx + static_cast<long>(r0) * K);
-    const auto ew = global(pl.ell_w + static_cast<long>(r0) * K);
-    const auto cf = global(pl.coef + static_cast<long>(r0) * Band<W>::kStride);
+    const int m1 = pl.m - 1;
+    const auto ts = constant(pl.tap_start + r0);
+    const auto tw = constant(pl.tap_w + static_cast<long>(r0) * K);
+    const auto cf = constant(pl.coef + static_cast<long>(r0) * Band<W>::kStride);
+    const auto blk = constant((G == 16 ? pl.blk_lo16 : pl.blk_lo) + r0 / G);
+    const float *mine = ring + lane;   // this lane's column of the ring
+
+    // Each block stages the 32 source rows its taps need (one 256-byte run
+    // per row across the wavefront) into the ring, the next block's rows in
+    // flight meanwhile; the taps then read the ring, each source row loaded
+    // from memory once per block instead of once per tap.
+    float staged[32];
+    auto fetch = [&](int b) {
+        const int lo = blk[b];
+#pragma unroll
+        for (int t = 0; t < 32; ++t)
+            staged[t] = BufferLoad<float>::at(rs, voff, min(lo + t, m1) * row_bytes);
+    };
+    fetch(0);
 
     float st[W];
 #pragma unroll
     for (int d = 0; d < W; ++d) st[d] = 0.0F;
     float y[L + P];
 #pragma unroll
-    for (int k = 0; k < R; ++k) {
-        float v = 0.0F;
+    for (int b = 0; b < NB; ++b) {
+        const int lo = blk[b];
+        wave_sync();
 #pragma unroll
-        for (int e = 0; e < K; ++e)
-            v = __builtin_fmaf(ew[k * K + e], BufferLoad<float>::at(rs, voff, ei[k * K + e] * row_bytes), v);
-        const float yv = forward_step<W>(cf + k * Band<W>::kStride, v, st);
-        if (k >= P) y[k - P] = yv;
+        for (int t = 0; t < 32; ++t) {
+            asm volatile("" : "+v"(staged[t]));
+            ring[t * 64 + lane] = staged[t];
+        }
+        wave_sync();
+        if (b + 1 < NB) fetch(b + 1);
+#pragma unroll
+        for (int t = 0; t < G; ++t) {
+            const int k = b * G + t;
+            const float *row = mine + (ts[k] - lo) * 64;
+            float tap[K];
+#pragma unroll
+            for (int e = 0; e < K; ++e) tap[e] = row[e * 64];
+            float v = 0.0F;
+#pragma unroll
+            for (int e = 0; e < K; ++e) v = __builtin_fmaf(tw[k * K + e], tap[e], v);
+            const float yv = forward_step<W>(cf + k * Band<W>::kStride, v, st);
+            if (k >= P) y[k - P] = yv;
+        }
     }
 #pragma unroll
     for (int d = 0; d < W; ++d) st[d] = 0.0F;
@@ -169,23 +238,24 @@ __device__ __forceinline__ void columns_item(const WinJob &jb, const ItemPos &at
 }
 
 // One job: item = blockIdx.x * kWindowWaves + wave.
-template <int W, int K, class In>
+template <int W, int K, int G, class In>
 __global__ void __launch_bounds__(64 * kWindowWaves) rows_window(WinJob jb, int items) {
     __shared__ float lds[kWindowWaves * kWinRing];
     const int lane = threadIdx.x & 63;
     const int wave = uniform(threadIdx.x >> 6);
     const int item = static_cast<int>(blockIdx.x) * kWindowWaves + wave;
     if (item >= items) return;
-    rows_item<W, K, In>(jb, decode_item(jb, item), lds + wave * kWinRing, lane);
+    rows_item<W, K, G, In>(jb, decode_item(jb, item), lds + wave * kWinRing, lane);
 }
 
-template <int W, int K, class Out>
+template <int W, int K, int G, class Out>
 __global__ void __launch_bounds__(64 * kWindowWaves) columns_window(WinJob jb, int items) {
+    __shared__ float lds[kWindowWaves * kWinRing];
     const int lane = threadIdx.x & 63;
     const int wave = uniform(threadIdx.x >> 6);
     const int item = static_cast<int>(blockIdx.x) * kWindowWaves + wave;
     if (item >= items) return;
-    columns_item<W, K, Out>(jb, decode_item(jb, item), lane);
+    columns_item<W, K, G, Out>(jb, decode_item(jb, item), lds + wave * kWinRing, lane);
 }
 
 // A job record, read with scalar loads.
@@ -214,7 +284,7 @@ __device__ __forceinline__ int find_job(const WinJob *__restrict__ jobs, int cou
 // table; wave w of workgroup b takes item b * kWindowWaves + w and finds its
 // job by a binary search over the table.  Each wave solves one item and
 // exits, so the hardware dispatcher balances plans of uneven size.
-template <int W, int K, class In>
+template <int W, int K, int G, class In>
 __global__ void __launch_bounds__(64 * kWindowWaves)
 rows_window_table(const WinJob *__restrict__ jobs, int count, long items) {
     __shared__ float lds[kWindowWaves * kWinRing];
@@ -223,19 +293,21 @@ rows_window_table(const WinJob *__restrict__ jobs, int count, long items) {
     const long item = static_cast<long>(blockIdx.x) * kWindowWaves + wave;
     if (item >= items) return;
     const WinJob jb = load_job(jobs + find_job(jobs, count, item));
-    rows_item<W, K, In>(jb, decode_item(jb, static_cast<int>(item - jb.first_item)),
-                        lds + wave * kWinRing, lane);
+    rows_item<W, K, G, In>(jb, decode_item(jb, static_cast<int>(item - jb.first_item)),
+                           lds + wave * kWinRing, lane);
 }
 
-template <int W, int K, class Out>
+template <int W, int K, int G, class Out>
 __global__ void __launch_bounds__(64 * kWindowWaves)
 columns_window_table(const WinJob *__restrict__ jobs, int count, long items) {
+    __shared__ float lds[kWindowWaves * kWinRing];
     const int lane = threadIdx.x & 63;
     const int wave = uniform(threadIdx.x >> 6);
     const long item = static_cast<long>(blockIdx.x) * kWindowWaves + wave;
     if (item >= items) return;
     const WinJob jb = load_job(jobs + find_job(jobs, count, item));
-    columns_item<W, K, Out>(jb, decode_item(jb, static_cast<int>(item - jb.first_item)), lane);
+    columns_item<W, K, G, Out>(jb, decode_item(jb, static_cast<int>(item - jb.first_item)),
+                               lds + wave * kWinRing, lane);
 }
 
 inline long job_items(const WinJob &jb) {
@@ -248,23 +320,33 @@ hipError_t launch_one(const WinJob &jb, bool rows, hipStream_t s) {
     if (items <= 0) return hipSuccess;
     if (items > 0x7FFFFFFFL) return hipErrorInvalidValue;
     const dim3 grid(static_cast<unsigned>((items + kWindowWaves - 1) / kWindowWaves));
-    if (rows)
-        rows_window<W, K, T><<<grid, 64 * kWindowWaves, 0, s>>>(jb, static_cast<int>(items));
+    if (!rows && jb.plan.block16)
+        columns_window<W, K, 16, T><<<grid, 64 * kWindowWaves, 0, s>>>(jb, static_cast<int>(items));
+    else if (!rows)
+        columns_window<W, K, 8, T><<<grid, 64 * kWindowWaves, 0, s>>>(jb, static_cast<int>(items));
+    else if (jb.plan.block16)
+        rows_window<W, K, 16, T><<<grid, 64 * kWindowWaves, 0, s>>>(jb, static_cast<int>(items));
     else
-        columns_window<W, K, T><<<grid, 64 * kWindowWaves, 0, s>>>(jb, static_cast<int>(items));
+        rows_window<W, K, 8, T><<<grid, 64 * kWindowWaves, 0, s>>>(jb, static_cast<int>(items));
     return hipGetLastError();
 }
 
+// `block16`: every job of the table has 16-row staging blocks.
 template <int W, int K, class T>
-hipError_t launch_table(const WinJob *jobs, int count, long items, bool rows, hipStream_t s) {
+hipError_t launch_table(const WinJob *jobs, int count, long items, bool rows, bool block16,
+                        hipStream_t s) {
     if (items <= 0 || count <= 0) return hipSuccess;
     const long needed = (items + kWindowWaves - 1) / kWindowWaves;
     if (needed > 0x7FFFFFFFL) return hipErrorInvalidValue;
     const dim3 grid(static_cast<unsigned>(needed));
-    if (rows)
-        rows_window_table<W, K, T><<<grid, 64 * kWindowWaves, 0, s>>>(jobs, count, items);
+    if (!rows && block16)
+        columns_window_table<W, K, 16, T><<<grid, 64 * kWindowWaves, 0, s>>>(jobs, count, items);
+    else if (!rows)
+        columns_window_table<W, K, 8, T><<<grid, 64 * kWindowWaves, 0, s>>>(jobs, count, items);
+    else if (block16)
+        rows_window_table<W, K, 16, T><<<grid, 64 * kWindowWaves, 0, s>>>(jobs, count, items);
     else
-        columns_window_table<W, K, T><<<grid, 64 * kWindowWaves, 0, s>>>(jobs, count, items);
+        rows_window_table<W, K, 8, T><<<grid, 64 * kWindowWaves, 0, s>>>(jobs, count, items);
     return hipGetLastError();
 }
 
@@ -291,8 +373,8 @@ hipError_t launch_window_band(const WinJob &job, bool rows, Sample kind, hipStre
 // Multi-job launches run F32 samples only (the batched and served formats).
 template <int W, int K>
 hipError_t launch_window_table_band(const WinJob *jobs, int count, long items, bool rows,
-                                    hipStream_t s) {
-    return dev::launch_table<W, K, float>(jobs, count, items, rows, s);
+                                    bool block16, hipStream_t s) {
+    return dev::launch_table<W, K, float>(jobs, count, items, rows, block16, s);
 }
 
 } // namespace dsmvc::hip

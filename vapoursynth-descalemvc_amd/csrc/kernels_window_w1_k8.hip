@@ -3,5 +3,6 @@
 
 namespace dsmvc::hip {
 template hipError_t launch_window_band<1, 8>(const WinJob &, bool, Sample, hipStream_t);
-template hipError_t launch_window_table_band<1, 8>(const WinJob *, int, long, bool, hipStream_t);
+template hipError_t launch_window_table_band<1, 8>(const WinJob *, int, long, bool, bool,
+                                                        hipStream_t);
 } // namespace dsmvc::hip

@@ -38,7 +38,7 @@ ZERO, REPEAT, REFLECT101, SYMMETRIC, MIRROR = range(5)
 KERNELS = {"bilinear": BILINEAR, "bicubic": BICUBIC, "lanczos": LANCZOS,
            "spline16": SPLINE16, "spline36": SPLINE36, "spline64": SPLINE64}
 BACKENDS = {0: "auto", 1: "cpu", 2: "metal", 3: "vulkan", 4: "cuda", 5: "hip"}
-SOLVE_MODES = {"auto": 0, "exact": 1, "chunked": 2, "windowed": 3}
+SOLVE_MODES = {"auto": 0, "exact": 1, "chunked": 2, "windowed": 3, "refined": 4}
 SAMPLE_KINDS = {"float32": 0, "uint8": 1, "uint16": 2}
 
 
@@ -241,6 +241,10 @@ class Executor:
 
     def tune(self, rows_chunks: int = 0, columns_chunks: int = 0):
         check(load().dsmvc_hip_executor_tune(self._handle, rows_chunks, columns_chunks))
+
+    def set_refinement(self, steps: int):
+        """Refinement steps per axis for executors in "refined" mode."""
+        check(load().dsmvc_hip_executor_set_refinement(self._handle, steps))
 
     def synchronize(self, stream=None):
         check(load().dsmvc_hip_synchronize(self._handle, _stream_handle(stream)))

@@ -21,7 +21,8 @@ EXPORTS = (
     "dsmvc_hip_clear_planner_caches", "dsmvc_hip_padding_index", "dsmvc_hip_parse_backend",
     "dsmvc_hip_resolve_backend", "dsmvc_hip_device_count", "dsmvc_hip_executor_create",
     "dsmvc_hip_executor_destroy", "dsmvc_hip_executor_prepare", "dsmvc_hip_executor_seal",
-    "dsmvc_hip_executor_tune", "dsmvc_hip_inverse_rows", "dsmvc_hip_inverse_columns",
+    "dsmvc_hip_executor_tune", "dsmvc_hip_executor_set_refinement", "dsmvc_hip_inverse_rows",
+    "dsmvc_hip_inverse_columns",
     "dsmvc_hip_inverse_2d", "dsmvc_hip_inverse_2d_u8", "dsmvc_hip_inverse_2d_u16",
     "dsmvc_hip_inverse_2d_batch", "dsmvc_hip_inverse_rows_batch",
     "dsmvc_hip_inverse_columns_batch", "dsmvc_hip_batch_create", "dsmvc_hip_batch_run",
@@ -206,6 +207,7 @@ def load():
     lib.dsmvc_hip_executor_prepare.argtypes = [vp, vp]
     lib.dsmvc_hip_executor_seal.argtypes = [vp]
     lib.dsmvc_hip_executor_tune.argtypes = [vp, i32, i32]
+    lib.dsmvc_hip_executor_set_refinement.argtypes = [vp, i32]
     for name in ("dsmvc_hip_inverse_rows", "dsmvc_hip_inverse_columns"):
         getattr(lib, name).argtypes = [vp, vp, vp, pd, vp, pd, i32, vp, u32]
     lib.dsmvc_hip_inverse_2d.argtypes = [vp, vp, vp, vp, pd, vp, pd, vp, u32]
