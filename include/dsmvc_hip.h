@@ -153,6 +153,11 @@ int dsmvc_hip_executor_seal(dsmvc_hip_executor *executor);
  * the residual A^T y - G x formed in compensated F32 (two-product, two-sum)
  * from the FP64 weights and normal matrix.  Float32 planes only. */
 int dsmvc_hip_executor_set_refinement(dsmvc_hip_executor *executor, int32_t steps);
+/* No counterpart in the reference: the route the autotuner chose for a
+ * plan, direction (rows != 0: horizontal) and sample kind -- a chunk count,
+ * -1 for the windowed kernels, 0 when not tuned yet. */
+int dsmvc_hip_executor_tuned_route(dsmvc_hip_executor *executor, const dsmvc_hip_plan *plan,
+                                   int32_t rows, int32_t sample_kind, int32_t *route);
 /* Route of the parallel F32 solves per direction: a chunk count for the
  * carry-chain chunked kernels, -1 for the windowed kernels, 0 = automatic
  * (autotuned).  A route the plan cannot use falls back to automatic. */
@@ -222,6 +227,32 @@ int dsmvc_hip_inverse_columns_batch(dsmvc_hip_executor *executor, const dsmvc_hi
                                     void *stream, uint32_t flags);
 
 /* Waits for `stream`; reports a deferred Float64 non-finite failure. */
+/* One plane of a multi-plane call (dsmvc_hip_inverse_2d_planes). */
+typedef struct dsmvc_hip_plane_job {
+    const dsmvc_hip_plan *horizontal;
+    const dsmvc_hip_plan *vertical;
+    const void *input;               /* device, samples of the call's kind */
+    ptrdiff_t input_row_stride;      /* samples */
+    ptrdiff_t input_frame_stride;    /* samples; read when frames > 1 */
+    void *output;
+    ptrdiff_t output_row_stride;
+    ptrdiff_t output_frame_stride;
+    int32_t frames;
+    int32_t reserved;
+    const dsmvc_hip_integer_conversion *conversion;   /* integer kinds only */
+} dsmvc_hip_plane_job;
+
+/* Every plane of a frame batch in one set of launches.  The reference's
+ * filter_get_frame solves a frame plane by plane, one Executor::inverse_2d /
+ * inverse_2d_u16 call each (src/vs_plugin.cpp:1636-1745); here the
+ * horizontal passes of all planes (luma and chroma, each with its own plans
+ * and conversion) share one windowed launch per kernel class, then the
+ * vertical passes.  Planes those kernels cannot take are solved one by one.
+ * sample_kind: DSMVC_HIP_SAMPLE_*; results match per-plane calls within the
+ * F32 route's tolerance. */
+int dsmvc_hip_inverse_2d_planes(dsmvc_hip_executor *executor, const dsmvc_hip_plane_job *planes,
+                                int32_t count, int32_t sample_kind, void *stream, uint32_t flags);
+
 /* ---- multi-plan batches --------------------------------------------------
  * No single counterpart in the reference, whose executors solve one plane per
  * call (src/vs_plugin.cpp:1636-1745); its closest analogue is the Metal
@@ -331,6 +362,11 @@ int dsmvc_hip_server_stats_get(const dsmvc_hip_server *server, dsmvc_hip_server_
 /* Finishes every accepted request, then stops. */
 void dsmvc_hip_server_destroy(dsmvc_hip_server *server);
 
+/* No counterpart in the reference: waits for `stream` and frees the scratch
+ * and job tables the executor keeps for it.  Call before destroying a stream
+ * the executor has run work on; otherwise that state lives until
+ * dsmvc_hip_executor_destroy. */
+int dsmvc_hip_executor_release_stream(dsmvc_hip_executor *executor, void *stream);
 int dsmvc_hip_synchronize(dsmvc_hip_executor *executor, void *stream);
 
 #ifdef __cplusplus

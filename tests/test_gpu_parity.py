@@ -319,6 +319,41 @@ def test_autotuned_partition_stable_and_close(oracle):
             assert err <= 4e-6 * scale
 
 
+def test_planes_call_matches_per_plane_calls(executors, oracle):
+    """dsmvc_hip_inverse_2d_planes: luma and chroma planes (u16, own plans
+    and conversions) in one call equal the oracle within one code, and a
+    float32 call mixing a windowed plane with a Float64-route plane equals
+    the per-plane executor results (the Float64 plane bit for bit)."""
+    reqs = config_plan_requests("config2")
+    rng = np.random.default_rng(31)
+    planes, jobs = [], []
+    for index, (h, v, sw, sh) in enumerate([(reqs[0], reqs[1], 1920, 1080),
+                                            (reqs[2], reqs[3], 960, 540),
+                                            (reqs[2], reqs[3], 960, 540)]):
+        x = rng.integers(0, 65536, (3, sh, sw)).astype(np.uint16)
+        conv = integer_conversion(16, chroma=index > 0, full_range=False)
+        dconv = dh.IntegerConversion(conv.input_offset, conv.input_scale, conv.output_scale,
+                                     conv.output_offset, conv.output_maximum)
+        planes.append((h, v, x, conv))
+        jobs.append((dh.Plan(_hreq(h)), dh.Plan(_hreq(v)),
+                     torch.from_numpy(x.astype(np.int32)).to(DEV).to(torch.uint16), dconv))
+    outs = executors["auto"].inverse_2d_planes(jobs)
+    for (h, v, x, conv), out in zip(planes, outs):
+        got = out.to(torch.int32).cpu().numpy()
+        for f in (0, 2):
+            expected = oracle.inverse_2d_int(oracle.build(h), oracle.build(v), x[f], conv)
+            assert int(np.abs(got[f] - expected.astype(np.int32)).max()) <= 1
+    x = torch.rand(2, 270, 480, device=DEV)
+    h64 = dh.Plan(_hreq(make_request(480, 320, kernel=BICUBIC, f64_mode=F64_ALWAYS)))
+    v64 = dh.Plan(_hreq(make_request(270, 180, kernel=BICUBIC, f64_mode=F64_ALWAYS)))
+    h32 = dh.Plan(_hreq(make_request(480, 300, kernel=LANCZOS, f64_mode=F64_NEVER)))
+    v32 = dh.Plan(_hreq(make_request(270, 170, kernel=LANCZOS, f64_mode=F64_NEVER)))
+    a, b = executors["auto"].inverse_2d_planes([(h64, v64, x, None), (h32, v32, x, None)])
+    assert torch.equal(a, executors["auto"].inverse_2d(h64, v64, x))
+    ref = executors["exact"].inverse_2d(h32, v32, x)
+    assert float((b - ref).abs().max()) <= 4e-6 * max(1.0, float(ref.abs().max()))
+
+
 def test_plugin_descale_yuv420(executors, oracle):
     """dsmvc_hip.Debicubic on a YUV420P16 clip equals the per-plane oracle
     within one code (chroma uses the shifted chroma plans)."""

@@ -131,7 +131,8 @@ def test_admission_waits_for_memory(executor):
         server.wait(rid)
         assert all(bool(torch.isfinite(o).all()) for o in outs)
     assert server.stats()["memory_high_water"] <= 3 * big
+    tiny = dh.Server(executor, memory_budget=4096, slots=1)
     with pytest.raises(ValueError, match="budget"):
-        tiny = dh.Server(executor, memory_budget=4096, slots=1)
         _submit(tiny, reqs[0])
+    tiny.close()
     server.close()

@@ -36,22 +36,28 @@ hip::DevConv identity_conv() {
 // WindowTables
 // ---------------------------------------------------------------------------
 WindowTables::~WindowTables() {
-    if (device_jobs_) {
-        (void)hipSetDevice(device_);
-        (void)hipFree(device_jobs_);
+    (void)hipSetDevice(device_);
+    for (Slot &slot : slots_) {
+        if (slot.done) {
+            if (slot.pending) (void)hipEventSynchronize(slot.done);
+            (void)hipEventDestroy(slot.done);
+        }
+        if (slot.device) (void)hipFree(slot.device);
+        if (slot.host) (void)hipHostFree(slot.host);
     }
 }
 
 void WindowTables::clear() { groups_.clear(); }
 
-void WindowTables::add(const hip::WinJob &job, int band, bool rows) {
+void WindowTables::add(const hip::WinJob &job, int band, bool rows, hip::Sample kind) {
     const int width = job.plan.ell_width;
     const bool block16 = job.plan.block16 != 0;
     auto it = std::find_if(groups_.begin(), groups_.end(), [&](const Group &g) {
-        return g.rows == rows && g.band == band && g.width == width && g.block16 == block16;
+        return g.rows == rows && g.band == band && g.width == width && g.block16 == block16
+            && g.kind == kind;
     });
     if (it == groups_.end()) {
-        groups_.push_back(Group{band, width, rows, block16, {}, 0, 0});
+        groups_.push_back(Group{band, width, rows, block16, kind, {}, 0, 0});
         it = groups_.end() - 1;
     }
     hip::WinJob j = job;
@@ -60,46 +66,65 @@ void WindowTables::add(const hip::WinJob &job, int band, bool rows) {
     it->jobs.push_back(j);
 }
 
+// Tables go through a ring of slots, each a pinned host buffer, a device
+// buffer and an event recorded after the launches that read it: an upload
+// reuses the oldest slot once its event has passed, so uploads and launches
+// stay asynchronous (a copy from pageable memory would wait for the stream).
 int WindowTables::upload(hipStream_t stream) {
     // Horizontal groups first: every vertical pass reads an intermediate
     // some horizontal group writes.
     std::stable_partition(groups_.begin(), groups_.end(), [](const Group &g) { return g.rows; });
-    staging_.clear();
+    std::size_t count = 0;
     for (Group &g : groups_) {
-        g.offset = staging_.size();
-        staging_.insert(staging_.end(), g.jobs.begin(), g.jobs.end());
+        g.offset = count;
+        count += g.jobs.size();
     }
     check(hipSetDevice(device_), "hipSetDevice failed");
-    if (staging_.size() > capacity_) {
-        if (device_jobs_) {
-            // A previous launch may still read the old table.
-            check(stream ? hipStreamSynchronize(stream) : hipDeviceSynchronize(), "table wait failed");
-            check(hipFree(device_jobs_), "job table release failed");
-            device_jobs_ = nullptr;
-        }
-        const std::size_t want = std::max<std::size_t>(staging_.size(), capacity_ * 2);
-        check(hipMalloc(reinterpret_cast<void **>(&device_jobs_), want * sizeof(hip::WinJob)),
-              "job table allocation failed");
-        capacity_ = want;
+    current_ = (current_ + 1) % kSlots;
+    Slot &slot = slots_[current_];
+    if (slot.pending) {
+        check(hipEventSynchronize(slot.done), "table slot wait failed");
+        slot.pending = false;
     }
-    if (!staging_.empty()) {
-        const std::size_t bytes = staging_.size() * sizeof(hip::WinJob);
+    if (!slot.done) check(hipEventCreateWithFlags(&slot.done, hipEventDisableTiming), "event creation failed");
+    if (count > slot.capacity) {
+        if (slot.device) check(hipFree(slot.device), "job table release failed");
+        if (slot.host) check(hipHostFree(slot.host), "job table release failed");
+        slot.device = nullptr;
+        slot.host = nullptr;
+        const std::size_t want = std::max<std::size_t>(count, slot.capacity * 2);
+        check(hipHostMalloc(reinterpret_cast<void **>(&slot.host), want * sizeof(hip::WinJob)),
+              "job table allocation failed");
+        check(hipMalloc(reinterpret_cast<void **>(&slot.device), want * sizeof(hip::WinJob)),
+              "job table allocation failed");
+        slot.capacity = want;
+    }
+    for (const Group &g : groups_)
+        std::copy(g.jobs.begin(), g.jobs.end(), slot.host + g.offset);
+    if (count) {
+        const std::size_t bytes = count * sizeof(hip::WinJob);
         if (stream)
-            check(hipMemcpyAsync(device_jobs_, staging_.data(), bytes, hipMemcpyHostToDevice, stream),
+            check(hipMemcpyAsync(slot.device, slot.host, bytes, hipMemcpyHostToDevice, stream),
                   "job table upload failed");
         else
-            check(hipMemcpy(device_jobs_, staging_.data(), bytes, hipMemcpyHostToDevice),
+            check(hipMemcpy(slot.device, slot.host, bytes, hipMemcpyHostToDevice),
                   "job table upload failed");
     }
+    record_ = stream != nullptr;
     return static_cast<int>(groups_.size());
 }
 
-void WindowTables::launch(hipStream_t stream) const {
+void WindowTables::launch(hipStream_t stream) {
+    Slot &slot = slots_[current_];
     for (const Group &g : groups_) {
         if (g.items == 0) continue;
-        check(hip::launch_window_table(device_jobs_ + g.offset, static_cast<int>(g.jobs.size()),
-                                       g.items, g.band, g.width, g.rows, g.block16, stream),
+        check(hip::launch_window_table(slot.device + g.offset, static_cast<int>(g.jobs.size()),
+                                       g.items, g.band, g.width, g.rows, g.block16, g.kind, stream),
               "table launch failed");
+    }
+    if (record_) {
+        check(hipEventRecord(slot.done, stream), "event record failed");
+        slot.pending = true;
     }
 }
 
@@ -166,7 +191,7 @@ HipBatch::HipBatch(HipExecutor &executor, std::vector<BatchJob> jobs, bool graph
         a.frames = j.frames;
         a.groups = (a.nvec + 63) / 64;
         a.conv = identity_conv();
-        tables_.add(a, hip::window_band_class(h.half_bandwidth), true);
+        tables_.add(a, hip::window_band_class(h.half_bandwidth), true, hip::Sample::f32);
         hip::WinJob b{};
         b.plan = *wv[k];
         b.src = inter;
@@ -179,7 +204,7 @@ HipBatch::HipBatch(HipExecutor &executor, std::vector<BatchJob> jobs, bool graph
         b.frames = j.frames;
         b.groups = (b.nvec + 63) / 64;
         b.conv = identity_conv();
-        tables_.add(b, hip::window_band_class(v.half_bandwidth), false);
+        tables_.add(b, hip::window_band_class(v.half_bandwidth), false, hip::Sample::f32);
         ++stats_.table_jobs;
     }
     stats_.jobs = static_cast<std::int32_t>(jobs_.size());

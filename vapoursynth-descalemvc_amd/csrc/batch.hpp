@@ -50,7 +50,7 @@ struct BatchStats {
 };
 
 // Groups windowed axis jobs by (direction, band class, tap width, staging
-// block size) into one
+// block size, sample kind) into one
 // device buffer of job tables and launches them: all horizontal groups
 // first, then all vertical groups, on one stream.  Reusable: `build` lays
 // out a new set of jobs (growing the device buffer), `launch` enqueues.
@@ -62,11 +62,15 @@ public:
     WindowTables &operator=(const WindowTables &) = delete;
 
     void clear();
-    void add(const hip::WinJob &job, int band, bool rows);
+    // `kind`: input samples of a horizontal job, output samples of a vertical one.
+    void add(const hip::WinJob &job, int band, bool rows, hip::Sample kind);
     // Uploads the tables (stream-ordered when `stream` is given, else
     // synchronously) and returns the number of launches `launch` makes.
     int upload(hipStream_t stream);
-    void launch(hipStream_t stream) const;
+    // Launches the uploaded tables.  After a synchronous upload the launches
+    // may be captured into a graph and replayed: that slot is never reused
+    // by this object unless upload() is called again.
+    void launch(hipStream_t stream);
     [[nodiscard]] std::int64_t items() const;
 
 private:
@@ -74,15 +78,24 @@ private:
         int band = 0, width = 0;
         bool rows = false;
         bool block16 = false;       // 16-row staging blocks
+        hip::Sample kind = hip::Sample::f32;
         std::vector<hip::WinJob> jobs;
         std::size_t offset = 0;     // first job's index in the device buffer
         long items = 0;
     };
+    struct Slot {
+        hip::WinJob *host = nullptr;     // pinned
+        hip::WinJob *device = nullptr;
+        std::size_t capacity = 0;
+        hipEvent_t done = nullptr;
+        bool pending = false;
+    };
+    static constexpr int kSlots = 3;
     int device_;
     std::vector<Group> groups_;
-    hip::WinJob *device_jobs_ = nullptr;
-    std::size_t capacity_ = 0;
-    std::vector<hip::WinJob> staging_;
+    Slot slots_[kSlots];
+    int current_ = 0;
+    bool record_ = false;
 };
 
 class HipBatch {

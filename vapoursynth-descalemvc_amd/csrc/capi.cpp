@@ -261,6 +261,15 @@ int dsmvc_hip_executor_set_refinement(dsmvc_hip_executor *executor, int32_t step
     return guarded([&] { exec_of(executor).set_refinement(steps); });
 }
 
+int dsmvc_hip_executor_tuned_route(dsmvc_hip_executor *executor, const dsmvc_hip_plan *plan,
+                                   int32_t rows, int32_t sample_kind, int32_t *route) {
+    return guarded([&] {
+        require(route && sample_kind >= 0 && sample_kind <= 2, "dsmvc: invalid route query");
+        const int r = exec_of(executor).tuned_chunks(plan_of(plan), rows != 0, sample_kind);
+        *route = r < -1 ? 0 : r;
+    });
+}
+
 int dsmvc_hip_executor_tune(dsmvc_hip_executor *executor, int32_t rows_chunks,
                             int32_t columns_chunks) {
     return guarded([&] {
@@ -390,6 +399,33 @@ int dsmvc_hip_inverse_columns_batch(dsmvc_hip_executor *executor, const dsmvc_hi
                                                 static_cast<dsmvc::SampleKind>(output_kind),
                                                 batch, columns, conversion ? &conv : nullptr,
                                                 stream_of(stream), is_async(flags));
+    });
+}
+
+int dsmvc_hip_inverse_2d_planes(dsmvc_hip_executor *executor, const dsmvc_hip_plane_job *planes,
+                                int32_t count, int32_t sample_kind, void *stream, uint32_t flags) {
+    return guarded([&] {
+        require((planes || count == 0) && count >= 0 && sample_kind >= 0 && sample_kind <= 2,
+                "dsmvc: invalid plane arguments");
+        std::vector<dsmvc::IntegerConversion> conv(static_cast<std::size_t>(count));
+        std::vector<dsmvc::HipExecutor::PlaneJob> list(static_cast<std::size_t>(count));
+        for (int32_t k = 0; k < count; ++k) {
+            const dsmvc_hip_plane_job &p = planes[k];
+            require(p.horizontal && p.vertical, "dsmvc: invalid plane arguments");
+            auto &j = list[static_cast<std::size_t>(k)];
+            j.horizontal = &plan_of(p.horizontal);
+            j.vertical = &plan_of(p.vertical);
+            j.batch = dsmvc::PlaneBatch{p.input, p.input_row_stride, p.input_frame_stride, p.output,
+                                        p.output_row_stride, p.output_frame_stride, p.frames};
+            if (p.conversion) {
+                conv[static_cast<std::size_t>(k)] = to_conversion(*p.conversion);
+                j.conversion = &conv[static_cast<std::size_t>(k)];
+            }
+        }
+        exec_of(executor).inverse_2d_planes(list.data(), count,
+                                            static_cast<dsmvc::SampleKind>(sample_kind),
+                                            static_cast<hipStream_t>(stream),
+                                            (flags & DSMVC_HIP_ASYNC) != 0);
     });
 }
 
@@ -540,6 +576,10 @@ int dsmvc_hip_server_stats_get(const dsmvc_hip_server *server, dsmvc_hip_server_
 }
 
 void dsmvc_hip_server_destroy(dsmvc_hip_server *server) { delete server; }
+
+int dsmvc_hip_executor_release_stream(dsmvc_hip_executor *executor, void *stream) {
+    return guarded([&] { exec_of(executor).release_stream(static_cast<hipStream_t>(stream)); });
+}
 
 int dsmvc_hip_synchronize(dsmvc_hip_executor *executor, void *stream) {
     return guarded([&] { exec_of(executor).synchronize(stream_of(stream)); });

@@ -291,27 +291,21 @@ bool build_window_plan(const AxisPlan &p, WindowPlan &out) {
         out.tap_start[static_cast<std::size_t>(r)] = start;
     }
 
-    out.ell_idx.assign(ur * width, 0);
-    out.ell_w.assign(ur * width, 0.0F);
+    // Packed records: forward [tap weights | lld_1..lld_W | inverse diagonal],
+    // backward [ul_1..ul_W], zero in padding rows and past the plan's band.
+    const int fw = window_fwd_words(band, width), bw = window_bwd_words(band);
+    out.fwd.assign(ur * static_cast<std::size_t>(fw), 0.0F);
+    out.bwd.assign(ur * static_cast<std::size_t>(bw), 0.0F);
     for (int r = 0; r < rows; ++r) {
-        const int i = real(r);
-        const int window = out.blk_lo[static_cast<std::size_t>(r / kWindowBlock)];
-        std::int32_t *idx = out.ell_idx.data() + static_cast<std::size_t>(r) * width;
-        float *wt = out.ell_w.data() + static_cast<std::size_t>(r) * width;
-        if (i < 0 || i >= n) {
-            std::fill(idx, idx + width, window);
-            continue;
+        const float *c = out.coef.data() + static_cast<std::size_t>(r) * 16;
+        float *f = out.fwd.data() + static_cast<std::size_t>(r) * fw;
+        float *u = out.bwd.data() + static_cast<std::size_t>(r) * bw;
+        std::copy_n(out.tap_w.data() + static_cast<std::size_t>(r) * width, width, f);
+        for (int d = 1; d <= w; ++d) {
+            f[width + d - 1] = c[d - 1];
+            u[d - 1] = c[kMaxBand + d];
         }
-        const auto k0 = p.transpose_offsets[static_cast<std::size_t>(i)];
-        const auto k1 = p.transpose_offsets[static_cast<std::size_t>(i) + 1];
-        const int c = static_cast<int>(k1 - k0);
-        const int pad = width - c;
-        const std::int32_t fill = c > 0 ? p.transpose_indices[k0] : window;
-        for (int e = 0; e < pad; ++e) idx[e] = fill;
-        for (int e = 0; e < c; ++e) {
-            idx[pad + e] = p.transpose_indices[k0 + static_cast<std::uint32_t>(e)];
-            wt[pad + e] = p.transpose_weights[k0 + static_cast<std::uint32_t>(e)];
-        }
+        f[width + band] = c[kMaxBand];
     }
 
     // Warm-up check, in FP64 on the F32 coefficients the kernels use: from
@@ -468,22 +462,18 @@ const DevWindow *DevicePlan::window(bool rows) {
         image.built = true;
         WindowPlan wp;
         if (build_window_plan(*plan_, wp)) {
-            const std::size_t o_coef = 0;
-            const std::size_t o_idx = align256(o_coef + wp.coef.size() * 4);
-            const std::size_t o_w = align256(o_idx + wp.ell_idx.size() * 4);
-            const std::size_t o_blk = align256(o_w + wp.ell_w.size() * 4);
+            const std::size_t o_fwd = 0;
+            const std::size_t o_bwd = align256(o_fwd + wp.fwd.size() * 4);
+            const std::size_t o_ts = align256(o_bwd + wp.bwd.size() * 4);
+            const std::size_t o_blk = align256(o_ts + wp.tap_start.size() * 4);
             const std::size_t o_blk16 = align256(o_blk + wp.blk_lo.size() * 4);
-            const std::size_t o_ts = align256(o_blk16 + wp.blk_lo16.size() * 4);
-            const std::size_t o_tw = align256(o_ts + wp.tap_start.size() * 4);
-            const std::size_t total = align256(o_tw + wp.tap_w.size() * 4);
+            const std::size_t total = align256(o_blk16 + wp.blk_lo16.size() * 4);
             std::vector<unsigned char> host(total, 0);
-            std::memcpy(host.data() + o_coef, wp.coef.data(), wp.coef.size() * 4);
-            std::memcpy(host.data() + o_idx, wp.ell_idx.data(), wp.ell_idx.size() * 4);
-            std::memcpy(host.data() + o_w, wp.ell_w.data(), wp.ell_w.size() * 4);
+            std::memcpy(host.data() + o_fwd, wp.fwd.data(), wp.fwd.size() * 4);
+            std::memcpy(host.data() + o_bwd, wp.bwd.data(), wp.bwd.size() * 4);
+            std::memcpy(host.data() + o_ts, wp.tap_start.data(), wp.tap_start.size() * 4);
             std::memcpy(host.data() + o_blk, wp.blk_lo.data(), wp.blk_lo.size() * 4);
             std::memcpy(host.data() + o_blk16, wp.blk_lo16.data(), wp.blk_lo16.size() * 4);
-            std::memcpy(host.data() + o_ts, wp.tap_start.data(), wp.tap_start.size() * 4);
-            std::memcpy(host.data() + o_tw, wp.tap_w.data(), wp.tap_w.size() * 4);
             int current = 0;
             check(hipGetDevice(&current), "hipGetDevice failed");
             check(hipSetDevice(device_), "hipSetDevice failed");
@@ -498,14 +488,12 @@ const DevWindow *DevicePlan::window(bool rows) {
             image.dev.warmup = wp.warmup;
             image.dev.chunks = wp.chunks;
             image.dev.ell_width = wp.width;
-            image.dev.coef = reinterpret_cast<const float *>(b + o_coef);
-            image.dev.ell_idx = reinterpret_cast<const std::int32_t *>(b + o_idx);
-            image.dev.ell_w = reinterpret_cast<const float *>(b + o_w);
+            image.dev.block16 = wp.block16 ? 1 : 0;
+            image.dev.fwd = reinterpret_cast<const float *>(b + o_fwd);
+            image.dev.bwd = reinterpret_cast<const float *>(b + o_bwd);
+            image.dev.tap_start = reinterpret_cast<const std::int32_t *>(b + o_ts);
             image.dev.blk_lo = reinterpret_cast<const std::int32_t *>(b + o_blk);
             image.dev.blk_lo16 = reinterpret_cast<const std::int32_t *>(b + o_blk16);
-            image.dev.block16 = wp.block16 ? 1 : 0;
-            image.dev.tap_start = reinterpret_cast<const std::int32_t *>(b + o_ts);
-            image.dev.tap_w = reinterpret_cast<const float *>(b + o_tw);
             image.ok = true;
             image.rows_feasible = wp.rows_feasible;
         }

@@ -59,14 +59,14 @@ struct WindowPlan {
     std::int32_t width = 0;          // packed taps per row
     bool rows_feasible = false;      // every 8-row block's taps fit 32 columns
     double residual = 0.0;           // worst relative influence of a chunk's zero start
-    std::vector<float> coef;         // [rows][16]
-    std::vector<std::int32_t> ell_idx;
-    std::vector<float> ell_w;
+    std::vector<float> coef;         // [rows][16] kMaxBand records (warm-up check)
+    std::vector<float> fwd;          // [rows][window_fwd_words(band, width)]
+    std::vector<float> bwd;          // [rows][window_bwd_words(band)]
     std::vector<std::int32_t> blk_lo; // [rows / 8]
     std::vector<std::int32_t> blk_lo16; // [rows / 16]
     bool block16 = false;            // every 16-row block's taps fit 32 columns
-    std::vector<std::int32_t> tap_start; // [rows] contiguous tap runs (horizontal kernel)
-    std::vector<float> tap_w;            // [rows][K]
+    std::vector<std::int32_t> tap_start; // [rows] contiguous tap runs
+    std::vector<float> tap_w;            // [rows][K] (copied into fwd)
 };
 // Largest influence (relative) of a wrong starting state allowed at a
 // windowed chunk's first forward / last backward row: 2^-28.

@@ -72,22 +72,27 @@ constexpr int kStageCols = 32;     // source columns staged per block
 // Windowed partition for the independent-chunk kernels.  Chunk q owns the
 // destination rows [qL, qL + L); its wavefront runs the forward recurrence
 // over [qL - P, qL + L + P) from a zero state and the backward recurrence
-// over [qL, qL + L + P) from a zero state.  P (the warm-up) is chosen per
-// plan from its FP64 factors so that the influence of the wrong starting
-// state has decayed below 2^-28 of the carried values by the chunk's first
+// over [qL, qL + L + P) from a zero state.  L and P are fixed per band class
+// (window_length / window_warmup); a plan uses the windowed kernels only when
+// the influence of the wrong starting state, computed in FP64 from its F32
+// factors, has decayed below 2^-28 of the carried values by the chunk's first
 // (forward) and last (backward) row.  Rows are stored padded: padded row
 // r <-> plan row r - P, with zero rows before 0 and from n on, so every chunk
-// visits exactly L + 2P padded rows starting at padded row qL.
-//   coef     [rows][16]  coefficient records (kMaxBand layout)
-//   ell_idx / ell_w [rows][K] taps, front-padded with weight 0
-//   blk_lo   [rows / 8]  horizontal kernel: first staged source column of
-//                        each 8-row block (its taps lie in [lo, lo + 32))
-//   blk_lo16 [rows / 16] the same for 16-row blocks, valid when block16 = 1
-//                        (every 16-row block's taps fit 32 columns)
-//   tap_start / tap_w    horizontal kernel: each row's taps as one run of K
-//                        consecutive source columns [start, start + K) with
-//                        weight 0 where the row has no tap, inside its
-//                        staging window (both block sizes)
+// visits exactly L + 2P padded rows starting at padded row qL.  Per padded
+// row, for the plan's band class W and tap width K:
+//   fwd       [rows][window_fwd_words(W, K)] forward record: the K weights of
+//             the row's tap run, the forward factors lld_1..lld_W and the
+//             inverse diagonal (zero in padding rows and unused slots)
+//   bwd       [rows][window_bwd_words(W)] backward record: ul_1..ul_W
+//   tap_start [rows] first source sample of the row's tap run: the taps are
+//             the K consecutive samples [start, start + K), weight 0 where the
+//             row has no tap, inside its staging window (both block sizes)
+//   blk_lo    [rows / 8]  first staged source sample of each 8-row block (its
+//             taps lie in [lo, lo + 32))
+//   blk_lo16  [rows / 16] the same for 16-row blocks, valid when block16 = 1
+//             (every 16-row block's taps fit 32 samples)
+// The records are read a few rows at a time with wide scalar loads, so the
+// kernels wait for scalar memory once per batch of rows, not once per row.
 struct DevWindow {
     std::int32_t n;
     std::int32_t m;
@@ -97,13 +102,11 @@ struct DevWindow {
     std::int32_t ell_width;   // K
     std::int32_t block16;
     std::int32_t pad_;
-    const float *coef;
-    const std::int32_t *ell_idx;
-    const float *ell_w;
+    const float *fwd;
+    const float *bwd;
+    const std::int32_t *tap_start;
     const std::int32_t *blk_lo;
     const std::int32_t *blk_lo16;
-    const std::int32_t *tap_start;   // [rows]
-    const float *tap_w;              // [rows][K]
 };
 
 constexpr int kWindowBlock = 8;    // padded rows per staging block (rows kernel)
@@ -121,7 +124,7 @@ enum class Sample : std::int32_t { f32 = 0, u8 = 1, u16 = 2, f64 = 3 };
 
 // One axis pass of one plan over `frames` planes, as a unit of a
 // (possibly multi-plan) windowed launch.  Items are (frame, chunk, group of
-// 64 vectors), the vector group varying fastest.  Rows pass: vectors are
+// 64 vectors), the group varying fastest.  Rows pass: vectors are
 // rows, `src` holds samples of the launch's kind, `dst` F32.  Columns pass:
 // vectors are columns, `src` F32, `dst` samples of the launch's kind.
 struct WinJob {
@@ -154,5 +157,11 @@ struct DevRefine {
 // Windowed-kernel shape per band class: chunk length L and warm-up P.
 constexpr int window_length(int w) { return w <= 3 ? 64 : (w <= 5 ? 48 : 32); }
 constexpr int window_warmup(int w) { return w <= 3 ? 24 : (w <= 5 ? 32 : 48); }
+// Record sizes of the windowed tables (DevWindow) in words, and the rows one
+// batch of scalar loads covers (at most 64 words, a divisor of 8).
+constexpr int window_fwd_words(int w, int k) { return (k + w + 1 + 3) & ~3; }
+constexpr int window_bwd_words(int w) { return w <= 3 ? 4 : 8; }
+constexpr int window_fwd_batch(int w, int k) { return window_fwd_words(w, k) <= 16 ? 4 : 2; }
+constexpr int window_bwd_batch(int w) { return 64 / window_bwd_words(w); }
 
 } // namespace dsmvc::hip

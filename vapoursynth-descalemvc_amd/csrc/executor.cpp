@@ -10,6 +10,7 @@
 #include <stdexcept>
 #include <string>
 
+#include "batch.hpp"
 #include "kernels.hpp"
 
 namespace dsmvc {
@@ -102,14 +103,29 @@ HipExecutor::HipExecutor(int device, SolveMode mode) : device_(device), mode_(mo
 }
 
 HipExecutor::~HipExecutor() {
+    // The per-stream entries are keyed by streams this executor does not
+    // own; their owners may have destroyed them already, so only the device
+    // as a whole is waited for.
     (void)hipSetDevice(device_);
+    (void)hipDeviceSynchronize();
+    plane_tables_.clear();
     for (auto &entry : scratch_) {
-        if (entry.first) (void)hipStreamSynchronize(entry.first);
-        else (void)hipDeviceSynchronize();
         if (entry.second.buffer) (void)hipFree(entry.second.buffer);
         if (entry.second.flag) (void)hipFree(entry.second.flag);
     }
     plans_.clear();
+}
+
+void HipExecutor::release_stream(hipStream_t stream) {
+    std::lock_guard lock(mutex_);
+    check(hipSetDevice(device_), "hipSetDevice failed");
+    check(stream ? hipStreamSynchronize(stream) : hipDeviceSynchronize(), "stream synchronize failed");
+    plane_tables_.erase(stream);
+    if (auto it = scratch_.find(stream); it != scratch_.end()) {
+        if (it->second.buffer) (void)hipFree(it->second.buffer);
+        if (it->second.flag) (void)hipFree(it->second.flag);
+        scratch_.erase(it);
+    }
 }
 
 void HipExecutor::prepare(const std::shared_ptr<const AxisPlan> &plan) {
@@ -140,6 +156,12 @@ hip::DevicePlan &HipExecutor::device_plan(const AxisPlan &plan) {
     return ref;
 }
 
+WindowTables &HipExecutor::stream_tables(hipStream_t stream) {
+    auto &slot = plane_tables_[stream];
+    if (!slot) slot = std::make_unique<WindowTables>(device_);
+    return *slot;
+}
+
 hip::DevicePlan &HipExecutor::plan_image(const AxisPlan &plan) {
     if (!plan.valid()) throw std::invalid_argument("dsmvc: invalid HIP axis plan");
     std::lock_guard lock(mutex_);
@@ -151,13 +173,13 @@ HipExecutor::Scratch &HipExecutor::scratch(hipStream_t stream, std::size_t bytes
     Scratch &s = scratch_[stream];
     if (!s.flag) check(hipMalloc(reinterpret_cast<void **>(&s.flag), 256), "scratch allocation failed");
     if (s.bytes < bytes) {
+        const std::size_t grown = align256(std::max(bytes, s.bytes + s.bytes / 2));
         if (s.buffer) {
             check(stream ? hipStreamSynchronize(stream) : hipDeviceSynchronize(), "stream synchronize failed");
             check(hipFree(s.buffer), "scratch release failed");
             s.buffer = nullptr;
             s.bytes = 0;
         }
-        const std::size_t grown = align256(std::max(bytes, s.bytes + s.bytes / 2));
         check(hipMalloc(&s.buffer, grown), "scratch allocation failed");
         s.bytes = grown;
     }
@@ -207,7 +229,13 @@ hipError_t HipExecutor::launch_route(hip::DevicePlan &dp, int route, bool rows,
         job.groups = (a.nvec + 63) / 64;
         job.first_item = 0;
         job.conv = a.conv;
-        return hip::launch_window(job, hip::window_band_class(dp.axis().w), rows, kind, stream);
+        // A one-job table on this stream's table ring.
+        WindowTables &tables = stream_tables(stream);
+        tables.clear();
+        tables.add(job, hip::window_band_class(dp.axis().w), rows, kind);
+        tables.upload(stream);
+        tables.launch(stream);
+        return hipGetLastError();
     }
     const hip::DevChunks *ch = dp.chunks(route, rows);
     if (!ch) return hipErrorInvalidValue;
@@ -273,7 +301,7 @@ int HipExecutor::choose_partition(hip::DevicePlan &dp, bool rows, const hip::Chu
 int HipExecutor::tuned_chunks(const AxisPlan &plan, bool rows, int kind) {
     std::lock_guard lock(mutex_);
     auto it = plans_.find(&plan);
-    if (it == plans_.end() || !same_plan(it->second->host(), plan)) return -1;
+    if (it == plans_.end() || !same_plan(it->second->host(), plan)) return -2;
     auto t = tuned_.find(TuneKey{it->second.get(), rows, kind});
     return t == tuned_.end() ? 0 : t->second;
 }
@@ -407,8 +435,7 @@ void HipExecutor::inverse_rows(const AxisPlan &plan, const float *input,
     Scratch &s = scratch(stream, align256(static_cast<std::size_t>(row_count) * plan.destination_size * elem));
     const DevConv conv = make_conv(nullptr);
     if (f64) {
-        check(hipMemsetAsync(s.flag, 0, sizeof(int), stream), "flag reset failed");
-        s.flag_armed = true;
+        arm_flag(s, stream);
         check(hip::launch_check_finite(input, Sample::f32, input_row_stride, 0, row_count,
                                        plan.source_size, 1, s.flag, stream), "check launch failed");
     }
@@ -432,14 +459,108 @@ void HipExecutor::inverse_columns(const AxisPlan &plan, const float *input,
     Scratch &s = scratch(stream, align256(static_cast<std::size_t>(column_count) * plan.destination_size * elem));
     const DevConv conv = make_conv(nullptr);
     if (f64) {
-        check(hipMemsetAsync(s.flag, 0, sizeof(int), stream), "flag reset failed");
-        s.flag_armed = true;
+        arm_flag(s, stream);
         check(hip::launch_check_finite(input, Sample::f32, input_row_stride, 0, plan.source_size,
                                        column_count, 1, s.flag, stream), "check launch failed");
     }
     axis_pass(plan, false, input, Sample::f32, input_row_stride, 0, output, Sample::f32,
               output_row_stride, 0, column_count, 1, conv, f64, s, 0, stream);
     finish(stream, s, async, "dsmvc: HIP Float64 column execution produced NaN or infinity");
+}
+
+void HipExecutor::inverse_2d_planes(const PlaneJob *planes, std::int32_t count, SampleKind kind,
+                                    hipStream_t stream, bool async) {
+    const bool integer = kind != SampleKind::float32;
+    if (count < 0 || (count > 0 && !planes))
+        throw std::invalid_argument("dsmvc: invalid plane list");
+    for (std::int32_t k = 0; k < count; ++k) {
+        const PlaneJob &p = planes[k];
+        const PlaneBatch &b = p.batch;
+        if (!p.horizontal || !p.vertical || !p.horizontal->valid() || !p.vertical->valid()
+            || !b.input || !b.output || b.frames < 1
+            || b.input_row_stride < p.horizontal->source_size
+            || b.output_row_stride < p.horizontal->destination_size
+            || (b.frames > 1
+                && (b.input_frame_stride < b.input_row_stride * p.vertical->source_size
+                    || b.output_frame_stride < b.output_row_stride * p.vertical->destination_size)))
+            throw std::invalid_argument("dsmvc: invalid plane arguments");
+        if (integer) {
+            if (!p.conversion) throw std::invalid_argument("dsmvc: integer planes need a conversion");
+            validate_conversion(*p.conversion, kind == SampleKind::u8 ? 255U : 65535U,
+                                "dsmvc: invalid plane arguments");
+        }
+    }
+    std::lock_guard lock(mutex_);
+    check(hipSetDevice(device_), "hipSetDevice failed");
+    const bool windows = mode_ == SolveMode::automatic || mode_ == SolveMode::windowed;
+    std::vector<const hip::DevWindow *> wh(static_cast<std::size_t>(count), nullptr);
+    std::vector<const hip::DevWindow *> wv(static_cast<std::size_t>(count), nullptr);
+    std::vector<std::size_t> offset(static_cast<std::size_t>(count), 0);
+    std::size_t total = 0;
+    for (std::int32_t k = 0; k < count; ++k) {
+        const AxisPlan &h = *planes[k].horizontal, &v = *planes[k].vertical;
+        if (!windows || h.requires_float64() || v.requires_float64()) continue;
+        wh[static_cast<std::size_t>(k)] = device_plan(h).window(true);
+        wv[static_cast<std::size_t>(k)] = device_plan(v).window(false);
+        if (!wh[static_cast<std::size_t>(k)] || !wv[static_cast<std::size_t>(k)]) {
+            wh[static_cast<std::size_t>(k)] = wv[static_cast<std::size_t>(k)] = nullptr;
+            continue;
+        }
+        offset[static_cast<std::size_t>(k)] = total;
+        total += align256(static_cast<std::size_t>(padded_width(h.destination_size)) * v.source_size
+                          * static_cast<std::size_t>(planes[k].batch.frames) * sizeof(float));
+    }
+    Scratch *last = nullptr;
+    if (total) {
+        Scratch &s = scratch(stream, total);
+        last = &s;
+        WindowTables &tables = stream_tables(stream);
+        tables.clear();
+        const Sample sk = sample_of(kind);
+        for (std::int32_t k = 0; k < count; ++k) {
+            if (!wh[static_cast<std::size_t>(k)]) continue;
+            const PlaneJob &p = planes[k];
+            const AxisPlan &h = *p.horizontal, &v = *p.vertical;
+            const std::ptrdiff_t row = padded_width(h.destination_size);
+            const std::ptrdiff_t frame = row * v.source_size;
+            float *inter = reinterpret_cast<float *>(static_cast<unsigned char *>(s.buffer)
+                                                     + offset[static_cast<std::size_t>(k)]);
+            hip::WinJob a{};
+            a.plan = *wh[static_cast<std::size_t>(k)];
+            a.src = p.batch.input;
+            a.s_row = p.batch.input_row_stride;
+            a.s_frame = p.batch.input_frame_stride;
+            a.dst = inter;
+            a.d_row = row;
+            a.d_frame = frame;
+            a.nvec = v.source_size;
+            a.frames = p.batch.frames;
+            a.groups = (a.nvec + 63) / 64;
+            a.conv = make_conv(p.conversion);
+            tables.add(a, hip::window_band_class(h.half_bandwidth), true, sk);
+            hip::WinJob b{};
+            b.plan = *wv[static_cast<std::size_t>(k)];
+            b.src = inter;
+            b.s_row = row;
+            b.s_frame = frame;
+            b.dst = p.batch.output;
+            b.d_row = p.batch.output_row_stride;
+            b.d_frame = p.batch.output_frame_stride;
+            b.nvec = h.destination_size;
+            b.frames = p.batch.frames;
+            b.groups = (b.nvec + 63) / 64;
+            b.conv = a.conv;
+            tables.add(b, hip::window_band_class(v.half_bandwidth), false, sk);
+        }
+        tables.upload(stream);
+        tables.launch(stream);
+    }
+    for (std::int32_t k = 0; k < count; ++k) {
+        if (wh[static_cast<std::size_t>(k)]) continue;
+        const PlaneJob &p = planes[k];
+        last = &solve_2d(*p.horizontal, *p.vertical, kind, p.batch, p.conversion, stream);
+    }
+    if (last) finish(stream, *last, async, "dsmvc: HIP Float64 2D execution produced NaN or infinity");
 }
 
 void HipExecutor::inverse_2d_batch(const AxisPlan &h, const AxisPlan &v, SampleKind kind,
@@ -460,6 +581,14 @@ void HipExecutor::inverse_2d_batch(const AxisPlan &h, const AxisPlan &v, SampleK
     }
     std::lock_guard lock(mutex_);
     check(hipSetDevice(device_), "hipSetDevice failed");
+    Scratch &s = solve_2d(h, v, kind, b, conversion, stream);
+    finish(stream, s, async, "dsmvc: HIP Float64 2D execution produced NaN or infinity");
+}
+
+// Queues one plane batch's two passes on `stream` (caller holds the lock).
+HipExecutor::Scratch &HipExecutor::solve_2d(const AxisPlan &h, const AxisPlan &v, SampleKind kind,
+                                            const PlaneBatch &b, const IntegerConversion *conversion,
+                                            hipStream_t stream) {
     const bool f64 = float64_route(h) || float64_route(v);
     const std::size_t elem = f64 ? 8 : 4;
     const std::ptrdiff_t inter_row = padded_width(h.destination_size);
@@ -474,8 +603,7 @@ void HipExecutor::inverse_2d_batch(const AxisPlan &h, const AxisPlan &v, SampleK
     const DevConv conv = make_conv(conversion);
     const Sample in_kind = sample_of(kind);
     if (f64) {
-        check(hipMemsetAsync(s.flag, 0, sizeof(int), stream), "flag reset failed");
-        s.flag_armed = true;
+        arm_flag(s, stream);
         check(hip::launch_check_finite(b.input, in_kind, b.input_row_stride, b.input_frame_stride,
                                        v.source_size, h.source_size, b.frames, s.flag, stream),
               "check launch failed");
@@ -488,7 +616,14 @@ void HipExecutor::inverse_2d_batch(const AxisPlan &h, const AxisPlan &v, SampleK
     axis_pass(v, false, inter, inter_kind, inter_row, inter_frame, b.output, in_kind,
               b.output_row_stride, b.output_frame_stride, h.destination_size, b.frames, conv,
               f64, s, inter_bytes, stream);
-    finish(stream, s, async, "dsmvc: HIP Float64 2D execution produced NaN or infinity");
+    return s;
+}
+
+// Clears the stream's Float64 flag unless an earlier, unreported call armed
+// it (its failure must survive until finish or synchronize reads it).
+void HipExecutor::arm_flag(Scratch &s, hipStream_t stream) {
+    if (!s.flag_armed) check(hipMemsetAsync(s.flag, 0, sizeof(int), stream), "flag reset failed");
+    s.flag_armed = true;
 }
 
 void HipExecutor::inverse_rows_batch(const AxisPlan &plan, SampleKind in_kind,

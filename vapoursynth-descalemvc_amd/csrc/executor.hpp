@@ -25,6 +25,7 @@ namespace dsmvc {
 namespace hip {
 struct ChunkArgs;
 }
+class WindowTables;
 
 enum class SolveMode : std::uint8_t {
     automatic = 0,   // chunked or windowed where the plan allows it (autotuned), else sequential
@@ -86,8 +87,8 @@ public:
         if (steps < 1 || steps > 3) throw std::invalid_argument("dsmvc: refinement steps must be 1..3");
         refine_steps_ = steps;
     }
-    // The tuned route of a key: a chunk count, kWindowRoute, 0 before tuning,
-    // -1 for an unknown plan.
+    // The tuned route of a key: a chunk count, kWindowRoute (-1), 0 before
+    // tuning, -2 for a plan this executor has not seen.
     [[nodiscard]] int tuned_chunks(const AxisPlan &plan, bool rows, int kind);
 
     // Float32 planes (device pointers).  Synchronous unless `async`.
@@ -118,6 +119,21 @@ public:
                           const PlaneBatch &batch, const IntegerConversion *conversion,
                           hipStream_t stream, bool async);
 
+    // Several planes in one set of launches -- a YUV frame batch's luma and
+    // chroma, each with its own plans and conversion: every plane's
+    // horizontal pass goes into one windowed table launch per kernel class,
+    // then every vertical pass.  Planes the windowed kernels cannot take
+    // (Float64 route, refined, exact or chunked modes, wide bands) are solved
+    // one by one after them.
+    struct PlaneJob {
+        const AxisPlan *horizontal = nullptr;
+        const AxisPlan *vertical = nullptr;
+        PlaneBatch batch{};
+        const IntegerConversion *conversion = nullptr;
+    };
+    void inverse_2d_planes(const PlaneJob *planes, std::int32_t count, SampleKind kind,
+                           hipStream_t stream, bool async);
+
     // One axis over a batch, for F32 plans: rows of `kind` samples into a
     // float32 intermediate, or columns of a float32 intermediate into `kind`
     // samples -- the two halves of inverse_2d_batch, callable on their own.
@@ -130,6 +146,10 @@ public:
 
     // Waits for `stream` and reports a deferred Float64 non-finite failure.
     void synchronize(hipStream_t stream);
+    // Waits for `stream` and frees the scratch and job tables kept for it.
+    // Call before destroying a stream this executor has run work on (the
+    // executor keeps per-stream state until then, or until it is destroyed).
+    void release_stream(hipStream_t stream);
 
     // The device image of a plan (uploaded on first use), for callers that
     // launch the kernels themselves (multi-plan batches, the request loop).
@@ -155,6 +175,10 @@ private:
                     hip::Sample out_kind, std::ptrdiff_t out_row, std::ptrdiff_t out_frame,
                     std::int32_t vectors, std::int32_t frames, const hip::DevConv &conv, bool f64,
                     Scratch &s, std::size_t fwd_offset, hipStream_t stream);
+    Scratch &solve_2d(const AxisPlan &h, const AxisPlan &v, SampleKind kind, const PlaneBatch &b,
+                      const IntegerConversion *conversion, hipStream_t stream);
+    void arm_flag(Scratch &s, hipStream_t stream);
+    WindowTables &stream_tables(hipStream_t stream);
     void axis_pass(const AxisPlan &plan, bool rows, const void *input, hip::Sample in_kind,
                    std::ptrdiff_t in_row, std::ptrdiff_t in_frame, void *output,
                    hip::Sample out_kind, std::ptrdiff_t out_row, std::ptrdiff_t out_frame,
@@ -189,6 +213,7 @@ private:
     int refine_steps_ = 1;
     int compute_units_ = 256;
     std::unordered_map<TuneKey, int, TuneKeyHash> tuned_;
+    std::unordered_map<hipStream_t, std::unique_ptr<WindowTables>> plane_tables_;
     std::mutex mutex_;
     std::unordered_map<const AxisPlan *, std::unique_ptr<dsmvc::hip::DevicePlan>> plans_;
     std::unordered_map<hipStream_t, Scratch> scratch_;

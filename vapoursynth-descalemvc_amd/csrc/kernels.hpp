@@ -60,25 +60,21 @@ extern template hipError_t launch_chunked_band<5>(const ChunkArgs &, bool, Sampl
 extern template hipError_t launch_chunked_band<7>(const ChunkArgs &, bool, Sample, hipStream_t);
 // Windowed (independent-chunk) kernels.  The band class is 1, 3, 5 or 7
 // (smaller bands run in the next class with zero coefficients); the tap
-// width 8 or 16.  `launch_window` runs one job; `launch_window_table` runs
-// `count` jobs from a device-resident table (`items` items in all) in one
-// launch, all jobs sharing the band class, tap width, direction and (rows)
-// staging block size, with F32 samples.
+// width 8 or 16.  `launch_window_table` runs `count` jobs from a
+// device-resident table (`items` items in all) in one launch, all jobs
+// sharing the band class, tap width, direction, staging block size and
+// sample kind (the input kind of a horizontal pass, the output kind of a
+// vertical pass).  A single pass is a one-job table.
 int window_band_class(int w);   // 1, 3, 5, 7, or 0 when unsupported
 constexpr int kWindowWaves = 4;   // wavefronts (one item each) per workgroup
-hipError_t launch_window(const WinJob &job, int band, bool rows, Sample kind, hipStream_t s);
 hipError_t launch_window_table(const WinJob *jobs, int count, long items, int band, int width,
-                               bool rows, bool block16, hipStream_t s);
-template <int W, int K>
-hipError_t launch_window_band(const WinJob &job, bool rows, Sample kind, hipStream_t s);
+                               bool rows, bool block16, Sample kind, hipStream_t s);
 template <int W, int K>
 hipError_t launch_window_table_band(const WinJob *jobs, int count, long items, bool rows,
-                                    bool block16, hipStream_t s);
+                                    bool block16, Sample kind, hipStream_t s);
 #define DSMVC_WINDOW_EXTERN(W, K)                                                                 \
-    extern template hipError_t launch_window_band<W, K>(const WinJob &, bool, Sample,             \
-                                                        hipStream_t);                             \
     extern template hipError_t launch_window_table_band<W, K>(const WinJob *, int, long, bool,    \
-                                                              bool, hipStream_t);
+                                                              bool, Sample, hipStream_t);
 DSMVC_WINDOW_EXTERN(1, 8)
 DSMVC_WINDOW_EXTERN(1, 16)
 DSMVC_WINDOW_EXTERN(3, 8)

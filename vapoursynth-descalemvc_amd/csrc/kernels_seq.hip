@@ -175,27 +175,15 @@ int window_band_class(int w) {
     return 0;
 }
 
-hipError_t launch_window(const WinJob &job, int band, bool rows, Sample kind, hipStream_t s) {
-    const bool wide = job.plan.ell_width == 16;
-    if (!wide && job.plan.ell_width != 8) return hipErrorInvalidValue;
-    switch (band) {
-    case 1: return wide ? launch_window_band<1, 16>(job, rows, kind, s) : launch_window_band<1, 8>(job, rows, kind, s);
-    case 3: return wide ? launch_window_band<3, 16>(job, rows, kind, s) : launch_window_band<3, 8>(job, rows, kind, s);
-    case 5: return wide ? launch_window_band<5, 16>(job, rows, kind, s) : launch_window_band<5, 8>(job, rows, kind, s);
-    case 7: return wide ? launch_window_band<7, 16>(job, rows, kind, s) : launch_window_band<7, 8>(job, rows, kind, s);
-    default: return hipErrorInvalidValue;
-    }
-}
-
 hipError_t launch_window_table(const WinJob *jobs, int count, long items, int band, int width,
-                               bool rows, bool block16, hipStream_t s) {
+                               bool rows, bool block16, Sample kind, hipStream_t s) {
     if (width != 8 && width != 16) return hipErrorInvalidValue;
     const bool wide = width == 16;
     switch (band) {
-    case 1: return wide ? launch_window_table_band<1, 16>(jobs, count, items, rows, block16, s) : launch_window_table_band<1, 8>(jobs, count, items, rows, block16, s);
-    case 3: return wide ? launch_window_table_band<3, 16>(jobs, count, items, rows, block16, s) : launch_window_table_band<3, 8>(jobs, count, items, rows, block16, s);
-    case 5: return wide ? launch_window_table_band<5, 16>(jobs, count, items, rows, block16, s) : launch_window_table_band<5, 8>(jobs, count, items, rows, block16, s);
-    case 7: return wide ? launch_window_table_band<7, 16>(jobs, count, items, rows, block16, s) : launch_window_table_band<7, 8>(jobs, count, items, rows, block16, s);
+    case 1: return wide ? launch_window_table_band<1, 16>(jobs, count, items, rows, block16, kind, s) : launch_window_table_band<1, 8>(jobs, count, items, rows, block16, kind, s);
+    case 3: return wide ? launch_window_table_band<3, 16>(jobs, count, items, rows, block16, kind, s) : launch_window_table_band<3, 8>(jobs, count, items, rows, block16, kind, s);
+    case 5: return wide ? launch_window_table_band<5, 16>(jobs, count, items, rows, block16, kind, s) : launch_window_table_band<5, 8>(jobs, count, items, rows, block16, kind, s);
+    case 7: return wide ? launch_window_table_band<7, 16>(jobs, count, items, rows, block16, kind, s) : launch_window_table_band<7, 8>(jobs, count, items, rows, block16, kind, s);
     default: return hipErrorInvalidValue;
     }
 }

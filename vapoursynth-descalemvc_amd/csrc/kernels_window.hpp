@@ -11,8 +11,7 @@
 // of the window from a zero state, and stores the chunk's L rows.  Chunks
 // never exchange carries, so a workgroup's wavefronts are independent (no
 // workgroup barrier) and one launch can hold items of many plans.  The four
-// wavefronts of a workgroup take consecutive items, which are consecutive
-// vector groups of the same chunk, so they read the same plan rows.
+// wavefronts of a workgroup take consecutive items (decode_item).
 //
 // Vertical pass (columns_window): lanes are 64 adjacent columns; each block
 // of G rows stages the 32 source rows its taps need (256-byte runs) into the
@@ -41,6 +40,9 @@ struct ItemPos {
     int frame, chunk, group;
 };
 
+// Items run vector group fastest: a workgroup's four wavefronts usually
+// solve the same chunk of one plan for neighbouring vectors, so they read the
+// same records through the scalar cache.
 __device__ __forceinline__ ItemPos decode_item(const WinJob &jb, int local) {
     const int per_frame = jb.plan.chunks * jb.groups;
     ItemPos p;
@@ -51,13 +53,79 @@ __device__ __forceinline__ ItemPos decode_item(const WinJob &jb, int local) {
     return p;
 }
 
+// Forward sweep of one batch of B rows whose taps are in tap[u][e]: each
+// row's record (weights, factors, inverse diagonal) is one wide scalar load,
+// all B issued together with the batch's LDS reads, so the wave waits once
+// per batch.  The arithmetic is the ordered one: the gather FMA chain in
+// ascending tap order, then the forward terms d = W..1, then the diagonal.
+template <int W, int K, int B>
+__device__ __forceinline__ void forward_batch(ConstPtr<float> rec, const float (&tap)[B][K],
+                                              float (&st)[W], float (&out)[B]) {
+    constexpr int RW = window_fwd_words(W, K);
+    float c[B][K + W + 1];
+#pragma unroll
+    for (int u = 0; u < B; ++u)
+#pragma unroll
+        for (int i = 0; i < K + W + 1; ++i) c[u][i] = rec[u * RW + i];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+        float v = 0.0F;
+#pragma unroll
+        for (int e = 0; e < K; ++e) v = __builtin_fmaf(c[u][e], tap[u][e], v);
+#pragma unroll
+        for (int d = W; d >= 1; --d) v = __builtin_fmaf(-c[u][K + d - 1], st[d - 1], v);
+        const float y = __fmul_rn(v, c[u][K + W]);
+        push(st, y);
+        out[u] = y;
+    }
+}
+
+// Backward sweep over y[0, L + P) from the end, a batch of rows' backward
+// records per group of scalar loads; record row of y[k] is k + P.  W == 3
+// sums d ascending, other bands descending, as the reference solver does.
+template <int W, int L, int P>
+__device__ __forceinline__ void backward_sweep(ConstPtr<float> bwd, float (&y)[L + P]) {
+    constexpr int BW = window_bwd_words(W);
+    constexpr int BB = window_bwd_batch(W);
+    float st[W];
+#pragma unroll
+    for (int d = 0; d < W; ++d) st[d] = 0.0F;
+#pragma unroll
+    for (int top = L + P; top > 0; top -= BB) {
+        const int count = top < BB ? top : BB;
+        float c[BB][W];
+#pragma unroll
+        for (int u = 0; u < BB; ++u)
+            if (u < count)
+#pragma unroll
+                for (int d = 0; d < W; ++d) c[u][d] = bwd[(top - 1 - u + P) * BW + d];
+#pragma unroll
+        for (int u = 0; u < BB; ++u) {
+            if (u >= count) break;
+            const int k = top - 1 - u;
+            float v = y[k];
+            if constexpr (W == 3) {
+#pragma unroll
+                for (int d = 1; d <= W; ++d) v = __builtin_fmaf(-c[u][d - 1], st[d - 1], v);
+            } else {
+#pragma unroll
+                for (int d = W; d >= 1; --d) v = __builtin_fmaf(-c[u][d - 1], st[d - 1], v);
+            }
+            push(st, v);
+            y[k] = v;
+        }
+    }
+}
+
 template <int W, int K, int G, class In>
 __device__ __forceinline__ void rows_item(const WinJob &jb, const ItemPos &at, float *ring,
                                           int lane) {
     constexpr int L = window_length(W);
     constexpr int P = window_warmup(W);
     constexpr int NB = (L + 2 * P) / G;
-    static_assert((L + 2 * P) % G == 0 && L % G == 0, "blocks tile the window");
+    constexpr int RW = window_fwd_words(W, K);
+    constexpr int B = window_fwd_batch(W, K);
+    static_assert((L + 2 * P) % G == 0 && L % G == 0 && G % B == 0, "blocks tile the window");
     const DevWindow &pl = jb.plan;
     const int nrows = jb.nvec;
     const int row0 = at.group * 64;
@@ -70,8 +138,7 @@ __device__ __forceinline__ void rows_item(const WinJob &jb, const ItemPos &at, f
     // Plan tables through the constant address space: the loads stay scalar
     // across the inline-asm pins below (which count as memory side effects).
     const auto ts = constant(pl.tap_start + r0);
-    const auto tw = constant(pl.tap_w + static_cast<long>(r0) * K);
-    const auto cf = constant(pl.coef + static_cast<long>(r0) * Band<W>::kStride);
+    const auto fr = constant(pl.fwd + static_cast<long>(r0) * RW);
     const auto blk = constant((G == 16 ? pl.blk_lo16 : pl.blk_lo) + r0 / G);
     const DevConv cv = jb.conv;
     const float *mine = ring + lane * kRowPitch;   // this lane's row of the ring
@@ -99,6 +166,9 @@ __device__ __forceinline__ void rows_item(const WinJob &jb, const ItemPos &at, f
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
         const int lo = blk[b];
+        int start[G];   // the block's tap runs, one scalar load
+#pragma unroll
+        for (int t = 0; t < G; ++t) start[t] = ts[b * G + t] - lo;
         wave_sync();
 #pragma unroll
         for (int t = 0; t < 32; ++t) {
@@ -110,27 +180,25 @@ __device__ __forceinline__ void rows_item(const WinJob &jb, const ItemPos &at, f
         wave_sync();
         if (b + 1 < NB) fetch(b + 1);
 #pragma unroll
-        for (int t = 0; t < G; ++t) {
-            const int k = b * G + t;
-            // The row's K taps are consecutive columns of this lane's ring
-            // row: one address, K immediate offsets, all reads in flight
-            // before the FMA chain (ascending column = CSR order).
-            const float *at = mine + (ts[k] - lo);
-            float tap[K];
+        for (int j = 0; j < G; j += B) {
+            // Each row's K taps are consecutive columns of this lane's ring
+            // row: one address, K immediate offsets (ascending column = CSR
+            // order).
+            float tap[B][K];
 #pragma unroll
-            for (int e = 0; e < K; ++e) tap[e] = at[e];
-            float v = 0.0F;
+            for (int u = 0; u < B; ++u) {
+                const float *a = mine + start[j + u];
 #pragma unroll
-            for (int e = 0; e < K; ++e) v = __builtin_fmaf(tw[k * K + e], tap[e], v);
-            const float yv = forward_step<W>(cf + k * Band<W>::kStride, v, st);
-            if (k >= P) y[k - P] = yv;
+                for (int e = 0; e < K; ++e) tap[u][e] = a[e];
+            }
+            float out[B];
+            forward_batch<W, K, B>(fr + (b * G + j) * RW, tap, st, out);
+#pragma unroll
+            for (int u = 0; u < B; ++u)
+                if (b * G + j + u >= P) y[b * G + j + u - P] = out[u];
         }
     }
-#pragma unroll
-    for (int d = 0; d < W; ++d) st[d] = 0.0F;
-#pragma unroll
-    for (int k = L + P - 1; k >= 0; --k)
-        y[k] = backward_step<W>(cf + (k + P) * Band<W>::kStride, y[k], st);
+    backward_sweep<W, L, P>(constant(pl.bwd + static_cast<long>(r0) * window_bwd_words(W)), y);
 
     // Results leave through the ring, 32 destination columns at a time:
     // lanes 0-31 write row t, lanes 32-63 row t + 1 (128-byte runs).
@@ -162,7 +230,9 @@ __device__ __forceinline__ void columns_item(const WinJob &jb, const ItemPos &at
     constexpr int L = window_length(W);
     constexpr int P = window_warmup(W);
     constexpr int NB = (L + 2 * P) / G;
-    static_assert((L + 2 * P) % G == 0 && L % G == 0, "blocks tile the window");
+    constexpr int RW = window_fwd_words(W, K);
+    constexpr int B = window_fwd_batch(W, K);
+    static_assert((L + 2 * P) % G == 0 && L % G == 0 && G % B == 0, "blocks tile the window");
     static_assert(kStageCols * 64 <= kWinRing, "ring holds a block");
     const DevWindow &pl = jb.plan;
     const int ncols = jb.nvec;
@@ -176,8 +246,7 @@ __device__ __forceinline__ void columns_item(const WinJob &jb, const ItemPos &at
     const int row_bytes = static_cast<int>(jb.s_row * 4);
     const int m1 = pl.m - 1;
     const auto ts = constant(pl.tap_start + r0);
-    const auto tw = constant(pl.tap_w + static_cast<long>(r0) * K);
-    const auto cf = constant(pl.coef + static_cast<long>(r0) * Band<W>::kStride);
+    const auto fr = constant(pl.fwd + static_cast<long>(r0) * RW);
     const auto blk = constant((G == 16 ? pl.blk_lo16 : pl.blk_lo) + r0 / G);
     const float *mine = ring + lane;   // this lane's column of the ring
 
@@ -201,6 +270,9 @@ __device__ __forceinline__ void columns_item(const WinJob &jb, const ItemPos &at
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
         const int lo = blk[b];
+        int start[G];
+#pragma unroll
+        for (int t = 0; t < G; ++t) start[t] = ts[b * G + t] - lo;
         wave_sync();
 #pragma unroll
         for (int t = 0; t < 32; ++t) {
@@ -210,24 +282,22 @@ __device__ __forceinline__ void columns_item(const WinJob &jb, const ItemPos &at
         wave_sync();
         if (b + 1 < NB) fetch(b + 1);
 #pragma unroll
-        for (int t = 0; t < G; ++t) {
-            const int k = b * G + t;
-            const float *row = mine + (ts[k] - lo) * 64;
-            float tap[K];
+        for (int j = 0; j < G; j += B) {
+            float tap[B][K];
 #pragma unroll
-            for (int e = 0; e < K; ++e) tap[e] = row[e * 64];
-            float v = 0.0F;
+            for (int u = 0; u < B; ++u) {
+                const float *row = mine + start[j + u] * 64;
 #pragma unroll
-            for (int e = 0; e < K; ++e) v = __builtin_fmaf(tw[k * K + e], tap[e], v);
-            const float yv = forward_step<W>(cf + k * Band<W>::kStride, v, st);
-            if (k >= P) y[k - P] = yv;
+                for (int e = 0; e < K; ++e) tap[u][e] = row[e * 64];
+            }
+            float out[B];
+            forward_batch<W, K, B>(fr + (b * G + j) * RW, tap, st, out);
+#pragma unroll
+            for (int u = 0; u < B; ++u)
+                if (b * G + j + u >= P) y[b * G + j + u - P] = out[u];
         }
     }
-#pragma unroll
-    for (int d = 0; d < W; ++d) st[d] = 0.0F;
-#pragma unroll
-    for (int k = L + P - 1; k >= 0; --k)
-        y[k] = backward_step<W>(cf + (k + P) * Band<W>::kStride, y[k], st);
+    backward_sweep<W, L, P>(constant(pl.bwd + static_cast<long>(r0) * window_bwd_words(W)), y);
 
     Out *o = static_cast<Out *>(jb.dst) + static_cast<long>(at.frame) * jb.d_frame + cc;
     const int n = pl.n;
@@ -235,27 +305,6 @@ __device__ __forceinline__ void columns_item(const WinJob &jb, const ItemPos &at
 #pragma unroll
     for (int k = 0; k < L; ++k)
         if (r0 + k < n && live) Store<Out>::f32(o + static_cast<long>(r0 + k) * jb.d_row, y[k], cv);
-}
-
-// One job: item = blockIdx.x * kWindowWaves + wave.
-template <int W, int K, int G, class In>
-__global__ void __launch_bounds__(64 * kWindowWaves) rows_window(WinJob jb, int items) {
-    __shared__ float lds[kWindowWaves * kWinRing];
-    const int lane = threadIdx.x & 63;
-    const int wave = uniform(threadIdx.x >> 6);
-    const int item = static_cast<int>(blockIdx.x) * kWindowWaves + wave;
-    if (item >= items) return;
-    rows_item<W, K, G, In>(jb, decode_item(jb, item), lds + wave * kWinRing, lane);
-}
-
-template <int W, int K, int G, class Out>
-__global__ void __launch_bounds__(64 * kWindowWaves) columns_window(WinJob jb, int items) {
-    __shared__ float lds[kWindowWaves * kWinRing];
-    const int lane = threadIdx.x & 63;
-    const int wave = uniform(threadIdx.x >> 6);
-    const int item = static_cast<int>(blockIdx.x) * kWindowWaves + wave;
-    if (item >= items) return;
-    columns_item<W, K, G, Out>(jb, decode_item(jb, item), lds + wave * kWinRing, lane);
 }
 
 // A job record, read with scalar loads.
@@ -280,10 +329,11 @@ __device__ __forceinline__ int find_job(const WinJob *__restrict__ jobs, int cou
     return uniform(lo);
 }
 
-// Many jobs in one launch: the grid covers every item of every job in the
-// table; wave w of workgroup b takes item b * kWindowWaves + w and finds its
-// job by a binary search over the table.  Each wave solves one item and
-// exits, so the hardware dispatcher balances plans of uneven size.
+// One launch over a device table of jobs (one job or many): the grid covers
+// every item of every job; wave w of workgroup b takes item b * kWindowWaves
+// + w and finds its job by a binary search over the table.  Each wave solves
+// one item and exits, so the hardware dispatcher balances plans of uneven
+// size.
 template <int W, int K, int G, class In>
 __global__ void __launch_bounds__(64 * kWindowWaves)
 rows_window_table(const WinJob *__restrict__ jobs, int count, long items) {
@@ -310,27 +360,6 @@ columns_window_table(const WinJob *__restrict__ jobs, int count, long items) {
                                lds + wave * kWinRing, lane);
 }
 
-inline long job_items(const WinJob &jb) {
-    return static_cast<long>(jb.frames) * jb.plan.chunks * jb.groups;
-}
-
-template <int W, int K, class T>
-hipError_t launch_one(const WinJob &jb, bool rows, hipStream_t s) {
-    const long items = job_items(jb);
-    if (items <= 0) return hipSuccess;
-    if (items > 0x7FFFFFFFL) return hipErrorInvalidValue;
-    const dim3 grid(static_cast<unsigned>((items + kWindowWaves - 1) / kWindowWaves));
-    if (!rows && jb.plan.block16)
-        columns_window<W, K, 16, T><<<grid, 64 * kWindowWaves, 0, s>>>(jb, static_cast<int>(items));
-    else if (!rows)
-        columns_window<W, K, 8, T><<<grid, 64 * kWindowWaves, 0, s>>>(jb, static_cast<int>(items));
-    else if (jb.plan.block16)
-        rows_window<W, K, 16, T><<<grid, 64 * kWindowWaves, 0, s>>>(jb, static_cast<int>(items));
-    else
-        rows_window<W, K, 8, T><<<grid, 64 * kWindowWaves, 0, s>>>(jb, static_cast<int>(items));
-    return hipGetLastError();
-}
-
 // `block16`: every job of the table has 16-row staging blocks.
 template <int W, int K, class T>
 hipError_t launch_table(const WinJob *jobs, int count, long items, bool rows, bool block16,
@@ -350,31 +379,18 @@ hipError_t launch_table(const WinJob *jobs, int count, long items, bool rows, bo
     return hipGetLastError();
 }
 
-// Horizontal pass: the kind is the input sample (output F32).  Vertical
-// pass: the kind is the output sample (input F32).
-template <int W, int K>
-hipError_t dispatch_one(const WinJob &jb, bool rows, Sample kind, hipStream_t s) {
-    switch (kind) {
-    case Sample::f32: return launch_one<W, K, float>(jb, rows, s);
-    case Sample::u8: return launch_one<W, K, std::uint8_t>(jb, rows, s);
-    case Sample::u16: return launch_one<W, K, std::uint16_t>(jb, rows, s);
-    default: return hipErrorInvalidValue;
-    }
-}
-
 } // namespace
 } // namespace dev
 
 template <int W, int K>
-hipError_t launch_window_band(const WinJob &job, bool rows, Sample kind, hipStream_t s) {
-    return dev::dispatch_one<W, K>(job, rows, kind, s);
-}
-
-// Multi-job launches run F32 samples only (the batched and served formats).
-template <int W, int K>
 hipError_t launch_window_table_band(const WinJob *jobs, int count, long items, bool rows,
-                                    bool block16, hipStream_t s) {
-    return dev::launch_table<W, K, float>(jobs, count, items, rows, block16, s);
+                                    bool block16, Sample kind, hipStream_t s) {
+    switch (kind) {
+    case Sample::f32: return dev::launch_table<W, K, float>(jobs, count, items, rows, block16, s);
+    case Sample::u8: return dev::launch_table<W, K, std::uint8_t>(jobs, count, items, rows, block16, s);
+    case Sample::u16: return dev::launch_table<W, K, std::uint16_t>(jobs, count, items, rows, block16, s);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 } // namespace dsmvc::hip

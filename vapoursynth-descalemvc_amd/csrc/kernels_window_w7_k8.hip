@@ -2,7 +2,6 @@
 #include "kernels_window.hpp"
 
 namespace dsmvc::hip {
-template hipError_t launch_window_band<7, 8>(const WinJob &, bool, Sample, hipStream_t);
 template hipError_t launch_window_table_band<7, 8>(const WinJob *, int, long, bool, bool,
-                                                        hipStream_t);
+                                                        Sample, hipStream_t);
 } // namespace dsmvc::hip

@@ -71,7 +71,13 @@ HipServer::~HipServer() {
     if (worker_.joinable()) worker_.join();
     (void)hipSetDevice(executor_.device());
     for (auto &slot : slots_) {
-        if (slot->stream) (void)hipStreamSynchronize(slot->stream);
+        if (slot->stream) {
+            (void)hipStreamSynchronize(slot->stream);
+            try {
+                executor_.release_stream(slot->stream);   // Float64 planes ran on it
+            } catch (...) {
+            }
+        }
         slot->tables.reset();
         if (slot->done) (void)hipEventDestroy(slot->done);
         if (slot->stream) (void)hipStreamDestroy(slot->stream);
@@ -277,7 +283,7 @@ bool HipServer::admit_round(Slot &slot, std::unique_lock<std::mutex> &lock) {
                 a.frames = 1;
                 a.groups = (a.nvec + 63) / 64;
                 a.conv = identity_conv();
-                tables.add(a, hip::window_band_class(h.half_bandwidth), true);
+                tables.add(a, hip::window_band_class(h.half_bandwidth), true, hip::Sample::f32);
                 hip::WinJob b{};
                 b.plan = *executor_.plan_image(v).window(false);
                 b.src = inter;
@@ -288,7 +294,7 @@ bool HipServer::admit_round(Slot &slot, std::unique_lock<std::mutex> &lock) {
                 b.frames = 1;
                 b.groups = (b.nvec + 63) / 64;
                 b.conv = identity_conv();
-                tables.add(b, hip::window_band_class(v.half_bandwidth), false);
+                tables.add(b, hip::window_band_class(v.half_bandwidth), false, hip::Sample::f32);
                 ++table_planes;
             }
         }

@@ -242,6 +242,14 @@ class Executor:
     def tune(self, rows_chunks: int = 0, columns_chunks: int = 0):
         check(load().dsmvc_hip_executor_tune(self._handle, rows_chunks, columns_chunks))
 
+    def tuned_route(self, plan: Plan, rows: bool, sample_kind: int = 0) -> int:
+        """The autotuner's route for a plan and direction: a chunk count, -1
+        (windowed kernels), or 0 when not tuned yet."""
+        route = ctypes.c_int32()
+        check(load().dsmvc_hip_executor_tuned_route(self._handle, plan.handle, int(rows),
+                                                    sample_kind, ctypes.byref(route)))
+        return route.value
+
     def set_refinement(self, steps: int):
         """Refinement steps per axis for executors in "refined" mode."""
         check(load().dsmvc_hip_executor_set_refinement(self._handle, steps))
@@ -299,6 +307,41 @@ class Executor:
             xb.stride(0), _dev_ptr(ob), ob.stride(1), ob.stride(0), frames, conv,
             _stream_handle(stream), ASYNC if asynchronous else 0))
         return out if batched or out.dim() == 2 else out[0]
+
+
+    def inverse_2d_planes(self, jobs, stream=None, asynchronous=False):
+        """Several planes in one set of launches (dsmvc_hip_inverse_2d_planes).
+        jobs: (horizontal plan, vertical plan, x, conversion or None) with x
+        [H, W] or [frames, H, W], all of one dtype.  Returns the outputs."""
+        torch = _torch()
+        lib = load()
+        table = (_lib.PlaneJobC * max(1, len(jobs)))()
+        outs, keep = [], []
+        kind = None
+        for k, (hp, vp, x, conversion) in enumerate(jobs):
+            xb = x if x.dim() == 3 else x.unsqueeze(0)
+            frames, rows, cols = xb.shape
+            if rows != vp.source_size or cols < hp.source_size or xb.stride(2) != 1:
+                raise ValueError("dsmvc: input plane does not match the plans")
+            k_ = {torch.float32: 0, torch.uint8: 1, torch.uint16: 2}.get(xb.dtype)
+            if k_ is None or (kind is not None and k_ != kind):
+                raise ValueError("dsmvc: planes must share one sample type")
+            kind = k_
+            out = torch.empty((frames, vp.destination_size, hp.destination_size), dtype=xb.dtype,
+                              device=xb.device)
+            conv = conversion.to_c() if conversion is not None else None
+            keep.append(conv)
+            table[k] = _lib.PlaneJobC(hp.handle, vp.handle, _dev_ptr(xb), xb.stride(1),
+                                      xb.stride(0), _dev_ptr(out), out.stride(1), out.stride(0),
+                                      frames, 0,
+                                      ctypes.cast(ctypes.pointer(conv), ctypes.c_void_p)
+                                      if conv is not None else None)
+            outs.append(out if x.dim() == 3 else out[0])
+        check(lib.dsmvc_hip_inverse_2d_planes(self._handle, table, len(jobs),
+                                              0 if kind is None else kind,
+                                              _stream_handle(stream),
+                                              ASYNC if asynchronous else 0))
+        return outs
 
 
 class Batch:
@@ -521,6 +564,22 @@ def Descale(src, width: int, height: int, kernel: str = "bicubic", taps: int = 3
                                 src_height * float(ch) / float(sh),
                                 src_top * float(ch) / float(sh), **kw))
     ex = _executor(solve)
+    if process_h and process_v:
+        # Every plane of the frame batch in one set of launches.
+        jobs = []
+        for index, plane in enumerate(clip.planes):
+            chroma = index > 0
+            conv = None
+            if plane.dtype != _torch().float32:
+                conv = integer_conversion(clip.bits or (8 if plane.dtype == _torch().uint8 else 16),
+                                          chroma and clip.color_family == "yuv", clip.full_range)
+            jobs.append((Plan(hreq[1 if chroma and len(hreq) > 1 else 0]),
+                         Plan(vreq[1 if chroma and len(vreq) > 1 else 0]), plane, conv))
+        out_planes = ex.inverse_2d_planes(jobs)
+        if isinstance(src, Clip):
+            return Clip(out_planes, clip.color_family, clip.subsampling_w, clip.subsampling_h,
+                        clip.bits, clip.full_range)
+        return out_planes[0]
     out_planes = []
     for index, plane in enumerate(clip.planes):
         chroma = index > 0

@@ -21,15 +21,17 @@ EXPORTS = (
     "dsmvc_hip_clear_planner_caches", "dsmvc_hip_padding_index", "dsmvc_hip_parse_backend",
     "dsmvc_hip_resolve_backend", "dsmvc_hip_device_count", "dsmvc_hip_executor_create",
     "dsmvc_hip_executor_destroy", "dsmvc_hip_executor_prepare", "dsmvc_hip_executor_seal",
-    "dsmvc_hip_executor_tune", "dsmvc_hip_executor_set_refinement", "dsmvc_hip_inverse_rows",
+    "dsmvc_hip_executor_tune", "dsmvc_hip_executor_set_refinement",
+    "dsmvc_hip_executor_tuned_route", "dsmvc_hip_inverse_rows",
     "dsmvc_hip_inverse_columns",
     "dsmvc_hip_inverse_2d", "dsmvc_hip_inverse_2d_u8", "dsmvc_hip_inverse_2d_u16",
     "dsmvc_hip_inverse_2d_batch", "dsmvc_hip_inverse_rows_batch",
-    "dsmvc_hip_inverse_columns_batch", "dsmvc_hip_batch_create", "dsmvc_hip_batch_run",
+    "dsmvc_hip_inverse_columns_batch", "dsmvc_hip_inverse_2d_planes",
+    "dsmvc_hip_batch_create", "dsmvc_hip_batch_run",
     "dsmvc_hip_batch_info_get", "dsmvc_hip_batch_destroy", "dsmvc_hip_server_create",
     "dsmvc_hip_server_submit", "dsmvc_hip_server_wait", "dsmvc_hip_server_poll",
     "dsmvc_hip_server_release", "dsmvc_hip_server_stats_get", "dsmvc_hip_server_destroy",
-    "dsmvc_hip_synchronize",
+    "dsmvc_hip_executor_release_stream", "dsmvc_hip_synchronize",
 )
 BATCH_GRAPH = 2
 
@@ -71,6 +73,22 @@ class WindowInfoC(ctypes.Structure):
         ("width", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
         ("residual", ctypes.c_double),
+    ]
+
+
+class PlaneJobC(ctypes.Structure):
+    _fields_ = [
+        ("horizontal", ctypes.c_void_p),
+        ("vertical", ctypes.c_void_p),
+        ("input", ctypes.c_void_p),
+        ("input_row_stride", ctypes.c_ssize_t),
+        ("input_frame_stride", ctypes.c_ssize_t),
+        ("output", ctypes.c_void_p),
+        ("output_row_stride", ctypes.c_ssize_t),
+        ("output_frame_stride", ctypes.c_ssize_t),
+        ("frames", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("conversion", ctypes.c_void_p),
     ]
 
 
@@ -208,6 +226,7 @@ def load():
     lib.dsmvc_hip_executor_seal.argtypes = [vp]
     lib.dsmvc_hip_executor_tune.argtypes = [vp, i32, i32]
     lib.dsmvc_hip_executor_set_refinement.argtypes = [vp, i32]
+    lib.dsmvc_hip_executor_tuned_route.argtypes = [vp, vp, i32, i32, ctypes.POINTER(i32)]
     for name in ("dsmvc_hip_inverse_rows", "dsmvc_hip_inverse_columns"):
         getattr(lib, name).argtypes = [vp, vp, vp, pd, vp, pd, i32, vp, u32]
     lib.dsmvc_hip_inverse_2d.argtypes = [vp, vp, vp, vp, pd, vp, pd, vp, u32]
@@ -220,6 +239,7 @@ def load():
                                                  ctypes.POINTER(ConversionC), vp, u32]
     lib.dsmvc_hip_inverse_columns_batch.argtypes = [vp, vp, vp, pd, pd, i32, vp, pd, pd, i32,
                                                     i32, ctypes.POINTER(ConversionC), vp, u32]
+    lib.dsmvc_hip_inverse_2d_planes.argtypes = [vp, ctypes.POINTER(PlaneJobC), i32, i32, vp, u32]
     lib.dsmvc_hip_batch_create.argtypes = [vp, ctypes.POINTER(BatchJobC), i32, u32,
                                            ctypes.POINTER(vp)]
     lib.dsmvc_hip_batch_run.argtypes = [vp, vp, u32]
@@ -236,6 +256,7 @@ def load():
     lib.dsmvc_hip_server_stats_get.argtypes = [vp, ctypes.POINTER(ServerStatsC)]
     lib.dsmvc_hip_server_destroy.argtypes = [vp]
     lib.dsmvc_hip_server_destroy.restype = None
+    lib.dsmvc_hip_executor_release_stream.argtypes = [vp, vp]
     lib.dsmvc_hip_synchronize.argtypes = [vp, vp]
     _lib = lib
     return lib
