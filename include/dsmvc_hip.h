@@ -91,7 +91,8 @@ typedef struct dsmvc_hip_window_info {
     int32_t length;          /* destination rows per chunk (L) */
     int32_t warmup;          /* warm-up rows on each side (P) */
     int32_t width;           /* packed taps per row: 8 or 16 */
-    int32_t reserved;
+    int32_t float64_feasible; /* 1: the windowed FP64 kernels take this plan on
+                                 the Float64 route of F32 planes (2^-40 check) */
     double residual;         /* worst influence of a chunk's zero start, relative */
 } dsmvc_hip_window_info;
 

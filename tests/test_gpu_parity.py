@@ -270,6 +270,56 @@ def test_float64_route_within_one_ulp(executors, oracle):
         assert int(ulp_distance(got, expected).max()) <= 1
 
 
+def test_float64_windowed_route_matches_sequential(executors, oracle):
+    """Float64 route on plans whose FP64 recurrence decays within the windowed
+    FP64 kernels' warm-up (bicubic W=3, lanczos3 W=5, bilinear W=1, mixed with
+    an F32 axis).  Bars: <= 1 ULP vs the oracle's Float64 route and vs the
+    sequential Float64 kernel, with at most 1 in 1000 outputs differing from
+    the sequential kernel at all."""
+    cases = [
+        (make_request(1920, 1280, kernel=BICUBIC, f64_mode=F64_ALWAYS),
+         make_request(1080, 720, kernel=BICUBIC, f64_mode=F64_ALWAYS)),
+        (make_request(1280, 1216, kernel=LANCZOS, taps=3, f64_mode=F64_ALWAYS),
+         make_request(720, 684, kernel=LANCZOS, taps=3, f64_mode=F64_ALWAYS)),
+        (make_request(640, 448, kernel=BILINEAR, f64_mode=F64_ALWAYS),
+         make_request(360, 252, kernel=BICUBIC, f64_mode=F64_NEVER)),
+    ]
+    for h, v in cases:
+        hp, vp = dh.Plan(_hreq(h)), dh.Plan(_hreq(v))
+        assert hp.window_info()["float64_feasible"] == 1
+        x = np.random.default_rng(h.destination_size).uniform(
+            0, 1, (2, v.source_size, h.source_size)).astype(np.float32)
+        xt = torch.from_numpy(x).to(DEV)
+        seq = executors["exact"].inverse_2d(hp, vp, xt).cpu().numpy()
+        win = executors["auto"].inverse_2d(hp, vp, xt).cpu().numpy()
+        flips = ulp_distance(win, seq)
+        print(f"{h.destination_size}x{v.destination_size}: max ulp vs sequential "
+              f"{int(flips.max())}, differing {float((flips > 0).mean()):.2e}", flush=True)
+        assert int(flips.max()) <= 1 and float((flips > 0).mean()) <= 1e-3
+        expected = oracle.inverse_2d(oracle.build(h), oracle.build(v), x[1])
+        assert int(ulp_distance(win[1], expected).max()) <= 1
+
+
+def test_float64_windowed_route_is_taken(executors):
+    """The automatic executor runs a 4K Float64 plane on the windowed FP64
+    kernels: several times faster than the sequential Float64 kernel."""
+    import time
+    hp = dh.Plan(dh.AxisRequest(3840, 2048, 2048.0, f64_mode=2))
+    vp = dh.Plan(dh.AxisRequest(2160, 1152, 1152.0, f64_mode=2))
+    x = torch.rand(2160, 3840, device=DEV)
+    elapsed = {}
+    for mode in ("exact", "auto"):
+        executors[mode].inverse_2d(hp, vp, x)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            executors[mode].inverse_2d(hp, vp, x)
+        torch.cuda.synchronize()
+        elapsed[mode] = (time.perf_counter() - t0) / 3
+    print({k: round(v * 1e3, 3) for k, v in elapsed.items()}, "ms", flush=True)
+    assert elapsed["auto"] * 2 < elapsed["exact"]
+
+
 def test_float64_route_rejects_nonfinite(executors):
     h = dh.Plan(dh.AxisRequest(64, 40, 40.0, f64_mode=2))
     v = dh.Plan(dh.AxisRequest(48, 30, 30.0, f64_mode=2))

@@ -145,3 +145,81 @@ def test_near_identity_plans_report_their_decay():
             assert info["residual"] <= TOLERANCE
         else:
             assert info["residual"] > TOLERANCE or info["width"] == 0
+
+
+# ---- Float64 route --------------------------------------------------------
+# The windowed FP64 kernels run the Float64 route's recurrence (unit-L FP64
+# factors: z_i = b_i - sum_d L(i, i-d) z_{i-d}, y_i = z_i / (D_i + eps),
+# x_i = y_i - sum_d L(i+d, i) x_{i+d}; reference src/axis_plan.cpp:1124-1191)
+# on chunks of L rows with P warm-up rows per side, and accept a plan when
+# the wrong start has decayed below 2^-40.  Chunk shape per band class:
+F64_SHAPES = {1: (32, 32), 3: (32, 32), 5: (16, 48), 7: (16, 64)}
+TOLERANCE64 = 2.0 ** -40
+
+
+def _f64_factors(plan):
+    a = plan.arrays()
+    n, w = plan.destination_size, plan.half_bandwidth
+    bands = a["ldlt_bands_f64"].reshape(w + 1, n)
+    low, up = np.zeros((w, n)), np.zeros((w, n))
+    for d in range(1, w + 1):
+        low[d - 1, d:] = bands[d, : n - d]      # L(i, i - d), stored at column i - d
+        up[d - 1, : n - d] = bands[d, : n - d]  # L(i + d, i), stored at column i
+    return a, low, up, a["inverse_diagonal_f64"].astype(np.float64)
+
+
+def _forward_z(b, low, inv, start, stop, w):
+    """Unit-L forward recurrence over [start, stop) from a zero state; returns
+    the diagonal-scaled values."""
+    z = np.zeros((b.shape[0], stop - start))
+    for i in range(start, stop):
+        v = b[:, i].copy()
+        for d in range(1, w + 1):
+            if i - d >= start:
+                v -= low[d - 1, i] * z[:, i - d - start]
+        z[:, i - start] = v
+    return z * inv[start:stop]
+
+
+def _windowed64(b, low, up, inv, w, length, warmup):
+    n = b.shape[1]
+    out = np.zeros_like(b)
+    for a in range(0, n, length):
+        e = min(n, a + length)
+        f0, f1 = max(0, a - warmup), min(n, e + warmup)
+        y = _forward_z(b, low, inv, f0, f1, w)
+        x = _backward(y, up, a, f1, w, f0)
+        out[:, a:e] = x[:, : e - a]
+    return out
+
+
+F64_CASES = [
+    ("bilinear-1920-1280", make_request(1920, 1280, kernel=BILINEAR, f64_mode=2)),
+    ("bicubic-1080-756", make_request(1080, 756, kernel=BICUBIC, f64_mode=2)),
+    ("lanczos3-2160-1512", make_request(2160, 1512, kernel=LANCZOS, taps=3, f64_mode=2)),
+    ("spline36-1080-1026", make_request(1080, 1026, kernel=SPLINE36, f64_mode=2)),
+]
+
+
+@pytest.mark.parametrize("name,req", F64_CASES, ids=[c[0] for c in F64_CASES])
+def test_windowed_float64_solve_matches_sequential(name, req):
+    plan = _plan(req)
+    assert plan.requires_float64
+    info = plan.window_info()
+    assert info["float64_feasible"] == 1, info
+    a, low, up, inv = _f64_factors(plan)
+    w = plan.half_bandwidth
+    length, warmup = F64_SHAPES[info["band"]]
+    rng = np.random.default_rng(plan.source_size)
+    y = rng.uniform(0.0, 1.0, (3, plan.source_size))
+    off, idx = a["offsets"], a["indices"]
+    rows = np.repeat(np.arange(plan.destination_size), np.diff(off).astype(np.int64))
+    b = np.zeros((3, plan.destination_size))
+    np.add.at(b.T, rows, a["weights_f64"].astype(np.float64)[:, None] * y[:, idx].T)
+    exact = _backward(_forward_z(b, low, inv, 0, plan.destination_size, w), up, 0,
+                      plan.destination_size, w, 0)
+    scale = float(np.abs(exact).max())
+    err = float(np.abs(_windowed64(b, low, up, inv, w, length, warmup) - exact).max())
+    assert err <= 8 * TOLERANCE64 * scale, (err, scale)
+    short = _windowed64(b, low, up, inv, w, length, warmup // 4)
+    assert float(np.abs(short - exact).max()) > 8 * TOLERANCE64 * scale

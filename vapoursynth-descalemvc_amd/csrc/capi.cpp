@@ -160,6 +160,8 @@ int dsmvc_hip_plan_window_info(const dsmvc_hip_plan *plan, dsmvc_hip_window_info
         info->warmup = wp.warmup;
         info->width = wp.width;
         info->residual = wp.residual;
+        dsmvc::hip::WindowPlan64 wp64;
+        info->float64_feasible = dsmvc::hip::build_window_plan64(p, dsmvc::hip::pack_plan(p), wp64) ? 1 : 0;
     });
 }
 

@@ -197,7 +197,16 @@ bool build_chunk_plan(const AxisPlan &p, int max_chunks, ChunkPlan &out) {
     return true;
 }
 
-bool build_window_plan(const AxisPlan &p, WindowPlan &out) {
+namespace {
+
+// A windowed partition of either precision for `length`-row chunks with
+// `warm` warm-up rows.  `coef_of(i, slot)` is plan row i's coefficient record
+// in the kMaxBand layout and `weight_of(q)` CSR weight q.  `zform`: the
+// forward recurrence carries z = L^-1 b and the diagonal multiplies only the
+// stored value (retained FP64 factors).
+template <class T, class Coef, class Weight>
+bool build_window(const AxisPlan &p, int length, int warm, Coef coef_of, Weight weight_of,
+                  bool zform, double tolerance, WindowPlanT<T> &out) {
     const int n = p.destination_size, w = p.half_bandwidth;
     const int band = window_band_class(w);
     if (band == 0 || n < 1) return false;
@@ -207,7 +216,6 @@ bool build_window_plan(const AxisPlan &p, WindowPlan &out) {
                                                - p.transpose_offsets[static_cast<std::size_t>(i)]));
     const int width = chunked_width_for(std::max(taps, 1));
     if (width == 0) return false;
-    const int length = window_length(band), warm = window_warmup(band);
     const int chunks = (n + length - 1) / length;
     const int rows = chunks * length + 2 * warm;
     out.band = band;
@@ -215,21 +223,16 @@ bool build_window_plan(const AxisPlan &p, WindowPlan &out) {
     out.length = length;
     out.warmup = warm;
     out.width = width;
-    const auto un = static_cast<std::size_t>(n);
     const auto ur = static_cast<std::size_t>(rows);
     auto real = [&](int r) { return r - warm; };   // plan row of padded row r
 
-    out.coef.assign(ur * 16, 0.0F);
+    out.zform = zform;
+    out.coef.assign(ur * 16, T(0));
     for (int r = 0; r < rows; ++r) {
         const int i = real(r);
         if (i < 0 || i >= n) continue;
-        float *row = out.coef.data() + static_cast<std::size_t>(r) * 16;
-        row[kMaxBand] = p.inverse_diagonal[static_cast<std::size_t>(i)];
-        for (int d = 1; d <= w; ++d) {
-            const std::size_t at = static_cast<std::size_t>(d - 1) * un + static_cast<std::size_t>(i);
-            row[d - 1] = p.lower_ld[at];
-            row[kMaxBand + d] = p.upper_l[at];
-        }
+        T *row = out.coef.data() + static_cast<std::size_t>(r) * 16;
+        for (int slot = 0; slot < 16; ++slot) row[slot] = coef_of(i, slot);
     }
 
     // Staging windows of 8- and 16-row blocks; blocks of zero rows reuse a
@@ -266,7 +269,7 @@ bool build_window_plan(const AxisPlan &p, WindowPlan &out) {
     // whole row and lie inside the 8-row window (and the 16-row window when
     // those are used); otherwise the horizontal pass is not windowed.
     out.tap_start.assign(ur, 0);
-    out.tap_w.assign(ur * width, 0.0F);
+    out.tap_w.assign(ur * width, T(0));
     for (int r = 0; r < rows; ++r) {
         const int i = real(r);
         const int lo8 = out.blk_lo[static_cast<std::size_t>(r / kWindowBlock)];
@@ -279,10 +282,10 @@ bool build_window_plan(const AxisPlan &p, WindowPlan &out) {
                 const int first = p.transpose_indices[k0], last = p.transpose_indices[k1 - 1];
                 start = std::max(lo8, last - width + 1);
                 if (last - first + 1 > width) out.rows_feasible = false;
-                float *wt = out.tap_w.data() + static_cast<std::size_t>(r) * width;
+                T *wt = out.tap_w.data() + static_cast<std::size_t>(r) * width;
                 for (auto q = k0; q < k1; ++q) {
                     const int e = p.transpose_indices[q] - start;
-                    if (e >= 0 && e < width) wt[e] = p.transpose_weights[q];
+                    if (e >= 0 && e < width) wt[e] = weight_of(q);
                 }
             }
         }
@@ -294,12 +297,12 @@ bool build_window_plan(const AxisPlan &p, WindowPlan &out) {
     // Packed records: forward [tap weights | lld_1..lld_W | inverse diagonal],
     // backward [ul_1..ul_W], zero in padding rows and past the plan's band.
     const int fw = window_fwd_words(band, width), bw = window_bwd_words(band);
-    out.fwd.assign(ur * static_cast<std::size_t>(fw), 0.0F);
-    out.bwd.assign(ur * static_cast<std::size_t>(bw), 0.0F);
+    out.fwd.assign(ur * static_cast<std::size_t>(fw), T(0));
+    out.bwd.assign(ur * static_cast<std::size_t>(bw), T(0));
     for (int r = 0; r < rows; ++r) {
-        const float *c = out.coef.data() + static_cast<std::size_t>(r) * 16;
-        float *f = out.fwd.data() + static_cast<std::size_t>(r) * fw;
-        float *u = out.bwd.data() + static_cast<std::size_t>(r) * bw;
+        const T *c = out.coef.data() + static_cast<std::size_t>(r) * 16;
+        T *f = out.fwd.data() + static_cast<std::size_t>(r) * fw;
+        T *u = out.bwd.data() + static_cast<std::size_t>(r) * bw;
         std::copy_n(out.tap_w.data() + static_cast<std::size_t>(r) * width, width, f);
         for (int d = 1; d <= w; ++d) {
             f[width + d - 1] = c[d - 1];
@@ -308,7 +311,7 @@ bool build_window_plan(const AxisPlan &p, WindowPlan &out) {
         f[width + band] = c[kMaxBand];
     }
 
-    // Warm-up check, in FP64 on the F32 coefficients the kernels use: from
+    // Warm-up check, in FP64 on the coefficients the kernels use: from
     // each unit starting state P rows before a chunk boundary (forward) or
     // after it (backward), the state must stay below the tolerance over the
     // whole chunk.  Boundaries next to the plan's ends start in zero rows and
@@ -335,7 +338,7 @@ bool build_window_plan(const AxisPlan &p, WindowPlan &out) {
             for (int r = a - warm; r < a + length; ++r) {
                 double v = 0.0;
                 for (int d = band; d >= 1; --d) v -= coef(r, d - 1) * ring[static_cast<std::size_t>(d - 1)];
-                push(v * coef(r, kMaxBand));
+                push(zform ? v : v * coef(r, kMaxBand));
                 if (r >= a - 1) worst = std::max(worst, peak());
             }
         }
@@ -355,7 +358,38 @@ bool build_window_plan(const AxisPlan &p, WindowPlan &out) {
     }
     out.residual = worst;
     // Both passes stage 32 source samples per block, so the runs must fit.
-    return out.rows_feasible && worst <= kWindowTolerance;
+    return out.rows_feasible && worst <= tolerance;
+}
+
+} // namespace
+
+bool build_window_plan(const AxisPlan &p, WindowPlan &out) {
+    const int band = window_band_class(p.half_bandwidth);
+    if (band == 0) return false;
+    const auto un = static_cast<std::size_t>(p.destination_size);
+    const int w = p.half_bandwidth;
+    auto coef = [&](int i, int slot) -> float {
+        const auto ui = static_cast<std::size_t>(i);
+        if (slot == kMaxBand) return p.inverse_diagonal[ui];
+        const int d = slot < kMaxBand ? slot + 1 : slot - kMaxBand;
+        if (d > w || slot > 2 * kMaxBand) return 0.0F;
+        const std::size_t at = static_cast<std::size_t>(d - 1) * un + ui;
+        return slot < kMaxBand ? p.lower_ld[at] : p.upper_l[at];
+    };
+    auto weight = [&](std::uint32_t q) { return p.transpose_weights[q]; };
+    return build_window<float>(p, window_length(band), window_warmup(band), coef, weight, false,
+                               kWindowTolerance, out);
+}
+
+bool build_window_plan64(const AxisPlan &p, const PackedPlan &packed, WindowPlan64 &out) {
+    const int band = window_band_class(p.half_bandwidth);
+    if (band == 0 || packed.coef_stride != 16) return false;
+    auto coef = [&](int i, int slot) {
+        return packed.coef64[static_cast<std::size_t>(i) * 16 + static_cast<std::size_t>(slot)];
+    };
+    auto weight = [&](std::uint32_t q) { return packed.wt64[q]; };
+    return build_window<double>(p, window_length64(band), window_warmup64(band), coef, weight,
+                                packed.zform != 0, kWindowTolerance64, out);
 }
 
 DevicePlan::DevicePlan(const AxisPlan &plan, int device)
@@ -402,6 +436,7 @@ DevicePlan::~DevicePlan() {
     for (auto &entry : chunk_images_)
         if (entry.second.memory) (void)hipFree(entry.second.memory);
     if (window_image_.memory) (void)hipFree(window_image_.memory);
+    if (window64_image_.memory) (void)hipFree(window64_image_.memory);
     if (refine_memory_) (void)hipFree(refine_memory_);
     if (memory_) (void)hipFree(memory_);
 }
@@ -500,6 +535,48 @@ const DevWindow *DevicePlan::window(bool rows) {
     }
     if (!image.ok || (rows && !image.rows_feasible)) return nullptr;
     return &image.dev;
+}
+
+const DevWindow64 *DevicePlan::window64() {
+    std::lock_guard lock(mutex_);
+    Window64Image &image = window64_image_;
+    if (!image.built) {
+        image.built = true;
+        WindowPlan64 wp;
+        if (build_window_plan64(*plan_, pack_plan(*plan_), wp)) {
+            const std::size_t o_fwd = 0;
+            const std::size_t o_bwd = align256(o_fwd + wp.fwd.size() * 8);
+            const std::size_t o_ts = align256(o_bwd + wp.bwd.size() * 8);
+            const std::size_t o_blk = align256(o_ts + wp.tap_start.size() * 4);
+            const std::size_t total = align256(o_blk + wp.blk_lo.size() * 4);
+            std::vector<unsigned char> host(total, 0);
+            std::memcpy(host.data() + o_fwd, wp.fwd.data(), wp.fwd.size() * 8);
+            std::memcpy(host.data() + o_bwd, wp.bwd.data(), wp.bwd.size() * 8);
+            std::memcpy(host.data() + o_ts, wp.tap_start.data(), wp.tap_start.size() * 4);
+            std::memcpy(host.data() + o_blk, wp.blk_lo.data(), wp.blk_lo.size() * 4);
+            int current = 0;
+            check(hipGetDevice(&current), "hipGetDevice failed");
+            check(hipSetDevice(device_), "hipSetDevice failed");
+            check(hipMalloc(&image.memory, total), "window table allocation failed");
+            check(hipMemcpy(image.memory, host.data(), total, hipMemcpyHostToDevice),
+                  "window table upload failed");
+            (void)hipSetDevice(current);
+            auto *b = static_cast<unsigned char *>(image.memory);
+            image.dev.n = plan_->destination_size;
+            image.dev.m = plan_->source_size;
+            image.dev.length = wp.length;
+            image.dev.warmup = wp.warmup;
+            image.dev.chunks = wp.chunks;
+            image.dev.ell_width = wp.width;
+            image.dev.zform = wp.zform ? 1 : 0;
+            image.dev.fwd = reinterpret_cast<const double *>(b + o_fwd);
+            image.dev.bwd = reinterpret_cast<const double *>(b + o_bwd);
+            image.dev.tap_start = reinterpret_cast<const std::int32_t *>(b + o_ts);
+            image.dev.blk_lo = reinterpret_cast<const std::int32_t *>(b + o_blk);
+            image.ok = true;
+        }
+    }
+    return image.ok ? &image.dev : nullptr;
 }
 
 const DevRefine *DevicePlan::refine() {

@@ -48,33 +48,42 @@ struct ChunkPlan {
 // the chunked kernels (band, length or tap count out of range).
 bool build_chunk_plan(const AxisPlan &plan, int max_chunks, ChunkPlan &out);
 
-// A windowed partition (layout in device_types.hpp, DevWindow): padded
-// coefficient rows and ELL taps with P zero rows on each side, the staging
-// windows of the horizontal kernel, and the warm-up check.
-struct WindowPlan {
+// A windowed partition (layout in device_types.hpp, DevWindow and
+// DevWindow64): padded coefficient rows and tap runs with P zero rows on each
+// side, the staging windows and the warm-up check, in F32 (T = float) or for
+// the Float64 route (T = double).
+template <class T> struct WindowPlanT {
     std::int32_t band = 0;           // band class the kernels run (1, 3, 5, 7)
     std::int32_t chunks = 0;
     std::int32_t length = 0;
     std::int32_t warmup = 0;
-    std::int32_t width = 0;          // packed taps per row
-    bool rows_feasible = false;      // every 8-row block's taps fit 32 columns
+    std::int32_t width = 0;          // tap run length K
+    bool rows_feasible = false;      // every 8-row block's taps fit 32 samples
+    bool zform = false;              // forward state is z = L^-1 b (FP64 factors)
     double residual = 0.0;           // worst relative influence of a chunk's zero start
-    std::vector<float> coef;         // [rows][16] kMaxBand records (warm-up check)
-    std::vector<float> fwd;          // [rows][window_fwd_words(band, width)]
-    std::vector<float> bwd;          // [rows][window_bwd_words(band)]
+    std::vector<T> coef;             // [rows][16] kMaxBand records (warm-up check)
+    std::vector<T> fwd;              // [rows][window_fwd_words(band, width)]
+    std::vector<T> bwd;              // [rows][window_bwd_words(band)]
     std::vector<std::int32_t> blk_lo; // [rows / 8]
     std::vector<std::int32_t> blk_lo16; // [rows / 16]
-    bool block16 = false;            // every 16-row block's taps fit 32 columns
+    bool block16 = false;            // every 16-row block's taps fit 32 samples
     std::vector<std::int32_t> tap_start; // [rows] contiguous tap runs
-    std::vector<float> tap_w;            // [rows][K] (copied into fwd)
+    std::vector<T> tap_w;                // [rows][K] (copied into fwd)
 };
+using WindowPlan = WindowPlanT<float>;
+using WindowPlan64 = WindowPlanT<double>;
 // Largest influence (relative) of a wrong starting state allowed at a
-// windowed chunk's first forward / last backward row: 2^-28.
+// windowed chunk's first forward / last backward row: 2^-28 for the F32
+// kernels, 2^-40 for the Float64 route (so that rounding its result to F32
+// almost never differs from the sequential Float64 solve).
 constexpr double kWindowTolerance = 1.0 / 268435456.0;
+constexpr double kWindowTolerance64 = 1.0 / 1099511627776.0;
 // False when the plan cannot use the windowed kernels: band above 7, a tap
 // run longer than 16 or outside its 32-sample staging window, or a warm-up
 // longer than the band class's P would be needed.
 bool build_window_plan(const AxisPlan &plan, WindowPlan &out);
+// The same for the Float64 route, from the plan's FP64 packing.
+bool build_window_plan64(const AxisPlan &plan, const PackedPlan &packed, WindowPlan64 &out);
 
 // One device allocation holding a plan's image (and its chunk partitions).
 class DevicePlan {
@@ -93,6 +102,9 @@ public:
     // Windowed partition on the device; nullptr when the plan cannot use the
     // windowed kernels in that direction.  Built lazily, cached.
     const DevWindow *window(bool rows);
+    // Windowed FP64 partition (Float64 route); nullptr when the plan's FP64
+    // recurrence does not decay fast enough.  Built lazily, cached.
+    const DevWindow64 *window64();
     // Refinement tables (FP64 weights and normal matrix split into F32
     // pairs); nullptr for a plan without FP64 data.  Built lazily, cached.
     const DevRefine *refine();
@@ -121,6 +133,13 @@ private:
         bool rows_feasible = false;
     };
     WindowImage window_image_;
+    struct Window64Image {
+        DevWindow64 dev{};
+        void *memory = nullptr;
+        bool built = false;
+        bool ok = false;
+    };
+    Window64Image window64_image_;
     DevRefine refine_{};
     void *refine_memory_ = nullptr;
     bool refine_built_ = false;

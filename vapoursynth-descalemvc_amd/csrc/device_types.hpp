@@ -164,4 +164,46 @@ constexpr int window_bwd_words(int w) { return w <= 3 ? 4 : 8; }
 constexpr int window_fwd_batch(int w, int k) { return window_fwd_words(w, k) <= 16 ? 4 : 2; }
 constexpr int window_bwd_batch(int w) { return 64 / window_bwd_words(w); }
 
+// Windowed FP64 partition (the Float64 route of F32 planes): the same
+// padded-row layout as DevWindow with FP64 records, a warm-up long enough
+// for a 2^-40 decay check and chunks short enough that a lane keeps its
+// L + P values as doubles in registers.  Forward records hold the FP64 tap
+// weights, forward factors and the diagonal multiplier of the Float64 route
+// (unit-L factors when zform = 1: the recurrence carries z = L^-1 b and the
+// diagonal is applied to the stored value only).  Staging uses 8-row blocks.
+constexpr int window_length64(int w) { return w <= 3 ? 32 : 16; }
+constexpr int window_warmup64(int w) { return w <= 3 ? 32 : (w <= 5 ? 48 : 64); }
+constexpr int window_fwd_batch64(int w, int k) { return k + w + 1 <= 12 ? 2 : 1; }
+constexpr int window_bwd_batch64(int w) { return 32 / window_bwd_words(w); }
+struct DevWindow64 {
+    std::int32_t n;
+    std::int32_t m;
+    std::int32_t length;      // L
+    std::int32_t warmup;      // P
+    std::int32_t chunks;
+    std::int32_t ell_width;   // K
+    std::int32_t zform;
+    std::int32_t pad_;
+    const double *fwd;        // [rows][window_fwd_words(W, K)]
+    const double *bwd;        // [rows][window_bwd_words(W)]
+    const std::int32_t *tap_start;
+    const std::int32_t *blk_lo;   // [rows / 8]
+};
+
+// One windowed FP64 axis pass over `frames` planes: rows pass F32 source to
+// an FP64 intermediate, columns pass FP64 intermediate to F32.  `flag` is
+// set when a stored value is not finite.
+struct Win64Args {
+    DevWindow64 plan;
+    const void *src;
+    long s_row, s_frame;
+    void *dst;
+    long d_row, d_frame;
+    std::int32_t nvec;
+    std::int32_t frames;
+    std::int32_t groups;      // ceil(nvec / 64)
+    std::int32_t pad_;
+    int *flag;
+};
+
 } // namespace dsmvc::hip

@@ -367,6 +367,30 @@ void HipExecutor::solve_pass(const AxisPlan &plan, bool rows, const void *input,
             throw std::invalid_argument("dsmvc: input and output planes must not overlap");
     }
     hip::DevicePlan &dp = device_plan(plan);
+    // Float64 route of an F32 plane's 2D solve: the windowed FP64 kernels
+    // when the plan's FP64 recurrence decays within their warm-up.
+    const bool f64_pair = rows ? in_kind == Sample::f32 && out_kind == Sample::f64
+                               : in_kind == Sample::f64 && out_kind == Sample::f32;
+    if (f64 && f64_pair && (mode_ == SolveMode::automatic || mode_ == SolveMode::windowed)) {
+        if (const hip::DevWindow64 *win = dp.window64()) {
+            hip::Win64Args a{};
+            a.plan = *win;
+            a.src = input;
+            a.s_row = in_row;
+            a.s_frame = in_frame;
+            a.dst = output;
+            a.d_row = out_row;
+            a.d_frame = out_frame;
+            a.nvec = vectors;
+            a.frames = frames;
+            a.groups = (vectors + 63) / 64;
+            a.flag = s.flag;
+            check(hip::launch_window64(a, hip::window_band_class(plan.half_bandwidth), win->ell_width,
+                                       rows, stream),
+                  "windowed Float64 launch failed");
+            return;
+        }
+    }
     if (!f64 && mode_ != SolveMode::exact) {
         hip::ChunkArgs a{};
         a.axis = dp.axis();

@@ -130,8 +130,7 @@ class Plan:
         """How the windowed (independent-chunk) kernels partition this plan."""
         info = _lib.WindowInfoC()
         check(load().dsmvc_hip_plan_window_info(self._handle, ctypes.byref(info)))
-        return {name: getattr(info, name) for name, _ in _lib.WindowInfoC._fields_
-                if name != "reserved"}
+        return {name: getattr(info, name) for name, _ in _lib.WindowInfoC._fields_}
 
     def arrays(self) -> dict:
         """Copies of the AxisPlan arrays (reference include/dsmvc/engine.hpp:90-107)."""

@@ -71,7 +71,7 @@ class WindowInfoC(ctypes.Structure):
         ("length", ctypes.c_int32),
         ("warmup", ctypes.c_int32),
         ("width", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("float64_feasible", ctypes.c_int32),
         ("residual", ctypes.c_double),
     ]
 
