@@ -188,6 +188,9 @@ def batched_scan(torch, dh, dev, frame, repeats):
     }
 
 
+SAT_REQUESTS = 400
+
+
 def served_stream(torch, dh, dev, seconds, rate=400.0, seed=5):
     """Mixed-size serving: a seeded Poisson request stream (mean `rate`
     requests/s for `seconds`) of GRAYS / YUV420PS frames from sources
@@ -283,24 +286,36 @@ def served_stream(torch, dh, dev, seconds, rate=400.0, seed=5):
         return {"n": int(len(values)), "p50": round(float(np.percentile(values, 50)), 3),
                 "p99": round(float(np.percentile(values, 99)), 3)}
 
-    # Saturation: the same requests submitted back to back (plans and device
-    # images cached), against the plain executor one 2D plane per call.
-    server = dh.Server(ex, memory_budget=8 << 30, slots=4)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    sat_ids = [server.submit(ins, outs, kernel=k[0], taps=k[1], b=k[2], c=k[3], f64_mode=mode)
-               for (_, ins, outs, k, mode) in reqs]
-    for rid in sat_ids:
-        server.wait(rid)
-    served_s = time.perf_counter() - t0
-    server.close()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for (_, ins, outs, _, _), pp in zip(reqs, plans):
-        for x, o, (hp, vp) in zip(ins, outs, pp):
-            ex.inverse_2d(hp, vp, x, out=o)
-    torch.cuda.synchronize()
-    plain_s = time.perf_counter() - t0
+    # Saturation: the first SAT requests of the stream (whose plans fit the
+    # planner cache) submitted back to back, against the plain executor one
+    # 2D plane per call on the same plans; both warmed once, best of two.
+    sat = reqs[:SAT_REQUESTS]
+    sat_plans = plans[:SAT_REQUESTS]
+    sat_planes = sum(len(r[1]) for r in sat)
+
+    def served_pass():
+        server = dh.Server(ex, memory_budget=8 << 30, slots=4)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sat_ids = [server.submit(ins, outs, kernel=k[0], taps=k[1], b=k[2], c=k[3], f64_mode=mode)
+                   for (_, ins, outs, k, mode) in sat]
+        for rid in sat_ids:
+            server.wait(rid)
+        seconds = time.perf_counter() - t0
+        server.close()
+        return seconds
+
+    def plain_pass():
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for (_, ins, outs, _, _), pp in zip(sat, sat_plans):
+            for x, o, (hp, vp) in zip(ins, outs, pp):
+                ex.inverse_2d(hp, vp, x, out=o)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    served_s = min(served_pass() for _ in range(3))
+    plain_s = min(plain_pass() for _ in range(3))
     return {
         "workload": "Poisson %.0f req/s, GRAYS/YUV420PS from 3840x2160/1920x1080/1280x720, "
                     "h ~ U[0.5,0.95]H, 4 kernels, f64mode 0/1/2" % rate,
@@ -316,11 +331,13 @@ def served_stream(torch, dh, dev, seconds, rate=400.0, seed=5):
         "rounds": stats["rounds"], "largest_round": stats["largest_round"],
         "table_planes": stats["table_planes"], "fallback_planes": stats["fallback_planes"],
         "memory_high_water_bytes": stats["memory_high_water"],
-        "saturation": {"served_solves_per_s": round(2 * planes / served_s, 1),
-                       "plain_one_plane_per_call_solves_per_s": round(2 * planes / plain_s, 1),
+        "saturation": {"requests": len(sat),
+                       "served_solves_per_s": round(2 * sat_planes / served_s, 1),
+                       "plain_one_plane_per_call_solves_per_s": round(2 * sat_planes / plain_s, 1),
                        "speedup": round(plain_s / served_s, 2),
-                       "note": "all requests submitted at once; plain = dsmvc_hip_inverse_2d "
-                               "per plane, synchronous, same plans"},
+                       "note": "the stream's first requests submitted at once (plans cached); "
+                               "plain = dsmvc_hip_inverse_2d per plane, synchronous, same plans; "
+                               "best of 3 warm passes each"},
     }
 
 

@@ -112,20 +112,22 @@ def test_windowed_rows_and_columns_close(executors, oracle, name, req):
     info = plan.window_info()
     m, n = req.source_size, req.destination_size
     rng = np.random.default_rng(m * 5 + n)
-    x = rng.uniform(0, 1, (130, m)).astype(np.float32)
-    xt = torch.from_numpy(x).to(DEV)
-    if not info["feasible"]:
-        with pytest.raises(ValueError, match="windowed"):
-            executors["windowed"].inverse_columns(plan, xt.T.contiguous())
-        return
-    op = oracle.build(req)
-    expected = np.stack([oracle.axis_f32_ordered(op, row) for row in x])
-    scale = max(1.0, float(np.abs(expected).max()))
-    if info["rows_feasible"]:
-        got = executors["windowed"].inverse_rows(plan, xt).cpu().numpy()
-        assert float(np.abs(got - expected).max()) <= 4e-6 * scale
-    gotc = executors["windowed"].inverse_columns(plan, xt.T.contiguous()).cpu().numpy()
-    assert float(np.abs(gotc - expected.T).max()) <= 4e-6 * scale
+    # 130 vectors: a partial last group of 64; 40: fewer vectors than one group.
+    for vectors in (130, 40):
+        x = rng.uniform(0, 1, (vectors, m)).astype(np.float32)
+        xt = torch.from_numpy(x).to(DEV)
+        if not info["feasible"]:
+            with pytest.raises(ValueError, match="windowed"):
+                executors["windowed"].inverse_columns(plan, xt.T.contiguous())
+            return
+        op = oracle.build(req)
+        expected = np.stack([oracle.axis_f32_ordered(op, row) for row in x])
+        scale = max(1.0, float(np.abs(expected).max()))
+        if info["rows_feasible"]:
+            got = executors["windowed"].inverse_rows(plan, xt).cpu().numpy()
+            assert float(np.abs(got - expected).max()) <= 4e-6 * scale, vectors
+        gotc = executors["windowed"].inverse_columns(plan, xt.T.contiguous()).cpu().numpy()
+        assert float(np.abs(gotc - expected.T).max()) <= 4e-6 * scale, vectors
 
 
 LONG_CASES = [

@@ -2,6 +2,7 @@
 #include "batch.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <stdexcept>
 #include <string>
 
@@ -35,6 +36,12 @@ hip::DevConv identity_conv() {
 // ---------------------------------------------------------------------------
 // WindowTables
 // ---------------------------------------------------------------------------
+// Launches use XCD-contiguous workgroups with the vector group fastest
+// (launch_window_table order 2); DSMVC_HIP_WINDOW_ORDER (0-3) overrides it.
+WindowTables::WindowTables(int device) : device_(device) {
+    if (const char *env = std::getenv("DSMVC_HIP_WINDOW_ORDER")) order_ = std::atoi(env) & 3;
+}
+
 WindowTables::~WindowTables() {
     (void)hipSetDevice(device_);
     for (Slot &slot : slots_) {
@@ -119,7 +126,8 @@ void WindowTables::launch(hipStream_t stream) {
     for (const Group &g : groups_) {
         if (g.items == 0) continue;
         check(hip::launch_window_table(slot.device + g.offset, static_cast<int>(g.jobs.size()),
-                                       g.items, g.band, g.width, g.rows, g.block16, g.kind, stream),
+                                       g.items, g.band, g.width, g.rows, g.block16, g.kind, order_,
+                                       stream),
               "table launch failed");
     }
     if (record_) {

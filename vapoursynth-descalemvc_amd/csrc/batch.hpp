@@ -56,7 +56,7 @@ struct BatchStats {
 // out a new set of jobs (growing the device buffer), `launch` enqueues.
 class WindowTables {
 public:
-    explicit WindowTables(int device) : device_(device) {}
+    explicit WindowTables(int device);
     ~WindowTables();
     WindowTables(const WindowTables &) = delete;
     WindowTables &operator=(const WindowTables &) = delete;
@@ -96,6 +96,7 @@ private:
     Slot slots_[kSlots];
     int current_ = 0;
     bool record_ = false;
+    int order_ = 2;   // item order of the launches (launch_window_table)
 };
 
 class HipBatch {

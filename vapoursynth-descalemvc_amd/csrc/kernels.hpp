@@ -67,14 +67,15 @@ extern template hipError_t launch_chunked_band<7>(const ChunkArgs &, bool, Sampl
 // vertical pass).  A single pass is a one-job table.
 int window_band_class(int w);   // 1, 3, 5, 7, or 0 when unsupported
 constexpr int kWindowWaves = 4;   // wavefronts (one item each) per workgroup
+// `order`: bit 0 chunk-fastest item order, bit 1 XCD-contiguous workgroups.
 hipError_t launch_window_table(const WinJob *jobs, int count, long items, int band, int width,
-                               bool rows, bool block16, Sample kind, hipStream_t s);
+                               bool rows, bool block16, Sample kind, int order, hipStream_t s);
 template <int W, int K>
 hipError_t launch_window_table_band(const WinJob *jobs, int count, long items, bool rows,
-                                    bool block16, Sample kind, hipStream_t s);
+                                    bool block16, Sample kind, int order, hipStream_t s);
 #define DSMVC_WINDOW_EXTERN(W, K)                                                                 \
     extern template hipError_t launch_window_table_band<W, K>(const WinJob *, int, long, bool,    \
-                                                              bool, Sample, hipStream_t);
+                                                              bool, Sample, int, hipStream_t);
 DSMVC_WINDOW_EXTERN(1, 8)
 DSMVC_WINDOW_EXTERN(1, 16)
 DSMVC_WINDOW_EXTERN(3, 8)

@@ -186,14 +186,14 @@ hipError_t launch_window64(const Win64Args &a, int band, int width, bool rows, h
 }
 
 hipError_t launch_window_table(const WinJob *jobs, int count, long items, int band, int width,
-                               bool rows, bool block16, Sample kind, hipStream_t s) {
+                               bool rows, bool block16, Sample kind, int order, hipStream_t s) {
     if (width != 8 && width != 16) return hipErrorInvalidValue;
     const bool wide = width == 16;
     switch (band) {
-    case 1: return wide ? launch_window_table_band<1, 16>(jobs, count, items, rows, block16, kind, s) : launch_window_table_band<1, 8>(jobs, count, items, rows, block16, kind, s);
-    case 3: return wide ? launch_window_table_band<3, 16>(jobs, count, items, rows, block16, kind, s) : launch_window_table_band<3, 8>(jobs, count, items, rows, block16, kind, s);
-    case 5: return wide ? launch_window_table_band<5, 16>(jobs, count, items, rows, block16, kind, s) : launch_window_table_band<5, 8>(jobs, count, items, rows, block16, kind, s);
-    case 7: return wide ? launch_window_table_band<7, 16>(jobs, count, items, rows, block16, kind, s) : launch_window_table_band<7, 8>(jobs, count, items, rows, block16, kind, s);
+    case 1: return wide ? launch_window_table_band<1, 16>(jobs, count, items, rows, block16, kind, order, s) : launch_window_table_band<1, 8>(jobs, count, items, rows, block16, kind, order, s);
+    case 3: return wide ? launch_window_table_band<3, 16>(jobs, count, items, rows, block16, kind, order, s) : launch_window_table_band<3, 8>(jobs, count, items, rows, block16, kind, order, s);
+    case 5: return wide ? launch_window_table_band<5, 16>(jobs, count, items, rows, block16, kind, order, s) : launch_window_table_band<5, 8>(jobs, count, items, rows, block16, kind, order, s);
+    case 7: return wide ? launch_window_table_band<7, 16>(jobs, count, items, rows, block16, kind, order, s) : launch_window_table_band<7, 8>(jobs, count, items, rows, block16, kind, order, s);
     default: return hipErrorInvalidValue;
     }
 }

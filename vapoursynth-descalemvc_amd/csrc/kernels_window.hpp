@@ -11,7 +11,7 @@
 // of the window from a zero state, and stores the chunk's L rows.  Chunks
 // never exchange carries, so a workgroup's wavefronts are independent (no
 // workgroup barrier) and one launch can hold items of many plans.  The four
-// wavefronts of a workgroup take consecutive items (decode_item).
+// wavefronts of a workgroup take consecutive items (decode_item, xcd_block).
 //
 // Vertical pass (columns_window): lanes are 64 adjacent columns; each block
 // of G rows stages the 32 source rows its taps need (256-byte runs) into the
@@ -42,15 +42,36 @@ struct ItemPos {
 
 // Items run vector group fastest: a workgroup's four wavefronts usually
 // solve the same chunk of one plan for neighbouring vectors, so they read the
-// same records through the scalar cache.
-__device__ __forceinline__ ItemPos decode_item(const WinJob &jb, int local) {
-    const int per_frame = jb.plan.chunks * jb.groups;
+// same records through the scalar cache.  With bit 0 of `order` the chunk
+// varies fastest instead.
+__device__ __forceinline__ ItemPos decode_item(const WinJob &jb, int local, int order) {
+    const int chunks = jb.plan.chunks, groups = jb.groups;
     ItemPos p;
-    p.frame = local / per_frame;
-    const int r = local - p.frame * per_frame;
-    p.chunk = r / jb.groups;
-    p.group = r - p.chunk * jb.groups;
+    if (order & 1) {
+        const int rest = local / chunks;
+        p.chunk = local - rest * chunks;
+        p.frame = rest / groups;
+        p.group = rest - p.frame * groups;
+    } else {
+        const int per_frame = chunks * groups;
+        p.frame = local / per_frame;
+        const int r = local - p.frame * per_frame;
+        p.chunk = r / groups;
+        p.group = r - p.chunk * groups;
+    }
     return p;
+}
+
+// Workgroups are dispatched to the 8 XCDs round-robin (block b on XCD
+// b % 8), each XCD with its own L2.  With bit 1 of `order`, blocks are
+// renumbered so that every XCD receives a contiguous range of the item
+// order: neighbouring chunks, whose source windows overlap, then share an L2.
+constexpr int kXcds = 8;
+__device__ __forceinline__ long xcd_block(long b, long blocks, int order) {
+    if (!(order & 2)) return b;
+    const long q = blocks / kXcds, r = blocks % kXcds;
+    const long xcd = b % kXcds, slot = b / kXcds;
+    return xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
 }
 
 // Forward sweep of one batch of B rows whose taps are in tap[u][e]: each
@@ -147,15 +168,18 @@ __device__ __forceinline__ void rows_item(const WinJob &jb, const ItemPos &at, f
     // current block is solved, so the loads' latency overlaps the solve.
     // Held as 32-bit words so they can be pinned in registers below.
     using Word = std::conditional_t<std::is_same_v<In, float>, float, std::uint32_t>;
+    // Rows past the plane (last vector group) read 0 through the buffer's
+    // range check; their results are not stored.  The lane's address is
+    // rebuilt per block (pinned, so the 32 row offsets are not kept live).
     Word staged[32];
+    const int row_step = 2 * static_cast<int>(jb.s_row) * static_cast<int>(sizeof(In));
     auto fetch = [&](int b) {
         const int column = min(blk[b] + scol, m1);
+        int at = ((row0 + srow) * static_cast<int>(jb.s_row) + column) * static_cast<int>(sizeof(In));
+        asm volatile("" : "+v"(at));
 #pragma unroll
-        for (int t = 0; t < 32; ++t) {
-            const int row = min(row0 + srow + 2 * t, nrows - 1);
-            staged[t] = static_cast<Word>(BufferLoad<In>::at(
-                rs, (row * static_cast<int>(jb.s_row) + column) * static_cast<int>(sizeof(In)), 0));
-        }
+        for (int t = 0; t < 32; ++t)
+            staged[t] = static_cast<Word>(BufferLoad<In>::at(rs, at + t * row_step, 0));
     };
     fetch(0);
 
@@ -335,47 +359,69 @@ __device__ __forceinline__ int find_job(const WinJob *__restrict__ jobs, int cou
 // one item and exits, so the hardware dispatcher balances plans of uneven
 // size.
 template <int W, int K, int G, class In>
-__global__ void __launch_bounds__(64 * kWindowWaves)
-rows_window_table(const WinJob *__restrict__ jobs, int count, long items) {
-    __shared__ float lds[kWindowWaves * kWinRing];
+__device__ __forceinline__ void rows_table_item(const WinJob *__restrict__ jobs, int count,
+                                                long items, int order, float *lds) {
     const int lane = threadIdx.x & 63;
     const int wave = uniform(threadIdx.x >> 6);
-    const long item = static_cast<long>(blockIdx.x) * kWindowWaves + wave;
+    const long item = xcd_block(blockIdx.x, gridDim.x, order) * kWindowWaves + wave;
     if (item >= items) return;
     const WinJob jb = load_job(jobs + find_job(jobs, count, item));
-    rows_item<W, K, G, In>(jb, decode_item(jb, static_cast<int>(item - jb.first_item)),
+    rows_item<W, K, G, In>(jb, decode_item(jb, static_cast<int>(item - jb.first_item), order),
                            lds + wave * kWinRing, lane);
 }
 
+template <int W, int K, int G, class In>
+__global__ void __launch_bounds__(64 * kWindowWaves)
+rows_window_table(const WinJob *__restrict__ jobs, int count, long items, int order) {
+    __shared__ float lds[kWindowWaves * kWinRing];
+    rows_table_item<W, K, G, In>(jobs, count, items, order, lds);
+}
+
+// The same kernel held to 128 VGPRs, so that four wavefronts share each
+// SIMD (the LDS ring allows four workgroups per CU).  Used where that fits
+// without spilling much (see rows_four_waves).
+template <int W, int K, int G, class In>
+__global__ void __launch_bounds__(64 * kWindowWaves) __attribute__((amdgpu_waves_per_eu(4)))
+rows_window_table4(const WinJob *__restrict__ jobs, int count, long items, int order) {
+    __shared__ float lds[kWindowWaves * kWinRing];
+    rows_table_item<W, K, G, In>(jobs, count, items, order, lds);
+}
+
+// u16 samples with 8-row blocks need more registers than 128 without spilling.
+template <class In, int G>
+constexpr bool rows_four_waves = !(std::is_same_v<In, std::uint16_t> && G == 8);
+
 template <int W, int K, int G, class Out>
 __global__ void __launch_bounds__(64 * kWindowWaves)
-columns_window_table(const WinJob *__restrict__ jobs, int count, long items) {
+columns_window_table(const WinJob *__restrict__ jobs, int count, long items, int order) {
     __shared__ float lds[kWindowWaves * kWinRing];
     const int lane = threadIdx.x & 63;
     const int wave = uniform(threadIdx.x >> 6);
-    const long item = static_cast<long>(blockIdx.x) * kWindowWaves + wave;
+    const long item = xcd_block(blockIdx.x, gridDim.x, order) * kWindowWaves + wave;
     if (item >= items) return;
     const WinJob jb = load_job(jobs + find_job(jobs, count, item));
-    columns_item<W, K, G, Out>(jb, decode_item(jb, static_cast<int>(item - jb.first_item)),
+    columns_item<W, K, G, Out>(jb, decode_item(jb, static_cast<int>(item - jb.first_item), order),
                                lds + wave * kWinRing, lane);
 }
 
 // `block16`: every job of the table has 16-row staging blocks.
 template <int W, int K, class T>
 hipError_t launch_table(const WinJob *jobs, int count, long items, bool rows, bool block16,
-                        hipStream_t s) {
+                        int order, hipStream_t s) {
     if (items <= 0 || count <= 0) return hipSuccess;
     const long needed = (items + kWindowWaves - 1) / kWindowWaves;
     if (needed > 0x7FFFFFFFL) return hipErrorInvalidValue;
     const dim3 grid(static_cast<unsigned>(needed));
     if (!rows && block16)
-        columns_window_table<W, K, 16, T><<<grid, 64 * kWindowWaves, 0, s>>>(jobs, count, items);
+        columns_window_table<W, K, 16, T><<<grid, 64 * kWindowWaves, 0, s>>>(jobs, count, items, order);
     else if (!rows)
-        columns_window_table<W, K, 8, T><<<grid, 64 * kWindowWaves, 0, s>>>(jobs, count, items);
+        columns_window_table<W, K, 8, T><<<grid, 64 * kWindowWaves, 0, s>>>(jobs, count, items, order);
     else if (block16)
-        rows_window_table<W, K, 16, T><<<grid, 64 * kWindowWaves, 0, s>>>(jobs, count, items);
+        rows_window_table4<W, K, 16, T><<<grid, 64 * kWindowWaves, 0, s>>>(jobs, count, items, order);
+    else if constexpr (rows_four_waves<T, 8>)
+        rows_window_table4<W, K, 8, T><<<grid, 64 * kWindowWaves, 0, s>>>(jobs, count, items, order);
     else
-        rows_window_table<W, K, 8, T><<<grid, 64 * kWindowWaves, 0, s>>>(jobs, count, items);
+        rows_window_table<W, K, 8, T><<<grid, 64 * kWindowWaves, 0, s>>>(jobs, count, items, order);
     return hipGetLastError();
 }
 
@@ -384,11 +430,11 @@ hipError_t launch_table(const WinJob *jobs, int count, long items, bool rows, bo
 
 template <int W, int K>
 hipError_t launch_window_table_band(const WinJob *jobs, int count, long items, bool rows,
-                                    bool block16, Sample kind, hipStream_t s) {
+                                    bool block16, Sample kind, int order, hipStream_t s) {
     switch (kind) {
-    case Sample::f32: return dev::launch_table<W, K, float>(jobs, count, items, rows, block16, s);
-    case Sample::u8: return dev::launch_table<W, K, std::uint8_t>(jobs, count, items, rows, block16, s);
-    case Sample::u16: return dev::launch_table<W, K, std::uint16_t>(jobs, count, items, rows, block16, s);
+    case Sample::f32: return dev::launch_table<W, K, float>(jobs, count, items, rows, block16, order, s);
+    case Sample::u8: return dev::launch_table<W, K, std::uint8_t>(jobs, count, items, rows, block16, order, s);
+    case Sample::u16: return dev::launch_table<W, K, std::uint16_t>(jobs, count, items, rows, block16, order, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -3,5 +3,5 @@
 
 namespace dsmvc::hip {
 template hipError_t launch_window_table_band<7, 16>(const WinJob *, int, long, bool, bool,
-                                                        Sample, hipStream_t);
+                                                        Sample, int, hipStream_t);
 } // namespace dsmvc::hip
