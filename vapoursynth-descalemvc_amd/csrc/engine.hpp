@@ -79,6 +79,11 @@ struct AxisPlan {
     std::vector<double> ldlt_bands_f64;            // [0 * n + i] = D_i, [d * n + i] = L(i+d, i)
     std::vector<double> inverse_diagonal_f64;
 
+    // Nonzero for plans made by build_axis_plan: unique per build (copies
+    // keep it), so two plans with the same nonzero serial hold the same
+    // arrays.  Plans are immutable once built.
+    std::uint64_t serial = 0;
+
     [[nodiscard]] bool valid() const noexcept;
     [[nodiscard]] bool requires_float64() const noexcept { return !ldlt_bands_f64.empty(); }
     [[nodiscard]] std::size_t storage_bytes() const noexcept;

@@ -65,6 +65,8 @@ DevConv make_conv(const IntegerConversion *c) {
 }
 
 bool same_plan(const AxisPlan &a, const AxisPlan &b) {
+    // Built plans carry a unique serial: equal serials, equal plans.
+    if (a.serial != 0 || b.serial != 0) return a.serial == b.serial;
     return a.source_size == b.source_size && a.destination_size == b.destination_size
         && a.half_bandwidth == b.half_bandwidth
         && a.transpose_indices.size() == b.transpose_indices.size()

@@ -11,6 +11,7 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <bit>
 #include <cmath>
 #include <cstring>
@@ -532,6 +533,8 @@ AxisPlan build_axis_plan(const AxisRequest &r) {
         }
     }
     if (!p.valid()) throw std::runtime_error("failed to build inverse axis plan");
+    static std::atomic<std::uint64_t> next_serial{1};
+    p.serial = next_serial.fetch_add(1, std::memory_order_relaxed);
     return p;
 }
 

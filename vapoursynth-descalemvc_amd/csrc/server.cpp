@@ -152,8 +152,11 @@ std::uint64_t HipServer::submit(const ServeRequest &r) {
         // loop only builds tables and launches.
         hip::DevicePlan &ih = executor_.plan_image(*plane.h);
         hip::DevicePlan &iv = executor_.plan_image(*plane.v);
-        plane.table = parallel && !plane.h->requires_float64() && !plane.v->requires_float64()
-                      && ih.window(true) && iv.window(false);
+        if (parallel && !plane.h->requires_float64() && !plane.v->requires_float64()) {
+            plane.wh = ih.window(true);
+            plane.wv = iv.window(false);
+        }
+        plane.table = plane.wh && plane.wv;
         if (plane.table)
             plane.inter_bytes = align256(static_cast<std::size_t>(pitch_of(dw)) * sh * sizeof(float));
         entry->bytes += plane.inter_bytes;
@@ -274,7 +277,7 @@ bool HipServer::admit_round(Slot &slot, std::unique_lock<std::mutex> &lock) {
                 float *inter = reinterpret_cast<float *>(static_cast<unsigned char *>(arena_) + at);
                 at += p.inter_bytes;
                 hip::WinJob a{};
-                a.plan = *executor_.plan_image(h).window(true);
+                a.plan = *p.wh;
                 a.src = p.src;
                 a.s_row = p.src_stride;
                 a.dst = inter;
@@ -285,7 +288,7 @@ bool HipServer::admit_round(Slot &slot, std::unique_lock<std::mutex> &lock) {
                 a.conv = identity_conv();
                 tables.add(a, hip::window_band_class(h.half_bandwidth), true, hip::Sample::f32);
                 hip::WinJob b{};
-                b.plan = *executor_.plan_image(v).window(false);
+                b.plan = *p.wv;
                 b.src = inter;
                 b.s_row = row;
                 b.dst = p.dst;

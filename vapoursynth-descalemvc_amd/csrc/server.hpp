@@ -36,6 +36,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "device_types.hpp"
 #include "engine.hpp"
 
 namespace dsmvc {
@@ -105,6 +106,10 @@ private:
         std::ptrdiff_t dst_stride = 0;
         bool table = false;          // windowed table launch (else executor route)
         std::size_t inter_bytes = 0;
+        // The plans' window tables (table planes).  The executor keeps them
+        // while the plans live, which `h` and `v` guarantee.
+        const hip::DevWindow *wh = nullptr;
+        const hip::DevWindow *wv = nullptr;
     };
     struct Entry {
         std::uint64_t id = 0;
