@@ -274,8 +274,8 @@ def test_float64_route_within_one_ulp(executors, oracle):
 
 def test_float64_windowed_route_matches_sequential(executors, oracle):
     """Float64 route on plans whose FP64 recurrence decays within the windowed
-    FP64 kernels' warm-up (bicubic W=3, lanczos3 W=5, bilinear W=1, mixed with
-    an F32 axis).  Bars: <= 1 ULP vs the oracle's Float64 route and vs the
+    FP64 kernels' warm-up (bicubic W=3, lanczos3 W=5, bilinear W=1 mixed with
+    an F32 axis, spline64 and lanczos4 W=7 with 16-tap runs).  Bars: <= 1 ULP vs the oracle's Float64 route and vs the
     sequential Float64 kernel, with at most 1 in 1000 outputs differing from
     the sequential kernel at all."""
     cases = [
@@ -285,6 +285,8 @@ def test_float64_windowed_route_matches_sequential(executors, oracle):
          make_request(720, 684, kernel=LANCZOS, taps=3, f64_mode=F64_ALWAYS)),
         (make_request(640, 448, kernel=BILINEAR, f64_mode=F64_ALWAYS),
          make_request(360, 252, kernel=BICUBIC, f64_mode=F64_NEVER)),
+        (make_request(960, 640, kernel=SPLINE64, f64_mode=F64_ALWAYS),
+         make_request(540, 360, kernel=LANCZOS, taps=4, f64_mode=F64_ALWAYS)),
     ]
     for h, v in cases:
         hp, vp = dh.Plan(_hreq(h)), dh.Plan(_hreq(v))

@@ -153,7 +153,7 @@ def test_near_identity_plans_report_their_decay():
 # x_i = y_i - sum_d L(i+d, i) x_{i+d}; reference src/axis_plan.cpp:1124-1191)
 # on chunks of L rows with P warm-up rows per side, and accept a plan when
 # the wrong start has decayed below 2^-40.  Chunk shape per band class:
-F64_SHAPES = {1: (32, 32), 3: (32, 32), 5: (16, 48), 7: (16, 64)}
+F64_SHAPES = {1: (48, 32), 3: (48, 32), 5: (32, 48), 7: (24, 64)}
 TOLERANCE64 = 2.0 ** -40
 
 
@@ -198,6 +198,7 @@ F64_CASES = [
     ("bicubic-1080-756", make_request(1080, 756, kernel=BICUBIC, f64_mode=2)),
     ("lanczos3-2160-1512", make_request(2160, 1512, kernel=LANCZOS, taps=3, f64_mode=2)),
     ("spline36-1080-1026", make_request(1080, 1026, kernel=SPLINE36, f64_mode=2)),
+    ("spline64-1920-1280", make_request(1920, 1280, kernel=SPLINE64, f64_mode=2)),
 ]
 
 

@@ -171,7 +171,7 @@ constexpr int window_bwd_batch(int w) { return 64 / window_bwd_words(w); }
 // weights, forward factors and the diagonal multiplier of the Float64 route
 // (unit-L factors when zform = 1: the recurrence carries z = L^-1 b and the
 // diagonal is applied to the stored value only).  Staging uses 8-row blocks.
-constexpr int window_length64(int w) { return w <= 3 ? 32 : 16; }
+constexpr int window_length64(int w) { return w <= 3 ? 48 : (w <= 5 ? 32 : 24); }
 constexpr int window_warmup64(int w) { return w <= 3 ? 32 : (w <= 5 ? 48 : 64); }
 constexpr int window_fwd_batch64(int w, int k) { return k + w + 1 <= 12 ? 2 : 1; }
 constexpr int window_bwd_batch64(int w) { return 32 / window_bwd_words(w); }
