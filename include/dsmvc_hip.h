@@ -142,7 +142,16 @@ int dsmvc_hip_device_count(int32_t *count);
 
 /* ---- executor ----------------------------------------------------------- */
 /* dsmvc::Executor(BackendKind::hip) (engine.hpp:363-372, src/executor.cpp:39-59);
- * device -1 = current device.  solve_mode: DSMVC_HIP_SOLVE_*. */
+ * device -1 = current device.  solve_mode: DSMVC_HIP_SOLVE_*:
+ *   AUTO      F32 plans: the windowed or chunked (partitioned) kernels, chosen
+ *             per plan and direction by timing; Float64 route of F32 2D
+ *             planes: the windowed FP64 kernels where the plan's FP64
+ *             recurrence decays within their warm-up; otherwise sequential.
+ *   EXACT     sequential kernels, bit-identical to the reference's ordered
+ *             F32 solver (and its Float64 route).
+ *   CHUNKED / WINDOWED  force that F32 family (plans it cannot take fail).
+ *   REFINED   AUTO, with plans that retain FP64 data solved in F32 plus
+ *             refinement instead of the Float64 route. */
 int dsmvc_hip_executor_create(int32_t device, int32_t solve_mode, dsmvc_hip_executor **executor);
 void dsmvc_hip_executor_destroy(dsmvc_hip_executor *executor);
 /* Executor::prepare / seal (src/executor.cpp:313-349): uploads the plan image. */

@@ -1,7 +1,7 @@
 // Device helpers shared by the gfx950 kernel translation units:
 // sample conversion, coefficient-record layout, the recurrence steps and the
-// chunk carry chain.  Included by kernels_seq.hip, kernels_rows.hip and
-// kernels_cols.hip only.
+// chunk carry chain.  Included by the kernel translation units (the
+// sequential, chunked, windowed and refinement kernels) only.
 #pragma once
 
 #include <hip/hip_runtime.h>
