@@ -304,6 +304,34 @@ def test_float64_windowed_route_matches_sequential(executors, oracle):
         assert int(ulp_distance(win[1], expected).max()) <= 1
 
 
+def test_float64_windowed_integer_within_one_code(executors, oracle):
+    """Integer planes on the Float64 route (u16 and u8, forced FP64 plans the
+    windowed FP64 kernels take): <= 1 code vs the oracle's integer Float64
+    route (reference src/cpu_executor.cpp:1528-1651) and vs the sequential
+    Float64 kernel."""
+    cases = [(16, make_request(960, 640, kernel=BICUBIC, f64_mode=F64_ALWAYS),
+              make_request(540, 360, kernel=BICUBIC, f64_mode=F64_ALWAYS)),
+             (8, make_request(640, 480, kernel=LANCZOS, taps=3, f64_mode=F64_ALWAYS),
+              make_request(360, 270, kernel=SPLINE36, f64_mode=F64_ALWAYS))]
+    for bits, h, v in cases:
+        hp, vp = dh.Plan(_hreq(h)), dh.Plan(_hreq(v))
+        assert hp.window_info()["float64_feasible"] == 1 and vp.window_info()["float64_feasible"] == 1
+        rng = np.random.default_rng(bits)
+        dtype, tdtype = (np.uint16, torch.uint16) if bits == 16 else (np.uint8, torch.uint8)
+        x = rng.integers(0, 1 << bits, (2, v.source_size, h.source_size)).astype(dtype)
+        conv = integer_conversion(bits, chroma=False, full_range=bits == 8)
+        dconv = dh.IntegerConversion(conv.input_offset, conv.input_scale, conv.output_scale,
+                                     conv.output_offset, conv.output_maximum)
+        xt = torch.from_numpy(x.astype(np.int32)).to(DEV).to(tdtype)
+        win = executors["auto"].inverse_2d(hp, vp, xt, conversion=dconv).to(torch.int32).cpu().numpy()
+        seq = executors["exact"].inverse_2d(hp, vp, xt, conversion=dconv).to(torch.int32).cpu().numpy()
+        print(f"u{bits}: max code diff vs sequential {int(np.abs(win - seq).max())}", flush=True)
+        assert int(np.abs(win - seq).max()) <= 1
+        for f in range(2):
+            expected = oracle.inverse_2d_int(oracle.build(h), oracle.build(v), x[f], conv)
+            assert int(np.abs(win[f] - expected.astype(np.int32)).max()) <= 1
+
+
 def test_float64_windowed_route_is_taken(executors):
     """The automatic executor runs a 4K Float64 plane on the windowed FP64
     kernels: several times faster than the sequential Float64 kernel."""

@@ -190,9 +190,9 @@ struct DevWindow64 {
     const std::int32_t *blk_lo;   // [rows / 8]
 };
 
-// One windowed FP64 axis pass over `frames` planes: rows pass F32 source to
-// an FP64 intermediate, columns pass FP64 intermediate to F32.  `flag` is
-// set when a stored value is not finite.
+// One windowed FP64 axis pass over `frames` planes: rows pass F32 or integer
+// source to an FP64 intermediate, columns pass FP64 intermediate to F32 or
+// integer samples (`conv`).  `flag` is set when a stored value is not finite.
 struct Win64Args {
     DevWindow64 plan;
     const void *src;
@@ -204,6 +204,7 @@ struct Win64Args {
     std::int32_t groups;      // ceil(nvec / 64)
     std::int32_t pad_;
     int *flag;
+    DevConv conv;
 };
 
 } // namespace dsmvc::hip

@@ -369,10 +369,10 @@ void HipExecutor::solve_pass(const AxisPlan &plan, bool rows, const void *input,
             throw std::invalid_argument("dsmvc: input and output planes must not overlap");
     }
     hip::DevicePlan &dp = device_plan(plan);
-    // Float64 route of an F32 plane's 2D solve: the windowed FP64 kernels
-    // when the plan's FP64 recurrence decays within their warm-up.
-    const bool f64_pair = rows ? in_kind == Sample::f32 && out_kind == Sample::f64
-                               : in_kind == Sample::f64 && out_kind == Sample::f32;
+    // Float64 route of a 2D solve (FP64 intermediate): the windowed FP64
+    // kernels when the plan's FP64 recurrence decays within their warm-up.
+    const bool f64_pair = rows ? in_kind != Sample::f64 && out_kind == Sample::f64
+                               : in_kind == Sample::f64 && out_kind != Sample::f64;
     if (f64 && f64_pair && (mode_ == SolveMode::automatic || mode_ == SolveMode::windowed)) {
         if (const hip::DevWindow64 *win = dp.window64()) {
             hip::Win64Args a{};
@@ -387,8 +387,9 @@ void HipExecutor::solve_pass(const AxisPlan &plan, bool rows, const void *input,
             a.frames = frames;
             a.groups = (vectors + 63) / 64;
             a.flag = s.flag;
+            a.conv = conv;
             check(hip::launch_window64(a, hip::window_band_class(plan.half_bandwidth), win->ell_width,
-                                       rows, stream),
+                                       rows, rows ? in_kind : out_kind, stream),
                   "windowed Float64 launch failed");
             return;
         }

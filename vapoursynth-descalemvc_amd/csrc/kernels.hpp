@@ -86,16 +86,20 @@ DSMVC_WINDOW_EXTERN(7, 8)
 DSMVC_WINDOW_EXTERN(7, 16)
 #undef DSMVC_WINDOW_EXTERN
 
-// Windowed FP64 pass of the Float64 route (kernels_window64.hpp): rows F32
-// source -> FP64 intermediate, columns FP64 intermediate -> F32; one launch
-// per pass, instantiated per band class in kernels_window64_w{1,3,5,7}.hip.
-hipError_t launch_window64(const Win64Args &a, int band, int width, bool rows, hipStream_t s);
+// Windowed FP64 pass of the Float64 route (kernels_window64.hpp): rows
+// F32/u8/u16 source -> FP64 intermediate, columns FP64 intermediate ->
+// F32/u8/u16; one launch per pass, instantiated per band class in
+// kernels_window64_w{1,3,5,7}.hip.
+// `kind`: sample type of the rows pass's source or the columns pass's output.
+hipError_t launch_window64(const Win64Args &a, int band, int width, bool rows, Sample kind,
+                           hipStream_t s);
 template <int W>
-hipError_t launch_window64_band(const Win64Args &a, int width, bool rows, hipStream_t s);
-extern template hipError_t launch_window64_band<1>(const Win64Args &, int, bool, hipStream_t);
-extern template hipError_t launch_window64_band<3>(const Win64Args &, int, bool, hipStream_t);
-extern template hipError_t launch_window64_band<5>(const Win64Args &, int, bool, hipStream_t);
-extern template hipError_t launch_window64_band<7>(const Win64Args &, int, bool, hipStream_t);
+hipError_t launch_window64_band(const Win64Args &a, int width, bool rows, Sample kind,
+                                hipStream_t s);
+extern template hipError_t launch_window64_band<1>(const Win64Args &, int, bool, Sample, hipStream_t);
+extern template hipError_t launch_window64_band<3>(const Win64Args &, int, bool, Sample, hipStream_t);
+extern template hipError_t launch_window64_band<5>(const Win64Args &, int, bool, Sample, hipStream_t);
+extern template hipError_t launch_window64_band<7>(const Win64Args &, int, bool, Sample, hipStream_t);
 
 // Refinement of an F32 axis solve x (in place): `steps` times,
 //   r = A^T y - G x   in compensated F32 (two-product and two-sum, with the

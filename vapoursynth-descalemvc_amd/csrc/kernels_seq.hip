@@ -175,12 +175,13 @@ int window_band_class(int w) {
     return 0;
 }
 
-hipError_t launch_window64(const Win64Args &a, int band, int width, bool rows, hipStream_t s) {
+hipError_t launch_window64(const Win64Args &a, int band, int width, bool rows, Sample kind,
+                           hipStream_t s) {
     switch (band) {
-    case 1: return launch_window64_band<1>(a, width, rows, s);
-    case 3: return launch_window64_band<3>(a, width, rows, s);
-    case 5: return launch_window64_band<5>(a, width, rows, s);
-    case 7: return launch_window64_band<7>(a, width, rows, s);
+    case 1: return launch_window64_band<1>(a, width, rows, kind, s);
+    case 3: return launch_window64_band<3>(a, width, rows, kind, s);
+    case 5: return launch_window64_band<5>(a, width, rows, kind, s);
+    case 7: return launch_window64_band<7>(a, width, rows, kind, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -12,13 +12,15 @@
 // d = W..1; the carried forward state is z (unit-L FP64 factors) or y.
 //
 // Horizontal pass (rows_window64): lanes are 64 rows; each block of 8
-// destination rows stages the 32 F32 source columns its taps need through the
+// destination rows stages the 32 source columns its taps need through the
 // wavefront's LDS ring (transposed, pitch 33), the next block's loads in
 // flight meanwhile; the FP64 results leave through the same ring as pairs of
 // 32-bit words, row-contiguous.  Vertical pass (columns_window64): lanes are
 // 64 adjacent columns of the FP64 intermediate; each tap is one coalesced
 // 512-byte load, the next batch's loads in flight while a batch is solved.
 #pragma once
+
+#include <type_traits>
 
 #include "kernels_common.hpp"
 
@@ -45,12 +47,14 @@ __device__ __forceinline__ Pos64 decode64(const Win64Args &a, long item) {
     return p;
 }
 
-// Forward sweep of a batch of B rows whose taps are tap[u][e]: each row's
-// FP64 record (weights, factors, diagonal multiplier) comes from scalar
-// loads issued together for the batch.
-template <int W, int K, int B, class Tap>
+// Forward sweep of a batch of B rows whose taps are tap[u][e] (`sample`
+// turns one into the Float64 route's FP64 sample): each row's FP64 record
+// (weights, factors, diagonal multiplier) comes from scalar loads issued
+// together for the batch.
+template <int W, int K, int B, class Tap, class Sample64>
 __device__ __forceinline__ void forward_batch64(ConstPtr<double> rec, const Tap (&tap)[B][K],
-                                                bool zform, double (&st)[W], double (&out)[B]) {
+                                                Sample64 sample, bool zform, double (&st)[W],
+                                                double (&out)[B]) {
     constexpr int RW = window_fwd_words(W, K);
     double c[B][K + W + 1];
 #pragma unroll
@@ -61,7 +65,7 @@ __device__ __forceinline__ void forward_batch64(ConstPtr<double> rec, const Tap 
     for (int u = 0; u < B; ++u) {
         double v = 0.0;
 #pragma unroll
-        for (int e = 0; e < K; ++e) v = __builtin_fma(c[u][e], static_cast<double>(tap[u][e]), v);
+        for (int e = 0; e < K; ++e) v = __builtin_fma(c[u][e], sample(tap[u][e]), v);
 #pragma unroll
         for (int d = W; d >= 1; --d) v = __builtin_fma(-c[u][K + d - 1], st[d - 1], v);
         const double y = __dmul_rn(v, c[u][K + W]);
@@ -100,8 +104,10 @@ __device__ __forceinline__ void backward_sweep64(ConstPtr<double> bwd, double (&
     }
 }
 
-// Returns false when a stored value is not finite.
-template <int W, int K>
+// Returns false when a stored value is not finite.  Integer samples are
+// staged exactly as F32 and normalised per tap in FP64, as the Float64
+// route's load does: (x - offset) * scale.
+template <int W, int K, class In>
 __device__ __forceinline__ bool rows64_item(const Win64Args &a, const Pos64 &at,
                                             std::uint32_t *ring, int lane) {
     constexpr int L = window_length64(W);
@@ -115,8 +121,13 @@ __device__ __forceinline__ bool rows64_item(const Win64Args &a, const Pos64 &at,
     const int nrows = a.nvec;
     const int row0 = at.group * 64;
     const int r0 = at.chunk * L;
-    const float *src = static_cast<const float *>(a.src) + static_cast<long>(at.frame) * a.s_frame;
-    const auto rs = plane_rsrc(src, static_cast<long>(nrows) * a.s_row * 4);
+    const In *src = static_cast<const In *>(a.src) + static_cast<long>(at.frame) * a.s_frame;
+    const auto rs = plane_rsrc(src, static_cast<long>(nrows) * a.s_row * static_cast<long>(sizeof(In)));
+    const double in_offset = a.conv.in_offset, in_scale = a.conv.in_scale;
+    auto sample = [&](float x) {
+        if constexpr (std::is_same_v<In, float>) return static_cast<double>(x);
+        else return __dmul_rn(__dsub_rn(static_cast<double>(x), in_offset), in_scale);
+    };
     const int m1 = pl.m - 1;
     const int scol = lane & (kStageCols - 1);
     const int srow = lane >> 5;
@@ -127,13 +138,15 @@ __device__ __forceinline__ bool rows64_item(const Win64Args &a, const Pos64 &at,
     const float *mine = reinterpret_cast<const float *>(ring) + lane * kRowPitch64;
     float *fring = reinterpret_cast<float *>(ring);
 
-    float staged[32];
+    using Word = std::conditional_t<std::is_same_v<In, float>, float, std::uint32_t>;
+    Word staged[32];
     auto fetch = [&](int b) {
         const int column = min(blk[b] + scol, m1);
 #pragma unroll
         for (int t = 0; t < 32; ++t) {
             const int row = min(row0 + srow + 2 * t, nrows - 1);
-            staged[t] = BufferLoad<float>::at(rs, (row * static_cast<int>(a.s_row) + column) * 4, 0);
+            staged[t] = static_cast<Word>(BufferLoad<In>::at(
+                rs, (row * static_cast<int>(a.s_row) + column) * static_cast<int>(sizeof(In)), 0));
         }
     };
     fetch(0);
@@ -152,7 +165,7 @@ __device__ __forceinline__ bool rows64_item(const Win64Args &a, const Pos64 &at,
 #pragma unroll
         for (int t = 0; t < 32; ++t) {
             asm volatile("" : "+v"(staged[t]));
-            fring[(srow + 2 * t) * kRowPitch64 + scol] = staged[t];
+            fring[(srow + 2 * t) * kRowPitch64 + scol] = static_cast<float>(staged[t]);
         }
         wave_sync();
         if (b + 1 < NB) fetch(b + 1);
@@ -166,7 +179,7 @@ __device__ __forceinline__ bool rows64_item(const Win64Args &a, const Pos64 &at,
                 for (int e = 0; e < K; ++e) tap[u][e] = p[e];
             }
             double out[B];
-            forward_batch64<W, K, B>(fr + (b * G + j) * RW, tap, zform, st, out);
+            forward_batch64<W, K, B>(fr + (b * G + j) * RW, tap, sample, zform, st, out);
 #pragma unroll
             for (int u = 0; u < B; ++u)
                 if (b * G + j + u >= P) y[b * G + j + u - P] = out[u];
@@ -207,7 +220,7 @@ __device__ __forceinline__ bool rows64_item(const Win64Args &a, const Pos64 &at,
     return finite;
 }
 
-template <int W, int K>
+template <int W, int K, class Out>
 __device__ __forceinline__ bool columns64_item(const Win64Args &a, const Pos64 &at, int lane) {
     constexpr int L = window_length64(W);
     constexpr int P = window_warmup64(W);
@@ -255,7 +268,7 @@ __device__ __forceinline__ bool columns64_item(const Win64Args &a, const Pos64 &
     for (int j = 0; j < R; j += B) {
         if (j + B < R) fetch(j + B, nxt);
         double out[B];
-        forward_batch64<W, K, B>(fr + j * RW, cur, zform, st, out);
+        forward_batch64<W, K, B>(fr + j * RW, cur, [](double x) { return x; }, zform, st, out);
 #pragma unroll
         for (int u = 0; u < B; ++u) {
             if (j + u >= P) y[j + u - P] = out[u];
@@ -265,54 +278,75 @@ __device__ __forceinline__ bool columns64_item(const Win64Args &a, const Pos64 &
     }
     backward_sweep64<W, L, P>(constant(pl.bwd + static_cast<long>(r0) * window_bwd_words(W)), y);
 
-    float *o = static_cast<float *>(a.dst) + static_cast<long>(at.frame) * a.d_frame + cc;
+    Out *o = static_cast<Out *>(a.dst) + static_cast<long>(at.frame) * a.d_frame + cc;
     const int n = pl.n;
-    const DevConv cv{0.0F, 1.0F, 1.0F, 0.0F, 0.0F, 0};
+    const DevConv cv = a.conv;
     bool finite = true;
 #pragma unroll
     for (int k = 0; k < L; ++k)
         if (r0 + k < n && live)
-            finite = Store<float>::f64(o + static_cast<long>(r0 + k) * a.d_row, y[k], cv) && finite;
+            finite = Store<Out>::f64(o + static_cast<long>(r0 + k) * a.d_row, y[k], cv) && finite;
     return finite;
 }
 
-template <int W, int K>
+template <int W, int K, class In>
 __global__ void __launch_bounds__(64 * kWaves64) rows_window64(Win64Args a, long items) {
     __shared__ std::uint32_t lds[kWaves64 * kRing64];
     const int lane = threadIdx.x & 63;
     const int wave = uniform(threadIdx.x >> 6);
     const long item = static_cast<long>(blockIdx.x) * kWaves64 + wave;
     if (item >= items) return;
-    if (!rows64_item<W, K>(a, decode64(a, item), lds + wave * kRing64, lane) && a.flag)
+    if (!rows64_item<W, K, In>(a, decode64(a, item), lds + wave * kRing64, lane) && a.flag)
         atomicOr(a.flag, 1);
 }
 
-template <int W, int K>
+template <int W, int K, class Out>
 __global__ void __launch_bounds__(64 * kWaves64) columns_window64(Win64Args a, long items) {
     const int lane = threadIdx.x & 63;
     const int wave = uniform(threadIdx.x >> 6);
     const long item = static_cast<long>(blockIdx.x) * kWaves64 + wave;
     if (item >= items) return;
-    if (!columns64_item<W, K>(a, decode64(a, item), lane) && a.flag) atomicOr(a.flag, 1);
+    if (!columns64_item<W, K, Out>(a, decode64(a, item), lane) && a.flag) atomicOr(a.flag, 1);
 }
 
 } // namespace
 } // namespace dev
 
-template <int W>
-hipError_t launch_window64_band(const Win64Args &a, int width, bool rows, hipStream_t s) {
-    const long items = static_cast<long>(a.frames) * a.plan.chunks * a.groups;
-    if (items <= 0) return hipSuccess;
-    const long blocks = (items + dev::kWaves64 - 1) / dev::kWaves64;
+namespace dev {
+namespace {
+template <int W, int K, class T>
+hipError_t launch64(const Win64Args &a, long items, bool rows, hipStream_t s) {
+    const long blocks = (items + kWaves64 - 1) / kWaves64;
     if (blocks > 0x7FFFFFFFL) return hipErrorInvalidValue;
     const dim3 grid(static_cast<unsigned>(blocks));
-    const dim3 block(64 * dev::kWaves64);
-    if (width == 8 && rows) dev::rows_window64<W, 8><<<grid, block, 0, s>>>(a, items);
-    else if (width == 8) dev::columns_window64<W, 8><<<grid, block, 0, s>>>(a, items);
-    else if (width == 16 && rows) dev::rows_window64<W, 16><<<grid, block, 0, s>>>(a, items);
-    else if (width == 16) dev::columns_window64<W, 16><<<grid, block, 0, s>>>(a, items);
-    else return hipErrorInvalidValue;
+    const dim3 block(64 * kWaves64);
+    if (rows) rows_window64<W, K, T><<<grid, block, 0, s>>>(a, items);
+    else columns_window64<W, K, T><<<grid, block, 0, s>>>(a, items);
     return hipGetLastError();
+}
+} // namespace
+} // namespace dev
+
+// `kind`: the sample type of the rows pass's source or the columns pass's
+// destination (the intermediate is FP64).
+template <int W>
+hipError_t launch_window64_band(const Win64Args &a, int width, bool rows, Sample kind,
+                                hipStream_t s) {
+    const long items = static_cast<long>(a.frames) * a.plan.chunks * a.groups;
+    if (items <= 0) return hipSuccess;
+    if (width != 8 && width != 16) return hipErrorInvalidValue;
+    const bool wide = width == 16;
+    switch (kind) {
+    case Sample::f32:
+        return wide ? dev::launch64<W, 16, float>(a, items, rows, s) : dev::launch64<W, 8, float>(a, items, rows, s);
+    case Sample::u8:
+        return wide ? dev::launch64<W, 16, std::uint8_t>(a, items, rows, s)
+                    : dev::launch64<W, 8, std::uint8_t>(a, items, rows, s);
+    case Sample::u16:
+        return wide ? dev::launch64<W, 16, std::uint16_t>(a, items, rows, s)
+                    : dev::launch64<W, 8, std::uint16_t>(a, items, rows, s);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 } // namespace dsmvc::hip

@@ -1,7 +1,7 @@
 // Windowed FP64 kernels (Float64 route), band class 5: rows and columns,
-// tap widths 8 and 16 (kernels_window64.hpp).
+// tap widths 8 and 16, F32 and integer samples (kernels_window64.hpp).
 #include "kernels_window64.hpp"
 
 namespace dsmvc::hip {
-template hipError_t launch_window64_band<5>(const Win64Args &, int, bool, hipStream_t);
+template hipError_t launch_window64_band<5>(const Win64Args &, int, bool, Sample, hipStream_t);
 } // namespace dsmvc::hip
