@@ -549,6 +549,33 @@ def main():
                 "frac_of_8TBps": round(FRAME_BYTES_GRAY_F32 * gfps / 1e9 / HBM_PEAK_GBPS, 4)}
         del gin
 
+    if not args.no_gray:
+        # Config 3's cost per precision route: Delanczos taps=4 blur 0.9, GRAYS
+        # F32 1920x1080 -> 1904x1071, 32 frames per call.  FP32 plain (plans
+        # with f64mode 1), FP32 + one refinement step and the Float64 route
+        # (f64mode 2 plans); their errors against the FP64 oracle are in
+        # tests/test_gpu_precision.py.
+        lk = dict(kernel=dh.LANCZOS, taps=4, b=0.0, c=0.5, blur=0.9, border=dh.SYMMETRIC)
+        def lplans(mode):
+            return (dh.Plan(dh.AxisRequest(SRC_W, 1904, 1904.0, 0.0, f64_mode=mode, **lk)),
+                    dh.Plan(dh.AxisRequest(SRC_H, 1071, 1071.0, 0.0, f64_mode=mode, **lk)))
+        refined = dh.Executor(0, "refined")
+        routes = {"fp32": (ex, lplans(1)), "fp32_refine1": (refined, lplans(2)),
+                  "float64": (ex, lplans(2))}
+        pin = torch.rand((32, SRC_H, SRC_W), device=dev)
+        pout = torch.empty((32, 1071, 1904), device=dev)
+        prec = {}
+        for name, (rex, (hp, vp)) in routes.items():
+            rex.inverse_2d(hp, vp, pin, out=pout)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(5):
+                rex.inverse_2d(hp, vp, pin, out=pout, asynchronous=True)
+            rex.synchronize()
+            prec[name] = {"frames_per_s": round(32 * 5 / (time.perf_counter() - t0), 1)}
+        extra["config3_routes_lanczos4_1904x1071"] = prec
+        del pin, pout, refined
+
     if not args.no_batch:
         frame = (unique[0][0].to(torch.float32) / 65535.0).contiguous()
         extra["mixed_size_scan"] = batched_scan(torch, dh, dev, frame, repeats=5)

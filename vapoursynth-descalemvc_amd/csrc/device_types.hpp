@@ -156,7 +156,7 @@ struct DevRefine {
 
 // Windowed-kernel shape per band class: chunk length L and warm-up P.
 constexpr int window_length(int w) { return w <= 3 ? 64 : (w <= 5 ? 48 : 32); }
-constexpr int window_warmup(int w) { return w <= 3 ? 24 : (w <= 5 ? 32 : 48); }
+constexpr int window_warmup(int w) { return w <= 3 ? 24 : (w <= 5 ? 32 : 64); }
 // Record sizes of the windowed tables (DevWindow) in words, and the rows one
 // batch of scalar loads covers (at most 64 words, a divisor of 8).
 constexpr int window_fwd_words(int w, int k) { return (k + w + 1 + 3) & ~3; }

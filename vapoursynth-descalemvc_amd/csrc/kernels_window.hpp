@@ -387,9 +387,10 @@ rows_window_table4(const WinJob *__restrict__ jobs, int count, long items, int o
     rows_table_item<W, K, G, In>(jobs, count, items, order, lds);
 }
 
-// u16 samples with 8-row blocks need more registers than 128 without spilling.
-template <class In, int G>
-constexpr bool rows_four_waves = !(std::is_same_v<In, std::uint16_t> && G == 8);
+// 8-row blocks of u16 samples, and of band class 7 (64 warm-up rows), need
+// more than 128 registers without spilling.
+template <int W, class In, int G>
+constexpr bool rows_four_waves = !(G == 8 && (W == 7 || std::is_same_v<In, std::uint16_t>));
 
 template <int W, int K, int G, class Out>
 __global__ void __launch_bounds__(64 * kWindowWaves)
@@ -418,7 +419,7 @@ hipError_t launch_table(const WinJob *jobs, int count, long items, bool rows, bo
         columns_window_table<W, K, 8, T><<<grid, 64 * kWindowWaves, 0, s>>>(jobs, count, items, order);
     else if (block16)
         rows_window_table4<W, K, 16, T><<<grid, 64 * kWindowWaves, 0, s>>>(jobs, count, items, order);
-    else if constexpr (rows_four_waves<T, 8>)
+    else if constexpr (rows_four_waves<W, T, 8>)
         rows_window_table4<W, K, 8, T><<<grid, 64 * kWindowWaves, 0, s>>>(jobs, count, items, order);
     else
         rows_window_table<W, K, 8, T><<<grid, 64 * kWindowWaves, 0, s>>>(jobs, count, items, order);
