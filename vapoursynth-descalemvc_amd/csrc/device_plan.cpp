@@ -381,6 +381,34 @@ bool build_window_plan(const AxisPlan &p, WindowPlan &out) {
                                kWindowTolerance, out);
 }
 
+bool build_solve_window_plan(const AxisPlan &p, WindowPlan &out) {
+    const int band = window_band_class(p.half_bandwidth);
+    const int n = p.destination_size, w = p.half_bandwidth;
+    if (band == 0 || n < 1) return false;
+    // The same system with identity taps: row i reads element i of its
+    // right-hand side.
+    AxisPlan q;
+    q.source_size = n;
+    q.destination_size = n;
+    q.half_bandwidth = w;
+    q.transpose_offsets.resize(static_cast<std::size_t>(n) + 1);
+    q.transpose_indices.resize(static_cast<std::size_t>(n));
+    for (int i = 0; i <= n; ++i) q.transpose_offsets[static_cast<std::size_t>(i)] = static_cast<std::uint32_t>(i);
+    for (int i = 0; i < n; ++i) q.transpose_indices[static_cast<std::size_t>(i)] = i;
+    const auto un = static_cast<std::size_t>(n);
+    auto coef = [&](int i, int slot) -> float {
+        const auto ui = static_cast<std::size_t>(i);
+        if (slot == kMaxBand) return p.inverse_diagonal[ui];
+        const int d = slot < kMaxBand ? slot + 1 : slot - kMaxBand;
+        if (d > w || slot > 2 * kMaxBand) return 0.0F;
+        const std::size_t at = static_cast<std::size_t>(d - 1) * un + ui;
+        return slot < kMaxBand ? p.lower_ld[at] : p.upper_l[at];
+    };
+    auto weight = [](std::uint32_t) { return 1.0F; };
+    return build_window<float>(q, window_length(band), window_warmup(band), coef, weight, false,
+                               kWindowTolerance, out);
+}
+
 bool build_window_plan64(const AxisPlan &p, const PackedPlan &packed, WindowPlan64 &out) {
     const int band = window_band_class(p.half_bandwidth);
     if (band == 0 || packed.coef_stride != 16) return false;
@@ -437,6 +465,7 @@ DevicePlan::~DevicePlan() {
         if (entry.second.memory) (void)hipFree(entry.second.memory);
     if (window_image_.memory) (void)hipFree(window_image_.memory);
     if (window64_image_.memory) (void)hipFree(window64_image_.memory);
+    if (solve_image_.memory) (void)hipFree(solve_image_.memory);
     if (refine_memory_) (void)hipFree(refine_memory_);
     if (memory_) (void)hipFree(memory_);
 }
@@ -490,6 +519,44 @@ const DevChunks *DevicePlan::chunks(int max_chunks, bool rows) {
     return &it->second.dev;
 }
 
+// Uploads a window plan's tables as one allocation and fills `dev` (its
+// plan sizes from this plan).  Returns the allocation.
+void *DevicePlan::upload_window(const WindowPlan &wp, DevWindow &dev) {
+    const std::size_t o_fwd = 0;
+    const std::size_t o_bwd = align256(o_fwd + wp.fwd.size() * 4);
+    const std::size_t o_ts = align256(o_bwd + wp.bwd.size() * 4);
+    const std::size_t o_blk = align256(o_ts + wp.tap_start.size() * 4);
+    const std::size_t o_blk16 = align256(o_blk + wp.blk_lo.size() * 4);
+    const std::size_t total = align256(o_blk16 + wp.blk_lo16.size() * 4);
+    std::vector<unsigned char> host(total, 0);
+    std::memcpy(host.data() + o_fwd, wp.fwd.data(), wp.fwd.size() * 4);
+    std::memcpy(host.data() + o_bwd, wp.bwd.data(), wp.bwd.size() * 4);
+    std::memcpy(host.data() + o_ts, wp.tap_start.data(), wp.tap_start.size() * 4);
+    std::memcpy(host.data() + o_blk, wp.blk_lo.data(), wp.blk_lo.size() * 4);
+    std::memcpy(host.data() + o_blk16, wp.blk_lo16.data(), wp.blk_lo16.size() * 4);
+    int current = 0;
+    check(hipGetDevice(&current), "hipGetDevice failed");
+    check(hipSetDevice(device_), "hipSetDevice failed");
+    void *memory = nullptr;
+    check(hipMalloc(&memory, total), "window table allocation failed");
+    check(hipMemcpy(memory, host.data(), total, hipMemcpyHostToDevice), "window table upload failed");
+    (void)hipSetDevice(current);
+    auto *b = static_cast<unsigned char *>(memory);
+    dev.n = plan_->destination_size;
+    dev.m = plan_->source_size;
+    dev.length = wp.length;
+    dev.warmup = wp.warmup;
+    dev.chunks = wp.chunks;
+    dev.ell_width = wp.width;
+    dev.block16 = wp.block16 ? 1 : 0;
+    dev.fwd = reinterpret_cast<const float *>(b + o_fwd);
+    dev.bwd = reinterpret_cast<const float *>(b + o_bwd);
+    dev.tap_start = reinterpret_cast<const std::int32_t *>(b + o_ts);
+    dev.blk_lo = reinterpret_cast<const std::int32_t *>(b + o_blk);
+    dev.blk_lo16 = reinterpret_cast<const std::int32_t *>(b + o_blk16);
+    return memory;
+}
+
 const DevWindow *DevicePlan::window(bool rows) {
     std::lock_guard lock(mutex_);
     WindowImage &image = window_image_;
@@ -497,44 +564,29 @@ const DevWindow *DevicePlan::window(bool rows) {
         image.built = true;
         WindowPlan wp;
         if (build_window_plan(*plan_, wp)) {
-            const std::size_t o_fwd = 0;
-            const std::size_t o_bwd = align256(o_fwd + wp.fwd.size() * 4);
-            const std::size_t o_ts = align256(o_bwd + wp.bwd.size() * 4);
-            const std::size_t o_blk = align256(o_ts + wp.tap_start.size() * 4);
-            const std::size_t o_blk16 = align256(o_blk + wp.blk_lo.size() * 4);
-            const std::size_t total = align256(o_blk16 + wp.blk_lo16.size() * 4);
-            std::vector<unsigned char> host(total, 0);
-            std::memcpy(host.data() + o_fwd, wp.fwd.data(), wp.fwd.size() * 4);
-            std::memcpy(host.data() + o_bwd, wp.bwd.data(), wp.bwd.size() * 4);
-            std::memcpy(host.data() + o_ts, wp.tap_start.data(), wp.tap_start.size() * 4);
-            std::memcpy(host.data() + o_blk, wp.blk_lo.data(), wp.blk_lo.size() * 4);
-            std::memcpy(host.data() + o_blk16, wp.blk_lo16.data(), wp.blk_lo16.size() * 4);
-            int current = 0;
-            check(hipGetDevice(&current), "hipGetDevice failed");
-            check(hipSetDevice(device_), "hipSetDevice failed");
-            check(hipMalloc(&image.memory, total), "window table allocation failed");
-            check(hipMemcpy(image.memory, host.data(), total, hipMemcpyHostToDevice),
-                  "window table upload failed");
-            (void)hipSetDevice(current);
-            auto *b = static_cast<unsigned char *>(image.memory);
-            image.dev.n = plan_->destination_size;
-            image.dev.m = plan_->source_size;
-            image.dev.length = wp.length;
-            image.dev.warmup = wp.warmup;
-            image.dev.chunks = wp.chunks;
-            image.dev.ell_width = wp.width;
-            image.dev.block16 = wp.block16 ? 1 : 0;
-            image.dev.fwd = reinterpret_cast<const float *>(b + o_fwd);
-            image.dev.bwd = reinterpret_cast<const float *>(b + o_bwd);
-            image.dev.tap_start = reinterpret_cast<const std::int32_t *>(b + o_ts);
-            image.dev.blk_lo = reinterpret_cast<const std::int32_t *>(b + o_blk);
-            image.dev.blk_lo16 = reinterpret_cast<const std::int32_t *>(b + o_blk16);
+            image.memory = upload_window(wp, image.dev);
             image.ok = true;
             image.rows_feasible = wp.rows_feasible;
         }
     }
     if (!image.ok || (rows && !image.rows_feasible)) return nullptr;
     return &image.dev;
+}
+
+const DevWindow *DevicePlan::solve_window() {
+    std::lock_guard lock(mutex_);
+    WindowImage &image = solve_image_;
+    if (!image.built) {
+        image.built = true;
+        WindowPlan wp;
+        if (build_solve_window_plan(*plan_, wp)) {
+            image.memory = upload_window(wp, image.dev);
+            image.dev.m = plan_->destination_size;
+            image.ok = true;
+            image.rows_feasible = wp.rows_feasible;
+        }
+    }
+    return image.ok ? &image.dev : nullptr;
 }
 
 const DevWindow64 *DevicePlan::window64() {
@@ -605,16 +657,48 @@ const DevRefine *DevicePlan::refine() {
                 const std::size_t at = static_cast<std::size_t>(i) * span + static_cast<std::size_t>(d + w);
                 split(g, g_hi[at], g_lo[at]);
             }
+        // Tap-major copies: k = the widest run of source samples a row spans.
+        int k = 1;
+        std::vector<std::int32_t> first(un, 0);
+        for (int i = 0; i < n; ++i) {
+            const auto k0 = p.transpose_offsets[static_cast<std::size_t>(i)];
+            const auto k1 = p.transpose_offsets[static_cast<std::size_t>(i) + 1];
+            if (k1 <= k0) continue;
+            first[static_cast<std::size_t>(i)] = p.transpose_indices[k0];
+            k = std::max(k, p.transpose_indices[k1 - 1] - p.transpose_indices[k0] + 1);
+        }
+        const auto uk = static_cast<std::size_t>(k);
+        std::vector<double> t64(uk * un, 0.0), b64(un * span, 0.0);
+        for (int i = 0; i < n; ++i) {
+            const auto ui = static_cast<std::size_t>(i);
+            for (auto q = p.transpose_offsets[ui]; q < p.transpose_offsets[ui + 1]; ++q) {
+                const auto e = static_cast<std::size_t>(p.transpose_indices[q] - first[ui]);
+                t64[e * un + ui] = p.transpose_weights_f64[q];
+            }
+            for (int d = -w; d <= w; ++d) {
+                const int j = i + d;
+                if (j < 0 || j >= n) continue;
+                b64[static_cast<std::size_t>(d + w) * un + ui]
+                    = p.normal_bands_f64[static_cast<std::size_t>(std::abs(d)) * un
+                                         + static_cast<std::size_t>(std::min(i, j))];
+            }
+        }
         const std::size_t o_wh = 0;
         const std::size_t o_wl = align256(o_wh + std::max<std::size_t>(nnz, 1) * 4);
         const std::size_t o_gh = align256(o_wl + std::max<std::size_t>(nnz, 1) * 4);
         const std::size_t o_gl = align256(o_gh + g_hi.size() * 4);
-        const std::size_t total = align256(o_gl + g_lo.size() * 4);
+        const std::size_t o_fi = align256(o_gl + g_lo.size() * 4);
+        const std::size_t o_t = align256(o_fi + first.size() * 4);
+        const std::size_t o_b = align256(o_t + t64.size() * 8);
+        const std::size_t total = align256(o_b + b64.size() * 8);
         std::vector<unsigned char> host(total, 0);
         std::memcpy(host.data() + o_wh, w_hi.data(), nnz * 4);
         std::memcpy(host.data() + o_wl, w_lo.data(), nnz * 4);
         std::memcpy(host.data() + o_gh, g_hi.data(), g_hi.size() * 4);
         std::memcpy(host.data() + o_gl, g_lo.data(), g_lo.size() * 4);
+        std::memcpy(host.data() + o_fi, first.data(), first.size() * 4);
+        std::memcpy(host.data() + o_t, t64.data(), t64.size() * 8);
+        std::memcpy(host.data() + o_b, b64.data(), b64.size() * 8);
         int current = 0;
         check(hipGetDevice(&current), "hipGetDevice failed");
         check(hipSetDevice(device_), "hipSetDevice failed");
@@ -629,6 +713,11 @@ const DevRefine *DevicePlan::refine() {
         refine_.w_lo = reinterpret_cast<const float *>(b + o_wl);
         refine_.g_hi = reinterpret_cast<const float *>(b + o_gh);
         refine_.g_lo = reinterpret_cast<const float *>(b + o_gl);
+        refine_.m = p.source_size;
+        refine_.k = k;
+        refine_.first = reinterpret_cast<const std::int32_t *>(b + o_fi);
+        refine_.t64 = reinterpret_cast<const double *>(b + o_t);
+        refine_.b64 = reinterpret_cast<const double *>(b + o_b);
     }
     return refine_memory_ ? &refine_ : nullptr;
 }

@@ -84,6 +84,10 @@ constexpr double kWindowTolerance64 = 1.0 / 1099511627776.0;
 bool build_window_plan(const AxisPlan &plan, WindowPlan &out);
 // The same for the Float64 route, from the plan's FP64 packing.
 bool build_window_plan64(const AxisPlan &plan, const PackedPlan &packed, WindowPlan64 &out);
+// The plan's system G d = r with identity taps (row i reads element i of r),
+// for the correction solve of refinement: the windowed kernels then solve a
+// given right-hand side of n elements per vector.
+bool build_solve_window_plan(const AxisPlan &plan, WindowPlan &out);
 
 // One device allocation holding a plan's image (and its chunk partitions).
 class DevicePlan {
@@ -105,6 +109,9 @@ public:
     // Windowed FP64 partition (Float64 route); nullptr when the plan's FP64
     // recurrence does not decay fast enough.  Built lazily, cached.
     const DevWindow64 *window64();
+    // Windowed tables of build_solve_window_plan (source size n); nullptr
+    // when the windowed kernels cannot take the plan.  Built lazily, cached.
+    const DevWindow *solve_window();
     // Refinement tables (FP64 weights and normal matrix split into F32
     // pairs); nullptr for a plan without FP64 data.  Built lazily, cached.
     const DevRefine *refine();
@@ -132,7 +139,9 @@ private:
         bool ok = false;
         bool rows_feasible = false;
     };
+    void *upload_window(const WindowPlan &wp, DevWindow &dev);
     WindowImage window_image_;
+    WindowImage solve_image_;
     struct Window64Image {
         DevWindow64 dev{};
         void *memory = nullptr;

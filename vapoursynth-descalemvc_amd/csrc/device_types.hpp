@@ -152,6 +152,15 @@ struct DevRefine {
     const float *w_lo;
     const float *g_hi;     // [n][2w+1]
     const float *g_lo;
+    // The same data in FP64 and tap-major for element-parallel kernels,
+    // whose lanes run along consecutive elements: row i reads source samples
+    // [first[i], first[i] + k) with weights t64[q * n + i] (0 past the row's
+    // taps), and G(i, i - w + d) = b64[d * n + i] (0 outside G).
+    std::int32_t m;
+    std::int32_t k;
+    const std::int32_t *first;  // [n]
+    const double *t64;          // [k][n]
+    const double *b64;          // [2w+1][n]
 };
 
 // Windowed-kernel shape per band class: chunk length L and warm-up P.

@@ -343,8 +343,41 @@ void HipExecutor::axis_pass(const AxisPlan &plan, bool rows, const void *input, 
     a.scratch = reinterpret_cast<float *>(static_cast<unsigned char *>(s.buffer) + fwd_offset);
     a.nvec = vectors;
     a.frames = frames;
-    for (int step = 0; step < refine_steps_; ++step)
-        check(hip::launch_refine_step(a, stream), "refinement launch failed");
+    const hip::DevWindow *solve = dp.solve_window();
+    if (!solve) {
+        for (int step = 0; step < refine_steps_; ++step)
+            check(hip::launch_refine_step(a, stream), "refinement launch failed");
+        return;
+    }
+    // Residual element by element, the correction G d = r on the windowed
+    // kernels (identity taps over r), then x += d.  r and d are laid out
+    // vector-major for a rows pass, element-major for a columns pass.
+    const std::ptrdiff_t n = plan.destination_size;
+    const std::size_t region = align256(static_cast<std::size_t>(n) * vectors * frames * sizeof(float));
+    float *r = a.scratch;
+    float *d = reinterpret_cast<float *>(reinterpret_cast<unsigned char *>(r) + region);
+    hip::WinJob job{};
+    job.plan = *solve;
+    job.src = r;
+    job.s_row = rows ? n : vectors;
+    job.s_frame = n * vectors;
+    job.dst = d;
+    job.d_row = rows ? n : vectors;
+    job.d_frame = n * vectors;
+    job.nvec = vectors;
+    job.frames = frames;
+    job.groups = (vectors + 63) / 64;
+    job.conv = make_conv(nullptr);
+    for (int step = 0; step < refine_steps_; ++step) {
+        check(hip::launch_refine_residual(a, r, rows, stream), "refinement launch failed");
+        WindowTables &tables = stream_tables(stream);
+        tables.clear();
+        tables.add(job, hip::window_band_class(dp.axis().w), rows, Sample::f32);
+        tables.upload(stream);
+        tables.launch(stream);
+        check(hipGetLastError(), "refinement launch failed");
+        check(hip::launch_refine_add(a, d, rows, stream), "refinement launch failed");
+    }
 }
 
 void HipExecutor::solve_pass(const AxisPlan &plan, bool rows, const void *input, Sample in_kind,
@@ -459,7 +492,8 @@ void HipExecutor::inverse_rows(const AxisPlan &plan, const float *input,
     check(hipSetDevice(device_), "hipSetDevice failed");
     const bool f64 = float64_route(plan);
     const std::size_t elem = f64 ? 8 : 4;
-    Scratch &s = scratch(stream, align256(static_cast<std::size_t>(row_count) * plan.destination_size * elem));
+    Scratch &s = scratch(stream, refine_regions() * align256(static_cast<std::size_t>(row_count)
+                                                             * plan.destination_size * elem));
     const DevConv conv = make_conv(nullptr);
     if (f64) {
         arm_flag(s, stream);
@@ -483,7 +517,8 @@ void HipExecutor::inverse_columns(const AxisPlan &plan, const float *input,
     check(hipSetDevice(device_), "hipSetDevice failed");
     const bool f64 = float64_route(plan);
     const std::size_t elem = f64 ? 8 : 4;
-    Scratch &s = scratch(stream, align256(static_cast<std::size_t>(column_count) * plan.destination_size * elem));
+    Scratch &s = scratch(stream, refine_regions() * align256(static_cast<std::size_t>(column_count)
+                                                             * plan.destination_size * elem));
     const DevConv conv = make_conv(nullptr);
     if (f64) {
         arm_flag(s, stream);
@@ -622,7 +657,7 @@ HipExecutor::Scratch &HipExecutor::solve_2d(const AxisPlan &h, const AxisPlan &v
     const std::ptrdiff_t inter_frame = inter_row * v.source_size;
     const std::size_t inter_bytes = align256(static_cast<std::size_t>(inter_frame) * b.frames * elem);
     // Sequential forward sweeps, the larger of the two passes.
-    const std::size_t fwd_bytes = align256(
+    const std::size_t fwd_bytes = refine_regions() * align256(
         std::max(static_cast<std::size_t>(v.source_size) * h.destination_size,
                  static_cast<std::size_t>(h.destination_size) * v.destination_size)
         * b.frames * elem);
@@ -672,8 +707,8 @@ void HipExecutor::inverse_rows_batch(const AxisPlan &plan, SampleKind in_kind,
     }
     std::lock_guard lock(mutex_);
     check(hipSetDevice(device_), "hipSetDevice failed");
-    Scratch &s = scratch(stream, align256(static_cast<std::size_t>(rows) * plan.destination_size
-                                          * b.frames * 4));
+    Scratch &s = scratch(stream, refine_regions() * align256(static_cast<std::size_t>(rows)
+                                                             * plan.destination_size * b.frames * 4));
     axis_pass(plan, true, b.input, sample_of(in_kind), b.input_row_stride, b.input_frame_stride,
               b.output, Sample::f32, b.output_row_stride, b.output_frame_stride, rows, b.frames,
               make_conv(conversion), false, s, 0, stream);
@@ -699,8 +734,8 @@ void HipExecutor::inverse_columns_batch(const AxisPlan &plan, SampleKind out_kin
     }
     std::lock_guard lock(mutex_);
     check(hipSetDevice(device_), "hipSetDevice failed");
-    Scratch &s = scratch(stream, align256(static_cast<std::size_t>(columns) * plan.destination_size
-                                          * b.frames * 4));
+    Scratch &s = scratch(stream, refine_regions() * align256(static_cast<std::size_t>(columns)
+                                                             * plan.destination_size * b.frames * 4));
     axis_pass(plan, false, b.input, Sample::f32, b.input_row_stride, b.input_frame_stride,
               b.output, sample_of(out_kind), b.output_row_stride, b.output_frame_stride, columns,
               b.frames, make_conv(conversion), false, s, 0, stream);

@@ -167,6 +167,9 @@ private:
     dsmvc::hip::DevicePlan &device_plan(const AxisPlan &plan);
     Scratch &scratch(hipStream_t stream, std::size_t bytes);
     void finish(hipStream_t stream, Scratch &s, bool async, const char *what);
+    // Refined solves keep the residual and its correction side by side in
+    // the scratch region that otherwise holds one forward sweep.
+    std::size_t refine_regions() const { return mode_ == SolveMode::refined ? 2 : 1; }
     bool float64_route(const AxisPlan &plan) const {
         return plan.requires_float64() && mode_ != SolveMode::refined;
     }

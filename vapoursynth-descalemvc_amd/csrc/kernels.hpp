@@ -121,6 +121,11 @@ struct RefineArgs {
     int frames;
 };
 hipError_t launch_refine_step(const RefineArgs &a, hipStream_t s);
+// The residual r (n * nvec * frames floats) element by element, laid out
+// [frame][vector][element] for a rows pass and [frame][element][vector] for a
+// columns pass; then x += d for a correction d laid out the same way.
+hipError_t launch_refine_residual(const RefineArgs &a, float *r, bool rows, hipStream_t s);
+hipError_t launch_refine_add(const RefineArgs &a, const float *d, bool rows, hipStream_t s);
 
 hipError_t launch_check_finite(const void *src, Sample in, long s_row, long s_frame, int rows,
                                int cols, int frames, int *flag, hipStream_t s);

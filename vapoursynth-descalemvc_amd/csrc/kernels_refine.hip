@@ -12,9 +12,11 @@
 // The FP64 route (reference src/axis_plan.cpp:1124-1248) solves the same
 // system in double; refinement converges to its solution at F32 rounding.
 //
-// Lanes own vectors (one lane per vector, like the sequential solve); the
-// band width is a runtime value.  This is the precision route, not the
-// throughput route.
+// Two forms: the residual element by element in FP64 with the correction
+// solved by the windowed kernels (launch_refine_residual / launch_refine_add,
+// used when the plan's windowed tables exist), or one lane per vector walking
+// the whole vector for both, with the compensated F32 residual above
+// (launch_refine_step).  The band width is a runtime value.
 #include "kernels_common.hpp"
 
 namespace dsmvc::hip {
@@ -92,8 +94,144 @@ __global__ void __launch_bounds__(256) refine_correct(RefineArgs a) {
     }
 }
 
+// The residual element by element.  r is laid out [frame][vector][element]
+// for a rows pass and [frame][element][vector] for a columns pass.  The sums
+// run in FP64 over the tap-major FP64 tables (FP64 arithmetic runs at the F32
+// rate on this GPU), and r is rounded to F32 once.  Indices past a row's
+// samples are clamped; their weights are 0, so the terms are exactly 0 (the
+// inputs are finite).
+//
+// Rows pass: a block covers 256 consecutive elements of V vectors; each
+// thread sums one element of V vectors, so one coefficient load serves V
+// sums.  Elements run along the lanes, so the tap-major tables and the
+// samples (y and x are row-contiguous) are read coalesced.
+template <int V>
+__global__ void __launch_bounds__(256) residual_rows(RefineArgs a, float *r) {
+    const int n = a.axis.n, w = a.refine.w;
+    const int iblocks = (n + 255) / 256;
+    const int i = static_cast<int>(blockIdx.x % iblocks) * 256 + static_cast<int>(threadIdx.x);
+    const int v0 = static_cast<int>(blockIdx.x / iblocks) * V;
+    if (i >= n) return;
+    const long f = blockIdx.y;
+    const float *y[V];
+    const float *x[V];
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+        const long v = min(v0 + u, a.nvec - 1);
+        y[u] = a.y + f * a.y_frame + v * a.y_vec;
+        x[u] = a.x + f * a.x_frame + v * a.x_vec;
+    }
+    const int j0 = a.refine.first[i], m1 = a.refine.m - 1;
+    const double *t = a.refine.t64 + i;
+    const double *g = a.refine.b64 + i;
+    double s[V];
+#pragma unroll
+    for (int u = 0; u < V; ++u) s[u] = 0.0;
+#pragma unroll 2
+    for (int q = 0; q < a.refine.k; ++q) {
+        const double c = t[static_cast<long>(q) * n];
+        const int j = min(j0 + q, m1);
+#pragma unroll
+        for (int u = 0; u < V; ++u) s[u] = __builtin_fma(c, static_cast<double>(y[u][j]), s[u]);
+    }
+    const int span = 2 * w + 1;
+#pragma unroll 2
+    for (int d = 0; d < span; ++d) {
+        const double c = -g[static_cast<long>(d) * n];
+        const int j = min(max(i - w + d, 0), n - 1);
+#pragma unroll
+        for (int u = 0; u < V; ++u) s[u] = __builtin_fma(c, static_cast<double>(x[u][j]), s[u]);
+    }
+    const long per = static_cast<long>(n) * a.nvec;
+#pragma unroll
+    for (int u = 0; u < V; ++u)
+        if (v0 + u < a.nvec)
+            r[f * per + static_cast<long>(v0 + u) * n + i] = static_cast<float>(s[u]);
+}
+
+// Columns pass: a block covers one element of 256 adjacent vectors, so its
+// coefficients are workgroup-uniform (scalar loads) and the samples of one
+// source row are read coalesced.  Two FMA chains keep loads in flight.
+__global__ void __launch_bounds__(256) residual_columns(RefineArgs a, float *r) {
+    const int n = a.axis.n, w = a.refine.w;
+    const int vblocks = (a.nvec + 255) / 256;
+    const int i = static_cast<int>(blockIdx.x) / vblocks;
+    const int v = static_cast<int>(blockIdx.x % vblocks) * 256 + static_cast<int>(threadIdx.x);
+    if (v >= a.nvec) return;
+    const long f = blockIdx.y;
+    const float *y = a.y + f * a.y_frame + v;
+    const float *x = a.x + f * a.x_frame + v;
+    const int j0 = a.refine.first[i], m1 = a.refine.m - 1;
+    const double *t = a.refine.t64 + i;
+    const double *g = a.refine.b64 + i;
+    const int k = a.refine.k, span = 2 * w + 1;
+    double s0 = 0.0, s1 = 0.0;
+    int q = 0;
+#pragma unroll 2
+    for (; q + 1 < k; q += 2) {
+        s0 = __builtin_fma(t[static_cast<long>(q) * n],
+                           static_cast<double>(y[min(j0 + q, m1) * a.y_el]), s0);
+        s1 = __builtin_fma(t[static_cast<long>(q + 1) * n],
+                           static_cast<double>(y[min(j0 + q + 1, m1) * a.y_el]), s1);
+    }
+    if (q < k)
+        s0 = __builtin_fma(t[static_cast<long>(q) * n], static_cast<double>(y[min(j0 + q, m1) * a.y_el]), s0);
+    int d = 0;
+#pragma unroll 2
+    for (; d + 1 < span; d += 2) {
+        s0 = __builtin_fma(-g[static_cast<long>(d) * n],
+                           static_cast<double>(x[min(max(i - w + d, 0), n - 1) * a.x_el]), s0);
+        s1 = __builtin_fma(-g[static_cast<long>(d + 1) * n],
+                           static_cast<double>(x[min(max(i - w + d + 1, 0), n - 1) * a.x_el]), s1);
+    }
+    if (d < span)
+        s0 = __builtin_fma(-g[static_cast<long>(d) * n],
+                           static_cast<double>(x[min(max(i - w + d, 0), n - 1) * a.x_el]), s0);
+    r[f * static_cast<long>(n) * a.nvec + static_cast<long>(i) * a.nvec + v] = static_cast<float>(s0 + s1);
+}
+
+// x += d, d laid out as the residual kernels lay out r.
+__global__ void __launch_bounds__(256) add_correction(RefineArgs a, const float *d, int rows) {
+    const long per = static_cast<long>(a.axis.n) * a.nvec;
+    const long idx = blockIdx.x * 256L + threadIdx.x;
+    if (idx >= per) return;
+    const long f = blockIdx.y;
+    const int n = a.axis.n;
+    int i, v;
+    if (rows) {
+        v = static_cast<int>(idx / n);
+        i = static_cast<int>(idx - static_cast<long>(v) * n);
+    } else {
+        i = static_cast<int>(idx / a.nvec);
+        v = static_cast<int>(idx - static_cast<long>(i) * a.nvec);
+    }
+    float *x = a.x + f * a.x_frame + v * a.x_vec + static_cast<long>(i) * a.x_el;
+    *x = __fadd_rn(*x, d[f * per + idx]);
+}
+
 } // namespace
 } // namespace dev
+
+hipError_t launch_refine_residual(const RefineArgs &a, float *r, bool rows, hipStream_t s) {
+    const long per = static_cast<long>(a.axis.n) * a.nvec;
+    if (per <= 0 || a.frames <= 0) return hipSuccess;
+    constexpr int V = 4;
+    const long blocks = rows ? static_cast<long>((a.axis.n + 255) / 256) * ((a.nvec + V - 1) / V)
+                             : static_cast<long>(a.axis.n) * ((a.nvec + 255) / 256);
+    if (blocks >= (1L << 31)) return hipErrorInvalidValue;
+    const dim3 grid(static_cast<unsigned>(blocks), a.frames);
+    if (rows) dev::residual_rows<V><<<grid, 256, 0, s>>>(a, r);
+    else dev::residual_columns<<<grid, 256, 0, s>>>(a, r);
+    return hipGetLastError();
+}
+
+hipError_t launch_refine_add(const RefineArgs &a, const float *d, bool rows, hipStream_t s) {
+    const long per = static_cast<long>(a.axis.n) * a.nvec;
+    if (per <= 0 || a.frames <= 0) return hipSuccess;
+    const dim3 grid(static_cast<unsigned>((per + 255) / 256), a.frames);
+    dev::add_correction<<<grid, 256, 0, s>>>(a, d, rows ? 1 : 0);
+    return hipGetLastError();
+}
 
 hipError_t launch_refine_step(const RefineArgs &a, hipStream_t s) {
     if (a.nvec <= 0 || a.frames <= 0) return hipSuccess;
