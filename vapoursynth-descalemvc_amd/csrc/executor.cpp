@@ -249,13 +249,17 @@ hipError_t HipExecutor::launch_route(hip::DevicePlan &dp, int route, bool rows,
 // Picks the route of an F32 pass: the executor's tuned choice, a fresh
 // measurement when the launch fills the GPU (every feasible route timed with
 // HIP events on the call's own buffers, whose output the caller's launch then
-// rewrites), or the default: the largest chunked partition, else the
-// windowed kernels.  0 = no parallel route (sequential solve).
+// rewrites), or the default: the windowed kernels, else the largest chunked
+// partition.  0 = no parallel route (sequential solve).
 int HipExecutor::choose_partition(hip::DevicePlan &dp, bool rows, const hip::ChunkArgs &base,
                                   hip::Sample kind, hipStream_t stream) {
     const TuneKey key{&dp, rows, static_cast<int>(kind)};
     if (auto it = tuned_.find(key); it != tuned_.end()) return it->second;
+    // The windowed route comes first: it is the default, kept unless a
+    // chunked partition is 10% faster.  Its timed launch includes the job
+    // table upload, which every call of that route pays.
     std::vector<int> candidates;
+    if (mode_ != SolveMode::chunked && dp.window(rows)) candidates.push_back(kWindowRoute);
     if (mode_ != SolveMode::windowed) {
         std::vector<std::pair<int, int>> shapes;
         for (const int c : {hip::kMaxChunks, 12, 10, 8, 6, 4}) {
@@ -267,7 +271,6 @@ int HipExecutor::choose_partition(hip::DevicePlan &dp, bool rows, const hip::Chu
             candidates.push_back(c);
         }
     }
-    if (mode_ != SolveMode::chunked && dp.window(rows)) candidates.push_back(kWindowRoute);
     if (candidates.empty()) return 0;
     const long workgroups = static_cast<long>((base.nvec + 63) / 64) * base.frames;
     if (!autotune_ || candidates.size() == 1) {
