@@ -2,7 +2,8 @@
 Float64 route, each scored against the FP64 oracle.
 
 Workload: Delanczos taps=4 and Despline64 with blur 0.9 on GRAYS F32
-1920x1080 -> 1904x1071 (near-identity, condition ~50) and -> 960x540; frames
+1920x1080 -> 1904x1071 (near-identity, condition ~50) and -> 960x540, and
+Delanczos taps=5 (band 9) -> 1904x1071; frames
 of seeded uniform [0, 1] noise plus a sum of 8 random 2-D sinusoids.  The
 oracle is the C restatement's Float64 route on forced-FP64 plans (reference
 src/axis_plan.cpp:1124-1248 with f64mode=2).
@@ -28,6 +29,9 @@ CASES = [
     ("spline64-1904x1071", SPLINE64, 3, 1904, 1071),
     ("lanczos4-960x540", LANCZOS, 4, 960, 540),
     ("spline64-960x540", SPLINE64, 3, 960, 540),
+    # Band 9: beyond the windowed kernels (bands up to 7), so refinement runs
+    # the one-lane-per-vector residual and correction kernels.
+    ("lanczos5-1904x1071", LANCZOS, 5, 1904, 1071),
 ]
 
 
@@ -65,6 +69,8 @@ def test_refinement_against_fp64_oracle(executors, oracle, name, kernel, taps, d
     h32, v32 = _plan(1920, dw, kernel, taps, F64_NEVER), _plan(1080, dh_, kernel, taps, F64_NEVER)
     h64, v64 = _plan(1920, dw, kernel, taps, F64_ALWAYS), _plan(1080, dh_, kernel, taps, F64_ALWAYS)
     assert h64.requires_float64 and v64.requires_float64
+    if taps == 5:
+        assert h64.half_bandwidth > 7 and v64.half_bandwidth > 7
     plain = executors["auto"].inverse_2d(h32, v32, x).cpu().numpy()
     f64 = executors["auto"].inverse_2d(h64, v64, x).cpu().numpy()
     ref1 = executors["refined1"].inverse_2d(h64, v64, x).cpu().numpy()
