@@ -3,7 +3,7 @@ Float64 route, each scored against the FP64 oracle.
 
 Workload: Delanczos taps=4 and Despline64 with blur 0.9 on GRAYS F32
 1920x1080 -> 1904x1071 (near-identity, condition ~50) and -> 960x540, and
-Delanczos taps=5 (band 9) -> 1904x1071; frames
+Delanczos taps=5 (band 9) -> 1904x1071, and Delanczos taps=4 -> 240x135; frames
 of seeded uniform [0, 1] noise plus a sum of 8 random 2-D sinusoids.  The
 oracle is the C restatement's Float64 route on forced-FP64 plans (reference
 src/axis_plan.cpp:1124-1248 with f64mode=2).
@@ -32,6 +32,10 @@ CASES = [
     # Band 9: beyond the windowed kernels (bands up to 7), so refinement runs
     # the one-lane-per-vector residual and correction kernels.
     ("lanczos5-1904x1071", LANCZOS, 5, 1904, 1071),
+    # 8x downscale: about 58 taps per element, so the vertical residual's
+    # four-element blocks sum element by element (their sample rows exceed the
+    # LDS tile).
+    ("lanczos4-240x135", LANCZOS, 4, 240, 135),
 ]
 
 

@@ -573,7 +573,7 @@ const DevWindow *DevicePlan::window(bool rows) {
     return &image.dev;
 }
 
-const DevWindow *DevicePlan::solve_window() {
+const DevWindow *DevicePlan::solve_window(bool rows) {
     std::lock_guard lock(mutex_);
     WindowImage &image = solve_image_;
     if (!image.built) {
@@ -586,7 +586,8 @@ const DevWindow *DevicePlan::solve_window() {
             image.rows_feasible = wp.rows_feasible;
         }
     }
-    return image.ok ? &image.dev : nullptr;
+    if (!image.ok || (rows && !image.rows_feasible)) return nullptr;
+    return &image.dev;
 }
 
 const DevWindow64 *DevicePlan::window64() {

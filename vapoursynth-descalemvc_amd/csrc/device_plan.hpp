@@ -110,10 +110,12 @@ public:
     // recurrence does not decay fast enough.  Built lazily, cached.
     const DevWindow64 *window64();
     // Windowed tables of build_solve_window_plan (source size n); nullptr
-    // when the windowed kernels cannot take the plan.  Built lazily, cached.
-    const DevWindow *solve_window();
+    // when the windowed kernels cannot take the plan in that direction.
+    // Built lazily, cached.
+    const DevWindow *solve_window(bool rows);
     // Refinement tables (FP64 weights and normal matrix split into F32
-    // pairs); nullptr for a plan without FP64 data.  Built lazily, cached.
+    // pairs, and tap-major FP64 copies); nullptr for a plan without FP64
+    // data.  Built lazily, cached.
     const DevRefine *refine();
     [[nodiscard]] std::size_t device_bytes() const { return bytes_; }
 

@@ -346,7 +346,7 @@ void HipExecutor::axis_pass(const AxisPlan &plan, bool rows, const void *input, 
     a.scratch = reinterpret_cast<float *>(static_cast<unsigned char *>(s.buffer) + fwd_offset);
     a.nvec = vectors;
     a.frames = frames;
-    const hip::DevWindow *solve = dp.solve_window();
+    const hip::DevWindow *solve = dp.solve_window(rows);
     if (!solve) {
         for (int step = 0; step < refine_steps_; ++step)
             check(hip::launch_refine_step(a, stream), "refinement launch failed");

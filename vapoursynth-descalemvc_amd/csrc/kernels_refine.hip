@@ -149,18 +149,12 @@ __global__ void __launch_bounds__(256) residual_rows(RefineArgs a, float *r) {
             r[f * per + static_cast<long>(v0 + u) * n + i] = static_cast<float>(s[u]);
 }
 
-// Columns pass: a block covers one element of 256 adjacent vectors, so its
-// coefficients are workgroup-uniform (scalar loads) and the samples of one
-// source row are read coalesced.  Two FMA chains keep loads in flight.
-__global__ void __launch_bounds__(256) residual_columns(RefineArgs a, float *r) {
+// Columns pass: the residual of element i for one vector (lanes = adjacent
+// vectors), coefficients as workgroup-uniform scalar loads.  Two FMA chains
+// keep loads in flight.
+__device__ __forceinline__ double column_residual(const RefineArgs &a, int i, const float *y,
+                                                  const float *x) {
     const int n = a.axis.n, w = a.refine.w;
-    const int vblocks = (a.nvec + 255) / 256;
-    const int i = static_cast<int>(blockIdx.x) / vblocks;
-    const int v = static_cast<int>(blockIdx.x % vblocks) * 256 + static_cast<int>(threadIdx.x);
-    if (v >= a.nvec) return;
-    const long f = blockIdx.y;
-    const float *y = a.y + f * a.y_frame + v;
-    const float *x = a.x + f * a.x_frame + v;
     const int j0 = a.refine.first[i], m1 = a.refine.m - 1;
     const double *t = a.refine.t64 + i;
     const double *g = a.refine.b64 + i;
@@ -187,7 +181,78 @@ __global__ void __launch_bounds__(256) residual_columns(RefineArgs a, float *r) 
     if (d < span)
         s0 = __builtin_fma(-g[static_cast<long>(d) * n],
                            static_cast<double>(x[min(max(i - w + d, 0), n - 1) * a.x_el]), s0);
-    r[f * static_cast<long>(n) * a.nvec + static_cast<long>(i) * a.nvec + v] = static_cast<float>(s0 + s1);
+    return s0 + s1;
+}
+
+// Columns pass: a block covers I consecutive elements of 256 adjacent vectors.
+// Neighbouring elements' sample runs overlap almost entirely, so the block
+// first writes its coefficients to LDS as dense rows over the union of the
+// runs (0 where an element has no weight), then each thread reads every
+// sample row of the union once and adds it to all I sums.  A block whose
+// union does not fit the LDS rows sums element by element instead.
+template <int I>
+__global__ void __launch_bounds__(256) residual_columns(RefineArgs a, float *r) {
+    constexpr int kCapY = 64, kCapX = 2 * 15 + I + 2;
+    __shared__ double cy[I][kCapY];
+    __shared__ double cx[I][kCapX];
+    const int n = a.axis.n, w = a.refine.w, k = a.refine.k, span = 2 * w + 1;
+    const int vblocks = (a.nvec + 255) / 256;
+    const int i0 = static_cast<int>(blockIdx.x) / vblocks * I;
+    const int v = static_cast<int>(blockIdx.x % vblocks) * 256 + static_cast<int>(threadIdx.x);
+    const long f = blockIdx.y;
+    const float *y = a.y + f * a.y_frame + min(v, a.nvec - 1);
+    const float *x = a.x + f * a.x_frame + min(v, a.nvec - 1);
+    const long plane = static_cast<long>(n) * a.nvec;
+    int i[I], j0[I];
+    int lo = 0x7fffffff, hi = -0x7fffffff;
+#pragma unroll
+    for (int u = 0; u < I; ++u) {
+        i[u] = min(i0 + u, n - 1);
+        j0[u] = a.refine.first[i[u]];
+        lo = min(lo, j0[u]);
+        hi = max(hi, j0[u] + k);
+    }
+    const int ylen = hi - lo, xlen = i[I - 1] - i[0] + span;
+    if (ylen > kCapY || xlen > kCapX) {
+        if (v >= a.nvec) return;
+#pragma unroll
+        for (int u = 0; u < I; ++u)
+            if (i0 + u < n)
+                r[f * plane + static_cast<long>(i0 + u) * a.nvec + v]
+                    = static_cast<float>(column_residual(a, i[u], y, x));
+        return;
+    }
+    for (int e = static_cast<int>(threadIdx.x); e < I * kCapY; e += 256) {
+        const int u = e / kCapY, jj = e % kCapY;
+        const int q = lo + jj - j0[u];
+        cy[u][jj] = (jj < ylen && q >= 0 && q < k) ? a.refine.t64[static_cast<long>(q) * n + i[u]] : 0.0;
+    }
+    for (int e = static_cast<int>(threadIdx.x); e < I * kCapX; e += 256) {
+        const int u = e / kCapX, dd = e % kCapX;
+        const int d = dd - (i[u] - i[0]);
+        cx[u][dd] = (dd < xlen && d >= 0 && d < span) ? -a.refine.b64[static_cast<long>(d) * n + i[u]] : 0.0;
+    }
+    __syncthreads();
+    if (v >= a.nvec) return;
+    double s[I];
+#pragma unroll
+    for (int u = 0; u < I; ++u) s[u] = 0.0;
+    const int m1 = a.refine.m - 1;
+#pragma unroll 4
+    for (int jj = 0; jj < ylen; ++jj) {
+        const double yv = static_cast<double>(y[min(lo + jj, m1) * a.y_el]);
+#pragma unroll
+        for (int u = 0; u < I; ++u) s[u] = __builtin_fma(cy[u][jj], yv, s[u]);
+    }
+#pragma unroll 4
+    for (int dd = 0; dd < xlen; ++dd) {
+        const double xv = static_cast<double>(x[min(max(i[0] - w + dd, 0), n - 1) * a.x_el]);
+#pragma unroll
+        for (int u = 0; u < I; ++u) s[u] = __builtin_fma(cx[u][dd], xv, s[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < I; ++u)
+        if (i0 + u < n) r[f * plane + static_cast<long>(i0 + u) * a.nvec + v] = static_cast<float>(s[u]);
 }
 
 // x += d, d laid out as the residual kernels lay out r.
@@ -215,13 +280,13 @@ __global__ void __launch_bounds__(256) add_correction(RefineArgs a, const float 
 hipError_t launch_refine_residual(const RefineArgs &a, float *r, bool rows, hipStream_t s) {
     const long per = static_cast<long>(a.axis.n) * a.nvec;
     if (per <= 0 || a.frames <= 0) return hipSuccess;
-    constexpr int V = 4;
+    constexpr int V = 4, I = 4;
     const long blocks = rows ? static_cast<long>((a.axis.n + 255) / 256) * ((a.nvec + V - 1) / V)
-                             : static_cast<long>(a.axis.n) * ((a.nvec + 255) / 256);
+                             : static_cast<long>((a.axis.n + I - 1) / I) * ((a.nvec + 255) / 256);
     if (blocks >= (1L << 31)) return hipErrorInvalidValue;
     const dim3 grid(static_cast<unsigned>(blocks), a.frames);
     if (rows) dev::residual_rows<V><<<grid, 256, 0, s>>>(a, r);
-    else dev::residual_columns<<<grid, 256, 0, s>>>(a, r);
+    else dev::residual_columns<I><<<grid, 256, 0, s>>>(a, r);
     return hipGetLastError();
 }
 
