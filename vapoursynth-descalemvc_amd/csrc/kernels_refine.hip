@@ -103,15 +103,24 @@ __global__ void __launch_bounds__(256) refine_correct(RefineArgs a) {
 //
 // Rows pass: a block covers 256 consecutive elements of V vectors; each
 // thread sums one element of V vectors, so one coefficient load serves V
-// sums.  Elements run along the lanes, so the tap-major tables and the
-// samples (y and x are row-contiguous) are read coalesced.
+// sums.  Elements run along the lanes, so the tap-major tables are read
+// coalesced.  The block first copies the sample spans its elements read
+// (source samples [min first, max first + k), solution elements i0 - w ..
+// i0 + 255 + w, both clamped to the vector) of its V vectors into LDS, so
+// every sample leaves HBM once per block; a block whose source span does not
+// fit the LDS rows reads its samples from global memory instead.
 template <int V>
 __global__ void __launch_bounds__(256) residual_rows(RefineArgs a, float *r) {
-    const int n = a.axis.n, w = a.refine.w;
+    constexpr int kCapY = 640, kCapX = 256 + 2 * 15 + 2;
+    __shared__ float ys[V][kCapY];
+    __shared__ float xs[V][kCapX];
+    __shared__ int bounds[2][4];
+    const int n = a.axis.n, w = a.refine.w, k = a.refine.k, span = 2 * w + 1;
     const int iblocks = (n + 255) / 256;
-    const int i = static_cast<int>(blockIdx.x % iblocks) * 256 + static_cast<int>(threadIdx.x);
+    const int i0 = static_cast<int>(blockIdx.x % iblocks) * 256;
+    const int tid = static_cast<int>(threadIdx.x);
+    const int i = i0 + tid;
     const int v0 = static_cast<int>(blockIdx.x / iblocks) * V;
-    if (i >= n) return;
     const long f = blockIdx.y;
     const float *y[V];
     const float *x[V];
@@ -121,26 +130,66 @@ __global__ void __launch_bounds__(256) residual_rows(RefineArgs a, float *r) {
         y[u] = a.y + f * a.y_frame + v * a.y_vec;
         x[u] = a.x + f * a.x_frame + v * a.x_vec;
     }
-    const int j0 = a.refine.first[i], m1 = a.refine.m - 1;
+    const int m1 = a.refine.m - 1;
+    const int j0 = a.refine.first[min(i, n - 1)];
+    // The block's source span: min and max of first[] over its elements.
+    int lo = j0, hi = j0;
+    for (int off = 32; off >= 1; off >>= 1) {
+        lo = min(lo, __shfl_xor(lo, off));
+        hi = max(hi, __shfl_xor(hi, off));
+    }
+    if ((tid & 63) == 0) {
+        bounds[0][tid >> 6] = lo;
+        bounds[1][tid >> 6] = hi;
+    }
+    __syncthreads();
+    lo = min(min(bounds[0][0], bounds[0][1]), min(bounds[0][2], bounds[0][3]));
+    hi = max(max(bounds[1][0], bounds[1][1]), max(bounds[1][2], bounds[1][3])) + k;
+    const bool staged = hi - lo <= kCapY && 256 + 2 * w <= kCapX;
+    if (staged) {
+        const int ylen = hi - lo, xlen = 256 + 2 * w;
+#pragma unroll
+        for (int u = 0; u < V; ++u) {
+            for (int e = tid; e < ylen; e += 256) ys[u][e] = y[u][min(lo + e, m1)];
+            for (int e = tid; e < xlen; e += 256) xs[u][e] = x[u][min(max(i0 - w + e, 0), n - 1)];
+        }
+        __syncthreads();
+    }
+    if (i >= n) return;
     const double *t = a.refine.t64 + i;
     const double *g = a.refine.b64 + i;
     double s[V];
 #pragma unroll
     for (int u = 0; u < V; ++u) s[u] = 0.0;
+    if (staged) {
+        const int jb = j0 - lo;
 #pragma unroll 2
-    for (int q = 0; q < a.refine.k; ++q) {
-        const double c = t[static_cast<long>(q) * n];
-        const int j = min(j0 + q, m1);
+        for (int q = 0; q < k; ++q) {
+            const double c = t[static_cast<long>(q) * n];
 #pragma unroll
-        for (int u = 0; u < V; ++u) s[u] = __builtin_fma(c, static_cast<double>(y[u][j]), s[u]);
-    }
-    const int span = 2 * w + 1;
+            for (int u = 0; u < V; ++u) s[u] = __builtin_fma(c, static_cast<double>(ys[u][jb + q]), s[u]);
+        }
 #pragma unroll 2
-    for (int d = 0; d < span; ++d) {
-        const double c = -g[static_cast<long>(d) * n];
-        const int j = min(max(i - w + d, 0), n - 1);
+        for (int d = 0; d < span; ++d) {
+            const double c = -g[static_cast<long>(d) * n];
 #pragma unroll
-        for (int u = 0; u < V; ++u) s[u] = __builtin_fma(c, static_cast<double>(x[u][j]), s[u]);
+            for (int u = 0; u < V; ++u) s[u] = __builtin_fma(c, static_cast<double>(xs[u][tid + d]), s[u]);
+        }
+    } else {
+#pragma unroll 2
+        for (int q = 0; q < k; ++q) {
+            const double c = t[static_cast<long>(q) * n];
+            const int j = min(j0 + q, m1);
+#pragma unroll
+            for (int u = 0; u < V; ++u) s[u] = __builtin_fma(c, static_cast<double>(y[u][j]), s[u]);
+        }
+#pragma unroll 2
+        for (int d = 0; d < span; ++d) {
+            const double c = -g[static_cast<long>(d) * n];
+            const int j = min(max(i - w + d, 0), n - 1);
+#pragma unroll
+            for (int u = 0; u < V; ++u) s[u] = __builtin_fma(c, static_cast<double>(x[u][j]), s[u]);
+        }
     }
     const long per = static_cast<long>(n) * a.nvec;
 #pragma unroll
