@@ -159,9 +159,11 @@ int dsmvc_hip_executor_prepare(dsmvc_hip_executor *executor, const dsmvc_hip_pla
 int dsmvc_hip_executor_seal(dsmvc_hip_executor *executor);
 /* No counterpart in the reference (which solves such plans in double only,
  * src/axis_plan.cpp:1124-1248): in DSMVC_HIP_SOLVE_REFINED mode, plans that
- * retain FP64 data are solved in F32 and refined `steps` times (1..3) with
- * the residual A^T y - G x formed in compensated F32 (two-product, two-sum)
- * from the FP64 weights and normal matrix.  Float32 planes only. */
+ * retain FP64 data are solved in F32 and refined `steps` times (1..3): the
+ * residual A^T y - G x from the FP64 weights and normal matrix (in FP64 for
+ * plans the windowed kernels take, else in compensated F32 with two-product
+ * and two-sum), rounded to F32, the correction solved with the F32 factors
+ * and added.  Float32 planes only. */
 int dsmvc_hip_executor_set_refinement(dsmvc_hip_executor *executor, int32_t steps);
 /* No counterpart in the reference: the route the autotuner chose for a
  * plan, direction (rows != 0: horizontal) and sample kind -- a chunk count,
