@@ -550,7 +550,7 @@ def main():
         del gin
 
     if not args.no_gray:
-        # Config 3's cost per precision route: Delanczos taps=4 blur 0.9, GRAYS
+        # Cost per precision route: Delanczos taps=4 blur 0.9, GRAYS
         # F32 1920x1080 -> 1904x1071, 32 frames per call.  FP32 plain (plans
         # with f64mode 1), FP32 + one refinement step and the Float64 route
         # (f64mode 2 plans); their errors against the FP64 oracle are in
@@ -573,7 +573,7 @@ def main():
                 rex.inverse_2d(hp, vp, pin, out=pout, asynchronous=True)
             rex.synchronize()
             prec[name] = {"frames_per_s": round(32 * 5 / (time.perf_counter() - t0), 1)}
-        extra["config3_routes_lanczos4_1904x1071"] = prec
+        extra["precision_routes_lanczos4_1904x1071"] = prec
         del pin, pout, refined
 
     if not args.no_batch:

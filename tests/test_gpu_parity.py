@@ -19,7 +19,7 @@ import dsmvc_hip as dh  # noqa: E402
 from support import (BICUBIC, BILINEAR, F64_ALWAYS, F64_AUTO, F64_NEVER, LANCZOS, REFLECT101,
                      REPEAT, SPLINE16, SPLINE36, SPLINE64, SYMMETRIC, ZERO, make_request,
                      ulp_distance)  # noqa: E402
-from workloads import config_plan_requests, integer_conversion  # noqa: E402
+from workloads import workload_plan_requests, integer_conversion  # noqa: E402
 
 DEV = torch.device("cuda:0")
 
@@ -161,9 +161,9 @@ def test_long_vectors_auto_close(executors, oracle, name, req):
 
 
 def test_windowed_yuv420p16_within_one_code(executors, oracle):
-    """Config 2 geometry on the windowed kernels: <= 1 code vs the oracle's
+    """YUV420P16 Debicubic 1080p -> 720p geometry on the windowed kernels: <= 1 code vs the oracle's
     scalar integer route."""
-    reqs = config_plan_requests("config2")
+    reqs = workload_plan_requests("yuv420p16_debicubic_1080p_720p")
     rng = np.random.default_rng(22)
     for plane, (h, v, sw, sh) in enumerate([(reqs[0], reqs[1], 1920, 1080),
                                             (reqs[2], reqs[3], 960, 540)]):
@@ -193,11 +193,11 @@ def test_exact_2d_bit_identical(executors, oracle, kernel):
     assert _identical(got, expected)
 
 
-def test_config1_debilinear_1080p_against_fp64(executors, oracle):
-    """Config 1 geometry (Debilinear 1920x1080 -> 1280x720, 8 U[0,1] frames).
+def test_debilinear_1080p_720p_against_fp64(executors, oracle):
+    """Debilinear geometry (Debilinear 1920x1080 -> 1280x720, 8 U[0,1] frames).
     Bars: exact == ordered oracle bit for bit; auto within 2e-6 of the FP64
     oracle (forced-F64 plans solved in double)."""
-    reqs = config_plan_requests("config1")
+    reqs = workload_plan_requests("debilinear_1080p_720p")
     h, v = reqs
     rng = np.random.default_rng(1)
     frames = rng.uniform(0, 1, (8, 1080, 1920)).astype(np.float32)
@@ -218,10 +218,10 @@ def test_config1_debilinear_1080p_against_fp64(executors, oracle):
         assert err <= 2e-6
 
 
-def test_config2_yuv420p16_within_one_code(executors, oracle):
-    """Config 2 geometry: Debicubic(0, 0.5) YUV420P16 1920x1080 -> 1280x720,
+def test_yuv420p16_debicubic_within_one_code(executors, oracle):
+    """YUV420P16 geometry: Debicubic(0, 0.5) YUV420P16 1920x1080 -> 1280x720,
     limited range.  Bar: <= 1 code vs the oracle's scalar integer route."""
-    reqs = config_plan_requests("config2")
+    reqs = workload_plan_requests("yuv420p16_debicubic_1080p_720p")
     rng = np.random.default_rng(2)
     for plane, (h, v, sw, sh) in enumerate([(reqs[0], reqs[1], 1920, 1080),
                                             (reqs[2], reqs[3], 960, 540)]):
@@ -377,7 +377,7 @@ def test_autotuned_partition_stable_and_close(oracle):
     planes of the 1080p -> 720p Debicubic request, 64 frames).  Bars: the
     tuned executor repeats its result bit for bit; every partition the tuner
     can pick stays within 4e-6 * max(1, |x|) of the ordered oracle."""
-    reqs = config_plan_requests("config2")
+    reqs = workload_plan_requests("yuv420p16_debicubic_1080p_720p")
     h, v = reqs[2], reqs[3]
     hp, vp = dh.Plan(_hreq(h)), dh.Plan(_hreq(v))
     x = torch.rand(64, 540, 960, device=DEV)
@@ -406,7 +406,7 @@ def test_planes_call_matches_per_plane_calls(executors, oracle):
     and conversions) in one call equal the oracle within one code, and a
     float32 call mixing a windowed plane with a Float64-route plane equals
     the per-plane executor results (the Float64 plane bit for bit)."""
-    reqs = config_plan_requests("config2")
+    reqs = workload_plan_requests("yuv420p16_debicubic_1080p_720p")
     rng = np.random.default_rng(31)
     planes, jobs = [], []
     for index, (h, v, sw, sh) in enumerate([(reqs[0], reqs[1], 1920, 1080),

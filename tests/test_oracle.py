@@ -24,7 +24,7 @@ from support import (BICUBIC, BILINEAR, F64_ALWAYS, F64_AUTO, F64_NEVER, LANCZOS
                      complete_plan_hash, conditioned_lanczos2_request, f32_plan_hash,
                      make_normal_input, make_request, mixed_horizontal_request, output_hash,
                      ulp_distance)
-from workloads import config_plan_requests, integer_conversion
+from workloads import workload_plan_requests, integer_conversion
 
 
 @pytest.mark.parametrize("fx", axis_fixtures(), ids=lambda f: f.name)
@@ -220,10 +220,11 @@ def _plan_equal(a, b):
 
 
 @pytest.mark.reference
-@pytest.mark.parametrize("config", ["config1", "config2", "config3", "config4_sample",
-                                    "config5_sample"])
-def test_plans_match_reference_engine(oracle, reference, config):
-    for req in config_plan_requests(config):
+@pytest.mark.parametrize("workload", ["debilinear_1080p_720p", "yuv420p16_debicubic_1080p_720p",
+                                      "near_identity_blur09", "mitchell_21_heights",
+                                      "served_mix_sample"])
+def test_plans_match_reference_engine(oracle, reference, workload):
+    for req in workload_plan_requests(workload):
         _plan_equal(oracle.build(req).arrays(), reference.build(req).arrays())
 
 
@@ -281,7 +282,7 @@ def test_integer_2d_f64_matches_reference_engine(oracle, reference):
 def test_full_size_1080p_scalar_matches_reference_engine(oracle, reference):
     """Config-1/2 geometry at full size: scalar 2D and u16 integer, bit for bit."""
     rng = np.random.default_rng(1)
-    reqs = config_plan_requests("config2")
+    reqs = workload_plan_requests("yuv420p16_debicubic_1080p_720p")
     h, v = reqs[0], reqs[1]
     x = rng.uniform(0, 1, (1080, 1920)).astype(np.float32)
     o = oracle.inverse_2d(oracle.build(h), oracle.build(v), x)

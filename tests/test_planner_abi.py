@@ -13,7 +13,7 @@ import dsmvc_hip as dh
 from dsmvc_hip import _lib
 from support import (Request, axis_fixtures, complete_plan_hash, f32_plan_hash, make_request,
                      PlanArrays)
-from workloads import config_plan_requests, integer_conversion as ref_integer_conversion
+from workloads import workload_plan_requests, integer_conversion as ref_integer_conversion
 
 ROOT = Path(__file__).resolve().parents[1]
 
@@ -57,9 +57,11 @@ def _equal(a: PlanArrays, b: PlanArrays):
         assert x.shape == y.shape and np.array_equal(x.view(np.uint8), y.view(np.uint8)), name
 
 
-@pytest.mark.parametrize("config", ["config1", "config2", "config3", "config4", "config5_sample"])
-def test_planner_matches_oracle_at_workloads(oracle, config):
-    for req in config_plan_requests(config):
+@pytest.mark.parametrize("workload", ["debilinear_1080p_720p", "yuv420p16_debicubic_1080p_720p",
+                                      "near_identity_blur09", "mitchell_501_heights",
+                                      "served_mix_sample"])
+def test_planner_matches_oracle_at_workloads(oracle, workload):
+    for req in workload_plan_requests(workload):
         _equal(_product_plan(req), oracle.build(req).arrays())
 
 

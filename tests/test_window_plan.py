@@ -17,7 +17,7 @@ import pytest
 
 import dsmvc_hip as dh
 from support import BICUBIC, BILINEAR, LANCZOS, SPLINE36, SPLINE64, make_request
-from workloads import config_plan_requests
+from workloads import workload_plan_requests
 
 TOLERANCE = 2.0 ** -28
 
@@ -123,11 +123,12 @@ def test_windowed_solve_matches_sequential(name, req):
     assert float(np.abs(short - exact).max()) > 8 * TOLERANCE * scale
 
 
-@pytest.mark.parametrize("config", ["config1", "config2", "config4_sample", "config5_sample"])
-def test_config_plans_are_windowed(config):
+@pytest.mark.parametrize("workload", ["debilinear_1080p_720p", "yuv420p16_debicubic_1080p_720p",
+                                      "mitchell_21_heights", "served_mix_sample"])
+def test_workload_plans_are_windowed(workload):
     """Every plan of the throughput workloads (W <= 7, taps <= 16) is
     accepted, in both directions."""
-    for req in config_plan_requests(config):
+    for req in workload_plan_requests(workload):
         plan = _plan(req)
         info = plan.window_info()
         assert info["feasible"] == 1 and info["rows_feasible"] == 1, (req.source_size,
@@ -136,10 +137,10 @@ def test_config_plans_are_windowed(config):
 
 
 def test_near_identity_plans_report_their_decay():
-    """Config 3's near-identity plans (1920 -> 1904, blur 0.9, W = 7) decay
+    """The near-identity plans (1920 -> 1904, blur 0.9, W = 7) decay
     slowly: whatever the verdict, a feasible plan must meet the tolerance and
     an infeasible one must report a residual above it."""
-    for req in config_plan_requests("config3"):
+    for req in workload_plan_requests("near_identity_blur09"):
         info = _plan(req).window_info()
         if info["feasible"]:
             assert info["residual"] <= TOLERANCE

@@ -1,4 +1,4 @@
-"""Test-side description of the benchmark workloads (axis requests per config) and
+"""Test-side description of the benchmark workloads (axis requests per workload) and
 a restatement of the reference's integer conversion constants.
 
 Request derivation follows the reference plugin: luma requests use src_width =
@@ -42,25 +42,25 @@ def chroma420_requests(sw, sh, dw, dh, kernel, taps=3, b=0.0, c=0.5, blur=1.0,
             make_request(ch, dh >> 1, active_h, shift_v, **kw)]
 
 
-def config_plan_requests(config):
-    if config == "config1":
+def workload_plan_requests(workload):
+    if workload == "debilinear_1080p_720p":
         return luma_requests(1920, 1080, 1280, 720, BILINEAR)
-    if config == "config2":
+    if workload == "yuv420p16_debicubic_1080p_720p":
         return (luma_requests(1920, 1080, 1280, 720, BICUBIC, b=0.0, c=0.5)
                 + chroma420_requests(1920, 1080, 1280, 720, BICUBIC, b=0.0, c=0.5))
-    if config == "config3":
+    if workload == "near_identity_blur09":
         out = []
         for kernel, taps in ((LANCZOS, 4), (SPLINE64, 3)):
             for dw, dh in ((1904, 1071), (960, 540)):
                 out += luma_requests(1920, 1080, dw, dh, kernel, taps=taps, blur=0.9)
         return out
-    if config in ("config4", "config4_sample"):
-        heights = range(500, 1001) if config == "config4" else range(500, 1001, 25)
+    if workload in ("mitchell_501_heights", "mitchell_21_heights"):
+        heights = range(500, 1001) if workload == "mitchell_501_heights" else range(500, 1001, 25)
         out = []
         for h in heights:
             out += luma_requests(1920, 1080, width_for_height(h), h, BICUBIC, b=1 / 3, c=1 / 3)
         return out
-    if config in ("config5_sample",):
+    if workload in ("served_mix_sample",):
         out = []
         kernels = [(BILINEAR, 3, 0.0, 0.5), (BICUBIC, 3, 0.0, 0.5), (LANCZOS, 3, 0.0, 0.5),
                    (SPLINE36, 3, 0.0, 0.5)]
@@ -73,7 +73,7 @@ def config_plan_requests(config):
             out += chroma420_requests(sw, sh, dw, dh, kernel, taps=taps, b=b, c=c,
                                       f64_mode=mode)
         return out
-    raise KeyError(config)
+    raise KeyError(workload)
 
 
 def integer_conversion(bits, chroma, full_range):
