@@ -92,3 +92,46 @@ def test_refinement_against_fp64_oracle(executors, oracle, name, kernel, taps, d
     assert err["fp32+refine1"] <= 2e-6
     assert err["fp32+refine1"] <= err["fp32"] + ulp
     assert err["fp32+refine2"] <= err["fp32+refine1"] + ulp
+
+
+def test_refined_frame_batch_matches_single_frames(executors, oracle):
+    """Refined 2D solves over a batch of pitched frames equal the same solves
+    frame by frame, bit for bit, and stay within 2e-6 of the FP64 oracle.
+    Delanczos taps=4 blur 0.9, 640x360 -> 600x338 (windowed, band 7)."""
+    hp, vp = _plan(640, 600, LANCZOS, 4, F64_ALWAYS), _plan(360, 338, LANCZOS, 4, F64_ALWAYS)
+    assert hp.window_info()["feasible"] and vp.window_info()["feasible"]
+    frames = [frame_with_sinusoids(11 + f, 640, 360) for f in range(5)]
+    x = torch.zeros((5, 360, 672), dtype=torch.float32, device=DEV)
+    x[:, :, :640] = torch.from_numpy(np.stack(frames)).to(DEV)
+    refined = executors["refined1"]
+    batch = refined.inverse_2d(hp, vp, x)
+    oh = oracle.build(make_request(640, 600, kernel=LANCZOS, taps=4, blur=0.9, f64_mode=F64_ALWAYS))
+    ov = oracle.build(make_request(360, 338, kernel=LANCZOS, taps=4, blur=0.9, f64_mode=F64_ALWAYS))
+    for f in range(5):
+        single = refined.inverse_2d(hp, vp, x[f])
+        assert torch.equal(single, batch[f]), f"frame {f}"
+    for f in (0, 4):
+        exact = oracle.inverse_2d_f64_exact(oh, ov, frames[f].astype(np.float64))
+        err = float(np.abs(batch[f].cpu().numpy().astype(np.float64) - exact).max())
+        print(f"frame {f}: refined max-abs {err:.3g}", flush=True)
+        assert err <= 2e-6
+
+
+@pytest.mark.parametrize("src,dst,vectors", [(640, 600, 300), (360, 338, 200), (20, 18, 3),
+                                             (24, 22, 5)])
+def test_refined_single_axis_against_fp64(executors, oracle, src, dst, vectors):
+    """Refined rows and columns calls (tiny planes and vector counts included:
+    fewer elements than one residual block, fewer vectors than a thread's
+    share) against the FP64 oracle's Float64 axis solve, <= 2e-6."""
+    plan = _plan(src, dst, LANCZOS, 4, F64_ALWAYS)
+    assert plan.window_info()["feasible"]
+    op = oracle.build(make_request(src, dst, kernel=LANCZOS, taps=4, blur=0.9, f64_mode=F64_ALWAYS))
+    rng = np.random.default_rng(src + vectors)
+    y = rng.uniform(0.0, 1.0, (vectors, src)).astype(np.float32)
+    refined = executors["refined1"]
+    rows = refined.inverse_rows(plan, torch.from_numpy(y).to(DEV)).cpu().numpy()
+    want = np.stack([oracle.axis_f64(op, v) for v in y])
+    assert float(np.abs(rows.astype(np.float64) - want).max()) <= 2e-6
+    yc = np.ascontiguousarray(y.T)
+    cols = refined.inverse_columns(plan, torch.from_numpy(yc).to(DEV)).cpu().numpy()
+    assert float(np.abs(cols.astype(np.float64) - want.T).max()) <= 2e-6
