@@ -304,23 +304,20 @@ __global__ void __launch_bounds__(256) residual_columns(RefineArgs a, float *r) 
         if (i0 + u < n) r[f * plane + static_cast<long>(i0 + u) * a.nvec + v] = static_cast<float>(s[u]);
 }
 
-// x += d, d laid out as the residual kernels lay out r.
-__global__ void __launch_bounds__(256) add_correction(RefineArgs a, const float *d, int rows) {
-    const long per = static_cast<long>(a.axis.n) * a.nvec;
-    const long idx = blockIdx.x * 256L + threadIdx.x;
+// x += d, d laid out as the residual kernels lay out r.  One thread per
+// element, threads along the contiguous dimension; strides are fixed per
+// pass and a plane is indexed in 32 bits.
+template <bool Rows>
+__global__ void __launch_bounds__(256) add_correction(RefineArgs a, const float *d) {
+    const unsigned per = static_cast<unsigned>(a.axis.n) * static_cast<unsigned>(a.nvec);
+    const unsigned idx = blockIdx.x * 256U + threadIdx.x;
     if (idx >= per) return;
     const long f = blockIdx.y;
-    const int n = a.axis.n;
-    int i, v;
-    if (rows) {
-        v = static_cast<int>(idx / n);
-        i = static_cast<int>(idx - static_cast<long>(v) * n);
-    } else {
-        i = static_cast<int>(idx / a.nvec);
-        v = static_cast<int>(idx - static_cast<long>(i) * a.nvec);
-    }
-    float *x = a.x + f * a.x_frame + v * a.x_vec + static_cast<long>(i) * a.x_el;
-    *x = __fadd_rn(*x, d[f * per + idx]);
+    const auto un = static_cast<unsigned>(a.axis.n), uv = static_cast<unsigned>(a.nvec);
+    const long v = Rows ? idx / un : idx % uv;
+    const long i = Rows ? idx % un : idx / uv;
+    float *x = a.x + f * a.x_frame + (Rows ? v * a.x_vec + i : i * a.x_el + v);
+    *x = __fadd_rn(*x, d[f * static_cast<long>(per) + idx]);
 }
 
 } // namespace
@@ -342,8 +339,10 @@ hipError_t launch_refine_residual(const RefineArgs &a, float *r, bool rows, hipS
 hipError_t launch_refine_add(const RefineArgs &a, const float *d, bool rows, hipStream_t s) {
     const long per = static_cast<long>(a.axis.n) * a.nvec;
     if (per <= 0 || a.frames <= 0) return hipSuccess;
+    if (per >= (1L << 31)) return hipErrorInvalidValue;   // the kernel indexes a plane in 32 bits
     const dim3 grid(static_cast<unsigned>((per + 255) / 256), a.frames);
-    dev::add_correction<<<grid, 256, 0, s>>>(a, d, rows ? 1 : 0);
+    if (rows) dev::add_correction<true><<<grid, 256, 0, s>>>(a, d);
+    else dev::add_correction<false><<<grid, 256, 0, s>>>(a, d);
     return hipGetLastError();
 }
 
