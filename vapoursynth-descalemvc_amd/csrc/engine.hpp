@@ -109,6 +109,8 @@ struct PlannerCacheStats {
 // with the reference's messages.
 [[nodiscard]] AxisPlan build_axis_plan(const AxisRequest &request);
 [[nodiscard]] std::shared_ptr<const AxisPlan> get_or_build_axis_plan(const AxisRequest &request);
+// The cached plan for `request` when it is built, else nullptr; never builds.
+[[nodiscard]] std::shared_ptr<const AxisPlan> find_axis_plan(const AxisRequest &request);
 [[nodiscard]] PlannerCacheStats planner_cache_stats();
 void clear_planner_caches();
 

@@ -624,6 +624,22 @@ public:
         }
     }
 
+    // A built entry (bytes set) counts as a hit, as in get(); an absent or
+    // still-building one returns nullptr without registering anything.
+    std::shared_ptr<const AxisPlan> find(const AxisRequest &r) {
+        const CacheKey key = key_of(r);
+        std::shared_future<std::shared_ptr<const AxisPlan>> ready;
+        {
+            std::lock_guard lock(mutex_);
+            auto it = map_.find(key);
+            if (it == map_.end() || it->second.bytes == 0) return nullptr;
+            ++hits_;
+            lru_.splice(lru_.begin(), lru_, it->second.where);
+            ready = it->second.value;
+        }
+        return ready.get();
+    }
+
     PlannerCacheStats stats() {
         std::lock_guard lock(mutex_);
         return {hits_, builds_, map_.size(), bytes_};
@@ -675,6 +691,8 @@ PlanCache &plan_cache() {
 std::shared_ptr<const AxisPlan> get_or_build_axis_plan(const AxisRequest &r) {
     return plan_cache().get(r);
 }
+
+std::shared_ptr<const AxisPlan> find_axis_plan(const AxisRequest &r) { return plan_cache().find(r); }
 
 PlannerCacheStats planner_cache_stats() { return plan_cache().stats(); }
 

@@ -2,6 +2,8 @@
 #include "server.hpp"
 
 #include <algorithm>
+#include <array>
+#include <future>
 #include <stdexcept>
 
 #include "batch.hpp"
@@ -112,49 +114,101 @@ std::uint64_t HipServer::submit(const ServeRequest &r) {
         q.destination_size = destination;
         q.active_length = active;
         q.shift = shift;
-        return get_or_build_axis_plan(q);
+        return q;
     };
-    auto entry = std::make_shared<Entry>();
-    entry->submitted = t0;
-    const bool parallel = executor_.mode() == SolveMode::automatic
-                          || executor_.mode() == SolveMode::windowed;
-    std::shared_ptr<const AxisPlan> luma_h, luma_v, chroma_h, chroma_v;
-    for (int p = 0; p < r.planes; ++p) {
+    auto plane_size = [&](int p, int &sw, int &sh, int &dw, int &dh) {
         const bool chroma = p > 0;
-        const int sw = chroma ? r.src_width >> ssw : r.src_width;
-        const int sh = chroma ? r.src_height >> ssh : r.src_height;
-        const int dw = chroma ? r.dst_width >> ssw : r.dst_width;
-        const int dh = chroma ? r.dst_height >> ssh : r.dst_height;
+        sw = chroma ? r.src_width >> ssw : r.src_width;
+        sh = chroma ? r.src_height >> ssh : r.src_height;
+        dw = chroma ? r.dst_width >> ssw : r.dst_width;
+        dh = chroma ? r.dst_height >> ssh : r.dst_height;
+    };
+    for (int p = 0; p < r.planes; ++p) {
+        int sw, sh, dw, dh;
+        plane_size(p, sw, sh, dw, dh);
         require(sw > 0 && sh > 0 && r.src[p] && r.dst[p] && r.src_stride[p] >= sw
                     && r.dst_stride[p] >= dw,
                 "dsmvc: invalid request plane");
-        if (!chroma) {
-            if (!luma_h) {
-                luma_h = axis(sw, dw, dw, 0.0);
-                luma_v = axis(sh, dh, dh, 0.0);
-            }
-        } else if (!chroma_h) {
-            const double cw = sw, fw = r.src_width;
-            chroma_h = ssw ? axis(sw, dw, double(r.dst_width) * cw / fw,
-                                  0.25 - 0.25 * double(r.dst_width) / fw)
-                           : axis(sw, dw, dw, 0.0);
-            chroma_v = ssh ? axis(sh, dh, double(r.dst_height) * double(sh) / double(r.src_height), 0.0)
-                           : axis(sh, dh, dh, 0.0);
+    }
+    // Axis requests: luma horizontal, luma vertical, then chroma.
+    std::array<AxisRequest, 4> wanted;
+    int kinds = 2;
+    wanted[0] = axis(r.src_width, r.dst_width, r.dst_width, 0.0);
+    wanted[1] = axis(r.src_height, r.dst_height, r.dst_height, 0.0);
+    if (r.planes == 3) {
+        int sw, sh, dw, dh;
+        plane_size(1, sw, sh, dw, dh);
+        const double cw = sw, fw = r.src_width;
+        wanted[2] = ssw ? axis(sw, dw, double(r.dst_width) * cw / fw, 0.25 - 0.25 * double(r.dst_width) / fw)
+                        : axis(sw, dw, dw, 0.0);
+        wanted[3] = ssh ? axis(sh, dh, double(r.dst_height) * double(sh) / double(r.src_height), 0.0)
+                        : axis(sh, dh, dh, 0.0);
+        kinds = 4;
+    }
+    const bool parallel = executor_.mode() == SolveMode::automatic
+                          || executor_.mode() == SolveMode::windowed;
+    // Device image plus the windowed tables the round (or, for a Float64
+    // pair, the executor) will use.
+    auto prepare = [&](const AxisPlan &plan, bool rows, bool f64_pair) {
+        hip::DevicePlan &image = executor_.plan_image(plan);
+        if (!parallel) return;
+        if (f64_pair) (void)image.window64();
+        else (void)image.window(rows);
+    };
+    // Cached plans are used as they are.  A request with several new plans
+    // (a new geometry) builds them, and then their device tables, on
+    // concurrent tasks: for a 4K source that is several milliseconds of FP64
+    // host work per plan, which the request's latency includes.
+    std::array<std::shared_ptr<const AxisPlan>, 4> plans;
+    int missing = 0;
+    for (int k = 0; k < kinds; ++k) {
+        plans[static_cast<std::size_t>(k)] = find_axis_plan(wanted[static_cast<std::size_t>(k)]);
+        missing += plans[static_cast<std::size_t>(k)] ? 0 : 1;
+    }
+    if (missing > 1) {
+        std::array<std::future<std::shared_ptr<const AxisPlan>>, 4> built;
+        for (int k = 0; k < kinds; ++k)
+            if (!plans[static_cast<std::size_t>(k)])
+                built[static_cast<std::size_t>(k)] = std::async(
+                    std::launch::async, [q = wanted[static_cast<std::size_t>(k)]] { return get_or_build_axis_plan(q); });
+        for (int k = 0; k < kinds; ++k)
+            if (!plans[static_cast<std::size_t>(k)]) plans[static_cast<std::size_t>(k)] = built[static_cast<std::size_t>(k)].get();
+        std::array<std::future<void>, 4> images;
+        for (int k = 0; k < kinds; ++k) {
+            const AxisPlan &plan = *plans[static_cast<std::size_t>(k)];
+            const AxisPlan &pair = *plans[static_cast<std::size_t>(k ^ 1)];
+            const bool f64_pair = plan.requires_float64() || pair.requires_float64();
+            images[static_cast<std::size_t>(k)] = std::async(
+                std::launch::async, [&prepare, &plan, k, f64_pair] { prepare(plan, (k & 1) == 0, f64_pair); });
         }
+        for (int k = 0; k < kinds; ++k) images[static_cast<std::size_t>(k)].get();
+    } else {
+        for (int k = 0; k < kinds; ++k)
+            if (!plans[static_cast<std::size_t>(k)])
+                plans[static_cast<std::size_t>(k)] = get_or_build_axis_plan(wanted[static_cast<std::size_t>(k)]);
+    }
+
+    auto entry = std::make_shared<Entry>();
+    entry->submitted = t0;
+    for (int p = 0; p < r.planes; ++p) {
+        int sw, sh, dw, dh;
+        plane_size(p, sw, sh, dw, dh);
+        const std::size_t kind = p > 0 ? 2 : 0;
         Plane plane;
-        plane.h = chroma ? chroma_h : luma_h;
-        plane.v = chroma ? chroma_v : luma_v;
+        plane.h = plans[kind];
+        plane.v = plans[kind + 1];
         plane.src = r.src[p];
         plane.src_stride = r.src_stride[p];
         plane.dst = r.dst[p];
         plane.dst_stride = r.dst_stride[p];
         // Device images are uploaded here, on the caller's thread, so the
         // loop only builds tables and launches.
-        hip::DevicePlan &ih = executor_.plan_image(*plane.h);
-        hip::DevicePlan &iv = executor_.plan_image(*plane.v);
-        if (parallel && !plane.h->requires_float64() && !plane.v->requires_float64()) {
-            plane.wh = ih.window(true);
-            plane.wv = iv.window(false);
+        const bool f64_pair = plane.h->requires_float64() || plane.v->requires_float64();
+        prepare(*plane.h, true, f64_pair);
+        prepare(*plane.v, false, f64_pair);
+        if (parallel && !f64_pair) {
+            plane.wh = executor_.plan_image(*plane.h).window(true);
+            plane.wv = executor_.plan_image(*plane.v).window(false);
         }
         plane.table = plane.wh && plane.wv;
         if (plane.table)
