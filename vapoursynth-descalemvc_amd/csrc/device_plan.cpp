@@ -2,6 +2,7 @@
 #include "device_plan.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -319,25 +320,37 @@ bool build_window(const AxisPlan &p, int length, int warm, Coef coef_of, Weight 
     auto coef = [&](int r, int slot) {
         return static_cast<double>(out.coef[static_cast<std::size_t>(r) * 16 + static_cast<std::size_t>(slot)]);
     };
-    std::vector<double> ring(static_cast<std::size_t>(band));
+    // The last `band` states in a circular buffer: back(d) is the state d
+    // rows back (d = 1..band), push() makes v the newest.
+    std::array<double, kMaxBand> ring{};
+    int head = 0;
+    auto back = [&](int d) {
+        int at = head + d - 1;
+        if (at >= band) at -= band;
+        return ring[static_cast<std::size_t>(at)];
+    };
     auto push = [&](double v) {
-        for (int d = band - 1; d > 0; --d) ring[static_cast<std::size_t>(d)] = ring[static_cast<std::size_t>(d - 1)];
-        ring[0] = v;
+        head = head == 0 ? band - 1 : head - 1;
+        ring[static_cast<std::size_t>(head)] = v;
+    };
+    auto reset = [&](int c) {
+        ring.fill(0.0);
+        head = 0;
+        ring[static_cast<std::size_t>(c)] = 1.0;
     };
     auto peak = [&] {
         double m = 0.0;
-        for (double v : ring) m = std::max(m, std::fabs(v));
+        for (int k = 0; k < band; ++k) m = std::max(m, std::fabs(ring[static_cast<std::size_t>(k)]));
         return m;
     };
     double worst = 0.0;
     for (int q = 1; q < chunks; ++q) {
         const int a = q * length + warm;      // padded row of the chunk's first row
         for (int c = 0; c < band; ++c) {
-            std::fill(ring.begin(), ring.end(), 0.0);
-            ring[static_cast<std::size_t>(c)] = 1.0;
+            reset(c);
             for (int r = a - warm; r < a + length; ++r) {
                 double v = 0.0;
-                for (int d = band; d >= 1; --d) v -= coef(r, d - 1) * ring[static_cast<std::size_t>(d - 1)];
+                for (int d = band; d >= 1; --d) v -= coef(r, d - 1) * back(d);
                 push(zform ? v : v * coef(r, kMaxBand));
                 if (r >= a - 1) worst = std::max(worst, peak());
             }
@@ -346,11 +359,10 @@ bool build_window(const AxisPlan &p, int length, int warm, Coef coef_of, Weight 
     for (int q = 0; q + 1 < chunks; ++q) {
         const int b = (q + 1) * length + warm;   // padded row just past the chunk
         for (int c = 0; c < band; ++c) {
-            std::fill(ring.begin(), ring.end(), 0.0);
-            ring[static_cast<std::size_t>(c)] = 1.0;
+            reset(c);
             for (int r = b + warm - 1; r >= b - length; --r) {
                 double v = 0.0;
-                for (int d = band; d >= 1; --d) v -= coef(r, kMaxBand + d) * ring[static_cast<std::size_t>(d - 1)];
+                for (int d = band; d >= 1; --d) v -= coef(r, kMaxBand + d) * back(d);
                 push(v);
                 if (r <= b) worst = std::max(worst, peak());
             }
