@@ -112,6 +112,15 @@ const char *dsmvc_hip_last_error(void);
 /* Replaces dsmvc::get_or_build_axis_plan (include/dsmvc/engine.hpp:177-178,
  * src/axis_plan.cpp:1367-1383): single-flight LRU keyed by the request's bits. */
 int dsmvc_hip_plan_build(const dsmvc_hip_axis_request *request, dsmvc_hip_plan **plan);
+/* No single counterpart in the reference, which plans one request at a time
+ * (src/axis_plan.cpp:1367-1383); for scans over many geometries.  Builds
+ * `count` plans through the same cache on up to `threads` host threads
+ * (<= 0: one per hardware thread, at most 16).  Every request is validated
+ * before any is built.  On failure no handle is returned (all of `plans` are
+ * NULL) and the error is that of the lowest failing index.  Each plan equals
+ * what dsmvc_hip_plan_build returns for its request. */
+int dsmvc_hip_plan_build_many(const dsmvc_hip_axis_request *requests, int32_t count,
+                              int32_t threads, dsmvc_hip_plan **plans);
 void dsmvc_hip_plan_release(dsmvc_hip_plan *plan);
 /* AxisPlan scalar fields. */
 int dsmvc_hip_plan_info_get(const dsmvc_hip_plan *plan, dsmvc_hip_plan_info *info);

@@ -57,6 +57,55 @@ def _equal(a: PlanArrays, b: PlanArrays):
         assert x.shape == y.shape and np.array_equal(x.view(np.uint8), y.view(np.uint8)), name
 
 
+def _arrays_of(p: "dh.Plan") -> PlanArrays:
+    a = p.arrays()
+    return PlanArrays(p.source_size, p.destination_size, p.support, p.half_bandwidth,
+                      p.active_length, p.shift, p.normal_rcond, p.normal_inf_norm,
+                      p.requires_float64, a["offsets"], a["indices"], a["weights"],
+                      a["lower_ld"], a["upper_l"], a["inverse_diagonal"], a["weights_f64"],
+                      a["normal_bands_f64"], a["ldlt_bands_f64"], a["inverse_diagonal_f64"])
+
+
+def _to_request(req: Request) -> "dh.AxisRequest":
+    return dh.AxisRequest(req.source_size, req.destination_size, req.active_length, req.shift,
+                          req.kernel, req.taps, req.b, req.c, req.blur, req.border, req.f64_mode)
+
+
+def test_build_plans_on_threads_match_the_oracle(oracle):
+    """dsmvc_hip_plan_build_many on 8 threads, from an empty cache: every plan
+    equals the oracle's bit for bit, duplicates share one build, and the
+    result is the same as one-at-a-time builds."""
+    reqs = (workload_plan_requests("mitchell_21_heights")
+            + workload_plan_requests("served_mix_sample")
+            + workload_plan_requests("near_identity_blur09"))
+    reqs = reqs + reqs[:6]                      # duplicates within one call
+    dh.clear_planner_caches()
+    plans = dh.build_plans([_to_request(r) for r in reqs], threads=8)
+    stats = dh.planner_cache_stats()
+    unique = len({tuple(getattr(r, f) for f, _ in r._fields_) for r in reqs})
+    assert stats["plan_builds"] == unique
+    for req, plan in zip(reqs, plans):
+        _equal(_arrays_of(plan), oracle.build(req).arrays())
+    dh.clear_planner_caches()
+    for req, plan in zip(reqs[:8], plans[:8]):
+        _equal(_arrays_of(plan), _product_plan(req))
+
+
+def test_build_plans_validates_before_building():
+    good = _to_request(make_request(1920, 1280, kernel=dh.BICUBIC))
+    bad = dh.AxisRequest(1920, 1280, 1280.0, 0.0, 9)             # no such kernel
+    dh.clear_planner_caches()
+    with pytest.raises(ValueError, match="kernel is invalid"):
+        dh.build_plans([good, bad, good], threads=2)
+    assert dh.planner_cache_stats()["plan_builds"] == 0
+    # A request that passes validation but cannot be planned: the error of the
+    # lowest failing index, and no handles.
+    empty = dh.AxisRequest(0, 10, 10.0, 0.0, dh.BICUBIC)
+    with pytest.raises((ValueError, dh.DsmvcError)):
+        dh.build_plans([good, empty], threads=2)
+    assert dh.build_plans([], threads=2) == []
+
+
 @pytest.mark.parametrize("workload", ["debilinear_1080p_720p", "yuv420p16_debicubic_1080p_720p",
                                       "near_identity_blur09", "mitchell_501_heights",
                                       "served_mix_sample"])

@@ -3,12 +3,16 @@
 // exceptions -> 2, missing device -> 3; the message is kept per thread.
 #include "dsmvc_hip.h"
 
+#include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <exception>
 #include <memory>
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "engine.hpp"
 #include "batch.hpp"
@@ -123,6 +127,45 @@ int dsmvc_hip_plan_build(const dsmvc_hip_axis_request *request, dsmvc_hip_plan *
         auto handle = std::make_unique<dsmvc_hip_plan>();
         handle->plan = dsmvc::get_or_build_axis_plan(to_request(*request));
         *plan = handle.release();
+    });
+}
+
+int dsmvc_hip_plan_build_many(const dsmvc_hip_axis_request *requests, int32_t count,
+                              int32_t threads, dsmvc_hip_plan **plans) {
+    return guarded([&] {
+        require(count >= 0 && (count == 0 || (requests && plans)), "dsmvc: null argument");
+        const auto n = static_cast<std::size_t>(count);
+        for (std::size_t k = 0; k < n; ++k) plans[k] = nullptr;
+        std::vector<dsmvc::AxisRequest> wanted;
+        wanted.reserve(n);
+        for (std::size_t k = 0; k < n; ++k) wanted.push_back(to_request(requests[k]));
+        std::vector<std::shared_ptr<const dsmvc::AxisPlan>> built(n);
+        std::vector<std::exception_ptr> failed(n);
+        std::atomic<std::size_t> next{0};
+        auto work = [&] {
+            for (std::size_t k = next++; k < n; k = next++) {
+                try {
+                    built[k] = dsmvc::get_or_build_axis_plan(wanted[k]);
+                } catch (...) {
+                    failed[k] = std::current_exception();
+                }
+            }
+        };
+        const unsigned hardware = std::max(1U, std::thread::hardware_concurrency());
+        const std::size_t workers = std::min<std::size_t>(
+            n, threads > 0 ? static_cast<std::size_t>(threads) : std::min(hardware, 16U));
+        std::vector<std::thread> pool;
+        for (std::size_t t = 1; t < workers; ++t) pool.emplace_back(work);
+        work();
+        for (auto &t : pool) t.join();
+        for (std::size_t k = 0; k < n; ++k)
+            if (failed[k]) std::rethrow_exception(failed[k]);
+        std::vector<std::unique_ptr<dsmvc_hip_plan>> handles(n);
+        for (std::size_t k = 0; k < n; ++k) {
+            handles[k] = std::make_unique<dsmvc_hip_plan>();
+            handles[k]->plan = std::move(built[k]);
+        }
+        for (std::size_t k = 0; k < n; ++k) plans[k] = handles[k].release();
     });
 }
 

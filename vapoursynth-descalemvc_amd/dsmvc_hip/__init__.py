@@ -24,7 +24,7 @@ from . import _lib
 from ._lib import ASYNC, AxisRequestC, ConversionC, DsmvcError, PlanInfoC, check, load
 
 __all__ = [
-    "AxisRequest", "Plan", "Executor", "Clip", "IntegerConversion", "build_plan",
+    "AxisRequest", "Plan", "Executor", "Clip", "IntegerConversion", "build_plan", "build_plans",
     "integer_conversion", "Descale", "Debilinear", "Debicubic", "Delanczos", "Despline16",
     "Despline36", "Despline64", "parse_backend", "resolve_backend", "device_count",
     "padding_index", "planner_cache_stats", "clear_planner_caches", "DsmvcError",
@@ -96,10 +96,13 @@ def integer_conversion(bits: int, chroma: bool, full_range: bool) -> IntegerConv
 class Plan:
     """Immutable axis plan built by the library's FP64 planner (cached)."""
 
-    def __init__(self, request: AxisRequest):
+    def __init__(self, request: AxisRequest, _handle: Optional[ctypes.c_void_p] = None):
         lib = load()
-        handle = ctypes.c_void_p()
-        check(lib.dsmvc_hip_plan_build(ctypes.byref(request.to_c()), ctypes.byref(handle)))
+        if _handle is None:
+            handle = ctypes.c_void_p()
+            check(lib.dsmvc_hip_plan_build(ctypes.byref(request.to_c()), ctypes.byref(handle)))
+        else:
+            handle = _handle   # already built (build_plans), owned from here on
         self._handle = handle
         self.request = request
         info = PlanInfoC()
@@ -151,6 +154,20 @@ class Plan:
 
 def build_plan(request: AxisRequest) -> Plan:
     return Plan(request)
+
+
+def build_plans(requests: Sequence[AxisRequest], threads: int = 0) -> List[Plan]:
+    """Many plans through the planner cache on host threads
+    (dsmvc_hip_plan_build_many; threads <= 0: one per hardware thread, at most
+    16).  Each plan equals Plan(request); a failure raises the error of the
+    lowest failing request and returns nothing."""
+    reqs = list(requests)
+    if not reqs:
+        return []
+    array = (AxisRequestC * len(reqs))(*[r.to_c() for r in reqs])
+    handles = (ctypes.c_void_p * len(reqs))()
+    check(load().dsmvc_hip_plan_build_many(array, len(reqs), int(threads), handles))
+    return [Plan(r, _handle=ctypes.c_void_p(h)) for r, h in zip(reqs, handles)]
 
 
 def planner_cache_stats() -> dict:

@@ -15,7 +15,8 @@ ASYNC = 1
 
 # Every symbol include/dsmvc_hip.h declares.
 EXPORTS = (
-    "dsmvc_hip_last_error", "dsmvc_hip_plan_build", "dsmvc_hip_plan_release",
+    "dsmvc_hip_last_error", "dsmvc_hip_plan_build", "dsmvc_hip_plan_build_many",
+    "dsmvc_hip_plan_release",
     "dsmvc_hip_plan_info_get", "dsmvc_hip_plan_arrays", "dsmvc_hip_plan_window_info",
     "dsmvc_hip_planner_cache_stats",
     "dsmvc_hip_clear_planner_caches", "dsmvc_hip_padding_index", "dsmvc_hip_parse_backend",
@@ -208,6 +209,8 @@ def load():
     vp, pd, i32, u32 = ctypes.c_void_p, ctypes.c_ssize_t, ctypes.c_int32, ctypes.c_uint32
     lib.dsmvc_hip_last_error.restype = ctypes.c_char_p
     lib.dsmvc_hip_plan_build.argtypes = [ctypes.POINTER(AxisRequestC), ctypes.POINTER(vp)]
+    lib.dsmvc_hip_plan_build_many.argtypes = [ctypes.POINTER(AxisRequestC), i32, i32,
+                                              ctypes.POINTER(vp)]
     lib.dsmvc_hip_plan_release.argtypes = [vp]
     lib.dsmvc_hip_plan_release.restype = None
     lib.dsmvc_hip_plan_info_get.argtypes = [vp, ctypes.POINTER(PlanInfoC)]
