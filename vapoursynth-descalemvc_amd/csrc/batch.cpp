@@ -2,9 +2,12 @@
 #include "batch.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
+#include <exception>
 #include <stdexcept>
 #include <string>
+#include <thread>
 
 #include "device_plan.hpp"
 #include "executor.hpp"
@@ -161,14 +164,38 @@ HipBatch::HipBatch(HipExecutor &executor, std::vector<BatchJob> jobs, bool graph
     // Intermediate layout of the table jobs: [frame][source row][pitch(n_h)].
     std::vector<std::size_t> offset(jobs_.size(), 0);
     std::vector<const hip::DevWindow *> wh(jobs_.size(), nullptr), wv(jobs_.size(), nullptr);
+    if (parallel) {
+        // Device images and window tables of every job, built on host
+        // threads (a scan brings hundreds of new plans).  Each task writes
+        // only its own job's slots; the executor and each device plan lock
+        // their own state.
+        std::vector<std::exception_ptr> failed(jobs_.size());
+        std::atomic<std::size_t> next{0};
+        auto work = [&] {
+            for (std::size_t k = next++; k < jobs_.size(); k = next++) {
+                const AxisPlan &h = *jobs_[k].horizontal, &v = *jobs_[k].vertical;
+                if (h.requires_float64() || v.requires_float64()) continue;
+                try {
+                    wh[k] = executor_.plan_image(h).window(true);
+                    wv[k] = executor_.plan_image(v).window(false);
+                } catch (...) {
+                    failed[k] = std::current_exception();
+                }
+            }
+        };
+        const std::size_t workers = std::min<std::size_t>(
+            jobs_.size(), std::min(16U, std::max(1U, std::thread::hardware_concurrency())));
+        std::vector<std::thread> pool;
+        for (std::size_t t = 1; t < workers; ++t) pool.emplace_back(work);
+        work();
+        for (auto &t : pool) t.join();
+        for (const auto &e : failed)
+            if (e) std::rethrow_exception(e);
+    }
     std::size_t total = 0;
     for (std::size_t k = 0; k < jobs_.size(); ++k) {
         const BatchJob &j = jobs_[k];
         const AxisPlan &h = *j.horizontal, &v = *j.vertical;
-        if (parallel && !h.requires_float64() && !v.requires_float64()) {
-            wh[k] = executor_.plan_image(h).window(true);
-            wv[k] = executor_.plan_image(v).window(false);
-        }
         if (!wh[k] || !wv[k]) {
             if (executor_.mode() == SolveMode::windowed)
                 throw std::invalid_argument("dsmvc: batch job cannot use the windowed HIP solve");
