@@ -7,6 +7,7 @@
 #include <exception>
 #include <stdexcept>
 #include <string>
+#include <system_error>
 #include <thread>
 
 #include "device_plan.hpp"
@@ -186,7 +187,15 @@ HipBatch::HipBatch(HipExecutor &executor, std::vector<BatchJob> jobs, bool graph
         const std::size_t workers = std::min<std::size_t>(
             jobs_.size(), std::min(16U, std::max(1U, std::thread::hardware_concurrency())));
         std::vector<std::thread> pool;
-        for (std::size_t t = 1; t < workers; ++t) pool.emplace_back(work);
+        // Fewer threads than asked for (thread creation failed) only means
+        // less parallelism: the calling thread takes whatever is left.
+        for (std::size_t t = 1; t < workers; ++t) {
+            try {
+                pool.emplace_back(work);
+            } catch (const std::system_error &) {
+                break;
+            }
+        }
         work();
         for (auto &t : pool) t.join();
         for (const auto &e : failed)

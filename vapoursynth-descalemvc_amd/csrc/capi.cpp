@@ -11,6 +11,7 @@
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -155,7 +156,15 @@ int dsmvc_hip_plan_build_many(const dsmvc_hip_axis_request *requests, int32_t co
         const std::size_t workers = std::min<std::size_t>(
             n, threads > 0 ? static_cast<std::size_t>(threads) : std::min(hardware, 16U));
         std::vector<std::thread> pool;
-        for (std::size_t t = 1; t < workers; ++t) pool.emplace_back(work);
+        // Fewer threads than asked for (thread creation failed) only means
+        // less parallelism: the calling thread takes whatever is left.
+        for (std::size_t t = 1; t < workers; ++t) {
+            try {
+                pool.emplace_back(work);
+            } catch (const std::system_error &) {
+                break;
+            }
+        }
         work();
         for (auto &t : pool) t.join();
         for (std::size_t k = 0; k < n; ++k)
