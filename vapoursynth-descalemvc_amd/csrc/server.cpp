@@ -5,6 +5,7 @@
 #include <array>
 #include <future>
 #include <stdexcept>
+#include <system_error>
 
 #include "batch.hpp"
 #include "device_plan.hpp"
@@ -165,7 +166,9 @@ std::uint64_t HipServer::submit(const ServeRequest &r) {
         plans[static_cast<std::size_t>(k)] = find_axis_plan(wanted[static_cast<std::size_t>(k)]);
         missing += plans[static_cast<std::size_t>(k)] ? 0 : 1;
     }
-    if (missing > 1) {
+    // A thread that cannot be started (std::system_error from std::async)
+    // leaves the remaining work to the inline path below.
+    if (missing > 1) try {
         std::array<std::future<std::shared_ptr<const AxisPlan>>, 4> built;
         for (int k = 0; k < kinds; ++k)
             if (!plans[static_cast<std::size_t>(k)])
@@ -182,11 +185,11 @@ std::uint64_t HipServer::submit(const ServeRequest &r) {
                 std::launch::async, [&prepare, &plan, k, f64_pair] { prepare(plan, (k & 1) == 0, f64_pair); });
         }
         for (int k = 0; k < kinds; ++k) images[static_cast<std::size_t>(k)].get();
-    } else {
-        for (int k = 0; k < kinds; ++k)
-            if (!plans[static_cast<std::size_t>(k)])
-                plans[static_cast<std::size_t>(k)] = get_or_build_axis_plan(wanted[static_cast<std::size_t>(k)]);
+    } catch (const std::system_error &) {
     }
+    for (int k = 0; k < kinds; ++k)
+        if (!plans[static_cast<std::size_t>(k)])
+            plans[static_cast<std::size_t>(k)] = get_or_build_axis_plan(wanted[static_cast<std::size_t>(k)]);
 
     auto entry = std::make_shared<Entry>();
     entry->submitted = t0;
