@@ -379,7 +379,7 @@ static void band_condition_bounds(const double *bands, int32_t n, int32_t w,
             work[col + d] += value;
         }
     }
-    double norm = work[0];
+    double norm = n > 0 ? work[0] : 0.0;
     for (int32_t i = 1; i < n; ++i)
         if (norm < work[i]) norm = work[i];
     double margin = INFINITY;
