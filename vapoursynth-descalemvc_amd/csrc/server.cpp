@@ -150,11 +150,13 @@ std::uint64_t HipServer::submit(const ServeRequest &r) {
                           || executor_.mode() == SolveMode::windowed;
     // Device image plus the windowed tables the round (or, for a Float64
     // pair, the executor) will use.
-    auto prepare = [&](const AxisPlan &plan, bool rows, bool f64_pair) {
+    auto prepare = [&](const AxisPlan &plan, bool rows, bool f64_pair) -> hip::DevicePlan & {
         hip::DevicePlan &image = executor_.plan_image(plan);
-        if (!parallel) return;
-        if (f64_pair) (void)image.window64();
-        else (void)image.window(rows);
+        if (parallel) {
+            if (f64_pair) (void)image.window64();
+            else (void)image.window(rows);
+        }
+        return image;
     };
     // Cached plans are used as they are.  A request with several new plans
     // (a new geometry) builds them, and then their device tables, on
@@ -182,7 +184,7 @@ std::uint64_t HipServer::submit(const ServeRequest &r) {
             const AxisPlan &pair = *plans[static_cast<std::size_t>(k ^ 1)];
             const bool f64_pair = plan.requires_float64() || pair.requires_float64();
             images[static_cast<std::size_t>(k)] = std::async(
-                std::launch::async, [&prepare, &plan, k, f64_pair] { prepare(plan, (k & 1) == 0, f64_pair); });
+                std::launch::async, [&prepare, &plan, k, f64_pair] { (void)prepare(plan, (k & 1) == 0, f64_pair); });
         }
         for (int k = 0; k < kinds; ++k) images[static_cast<std::size_t>(k)].get();
     } catch (const std::system_error &) {
@@ -207,11 +209,11 @@ std::uint64_t HipServer::submit(const ServeRequest &r) {
         // Device images are uploaded here, on the caller's thread, so the
         // loop only builds tables and launches.
         const bool f64_pair = plane.h->requires_float64() || plane.v->requires_float64();
-        prepare(*plane.h, true, f64_pair);
-        prepare(*plane.v, false, f64_pair);
+        hip::DevicePlan &ih = prepare(*plane.h, true, f64_pair);
+        hip::DevicePlan &iv = prepare(*plane.v, false, f64_pair);
         if (parallel && !f64_pair) {
-            plane.wh = executor_.plan_image(*plane.h).window(true);
-            plane.wv = executor_.plan_image(*plane.v).window(false);
+            plane.wh = ih.window(true);
+            plane.wv = iv.window(false);
         }
         plane.table = plane.wh && plane.wv;
         if (plane.table)
