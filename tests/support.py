@@ -588,3 +588,25 @@ MIXED_OUTPUT_HASH = "a9dcffe6649bd96e"
 
 def mixed_horizontal_request():
     return make_request(17, 13, 12.75, 0.125, BICUBIC, 3, border=SYMMETRIC, f64_mode=F64_NEVER)
+
+
+# --------------------------------------------------------------------------
+# The FP64 accuracy oracle and the Float32 bar
+# --------------------------------------------------------------------------
+def fp64_request(r: Request) -> Request:
+    """The same request with f64mode = 2: its plan keeps the FP64 factors for
+    any geometry, and the oracle's Float64 route on it is the FP64 accuracy
+    oracle (reference src/axis_plan.cpp:909-940, 1193-1248)."""
+    return make_request(r.source_size, r.destination_size, r.active_length, r.shift, r.kernel,
+                        r.taps, r.b, r.c, r.blur, r.border, F64_ALWAYS)
+
+
+F32_BAR = 2.0e-6
+
+
+def f32_bar(reference_error: float) -> float:
+    """Largest max-abs error against the FP64 oracle allowed for a Float32
+    route: 2e-6, or twice the reference's own ordered Float32 path's error on
+    the same input where that is larger (ill-conditioned plans, whose F32
+    error no F32 solve can avoid)."""
+    return max(F32_BAR, 2.0 * reference_error)

@@ -3,8 +3,10 @@
 Bars (stated per test):
   * exact route (sequential solve): bit-identical to the ordered-FMA oracle
     (reference src/axis_plan.cpp:1068-1122 restated in oracle/);
-  * chunked route: max-abs vs the ordered oracle and vs the FP64 oracle within
-    the tolerance written in the test;
+  * parallel F32 routes (chunked, windowed, auto): max-abs against the FP64
+    oracle (the oracle's Float64 route on the f64mode = 2 twin of each
+    request) <= support.f32_bar(e_ref) = max(2e-6, 2 * e_ref), e_ref being the
+    reference's ordered F32 path's own error on the same input;
   * integer output: <= 1 code vs the oracle's scalar integer route;
   * Float64 route: <= 1 output ULP vs the oracle's Float64 route;
   * padding outside the logical width is never written (guard sentinels).
@@ -17,8 +19,8 @@ pytestmark = pytest.mark.gpu
 
 import dsmvc_hip as dh  # noqa: E402
 from support import (BICUBIC, BILINEAR, F64_ALWAYS, F64_AUTO, F64_NEVER, LANCZOS, REFLECT101,
-                     REPEAT, SPLINE16, SPLINE36, SPLINE64, SYMMETRIC, ZERO, make_request,
-                     ulp_distance)  # noqa: E402
+                     REPEAT, SPLINE16, SPLINE36, SPLINE64, SYMMETRIC, ZERO, f32_bar,
+                     fp64_request, make_request, ulp_distance)  # noqa: E402
 from workloads import workload_plan_requests, integer_conversion  # noqa: E402
 
 DEV = torch.device("cuda:0")
@@ -34,6 +36,21 @@ def executors():
     if not torch.cuda.is_available():
         pytest.skip("no HIP device")
     return {m: dh.Executor(0, m) for m in ("exact", "auto", "windowed")}
+
+
+def _fp64_axis(oracle, req, x):
+    """FP64 oracle of one axis over the vectors x [V][m]: [V][n] float64."""
+    p = oracle.build(fp64_request(req))
+    return np.stack([oracle.axis_f64(p, row.astype(np.float64)) for row in x])
+
+
+def _check_fp64(got, ref64, ordered, label=""):
+    """got and ordered (the reference's F32 result) against the FP64 oracle:
+    max-abs <= f32_bar(the reference's own error)."""
+    err = float(np.abs(got.astype(np.float64) - ref64).max())
+    e_ref = float(np.abs(ordered.astype(np.float64) - ref64).max())
+    assert err <= f32_bar(e_ref), (label, err, e_ref)
+    return err
 
 
 def _identical(a, b):
@@ -86,7 +103,8 @@ def test_exact_rows_and_columns_bit_identical(executors, oracle, name, req):
 
 @pytest.mark.parametrize("name,req", AXIS_CASES, ids=[c[0] for c in AXIS_CASES])
 def test_auto_rows_and_columns_close(executors, oracle, name, req):
-    """Chunked (partitioned) route: |HIP - ordered oracle| <= 4e-6 * max(1, |x|_inf)."""
+    """Production (auto) route, rows and columns: max-abs against the FP64
+    oracle <= f32_bar(e_ref)."""
     print(f"case {name}", flush=True)
     op = oracle.build(req)
     plan = dh.Plan(_hreq(req))
@@ -95,18 +113,19 @@ def test_auto_rows_and_columns_close(executors, oracle, name, req):
     vectors = 130
     x = rng.uniform(0, 1, (vectors, m)).astype(np.float32)
     expected = np.stack([oracle.axis_f32_ordered(op, row) for row in x])
+    ref64 = _fp64_axis(oracle, req, x)
     got = executors["auto"].inverse_rows(plan, torch.from_numpy(x).to(DEV)).cpu().numpy()
-    scale = max(1.0, float(np.abs(expected).max()))
-    assert float(np.abs(got - expected).max()) <= 4e-6 * scale
+    _check_fp64(got, ref64, expected, "rows")
     xc = np.ascontiguousarray(x.T)
     gotc = executors["auto"].inverse_columns(plan, torch.from_numpy(xc).to(DEV)).cpu().numpy()
-    assert float(np.abs(gotc - expected.T).max()) <= 4e-6 * scale
+    _check_fp64(gotc.T, ref64, expected, "columns")
 
 
 @pytest.mark.parametrize("name,req", AXIS_CASES, ids=[c[0] for c in AXIS_CASES])
 def test_windowed_rows_and_columns_close(executors, oracle, name, req):
-    """Windowed (independent-chunk) route: |HIP - ordered oracle| <= 4e-6 * max(1, |x|_inf).
-    Plans the windowed kernels cannot take must be refused, not solved."""
+    """Windowed (independent-chunk) route: max-abs against the FP64 oracle <=
+    f32_bar(e_ref).  Plans the windowed kernels cannot take must be refused,
+    not solved."""
     print(f"case {name}", flush=True)
     plan = dh.Plan(_hreq(req))
     info = plan.window_info()
@@ -122,12 +141,12 @@ def test_windowed_rows_and_columns_close(executors, oracle, name, req):
             return
         op = oracle.build(req)
         expected = np.stack([oracle.axis_f32_ordered(op, row) for row in x])
-        scale = max(1.0, float(np.abs(expected).max()))
+        ref64 = _fp64_axis(oracle, req, x)
         if info["rows_feasible"]:
             got = executors["windowed"].inverse_rows(plan, xt).cpu().numpy()
-            assert float(np.abs(got - expected).max()) <= 4e-6 * scale, vectors
+            _check_fp64(got, ref64, expected, f"rows {vectors}")
         gotc = executors["windowed"].inverse_columns(plan, xt.T.contiguous()).cpu().numpy()
-        assert float(np.abs(gotc - expected.T).max()) <= 4e-6 * scale, vectors
+        _check_fp64(gotc.T, ref64, expected, f"columns {vectors}")
 
 
 LONG_CASES = [
@@ -141,8 +160,8 @@ LONG_CASES = [
 @pytest.mark.parametrize("name,req", LONG_CASES, ids=[c[0] for c in LONG_CASES])
 def test_long_vectors_auto_close(executors, oracle, name, req):
     """Vectors longer than the carry-chain kernels take (n > 1280) run on the
-    windowed kernels in auto mode: within 4e-6 * max(1, |x|) of the ordered
-    oracle, rows and columns, with enough vectors to fill the GPU."""
+    windowed kernels in auto mode: max-abs against the FP64 oracle <=
+    f32_bar(e_ref), rows and columns, with enough vectors to fill the GPU."""
     print(f"case {name}", flush=True)
     plan = dh.Plan(_hreq(req))
     assert plan.window_info()["feasible"] == 1
@@ -153,11 +172,11 @@ def test_long_vectors_auto_close(executors, oracle, name, req):
     got = executors["auto"].inverse_rows(plan, xt).cpu().numpy()
     gotc = executors["auto"].inverse_columns(plan, xt.T.contiguous()).cpu().numpy()
     op = oracle.build(req)
-    for r in (0, 1, 511, 1023):
-        expected = oracle.axis_f32_ordered(op, x[r])
-        scale = max(1.0, float(np.abs(expected).max()))
-        assert float(np.abs(got[r] - expected).max()) <= 4e-6 * scale
-        assert float(np.abs(gotc[:, r] - expected).max()) <= 4e-6 * scale
+    picks = [0, 1, 511, 1023]
+    expected = np.stack([oracle.axis_f32_ordered(op, x[r]) for r in picks])
+    ref64 = _fp64_axis(oracle, req, x[picks])
+    _check_fp64(got[picks], ref64, expected, "rows")
+    _check_fp64(gotc[:, picks].T, ref64, expected, "columns")
 
 
 def test_windowed_yuv420p16_within_one_code(executors, oracle):
@@ -376,7 +395,7 @@ def test_autotuned_partition_stable_and_close(oracle):
     """A batch that fills the GPU triggers the partition autotuner (chroma
     planes of the 1080p -> 720p Debicubic request, 64 frames).  Bars: the
     tuned executor repeats its result bit for bit; every partition the tuner
-    can pick stays within 4e-6 * max(1, |x|) of the ordered oracle."""
+    can pick stays within f32_bar(e_ref) of the FP64 oracle."""
     reqs = workload_plan_requests("yuv420p16_debicubic_1080p_720p")
     h, v = reqs[2], reqs[3]
     hp, vp = dh.Plan(_hreq(h)), dh.Plan(_hreq(v))
@@ -392,13 +411,13 @@ def test_autotuned_partition_stable_and_close(oracle):
         pinned = dh.Executor(0, "auto")
         pinned.tune(count, count)
         results[f"c{count}"] = pinned.inverse_2d(hp, vp, x[[0, 63]]).cpu().numpy()
+    fh, fv = oracle.build(fp64_request(h)), oracle.build(fp64_request(v))
     for f in range(2):
         expected = oracle.inverse_2d(oh, ov, frames[f], ordered=True)
-        scale = max(1.0, float(np.abs(expected).max()))
+        ref64 = oracle.inverse_2d_f64_exact(fh, fv, frames[f].astype(np.float64))
         for name, got in results.items():
-            err = float(np.abs(got[f] - expected).max())
-            print(f"{name} frame {f}: max-abs {err:.3g}", flush=True)
-            assert err <= 4e-6 * scale
+            err = _check_fp64(got[f], ref64, expected, name)
+            print(f"{name} frame {f}: max-abs vs FP64 {err:.3g}", flush=True)
 
 
 def test_planes_call_matches_per_plane_calls(executors, oracle):

@@ -217,7 +217,9 @@ HipBatch::HipBatch(HipExecutor &executor, std::vector<BatchJob> jobs, bool graph
     }
     if (total) check(hipMalloc(&intermediate_, total), "batch intermediate allocation failed");
     for (std::size_t k = 0; k < jobs_.size(); ++k) {
-        if (!wh[k]) continue;
+        // Jobs with either pass outside the windowed kernels went to the
+        // fallback list above and are solved there only.
+        if (!wh[k] || !wv[k]) continue;
         const BatchJob &j = jobs_[k];
         const AxisPlan &h = *j.horizontal, &v = *j.vertical;
         const std::ptrdiff_t row = pitch_of(h.destination_size);

@@ -507,8 +507,12 @@ const DevChunks *DevicePlan::chunks(int max_chunks, bool rows) {
             check(hipGetDevice(&current), "hipGetDevice failed");
             check(hipSetDevice(device_), "hipSetDevice failed");
             check(hipMalloc(&image.memory, total), "chunk table allocation failed");
-            check(hipMemcpy(image.memory, host.data(), total, hipMemcpyHostToDevice),
-                  "chunk table upload failed");
+            if (const hipError_t e = hipMemcpy(image.memory, host.data(), total,
+                                               hipMemcpyHostToDevice);
+                e != hipSuccess) {
+                (void)hipFree(image.memory);
+                check(e, "chunk table upload failed");
+            }
             (void)hipSetDevice(current);
             auto *b = static_cast<unsigned char *>(image.memory);
             image.dev.chunks = cp.chunks;
@@ -551,7 +555,11 @@ void *DevicePlan::upload_window(const WindowPlan &wp, DevWindow &dev) {
     check(hipSetDevice(device_), "hipSetDevice failed");
     void *memory = nullptr;
     check(hipMalloc(&memory, total), "window table allocation failed");
-    check(hipMemcpy(memory, host.data(), total, hipMemcpyHostToDevice), "window table upload failed");
+    if (const hipError_t e = hipMemcpy(memory, host.data(), total, hipMemcpyHostToDevice);
+        e != hipSuccess) {
+        (void)hipFree(memory);
+        check(e, "window table upload failed");
+    }
     (void)hipSetDevice(current);
     auto *b = static_cast<unsigned char *>(memory);
     dev.n = plan_->destination_size;
@@ -573,13 +581,15 @@ const DevWindow *DevicePlan::window(bool rows) {
     std::lock_guard lock(mutex_);
     WindowImage &image = window_image_;
     if (!image.built) {
-        image.built = true;
+        // Marked built only once the tables are built and uploaded: a failed
+        // upload throws here and the next call tries again.
         WindowPlan wp;
         if (build_window_plan(*plan_, wp)) {
             image.memory = upload_window(wp, image.dev);
             image.ok = true;
             image.rows_feasible = wp.rows_feasible;
         }
+        image.built = true;
     }
     if (!image.ok || (rows && !image.rows_feasible)) return nullptr;
     return &image.dev;
@@ -589,7 +599,6 @@ const DevWindow *DevicePlan::solve_window(bool rows) {
     std::lock_guard lock(mutex_);
     WindowImage &image = solve_image_;
     if (!image.built) {
-        image.built = true;
         WindowPlan wp;
         if (build_solve_window_plan(*plan_, wp)) {
             image.memory = upload_window(wp, image.dev);
@@ -597,6 +606,7 @@ const DevWindow *DevicePlan::solve_window(bool rows) {
             image.ok = true;
             image.rows_feasible = wp.rows_feasible;
         }
+        image.built = true;
     }
     if (!image.ok || (rows && !image.rows_feasible)) return nullptr;
     return &image.dev;
@@ -606,7 +616,6 @@ const DevWindow64 *DevicePlan::window64() {
     std::lock_guard lock(mutex_);
     Window64Image &image = window64_image_;
     if (!image.built) {
-        image.built = true;
         WindowPlan64 wp;
         if (build_window_plan64(*plan_, pack_plan(*plan_), wp)) {
             const std::size_t o_fwd = 0;
@@ -622,10 +631,15 @@ const DevWindow64 *DevicePlan::window64() {
             int current = 0;
             check(hipGetDevice(&current), "hipGetDevice failed");
             check(hipSetDevice(device_), "hipSetDevice failed");
-            check(hipMalloc(&image.memory, total), "window table allocation failed");
-            check(hipMemcpy(image.memory, host.data(), total, hipMemcpyHostToDevice),
-                  "window table upload failed");
+            void *memory = nullptr;
+            check(hipMalloc(&memory, total), "window table allocation failed");
+            if (const hipError_t e = hipMemcpy(memory, host.data(), total, hipMemcpyHostToDevice);
+                e != hipSuccess) {
+                (void)hipFree(memory);
+                check(e, "window table upload failed");
+            }
             (void)hipSetDevice(current);
+            image.memory = memory;
             auto *b = static_cast<unsigned char *>(image.memory);
             image.dev.n = plan_->destination_size;
             image.dev.m = plan_->source_size;
@@ -640,6 +654,7 @@ const DevWindow64 *DevicePlan::window64() {
             image.dev.blk_lo = reinterpret_cast<const std::int32_t *>(b + o_blk);
             image.ok = true;
         }
+        image.built = true;
     }
     return image.ok ? &image.dev : nullptr;
 }
@@ -647,9 +662,11 @@ const DevWindow64 *DevicePlan::window64() {
 const DevRefine *DevicePlan::refine() {
     std::lock_guard lock(mutex_);
     if (!refine_built_) {
-        refine_built_ = true;
         const AxisPlan &p = *plan_;
-        if (!p.requires_float64()) return nullptr;
+        if (!p.requires_float64()) {
+            refine_built_ = true;
+            return nullptr;
+        }
         const int n = p.destination_size, w = p.half_bandwidth, span = 2 * w + 1;
         const std::size_t nnz = p.transpose_weights_f64.size();
         const auto un = static_cast<std::size_t>(n);
@@ -715,10 +732,16 @@ const DevRefine *DevicePlan::refine() {
         int current = 0;
         check(hipGetDevice(&current), "hipGetDevice failed");
         check(hipSetDevice(device_), "hipSetDevice failed");
-        check(hipMalloc(&refine_memory_, total), "refinement table allocation failed");
-        check(hipMemcpy(refine_memory_, host.data(), total, hipMemcpyHostToDevice),
-              "refinement table upload failed");
+        void *memory = nullptr;
+        check(hipMalloc(&memory, total), "refinement table allocation failed");
+        if (const hipError_t e = hipMemcpy(memory, host.data(), total, hipMemcpyHostToDevice);
+            e != hipSuccess) {
+            (void)hipFree(memory);
+            check(e, "refinement table upload failed");
+        }
         (void)hipSetDevice(current);
+        refine_memory_ = memory;
+        refine_built_ = true;
         auto *b = static_cast<unsigned char *>(refine_memory_);
         refine_.n = n;
         refine_.w = w;

@@ -25,6 +25,7 @@ from ._lib import ASYNC, AxisRequestC, ConversionC, DsmvcError, PlanInfoC, check
 
 __all__ = [
     "AxisRequest", "Plan", "Executor", "Clip", "IntegerConversion", "build_plan", "build_plans",
+    "filter_requests",
     "integer_conversion", "Descale", "Debilinear", "Debicubic", "Delanczos", "Despline16",
     "Despline36", "Despline64", "parse_backend", "resolve_backend", "device_count",
     "padding_index", "planner_cache_stats", "clear_planner_caches", "DsmvcError",
@@ -227,6 +228,34 @@ def _dev_ptr(t) -> ctypes.c_void_p:
     return ctypes.c_void_p(t.data_ptr())
 
 
+def _sample_kind(t) -> Optional[int]:
+    torch = _torch()
+    return {torch.float32: 0, torch.uint8: 1, torch.uint16: 2}.get(t.dtype)
+
+
+def _check_planes(t, what: str, rows: int, cols: int, dtype=None, frames: Optional[int] = None):
+    """Checks, before any device call, that tensor `t` ([H, W] or [frames, H, W])
+    holds at least `rows` x `cols` samples per plane, of `dtype` when given, and
+    `frames` planes when given, with contiguous rows and strides that keep rows
+    and planes apart -- so that no kernel reads or writes past the buffer.
+    Returns t as [frames, H, W]."""
+    if t.dim() not in (2, 3):
+        raise ValueError(f"dsmvc: {what} must be a 2-D plane or a 3-D batch of planes")
+    if dtype is not None and t.dtype != dtype:
+        raise ValueError(f"dsmvc: {what} must be {str(dtype).replace('torch.', '')}")
+    tb = t if t.dim() == 3 else t.unsqueeze(0)
+    n, h, w = tb.shape
+    if frames is not None and n != frames:
+        raise ValueError(f"dsmvc: {what} has {n} frames, expected {frames}")
+    if h < rows or w < cols:
+        raise ValueError(f"dsmvc: {what} is {h}x{w}, needs at least {rows}x{cols} samples")
+    if tb.stride(2) != 1:
+        raise ValueError(f"dsmvc: {what} rows must be contiguous")
+    if (rows > 1 and tb.stride(1) < cols) or (n > 1 and tb.stride(0) < tb.stride(1) * rows):
+        raise ValueError(f"dsmvc: {what} strides overlap its rows or planes")
+    return tb
+
+
 class Executor:
     """dsmvc::Executor(backend="hip") on one GPU (reference include/dsmvc/engine.hpp:363-504).
 
@@ -276,9 +305,15 @@ class Executor:
     # -- single axis, float32 ------------------------------------------------
     def inverse_rows(self, plan: Plan, x, out=None, stream=None, asynchronous=False):
         torch = _torch()
+        if x.dim() != 2:
+            raise ValueError("dsmvc: inverse_rows takes a 2-D plane")
         rows = x.shape[0]
+        _check_planes(x, "input", rows, plan.source_size, torch.float32)
         if out is None:
             out = torch.empty((rows, plan.destination_size), dtype=torch.float32, device=x.device)
+        if out.dim() != 2:
+            raise ValueError("dsmvc: inverse_rows writes a 2-D plane")
+        _check_planes(out, "output", rows, plan.destination_size, torch.float32)
         check(load().dsmvc_hip_inverse_rows(
             self._handle, plan.handle, _dev_ptr(x), x.stride(0), _dev_ptr(out), out.stride(0),
             rows, _stream_handle(stream), ASYNC if asynchronous else 0))
@@ -286,11 +321,17 @@ class Executor:
 
     def inverse_columns(self, plan: Plan, x, out=None, stream=None, asynchronous=False):
         torch = _torch()
+        if x.dim() != 2:
+            raise ValueError("dsmvc: inverse_columns takes a 2-D plane")
         cols = x.shape[1]
-        if out is None:
-            out = torch.empty((plan.destination_size, cols), dtype=torch.float32, device=x.device)
         if x.shape[0] < plan.source_size:
             raise ValueError("dsmvc: input has fewer rows than the plan's source size")
+        _check_planes(x, "input", plan.source_size, cols, torch.float32)
+        if out is None:
+            out = torch.empty((plan.destination_size, cols), dtype=torch.float32, device=x.device)
+        if out.dim() != 2:
+            raise ValueError("dsmvc: inverse_columns writes a 2-D plane")
+        _check_planes(out, "output", plan.destination_size, cols, torch.float32)
         check(load().dsmvc_hip_inverse_columns(
             self._handle, plan.handle, _dev_ptr(x), x.stride(0), _dev_ptr(out), out.stride(0),
             cols, _stream_handle(stream), ASYNC if asynchronous else 0))
@@ -303,20 +344,24 @@ class Executor:
         """x: [H, W] or [frames, H, W]; float32, uint8 or uint16 (integer needs a
         conversion).  Returns [h, w] / [frames, h, w] of x's dtype."""
         torch = _torch()
+        if x.dim() not in (2, 3):
+            raise ValueError("dsmvc: input must be a 2-D plane or a 3-D batch of planes")
         batched = x.dim() == 3
         xb = x if batched else x.unsqueeze(0)
         frames, rows, cols = xb.shape
         if rows != vertical.source_size or cols < horizontal.source_size:
             raise ValueError("dsmvc: input plane does not match the plans")
-        if xb.stride(2) != 1:
-            raise ValueError("dsmvc: rows must be contiguous")
-        kind = {torch.float32: 0, torch.uint8: 1, torch.uint16: 2}.get(xb.dtype)
+        kind = _sample_kind(xb)
         if kind is None:
             raise ValueError("dsmvc: samples must be float32, uint8 or uint16")
+        if kind != 0 and conversion is None:
+            raise ValueError("dsmvc: integer planes need a conversion")
+        _check_planes(xb, "input", vertical.source_size, horizontal.source_size)
         if out is None:
             out = torch.empty((frames, vertical.destination_size, horizontal.destination_size),
                               dtype=xb.dtype, device=xb.device)
-        ob = out if out.dim() == 3 else out.unsqueeze(0)
+        ob = _check_planes(out, "output", vertical.destination_size,
+                           horizontal.destination_size, xb.dtype, frames)
         conv = ctypes.byref(conversion.to_c()) if conversion is not None else None
         check(load().dsmvc_hip_inverse_2d_batch(
             self._handle, horizontal.handle, vertical.handle, kind, _dev_ptr(xb), xb.stride(1),
@@ -335,13 +380,18 @@ class Executor:
         outs, keep = [], []
         kind = None
         for k, (hp, vp, x, conversion) in enumerate(jobs):
+            if x.dim() not in (2, 3):
+                raise ValueError("dsmvc: input must be a 2-D plane or a 3-D batch of planes")
             xb = x if x.dim() == 3 else x.unsqueeze(0)
             frames, rows, cols = xb.shape
             if rows != vp.source_size or cols < hp.source_size or xb.stride(2) != 1:
                 raise ValueError("dsmvc: input plane does not match the plans")
-            k_ = {torch.float32: 0, torch.uint8: 1, torch.uint16: 2}.get(xb.dtype)
+            k_ = _sample_kind(xb)
             if k_ is None or (kind is not None and k_ != kind):
                 raise ValueError("dsmvc: planes must share one sample type")
+            if k_ != 0 and conversion is None:
+                raise ValueError("dsmvc: integer planes need a conversion")
+            _check_planes(xb, "input", vp.source_size, hp.source_size)
             kind = k_
             out = torch.empty((frames, vp.destination_size, hp.destination_size), dtype=xb.dtype,
                               device=xb.device)
@@ -374,11 +424,15 @@ class Batch:
         self._executor = executor
         self._keep = []
         table = (_lib.BatchJobC * max(1, len(jobs)))()
+        torch = _torch()
         for k, (hp, vp, x, out) in enumerate(jobs):
             frames = x.shape[0] if x.dim() == 3 else 1
             if out.shape[-2:] != (vp.destination_size, hp.destination_size) or \
                     x.shape[-2] < vp.source_size or x.shape[-1] < hp.source_size:
                 raise ValueError("dsmvc: batch job shapes do not match its plans")
+            _check_planes(x, "batch input", vp.source_size, hp.source_size, torch.float32)
+            _check_planes(out, "batch output", vp.destination_size, hp.destination_size,
+                          torch.float32, frames)
             table[k] = _lib.BatchJobC(
                 hp.handle, vp.handle, _dev_ptr(x), x.stride(-2),
                 x.stride(0) if x.dim() == 3 else 0, _dev_ptr(out), out.stride(-2),
@@ -452,6 +506,15 @@ class Server:
         for k in range(n):
             if planes_in[k].dtype != _torch().float32 or planes_out[k].dtype != _torch().float32:
                 raise ValueError("dsmvc: served planes are float32")
+            if planes_in[k].dim() != 2 or planes_out[k].dim() != 2:
+                raise ValueError("dsmvc: served planes are 2-D")
+            # The library derives chroma sizes from luma by the subsampling.
+            sw_, sh_ = r.subsampling_w if k else 0, r.subsampling_h if k else 0
+            _check_planes(planes_in[k], f"input plane {k}", r.src_height >> sh_,
+                          r.src_width >> sw_)
+            _check_planes(planes_out[k], f"output plane {k}", r.dst_height >> sh_,
+                          r.dst_width >> sw_)
+        for k in range(n):
             r.src[k] = _dev_ptr(planes_in[k]).value
             r.src_stride[k] = planes_in[k].stride(0)
             r.dst[k] = _dev_ptr(planes_out[k]).value
@@ -517,6 +580,57 @@ def _executor(solve: str = "auto"):
     return _executors[solve]
 
 
+def filter_requests(source_width: int, source_height: int, width: int, height: int,
+                    num_planes: int = 1, subsampling_w: int = 0, subsampling_h: int = 0,
+                    src_left: float = 0.0, src_top: float = 0.0,
+                    src_width: Optional[float] = None, src_height: Optional[float] = None,
+                    kernel: int = BICUBIC, taps: int = 3, b: float = 0.0, c: float = 0.5,
+                    blur: float = 1.0, border: int = SYMMETRIC, f64_mode: int = 0):
+    """Axis requests per plane kind, as the reference's prepare_filter_requests
+    derives them (src/vs_plugin.cpp:656-705): ([luma, chroma] horizontal,
+    [luma, chroma] vertical), chroma present only for subsampled multi-plane
+    clips.  Horizontal chroma is left-sited (shift 0.25 - 0.25 * width /
+    source_width, plus src_left scaled to the chroma plane); vertical chroma
+    takes src_top scaled only.  The active lengths are src_width / src_height
+    (defaulting to the output size) scaled to each plane."""
+    src_width = float(width) if src_width is None else float(src_width)
+    src_height = float(height) if src_height is None else float(src_height)
+    kw = dict(kernel=kernel, taps=taps, b=b, c=c, blur=blur, border=border, f64_mode=f64_mode)
+    hreq = [AxisRequest(source_width, width, src_width, src_left, **kw)]
+    vreq = [AxisRequest(source_height, height, src_height, src_top, **kw)]
+    if num_planes > 1 and subsampling_w > 0:
+        cw = source_width >> subsampling_w
+        hreq.append(AxisRequest(cw, width >> subsampling_w,
+                                src_width * float(cw) / float(source_width),
+                                0.25 - 0.25 * float(width) / float(source_width)
+                                + src_left * float(cw) / float(source_width), **kw))
+    if num_planes > 1 and subsampling_h > 0:
+        ch = source_height >> subsampling_h
+        vreq.append(AxisRequest(ch, height >> subsampling_h,
+                                src_height * float(ch) / float(source_height),
+                                src_top * float(ch) / float(source_height), **kw))
+    return hreq, vreq
+
+
+def _to_float(plane, conv: IntegerConversion):
+    """Integer samples to normalised float32, the same F32 operations as the
+    fused kernels' load: (float(x) - offset) * scale."""
+    torch = _torch()
+    off = torch.tensor(conv.input_offset, dtype=torch.float32, device=plane.device)
+    scale = torch.tensor(conv.input_scale, dtype=torch.float32, device=plane.device)
+    return (plane.to(torch.float32) - off) * scale
+
+
+def _to_integer(plane, conv: IntegerConversion, dtype):
+    """float32 to integer samples: nearbyint(clamp(v * scale + offset, 0, max))
+    under round-half-even, as the reference's integer output conversion."""
+    torch = _torch()
+    scale = torch.tensor(conv.output_scale, dtype=torch.float32, device=plane.device)
+    off = torch.tensor(conv.output_offset, dtype=torch.float32, device=plane.device)
+    v = (plane * scale + off).clamp(0.0, float(conv.output_maximum))
+    return torch.round(v).to(torch.int32).to(dtype)
+
+
 def Descale(src, width: int, height: int, kernel: str = "bicubic", taps: int = 3,
             b: float = 0.0, c: float = 0.5, src_left: float = 0.0, src_top: float = 0.0,
             src_width: Optional[float] = None, src_height: Optional[float] = None,
@@ -525,7 +639,11 @@ def Descale(src, width: int, height: int, kernel: str = "bicubic", taps: int = 3
             force_h: Optional[bool] = None, force_v: Optional[bool] = None,
             backend: str = "hip", solve: str = "auto"):
     """core.dsmvc.Descale with backend="hip" (reference src/vs_plugin.cpp:362-509,
-    656-705, 1774-1987).  `src` is a Clip or a single float32 plane tensor."""
+    656-705, 1774-1987).  `src` is a Clip or a single float32 plane tensor.
+    Integer clips descaled on both axes take the fused integer route; on one
+    axis they are converted to float32, descaled and converted back, as the
+    reference does through its resize calls (convert_and_invoke,
+    vs_plugin.cpp:570-626)."""
     if resolve_backend(backend) != "hip":
         raise ValueError("dsmvc: this build provides backend=\"hip\" only")
     clip = src if isinstance(src, Clip) else Clip([src])
@@ -549,76 +667,62 @@ def Descale(src, width: int, height: int, kernel: str = "bicubic", taps: int = 3
         raise ValueError("f64mode must be 0, 1, or 2")
     if not (blur > 0.0) or not math.isfinite(blur):
         raise ValueError("blur must be finite and greater than zero")
+    sw, sh = clip.width, clip.height
+    extent = min(sw, sh)
+    if len(clip.planes) > 1:
+        extent = min(extent, sw >> clip.subsampling_w, sh >> clip.subsampling_h)
+    if blur >= float(extent):
+        raise ValueError("blur exceeds the supported source-plane extent")
     if kernel == "lanczos" and taps < 1:
         raise ValueError("taps must be greater than zero")
-    sw, sh = clip.width, clip.height
     if clip.color_family == "yuv" and clip.subsampling_w | clip.subsampling_h:
         if width % (1 << clip.subsampling_w) or height % (1 << clip.subsampling_h):
             raise ValueError("output width is incompatible with subsampling")
-    src_width = float(width) if src_width is None else float(src_width)
-    src_height = float(height) if src_height is None else float(src_height)
+    sw_act = float(width) if src_width is None else float(src_width)
+    sh_act = float(height) if src_height is None else float(src_height)
     force_h = force if force_h is None else force_h
     force_v = force if force_v is None else force_v
-    process_h = width != sw or src_left != 0.0 or src_width != float(width) or force_h
-    process_v = height != sh or src_top != 0.0 or src_height != float(height) or force_v
+    process_h = width != sw or src_left != 0.0 or sw_act != float(width) or force_h
+    process_v = height != sh or src_top != 0.0 or sh_act != float(height) or force_v
     if not process_h and not process_v:
         return clip if isinstance(src, Clip) else src
-    kw = dict(kernel=KERNELS[kernel], taps=taps, b=b, c=c, blur=blur, border=border,
-              f64_mode=f64mode)
-    # Requests per plane kind (reference prepare_filter_requests, vs_plugin.cpp:656-705).
-    hreq = [AxisRequest(sw, width, src_width, src_left, **kw)]
-    vreq = [AxisRequest(sh, height, src_height, src_top, **kw)]
-    if len(clip.planes) > 1 and clip.subsampling_w > 0:
-        cw = sw >> clip.subsampling_w
-        hreq.append(AxisRequest(cw, width >> clip.subsampling_w,
-                                src_width * float(cw) / float(sw),
-                                0.25 - 0.25 * float(width) / float(sw) + src_left * float(cw) / float(sw),
-                                **kw))
-    if len(clip.planes) > 1 and clip.subsampling_h > 0:
-        ch = sh >> clip.subsampling_h
-        vreq.append(AxisRequest(ch, height >> clip.subsampling_h,
-                                src_height * float(ch) / float(sh),
-                                src_top * float(ch) / float(sh), **kw))
+    hreq, vreq = filter_requests(sw, sh, width, height, len(clip.planes), clip.subsampling_w,
+                                 clip.subsampling_h, src_left, src_top, sw_act, sh_act,
+                                 kernel=KERNELS[kernel], taps=taps, b=b, c=c, blur=blur,
+                                 border=border, f64_mode=f64mode)
     ex = _executor(solve)
+    torch = _torch()
+
+    def conversion_of(index, plane):
+        if plane.dtype == torch.float32:
+            return None
+        return integer_conversion(clip.bits or (8 if plane.dtype == torch.uint8 else 16),
+                                  index > 0 and clip.color_family == "yuv", clip.full_range)
+
+    def plans_of(index):
+        chroma = index > 0
+        return (Plan(hreq[1 if chroma and len(hreq) > 1 else 0]),
+                Plan(vreq[1 if chroma and len(vreq) > 1 else 0]))
+
     if process_h and process_v:
         # Every plane of the frame batch in one set of launches.
-        jobs = []
-        for index, plane in enumerate(clip.planes):
-            chroma = index > 0
-            conv = None
-            if plane.dtype != _torch().float32:
-                conv = integer_conversion(clip.bits or (8 if plane.dtype == _torch().uint8 else 16),
-                                          chroma and clip.color_family == "yuv", clip.full_range)
-            jobs.append((Plan(hreq[1 if chroma and len(hreq) > 1 else 0]),
-                         Plan(vreq[1 if chroma and len(vreq) > 1 else 0]), plane, conv))
+        jobs = [plans_of(i) + (plane, conversion_of(i, plane))
+                for i, plane in enumerate(clip.planes)]
         out_planes = ex.inverse_2d_planes(jobs)
-        if isinstance(src, Clip):
-            return Clip(out_planes, clip.color_family, clip.subsampling_w, clip.subsampling_h,
-                        clip.bits, clip.full_range)
-        return out_planes[0]
-    out_planes = []
-    for index, plane in enumerate(clip.planes):
-        chroma = index > 0
-        hp = Plan(hreq[1 if chroma and len(hreq) > 1 else 0])
-        vp = Plan(vreq[1 if chroma and len(vreq) > 1 else 0])
-        integer = plane.dtype != _torch().float32
-        conv = None
-        if integer:
-            if not (process_h and process_v):
-                raise ValueError("dsmvc: the HIP executor descales integer clips on both axes "
-                                 "only; convert to float32 for single-axis descales")
-            conv = integer_conversion(clip.bits or (8 if plane.dtype == _torch().uint8 else 16),
-                                      chroma and clip.color_family == "yuv", clip.full_range)
-        if process_h and process_v:
-            out_planes.append(ex.inverse_2d(hp, vp, plane, conversion=conv))
-        elif process_h:
+    else:
+        out_planes = []
+        for index, plane in enumerate(clip.planes):
+            hp, vp = plans_of(index)
+            conv = conversion_of(index, plane)
             x = plane if plane.dim() == 3 else plane.unsqueeze(0)
-            res = [ex.inverse_rows(hp, f) for f in x]
-            out_planes.append(_torch().stack(res) if plane.dim() == 3 else res[0])
-        else:
-            x = plane if plane.dim() == 3 else plane.unsqueeze(0)
-            res = [ex.inverse_columns(vp, f) for f in x]
-            out_planes.append(_torch().stack(res) if plane.dim() == 3 else res[0])
+            xf = _to_float(x, conv) if conv is not None else x
+            if process_h:
+                res = torch.stack([ex.inverse_rows(hp, f) for f in xf])
+            else:
+                res = torch.stack([ex.inverse_columns(vp, f) for f in xf])
+            if conv is not None:
+                res = _to_integer(res, conv, plane.dtype)
+            out_planes.append(res if plane.dim() == 3 else res[0])
     if isinstance(src, Clip):
         return Clip(out_planes, clip.color_family, clip.subsampling_w, clip.subsampling_h,
                     clip.bits, clip.full_range)
