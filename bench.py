@@ -12,9 +12,10 @@ more than 1 GiB of distinct buffers so the 256 MB Infinity Cache cannot serve
 them.
 
 Prints one JSON line: frames/s (whole job), the per-kernel roofline of the
-dominant kernel (HIP events on the launch stream over the timed region) and
-the reference CPU engine's scalar path (opt=1) on a bounded sample of the same
-workload on this host.
+dominant kernel (HIP events on the launch stream over the timed region) with
+the frame-level fraction beside it, and the reference CPU engine's automatic
+SIMD path on every host core (its scalar path on one core beside it) on a
+bounded sample of the same workload on this host.
 """
 import argparse
 import ctypes
@@ -87,38 +88,150 @@ def forward_operator(torch, plan, device):
     return torch.from_numpy(A).to(device)
 
 
-def cpu_baseline(planes_host, seconds=10.0):
-    """Reference engine (oracle/_ref, built from the reference's own sources)
-    CpuExecutor scalar path (opt=1), one thread, whole YUV420P16 frames; falls
-    back to the oracle's C restatement when the reference build is absent."""
+def host_cores():
+    """Host threads this process may use, at most 16 (one GPU's share of the host)."""
+    try:
+        count = len(os.sched_getaffinity(0))
+    except AttributeError:
+        count = os.cpu_count() or 1
+    return max(1, min(16, count))
+
+
+def cpu_baseline(planes_host, seconds=12.0, scalar_seconds=4.0):
+    """The reference engine (oracle/_ref, built from the reference's own
+    sources) on this host: CpuExecutor with CpuPath::automatic (AVX-512 where
+    the host has it, else AVX2; reference src/cpu_executor.cpp:787-799), one
+    worker thread per host core, each taking whole YUV420P16 frames and
+    calling inverse_2d_u16 per plane on one shared executor -- the way
+    VapourSynth's fmParallel drives the plugin.  The scalar path (opt=1) on
+    one thread is timed beside it.  Falls back to the oracle's C restatement
+    (one thread, "port") when the reference build is absent."""
+    import threading
     sys.path.insert(0, str(ROOT / "tests"))
-    from support import Oracle, Reference, make_request, reference_available
+    from support import Oracle, Reference, reference_available
     from workloads import chroma420_requests, integer_conversion, luma_requests
     lreq = luma_requests(SRC_W, SRC_H, DST_W, DST_H, 1, b=0.0, c=0.5)
     creq = chroma420_requests(SRC_W, SRC_H, DST_W, DST_H, 1, b=0.0, c=0.5)
     convs = [integer_conversion(16, False, False), integer_conversion(16, True, False)]
-    if reference_available():
-        eng, kind = Reference(), "reference"
-        run = lambda h, v, x, conv: eng.inverse_2d_int(h, v, x, conv, path=Reference.CPU_SCALAR)  # noqa: E731
-    else:
-        eng, kind = Oracle(), "port"
-        run = lambda h, v, x, conv: eng.inverse_2d_int(h, v, x, conv)  # noqa: E731
-    plans = [(eng.build(lreq[0]), eng.build(lreq[1])), (eng.build(creq[0]), eng.build(creq[1]))]
-    frames = 0
-    start = time.perf_counter()
-    while True:
-        f = frames % planes_host[0].shape[0]
-        for p in range(3):
-            h, v = plans[0 if p == 0 else 1]
-            run(h, v, planes_host[p][f], convs[0 if p == 0 else 1])
-        frames += 1
+    count = planes_host[0].shape[0]
+
+    def timed(run, threads, limit):
+        stop = time.perf_counter() + limit
+        done = [0] * threads
+
+        def worker(t):
+            k = t
+            while time.perf_counter() < stop:
+                f = k % count
+                for p in range(3):
+                    h, v = plans[0 if p == 0 else 1]
+                    run(h, v, planes_host[p][f], convs[0 if p == 0 else 1])
+                done[t] += 1
+                k += threads
+        start = time.perf_counter()
+        pool = [threading.Thread(target=worker, args=(t,)) for t in range(threads)]
+        for th in pool:
+            th.start()
+        for th in pool:
+            th.join()
         elapsed = time.perf_counter() - start
-        if elapsed >= seconds or frames >= 400:
-            break
-    return {"value": round(frames / elapsed, 3), "unit": "frames/s", "cores": 1, "kind": kind,
-            "sample": f"{frames} YUV420P16 1920x1080->1280x720 Debicubic frames, CpuExecutor "
-                      f"scalar (opt=1) inverse_2d_u16 per plane, single thread, "
-                      f"{elapsed:.1f} s"}
+        return sum(done), elapsed
+
+    if not reference_available():
+        eng = Oracle()
+        plans = [(eng.build(lreq[0]), eng.build(lreq[1])), (eng.build(creq[0]), eng.build(creq[1]))]
+        frames, elapsed = timed(lambda h, v, x, c: eng.inverse_2d_int(h, v, x, c), 1, seconds)
+        return {"value": round(frames / elapsed, 3), "unit": "frames/s", "cores": 1,
+                "kind": "port", "sample": f"{frames} YUV420P16 frames through the oracle's C "
+                                          f"restatement, one thread, {elapsed:.1f} s"}
+    eng = Reference()
+    plans = [(eng.build(lreq[0]), eng.build(lreq[1])), (eng.build(creq[0]), eng.build(creq[1]))]
+    threads = host_cores()
+    path_name = eng.executor_name(Reference.CPU_AUTOMATIC)
+    auto = lambda h, v, x, c: eng.inverse_2d_int(h, v, x, c, path=Reference.CPU_AUTOMATIC)  # noqa: E731
+    scalar = lambda h, v, x, c: eng.inverse_2d_int(h, v, x, c, path=Reference.CPU_SCALAR)  # noqa: E731
+    timed(auto, threads, 1.0)   # warm: plan packing, worker pool start
+    frames, elapsed = timed(auto, threads, seconds)
+    sframes, selapsed = timed(scalar, 1, scalar_seconds)
+    return {"value": round(frames / elapsed, 3), "unit": "frames/s", "cores": threads,
+            "kind": "reference",
+            "sample": f"{frames} YUV420P16 1920x1080->1280x720 Debicubic frames in "
+                      f"{elapsed:.1f} s: reference CpuExecutor CpuPath::automatic "
+                      f"({path_name}), {threads} threads each taking whole frames, "
+                      f"inverse_2d_u16 per plane",
+            "scalar_one_thread": {"value": round(sframes / selapsed, 3), "unit": "frames/s",
+                                  "cores": 1, "sample": f"{sframes} frames, CpuPath::scalar "
+                                                        f"(opt=1), {selapsed:.1f} s"}}
+
+
+def cpu_calibration(seconds=3.0):
+    """CPU-only: the reference engine's scalar and automatic paths at 1..N
+    threads on the GRAYS F32 1080p -> 720p workloads (Debilinear, and
+    Debicubic) and the YUV420P16 Debicubic workload, plus the ratio of the
+    oracle's C restatement to the reference scalar path on one thread.  Each
+    worker takes whole frames; frames/s per entry."""
+    import threading
+    sys.path.insert(0, str(ROOT / "tests"))
+    from support import BICUBIC, BILINEAR, Oracle, Reference, reference_available
+    from workloads import chroma420_requests, integer_conversion, luma_requests
+    if not reference_available():
+        raise SystemExit("bench.py --cpu-calibration needs oracle/_ref (the reference build)")
+    ref, orc = Reference(), Oracle()
+    rng = np.random.default_rng(1)
+    gray = rng.uniform(0, 1, (8, SRC_H, SRC_W)).astype(np.float32)
+    yuv = [rng.integers(0, 65536, (4, SRC_H >> (p > 0), SRC_W >> (p > 0))).astype(np.uint16)
+           for p in range(3)]
+    convs = [integer_conversion(16, False, False), integer_conversion(16, True, False)]
+
+    def rate(work, threads):
+        stop = time.perf_counter() + seconds
+        done = [0] * threads
+
+        def worker(t):
+            k = t
+            while time.perf_counter() < stop:
+                work(k)
+                done[t] += 1
+                k += threads
+        start = time.perf_counter()
+        pool = [threading.Thread(target=worker, args=(t,)) for t in range(threads)]
+        for th in pool:
+            th.start()
+        for th in pool:
+            th.join()
+        return round(sum(done) / (time.perf_counter() - start), 2)
+
+    out = {"host_threads": host_cores(), "seconds_per_entry": seconds,
+           "automatic_resolves_to": ref.executor_name(Reference.CPU_AUTOMATIC)}
+    counts = sorted({1, 2, 4, host_cores()})
+    for name, kernel in (("grays_f32_debilinear", BILINEAR), ("grays_f32_debicubic", BICUBIC)):
+        h, v = luma_requests(SRC_W, SRC_H, DST_W, DST_H, kernel, b=0.0, c=0.5)
+        rh, rv = ref.build(h), ref.build(v)
+        entry = {"scalar_1": rate(lambda k: ref.inverse_2d(rh, rv, gray[k % 8],
+                                                           path=Reference.CPU_SCALAR), 1)}
+        for n in counts:
+            entry[f"automatic_{n}"] = rate(lambda k: ref.inverse_2d(
+                rh, rv, gray[k % 8], path=Reference.CPU_AUTOMATIC), n)
+        out[name] = entry
+    lreq = luma_requests(SRC_W, SRC_H, DST_W, DST_H, BICUBIC, b=0.0, c=0.5)
+    creq = chroma420_requests(SRC_W, SRC_H, DST_W, DST_H, BICUBIC, b=0.0, c=0.5)
+
+    def yuv_frame(eng, plans, **kw):
+        def work(k):
+            for p in range(3):
+                h, v = plans[0 if p == 0 else 1]
+                eng.inverse_2d_int(h, v, yuv[p][k % 4], convs[0 if p == 0 else 1], **kw)
+        return work
+    rplans = [(ref.build(lreq[0]), ref.build(lreq[1])), (ref.build(creq[0]), ref.build(creq[1]))]
+    oplans = [(orc.build(lreq[0]), orc.build(lreq[1])), (orc.build(creq[0]), orc.build(creq[1]))]
+    entry = {"scalar_1": rate(yuv_frame(ref, rplans, path=Reference.CPU_SCALAR), 1)}
+    for n in counts:
+        entry[f"automatic_{n}"] = rate(yuv_frame(ref, rplans, path=Reference.CPU_AUTOMATIC), n)
+    out["yuv420p16_debicubic"] = entry
+    port = rate(yuv_frame(orc, oplans), 1)
+    out["restatement_scalar_1_yuv420p16"] = port
+    out["restatement_over_reference_scalar"] = round(port / entry["scalar_1"], 3)
+    return out
 
 
 def batched_scan(torch, dh, dev, frame, repeats):
@@ -383,9 +496,15 @@ def main():
     ap.add_argument("--no-serve", action="store_true",
                     help="skip the served Poisson request stream")
     ap.add_argument("--serve-seconds", type=float, default=4.0)
+    ap.add_argument("--cpu-calibration", action="store_true",
+                    help="CPU only: time the reference engine's scalar and automatic paths at "
+                         "1..N threads and the oracle restatement, print JSON, exit")
     ap.add_argument("--calibration-copy", action="store_true",
                     help="run one 256 MiB device copy before the timed region (counter calibration)")
     args = ap.parse_args()
+    if args.cpu_calibration:
+        print(json.dumps(cpu_calibration()), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               f"(the path shards by frame with no collectives)", file=sys.stderr)
@@ -520,13 +639,17 @@ def main():
         dom, dom_bytes, dom_s = family(False), v_bytes, v_avg
     achieved = dom_bytes / dom_s / 1e9
     traffic, traffic_source = measured_traffic(dom)
+    frame_gbps = FRAME_BYTES_YUV16 * fps / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
                 "traffic_source": traffic_source,
                 "kernel": dom, "algorithmic_bytes_per_launch": dom_bytes,
                 "avg_launch_ms": round(dom_s * 1e3, 4),
-                "luma_rows_ms": round(h_avg * 1e3, 4), "luma_columns_ms": round(v_avg * 1e3, 4)}
-    frame_gbps = FRAME_BYTES_YUV16 * fps / 1e9
+                "luma_rows_ms": round(h_avg * 1e3, 4), "luma_columns_ms": round(v_avg * 1e3, 4),
+                # The whole frame (all passes of all planes, one call per step):
+                # algorithmic frame bytes x frames/s.
+                "frame_achieved": round(frame_gbps, 1),
+                "frame_frac": round(frame_gbps / HBM_PEAK_GBPS, 4)}
 
     extra = {"frame_hbm": {"algorithmic_bytes_per_frame": FRAME_BYTES_YUV16,
                            "achieved_GBps": round(frame_gbps, 1),
