@@ -226,6 +226,29 @@ int dsmvc_hip_inverse_2d_batch(dsmvc_hip_executor *executor, const dsmvc_hip_pla
                                ptrdiff_t output_row_stride, ptrdiff_t output_frame_stride,
                                int32_t frames, const dsmvc_hip_integer_conversion *conversion,
                                void *stream, uint32_t flags);
+/* No counterpart in the reference.  dsmvc_hip_inverse_2d_batch for callers
+ * that cannot allocate device memory during a call (code under HIP graph
+ * capture, tight serving loops, hosts that forbid allocation): it calls no
+ * hipMalloc, hipFree or allocating library routine.  Everything beyond the
+ * caller's input and output -- the intermediate, the sequential kernels'
+ * forward sweeps, the job records, the Float64 flag -- lives in a fixed
+ * 64 MiB device array per GPU, held until the process ends; a request that
+ * needs more (about a 4K F32 frame on the sequential route) fails with
+ * DSMVC_HIP_INVALID_ARGUMENT.  Both plans must have been prepared on this
+ * executor first (dsmvc_hip_executor_prepare); runs the windowed kernels
+ * where the prepared tables allow, else the sequential ones (always in EXACT
+ * mode), without autotuning; REFINED executors are refused; the Float64
+ * route needs a synchronous call.  All calls on one GPU share the array:
+ * calls that overlap on the device (other streams, other executors) race. */
+int dsmvc_hip_inverse_2d_batch_noalloc(dsmvc_hip_executor *executor,
+                                       const dsmvc_hip_plan *horizontal,
+                                       const dsmvc_hip_plan *vertical, int32_t sample_kind,
+                                       const void *input, ptrdiff_t input_row_stride,
+                                       ptrdiff_t input_frame_stride, void *output,
+                                       ptrdiff_t output_row_stride,
+                                       ptrdiff_t output_frame_stride, int32_t frames,
+                                       const dsmvc_hip_integer_conversion *conversion,
+                                       void *stream, uint32_t flags);
 /* The two halves of the 2D route as separate batched calls (Float32 plans):
  * the reference's GPU pipeline runs the horizontal stage into an intermediate
  * and the vertical stage out of it (src/cuda/cuda_executor.cpp:2866-2949).

@@ -416,6 +416,27 @@ int dsmvc_hip_inverse_2d_batch(dsmvc_hip_executor *executor, const dsmvc_hip_pla
     });
 }
 
+int dsmvc_hip_inverse_2d_batch_noalloc(dsmvc_hip_executor *executor,
+                                       const dsmvc_hip_plan *horizontal,
+                                       const dsmvc_hip_plan *vertical, int32_t sample_kind,
+                                       const void *input, ptrdiff_t input_row_stride,
+                                       ptrdiff_t input_frame_stride, void *output,
+                                       ptrdiff_t output_row_stride,
+                                       ptrdiff_t output_frame_stride, int32_t frames,
+                                       const dsmvc_hip_integer_conversion *conversion,
+                                       void *stream, uint32_t flags) {
+    return guarded([&] {
+        require(sample_kind >= 0 && sample_kind <= 2, "dsmvc: sample kind is invalid");
+        dsmvc::IntegerConversion conv;
+        if (conversion) conv = to_conversion(*conversion);
+        const dsmvc::PlaneBatch batch{input, input_row_stride, input_frame_stride, output,
+                                      output_row_stride, output_frame_stride, frames};
+        exec_of(executor).inverse_2d_batch_noalloc(
+            plan_of(horizontal), plan_of(vertical), static_cast<dsmvc::SampleKind>(sample_kind),
+            batch, conversion ? &conv : nullptr, stream_of(stream), is_async(flags));
+    });
+}
+
 int dsmvc_hip_inverse_rows_batch(dsmvc_hip_executor *executor, const dsmvc_hip_plan *plan,
                                  int32_t input_kind, const void *input,
                                  ptrdiff_t input_row_stride, ptrdiff_t input_frame_stride,

@@ -595,6 +595,18 @@ const DevWindow *DevicePlan::window(bool rows) {
     return &image.dev;
 }
 
+const DevWindow *DevicePlan::built_window(bool rows) {
+    std::lock_guard lock(mutex_);
+    const WindowImage &image = window_image_;
+    if (!image.built || !image.ok || (rows && !image.rows_feasible)) return nullptr;
+    return &image.dev;
+}
+
+const DevWindow64 *DevicePlan::built_window64() {
+    std::lock_guard lock(mutex_);
+    return window64_image_.built && window64_image_.ok ? &window64_image_.dev : nullptr;
+}
+
 const DevWindow *DevicePlan::solve_window(bool rows) {
     std::lock_guard lock(mutex_);
     WindowImage &image = solve_image_;

@@ -118,6 +118,10 @@ public:
     // data.  Built lazily, cached.
     const DevRefine *refine();
     [[nodiscard]] std::size_t device_bytes() const { return bytes_; }
+    // The windowed tables only if an earlier window() / window64() call
+    // built them; never builds or allocates (nullptr otherwise).
+    const DevWindow *built_window(bool rows);
+    const DevWindow64 *built_window64();
 
 private:
     struct ChunkImage {

@@ -102,6 +102,7 @@ HipExecutor::HipExecutor(int device, SolveMode mode) : device_(device), mode_(mo
     if (hipGetDeviceProperties(&prop, device_) == hipSuccess && prop.multiProcessorCount > 0)
         compute_units_ = prop.multiProcessorCount;
     if (const char *env = std::getenv("DSMVC_HIP_AUTOTUNE")) autotune_ = std::string(env) != "0";
+    noalloc_scratch_ = noalloc_setup(device_);
 }
 
 HipExecutor::~HipExecutor() {
@@ -134,7 +135,13 @@ void HipExecutor::prepare(const std::shared_ptr<const AxisPlan> &plan) {
     if (!plan || !plan->valid())
         throw std::invalid_argument("dsmvc: cannot prepare an invalid HIP axis plan");
     std::lock_guard lock(mutex_);
-    (void)device_plan(*plan);
+    hip::DevicePlan &dp = device_plan(*plan);
+    // The windowed tables too, so that a prepared plan runs without any
+    // further device allocation (inverse_2d_batch_noalloc relies on this).
+    if (mode_ == SolveMode::automatic || mode_ == SolveMode::windowed) {
+        (void)dp.window(false);
+        if (plan->requires_float64()) (void)dp.window64();
+    }
 }
 
 void HipExecutor::seal() {
@@ -156,6 +163,13 @@ hip::DevicePlan &HipExecutor::device_plan(const AxisPlan &plan) {
     auto &ref = *image;
     plans_[&plan] = std::move(image);
     return ref;
+}
+
+hip::DevicePlan *HipExecutor::prepared_plan(const AxisPlan &plan) {
+    // Looks the plan's device image up without building one.
+    auto it = plans_.find(&plan);
+    if (it == plans_.end() || !same_plan(it->second->host(), plan)) return nullptr;
+    return it->second.get();
 }
 
 WindowTables &HipExecutor::stream_tables(hipStream_t stream) {

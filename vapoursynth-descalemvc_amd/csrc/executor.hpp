@@ -144,6 +144,23 @@ public:
                                std::int32_t column_count, const IntegerConversion *conversion,
                                hipStream_t stream, bool async);
 
+    // inverse_2d_batch without any allocation, for callers that cannot
+    // allocate (code under HIP graph capture, tight serving loops): every
+    // buffer beyond the caller's input and output lives in a fixed device
+    // array of kNoallocScratchBytes per GPU (noalloc.cpp).  Both plans must
+    // have been prepared on this executor (prepare builds their device
+    // images and windowed tables).  Routes: the windowed kernels where the
+    // prepared tables allow, else the sequential kernels (always in exact
+    // mode); no autotuning, and refined mode is refused.  A request whose
+    // intermediate and forward sweeps exceed the array is refused.  The
+    // Float64 route needs a synchronous call (it reports non-finite results
+    // on return).  Calls on one GPU share the array: calls that overlap on
+    // the device (different streams) race.
+    void inverse_2d_batch_noalloc(const AxisPlan &horizontal, const AxisPlan &vertical,
+                                  SampleKind kind, const PlaneBatch &batch,
+                                  const IntegerConversion *conversion, hipStream_t stream,
+                                  bool async);
+
     // Waits for `stream` and reports a deferred Float64 non-finite failure.
     void synchronize(hipStream_t stream);
     // Waits for `stream` and frees the scratch and job tables kept for it.
@@ -157,6 +174,12 @@ public:
     hip::DevicePlan &plan_image(const AxisPlan &plan);
 
 private:
+    // The device address of this GPU's no-allocation scratch array,
+    // resolved once per device when an executor is created (noalloc.cpp).
+    static void *noalloc_setup(int device);
+    hip::DevicePlan *prepared_plan(const AxisPlan &plan);
+    void *noalloc_scratch_ = nullptr;
+
     struct Scratch {
         void *buffer = nullptr;
         std::size_t bytes = 0;

@@ -33,6 +33,7 @@ EXPORTS = (
     "dsmvc_hip_server_submit", "dsmvc_hip_server_wait", "dsmvc_hip_server_poll",
     "dsmvc_hip_server_release", "dsmvc_hip_server_stats_get", "dsmvc_hip_server_destroy",
     "dsmvc_hip_executor_release_stream", "dsmvc_hip_synchronize",
+    "dsmvc_hip_inverse_2d_batch_noalloc",
 )
 BATCH_GRAPH = 2
 
@@ -238,6 +239,8 @@ def load():
                                        vp, u32]
     lib.dsmvc_hip_inverse_2d_batch.argtypes = [vp, vp, vp, i32, vp, pd, pd, vp, pd, pd, i32,
                                                ctypes.POINTER(ConversionC), vp, u32]
+    lib.dsmvc_hip_inverse_2d_batch_noalloc.argtypes = [vp, vp, vp, i32, vp, pd, pd, vp, pd, pd,
+                                                       i32, ctypes.POINTER(ConversionC), vp, u32]
     lib.dsmvc_hip_inverse_rows_batch.argtypes = [vp, vp, i32, vp, pd, pd, vp, pd, pd, i32, i32,
                                                  ctypes.POINTER(ConversionC), vp, u32]
     lib.dsmvc_hip_inverse_columns_batch.argtypes = [vp, vp, vp, pd, pd, i32, vp, pd, pd, i32,
