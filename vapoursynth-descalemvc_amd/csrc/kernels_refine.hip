@@ -96,8 +96,9 @@ __global__ void __launch_bounds__(256) refine_correct(RefineArgs a) {
 
 // The residual element by element.  r is laid out [frame][vector][element]
 // for a rows pass and [frame][element][vector] for a columns pass.  The sums
-// run in FP64 over the tap-major FP64 tables (FP64 arithmetic runs at the F32
-// rate on this GPU), and r is rounded to F32 once.  Indices past a row's
+// run in FP64 over the tap-major FP64 tables (the FP64 vector rate of this GPU,
+// 78.6 TFLOP/s, is half its FP32 rate, ample for a pass bound by memory), and
+// r is rounded to F32 once.  Indices past a row's
 // samples are clamped; their weights are 0, so the terms are exactly 0 (the
 // inputs are finite).
 //
