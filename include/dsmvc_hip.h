@@ -133,6 +133,31 @@ int dsmvc_hip_plan_arrays(const dsmvc_hip_plan *plan, uint32_t *offsets, int32_t
                           float *inverse_diagonal, double *weights_f64,
                           double *normal_bands_f64, double *ldlt_bands_f64,
                           double *inverse_diagonal_f64);
+/* The arrays of an AxisPlan made elsewhere (reference include/dsmvc/engine.hpp:
+ * 90-107), sized as in dsmvc_hip_plan_arrays: offsets n+1, indices/weights
+ * nnz, lower_ld/upper_l W*n, inverse_diagonal n; the four F64 arrays (nnz,
+ * (W+1)*n, (W+1)*n, n) are read only when info->requires_float64. */
+typedef struct dsmvc_hip_plan_data {
+    const uint32_t *offsets;
+    const int32_t *indices;
+    const float *weights;
+    const float *lower_ld;
+    const float *upper_l;
+    const float *inverse_diagonal;
+    const double *weights_f64;
+    const double *normal_bands_f64;
+    const double *ldlt_bands_f64;
+    const double *inverse_diagonal_f64;
+} dsmvc_hip_plan_data;
+/* Takes a plan built by the reference's own planner (or any caller) instead
+ * of building one: the reference executors accept any valid
+ * `const AxisPlan &` (include/dsmvc/engine.hpp:455-490) and copy borrowed
+ * plans per call (src/cpu_executor.cpp:553-558).  The scalar fields come from
+ * `info` (its nnz and requires_float64 size the arrays); the arrays are
+ * copied.  The plan must pass the reference's AxisPlan::valid() rules
+ * (src/axis_plan.cpp:1252-1341), else DSMVC_HIP_INVALID_ARGUMENT. */
+int dsmvc_hip_plan_import(const dsmvc_hip_plan_info *info, const dsmvc_hip_plan_data *data,
+                          dsmvc_hip_plan **plan);
 /* dsmvc::planner_cache_stats / clear_planner_caches (engine.hpp:179-180). */
 /* No counterpart in the reference: reports how the windowed (independent
  * chunk) kernels would partition this plan, computed on the host. */
@@ -269,6 +294,48 @@ int dsmvc_hip_inverse_columns_batch(dsmvc_hip_executor *executor, const dsmvc_hi
                                     ptrdiff_t output_frame_stride, int32_t columns,
                                     int32_t frames, const dsmvc_hip_integer_conversion *conversion,
                                     void *stream, uint32_t flags);
+
+/* ---- host frames ----------------------------------------------------------
+ * The reference's executors take host buffers (include/dsmvc/engine.hpp:
+ * 455-490); its CUDA executor stages them through pinned memory on a stream
+ * of its own (src/cuda/cuda_executor.cpp:2798-2822).  These entries take HOST
+ * pointers and do the same on one of the executor's execution slots: the
+ * input is packed into the slot's pinned buffer, copied to the device,
+ * solved, copied back and written to the caller's rows (logical width only).
+ * Synchronous; concurrent callers overlap (up to 8 slots).  Same arguments
+ * and errors as the device-pointer entries.
+ * Executor::inverse_rows / inverse_columns (engine.hpp:468-478). */
+int dsmvc_hip_inverse_rows_host(dsmvc_hip_executor *executor, const dsmvc_hip_plan *plan,
+                                const float *input, ptrdiff_t input_row_stride, float *output,
+                                ptrdiff_t output_row_stride, int32_t row_count);
+int dsmvc_hip_inverse_columns_host(dsmvc_hip_executor *executor, const dsmvc_hip_plan *plan,
+                                   const float *input, ptrdiff_t input_row_stride,
+                                   float *output, ptrdiff_t output_row_stride,
+                                   int32_t column_count);
+/* Executor::inverse_2d / inverse_2d_u8 / inverse_2d_u16 (engine.hpp:480-498)
+ * by sample kind, for `frames` planes (frame strides read when frames > 1). */
+int dsmvc_hip_inverse_2d_host(dsmvc_hip_executor *executor, const dsmvc_hip_plan *horizontal,
+                              const dsmvc_hip_plan *vertical, int32_t sample_kind,
+                              const void *input, ptrdiff_t input_row_stride,
+                              ptrdiff_t input_frame_stride, void *output,
+                              ptrdiff_t output_row_stride, ptrdiff_t output_frame_stride,
+                              int32_t frames, const dsmvc_hip_integer_conversion *conversion);
+
+/* ---- device plan cache ----------------------------------------------------
+ * The executor keeps each plan's device image (and the tables built for it)
+ * in an LRU keyed by the plan's build serial -- one per planner request key
+ * (kernel, geometry, padding, precision) -- bounded by bytes: 1024 MiB by
+ * default, DSMVC_HIP_PLAN_CACHE_MB, or this call.  The reference bounds its
+ * packed-plan cache the same way (src/cuda/cuda_executor.cpp:219-235,
+ * 1936-2020).  Images in use by a call, a batch or a served request are
+ * never evicted. */
+typedef struct dsmvc_hip_plan_cache_stats {
+    uint64_t entries, bytes, limit;
+    uint64_t hits, misses, evictions;
+} dsmvc_hip_plan_cache_stats;
+int dsmvc_hip_executor_set_plan_cache(dsmvc_hip_executor *executor, int64_t limit_bytes);
+int dsmvc_hip_executor_plan_cache_stats(dsmvc_hip_executor *executor,
+                                        dsmvc_hip_plan_cache_stats *stats);
 
 /* Waits for `stream`; reports a deferred Float64 non-finite failure. */
 /* One plane of a multi-plane call (dsmvc_hip_inverse_2d_planes). */

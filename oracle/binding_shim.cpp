@@ -1,0 +1,157 @@
+// Test infrastructure only: drives the reference-side binding
+// (vapoursynth-descalemvc_amd/reference_binding/hip_executor.cpp) the way the
+// reference engine would, with plans from the reference's own planner and
+// host buffers, so that tests can check the drop-in end to end.  Built by
+// oracle/Makefile into oracle/_ref/ against the reference's headers and its
+// engine library (oracle/_ref/libdsmvc_ref.so) and against libdsmvc_hip.so.
+//
+// Interfaces used (reference paths):
+//   dsmvc::get_or_build_axis_plan   include/dsmvc/engine.hpp:177
+//   dsmvc::AxisPlan, IntegerConversion  include/dsmvc/engine.hpp:82-173
+#include <dsmvc/engine.hpp>
+
+#include "hip_executor.hpp"
+
+#include <cstdint>
+#include <exception>
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+namespace {
+
+thread_local std::string last_error;
+
+template <class F>
+int guarded(F &&f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::invalid_argument &e) {
+        last_error = e.what();
+        return 1;
+    } catch (const std::exception &e) {
+        last_error = e.what();
+        return 2;
+    }
+}
+
+} // namespace
+
+extern "C" {
+
+// Same layout as ref_shim.cpp's request (tests/support.py Request).
+struct bnd_request {
+    int32_t source_size;
+    int32_t destination_size;
+    double active_length;
+    double shift;
+    int32_t kernel;
+    int32_t taps;
+    double b;
+    double c;
+    double blur;
+    int32_t border;
+    int32_t f64_mode;
+};
+
+struct bnd_conversion {
+    float input_offset;
+    float input_scale;
+    float output_scale;
+    float output_offset;
+    uint32_t output_maximum;
+};
+
+}
+
+namespace {
+
+std::shared_ptr<const dsmvc::AxisPlan> reference_plan(const bnd_request *r) {
+    dsmvc::AxisRequest q;
+    q.source_size = r->source_size;
+    q.destination_size = r->destination_size;
+    q.active_length = r->active_length;
+    q.shift = r->shift;
+    q.kernel.kind = static_cast<dsmvc::KernelKind>(r->kernel);
+    q.kernel.taps = r->taps;
+    q.kernel.b = r->b;
+    q.kernel.c = r->c;
+    q.kernel.blur = r->blur;
+    q.border = static_cast<dsmvc::BorderMode>(r->border);
+    q.f64_mode = static_cast<dsmvc::F64Mode>(r->f64_mode);
+    return dsmvc::get_or_build_axis_plan(q);
+}
+
+dsmvc::hip_detail::HipExecutor &executor(void *ex) {
+    return *static_cast<dsmvc::hip_detail::HipExecutor *>(ex);
+}
+
+} // namespace
+
+extern "C" {
+
+const char *bnd_last_error(void) { return last_error.c_str(); }
+
+int bnd_available(void) { return dsmvc::hip_detail::backend_available() ? 1 : 0; }
+
+int bnd_create(int32_t solve_mode, void **out) {
+    return guarded([&] { *out = new dsmvc::hip_detail::HipExecutor(solve_mode); });
+}
+
+void bnd_free(void *ex) { delete static_cast<dsmvc::hip_detail::HipExecutor *>(ex); }
+
+// Registers the reference planner's plan for `r` (prepare), as the plugin
+// does for its filter's plans before the first frame.
+int bnd_prepare(void *ex, const bnd_request *r) {
+    return guarded([&] { executor(ex).prepare(reference_plan(r)); });
+}
+
+int bnd_seal(void *ex) {
+    return guarded([&] { executor(ex).seal(); });
+}
+
+// kind 0: float32 (Executor::inverse_2d), 1: u8, 2: u16 (inverse_2d_u8/_u16).
+int bnd_inverse_2d(void *ex, const bnd_request *h, const bnd_request *v, int32_t kind,
+                   const void *in, ptrdiff_t in_stride, void *out, ptrdiff_t out_stride,
+                   const bnd_conversion *conversion) {
+    return guarded([&] {
+        const auto hp = reference_plan(h), vp = reference_plan(v);
+        dsmvc::IntegerConversion c;
+        if (conversion) {
+            c.input_offset = conversion->input_offset;
+            c.input_scale = conversion->input_scale;
+            c.output_scale = conversion->output_scale;
+            c.output_offset = conversion->output_offset;
+            c.output_maximum = conversion->output_maximum;
+        }
+        auto &e = executor(ex);
+        if (kind == 0)
+            e.inverse_2d(*hp, *vp, static_cast<const float *>(in), in_stride,
+                         static_cast<float *>(out), out_stride, nullptr);
+        else if (kind == 1)
+            e.inverse_2d_u8(*hp, *vp, static_cast<const uint8_t *>(in), in_stride,
+                            static_cast<uint8_t *>(out), out_stride, c, nullptr);
+        else
+            e.inverse_2d_u16(*hp, *vp, static_cast<const uint16_t *>(in), in_stride,
+                             static_cast<uint16_t *>(out), out_stride, c, nullptr);
+    });
+}
+
+int bnd_inverse_rows(void *ex, const bnd_request *r, const float *in, ptrdiff_t in_stride,
+                     float *out, ptrdiff_t out_stride, int32_t rows) {
+    return guarded([&] {
+        const auto p = reference_plan(r);
+        executor(ex).inverse_rows(*p, in, in_stride, out, out_stride, rows, nullptr);
+    });
+}
+
+int bnd_inverse_columns(void *ex, const bnd_request *r, const float *in, ptrdiff_t in_stride,
+                        float *out, ptrdiff_t out_stride, int32_t columns) {
+    return guarded([&] {
+        const auto p = reference_plan(r);
+        executor(ex).inverse_columns(*p, in, in_stride, out, out_stride, columns, nullptr);
+    });
+}
+
+} // extern "C"

@@ -165,6 +165,9 @@ HipBatch::HipBatch(HipExecutor &executor, std::vector<BatchJob> jobs, bool graph
     // Intermediate layout of the table jobs: [frame][source row][pitch(n_h)].
     std::vector<std::size_t> offset(jobs_.size(), 0);
     std::vector<const hip::DevWindow *> wh(jobs_.size(), nullptr), wv(jobs_.size(), nullptr);
+    // Device images of the table jobs, held (pinned in the executor's plan
+    // cache) for the batch's lifetime: the tables point into them.
+    std::vector<std::shared_ptr<hip::DevicePlan>> ih(jobs_.size()), iv(jobs_.size());
     if (parallel) {
         // Device images and window tables of every job, built on host
         // threads (a scan brings hundreds of new plans).  Each task writes
@@ -177,8 +180,10 @@ HipBatch::HipBatch(HipExecutor &executor, std::vector<BatchJob> jobs, bool graph
                 const AxisPlan &h = *jobs_[k].horizontal, &v = *jobs_[k].vertical;
                 if (h.requires_float64() || v.requires_float64()) continue;
                 try {
-                    wh[k] = executor_.plan_image(h).window(true);
-                    wv[k] = executor_.plan_image(v).window(false);
+                    ih[k] = executor_.plan_image(h);
+                    iv[k] = executor_.plan_image(v);
+                    wh[k] = ih[k]->window(true);
+                    wv[k] = iv[k]->window(false);
                 } catch (...) {
                     failed[k] = std::current_exception();
                 }
@@ -211,6 +216,8 @@ HipBatch::HipBatch(HipExecutor &executor, std::vector<BatchJob> jobs, bool graph
             fallback_.push_back(k);
             continue;
         }
+        images_.push_back(std::move(ih[k]));
+        images_.push_back(std::move(iv[k]));
         offset[k] = total;
         total += align256(static_cast<std::size_t>(pitch_of(h.destination_size)) * v.source_size
                           * static_cast<std::size_t>(j.frames) * sizeof(float));

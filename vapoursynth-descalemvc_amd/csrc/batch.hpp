@@ -24,6 +24,9 @@
 namespace dsmvc {
 
 class HipExecutor;
+namespace hip {
+class DevicePlan;
+}
 
 // One 2D job: `frames` F32 planes of the horizontal plan's source width and
 // the vertical plan's source height, into planes of the destination size.
@@ -117,6 +120,7 @@ private:
     HipExecutor &executor_;
     std::vector<BatchJob> jobs_;
     std::vector<std::size_t> fallback_;
+    std::vector<std::shared_ptr<hip::DevicePlan>> images_;
     WindowTables tables_;
     void *intermediate_ = nullptr;
     hipGraph_t graph_ = nullptr;

@@ -180,6 +180,55 @@ int dsmvc_hip_plan_build_many(const dsmvc_hip_axis_request *requests, int32_t co
 
 void dsmvc_hip_plan_release(dsmvc_hip_plan *plan) { delete plan; }
 
+int dsmvc_hip_plan_import(const dsmvc_hip_plan_info *info, const dsmvc_hip_plan_data *data,
+                          dsmvc_hip_plan **plan) {
+    return guarded([&] {
+        require(info && data && plan, "dsmvc: null argument");
+        require(info->source_size > 0 && info->destination_size > 0 && info->nnz >= 0
+                    && info->half_bandwidth >= 0 && info->half_bandwidth < info->destination_size,
+                "dsmvc: imported axis plan is invalid");
+        const auto n = static_cast<std::size_t>(info->destination_size);
+        const auto nnz = static_cast<std::size_t>(info->nnz);
+        const auto factors = static_cast<std::size_t>(info->half_bandwidth) * n;
+        const auto bands = factors + n;
+        const bool f64 = info->requires_float64 != 0;
+        require(data->offsets && data->inverse_diagonal && (nnz == 0 || (data->indices && data->weights))
+                    && (factors == 0 || (data->lower_ld && data->upper_l)),
+                "dsmvc: imported axis plan is missing an array");
+        require(!f64 || (data->normal_bands_f64 && data->ldlt_bands_f64 && data->inverse_diagonal_f64
+                         && (nnz == 0 || data->weights_f64)),
+                "dsmvc: imported axis plan is missing an FP64 array");
+        dsmvc::AxisPlan p;
+        p.source_size = info->source_size;
+        p.destination_size = info->destination_size;
+        p.support = info->support;
+        p.half_bandwidth = info->half_bandwidth;
+        p.active_length = info->active_length;
+        p.shift = info->shift;
+        p.normal_rcond = info->normal_rcond;
+        p.normal_inf_norm = info->normal_inf_norm;
+        p.transpose_offsets.assign(data->offsets, data->offsets + n + 1);
+        if (nnz) {
+            p.transpose_indices.assign(data->indices, data->indices + nnz);
+            p.transpose_weights.assign(data->weights, data->weights + nnz);
+        }
+        if (factors) {
+            p.lower_ld.assign(data->lower_ld, data->lower_ld + factors);
+            p.upper_l.assign(data->upper_l, data->upper_l + factors);
+        }
+        p.inverse_diagonal.assign(data->inverse_diagonal, data->inverse_diagonal + n);
+        if (f64) {
+            if (nnz) p.transpose_weights_f64.assign(data->weights_f64, data->weights_f64 + nnz);
+            p.normal_bands_f64.assign(data->normal_bands_f64, data->normal_bands_f64 + bands);
+            p.ldlt_bands_f64.assign(data->ldlt_bands_f64, data->ldlt_bands_f64 + bands);
+            p.inverse_diagonal_f64.assign(data->inverse_diagonal_f64, data->inverse_diagonal_f64 + n);
+        }
+        auto handle = std::make_unique<dsmvc_hip_plan>();
+        handle->plan = dsmvc::import_axis_plan(std::move(p));
+        *plan = handle.release();
+    });
+}
+
 int dsmvc_hip_plan_info_get(const dsmvc_hip_plan *plan, dsmvc_hip_plan_info *info) {
     return guarded([&] {
         const auto &p = plan_of(plan);
@@ -434,6 +483,64 @@ int dsmvc_hip_inverse_2d_batch_noalloc(dsmvc_hip_executor *executor,
         exec_of(executor).inverse_2d_batch_noalloc(
             plan_of(horizontal), plan_of(vertical), static_cast<dsmvc::SampleKind>(sample_kind),
             batch, conversion ? &conv : nullptr, stream_of(stream), is_async(flags));
+    });
+}
+
+int dsmvc_hip_inverse_rows_host(dsmvc_hip_executor *executor, const dsmvc_hip_plan *plan,
+                                const float *input, ptrdiff_t input_row_stride, float *output,
+                                ptrdiff_t output_row_stride, int32_t row_count) {
+    return guarded([&] {
+        exec_of(executor).inverse_rows_host(plan_of(plan), input, input_row_stride, output,
+                                            output_row_stride, row_count);
+    });
+}
+
+int dsmvc_hip_inverse_columns_host(dsmvc_hip_executor *executor, const dsmvc_hip_plan *plan,
+                                   const float *input, ptrdiff_t input_row_stride,
+                                   float *output, ptrdiff_t output_row_stride,
+                                   int32_t column_count) {
+    return guarded([&] {
+        exec_of(executor).inverse_columns_host(plan_of(plan), input, input_row_stride, output,
+                                               output_row_stride, column_count);
+    });
+}
+
+int dsmvc_hip_inverse_2d_host(dsmvc_hip_executor *executor, const dsmvc_hip_plan *horizontal,
+                              const dsmvc_hip_plan *vertical, int32_t sample_kind,
+                              const void *input, ptrdiff_t input_row_stride,
+                              ptrdiff_t input_frame_stride, void *output,
+                              ptrdiff_t output_row_stride, ptrdiff_t output_frame_stride,
+                              int32_t frames, const dsmvc_hip_integer_conversion *conversion) {
+    return guarded([&] {
+        require(sample_kind >= 0 && sample_kind <= 2, "dsmvc: sample kind is invalid");
+        dsmvc::IntegerConversion conv;
+        if (conversion) conv = to_conversion(*conversion);
+        const dsmvc::PlaneBatch batch{input, input_row_stride, input_frame_stride, output,
+                                      output_row_stride, output_frame_stride, frames};
+        exec_of(executor).inverse_2d_host(plan_of(horizontal), plan_of(vertical),
+                                          static_cast<dsmvc::SampleKind>(sample_kind), batch,
+                                          conversion ? &conv : nullptr);
+    });
+}
+
+int dsmvc_hip_executor_set_plan_cache(dsmvc_hip_executor *executor, int64_t limit_bytes) {
+    return guarded([&] {
+        require(limit_bytes > 0, "dsmvc: the plan cache limit must be positive");
+        exec_of(executor).set_plan_cache_limit(static_cast<std::size_t>(limit_bytes));
+    });
+}
+
+int dsmvc_hip_executor_plan_cache_stats(dsmvc_hip_executor *executor,
+                                        dsmvc_hip_plan_cache_stats *stats) {
+    return guarded([&] {
+        require(stats, "dsmvc: null argument");
+        const auto s = exec_of(executor).plan_cache_stats();
+        stats->entries = s.entries;
+        stats->bytes = s.bytes;
+        stats->limit = s.limit;
+        stats->hits = s.hits;
+        stats->misses = s.misses;
+        stats->evictions = s.evictions;
     });
 }
 

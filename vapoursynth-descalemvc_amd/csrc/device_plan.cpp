@@ -458,6 +458,7 @@ DevicePlan::DevicePlan(const AxisPlan &plan, int device)
         memory_ = nullptr;
         check(copy, "device plan upload failed");
     }
+    total_bytes_ = bytes_;
     auto *base = static_cast<unsigned char *>(memory_);
     axis_.n = packed.n;
     axis_.m = packed.m;
@@ -507,6 +508,7 @@ const DevChunks *DevicePlan::chunks(int max_chunks, bool rows) {
             check(hipGetDevice(&current), "hipGetDevice failed");
             check(hipSetDevice(device_), "hipSetDevice failed");
             check(hipMalloc(&image.memory, total), "chunk table allocation failed");
+            image.bytes = total;
             if (const hipError_t e = hipMemcpy(image.memory, host.data(), total,
                                                hipMemcpyHostToDevice);
                 e != hipSuccess) {
@@ -528,6 +530,7 @@ const DevChunks *DevicePlan::chunks(int max_chunks, bool rows) {
             image.dev.blk_lo = reinterpret_cast<const std::int32_t *>(b + o_blk);
             image.ok = true;
             image.rows_feasible = cp.rows_feasible;
+            total_bytes_ += image.bytes;
         }
         it = chunk_images_.emplace(max_chunks, image).first;
     }
@@ -537,7 +540,7 @@ const DevChunks *DevicePlan::chunks(int max_chunks, bool rows) {
 
 // Uploads a window plan's tables as one allocation and fills `dev` (its
 // plan sizes from this plan).  Returns the allocation.
-void *DevicePlan::upload_window(const WindowPlan &wp, DevWindow &dev) {
+void *DevicePlan::upload_window(const WindowPlan &wp, DevWindow &dev, std::size_t &bytes) {
     const std::size_t o_fwd = 0;
     const std::size_t o_bwd = align256(o_fwd + wp.fwd.size() * 4);
     const std::size_t o_ts = align256(o_bwd + wp.bwd.size() * 4);
@@ -561,6 +564,7 @@ void *DevicePlan::upload_window(const WindowPlan &wp, DevWindow &dev) {
         check(e, "window table upload failed");
     }
     (void)hipSetDevice(current);
+    bytes = total;
     auto *b = static_cast<unsigned char *>(memory);
     dev.n = plan_->destination_size;
     dev.m = plan_->source_size;
@@ -585,7 +589,8 @@ const DevWindow *DevicePlan::window(bool rows) {
         // upload throws here and the next call tries again.
         WindowPlan wp;
         if (build_window_plan(*plan_, wp)) {
-            image.memory = upload_window(wp, image.dev);
+            image.memory = upload_window(wp, image.dev, image.bytes);
+            total_bytes_ += image.bytes;
             image.ok = true;
             image.rows_feasible = wp.rows_feasible;
         }
@@ -613,7 +618,8 @@ const DevWindow *DevicePlan::solve_window(bool rows) {
     if (!image.built) {
         WindowPlan wp;
         if (build_solve_window_plan(*plan_, wp)) {
-            image.memory = upload_window(wp, image.dev);
+            image.memory = upload_window(wp, image.dev, image.bytes);
+            total_bytes_ += image.bytes;
             image.dev.m = plan_->destination_size;
             image.ok = true;
             image.rows_feasible = wp.rows_feasible;
@@ -652,6 +658,8 @@ const DevWindow64 *DevicePlan::window64() {
             }
             (void)hipSetDevice(current);
             image.memory = memory;
+            image.bytes = total;
+            total_bytes_ += total;
             auto *b = static_cast<unsigned char *>(image.memory);
             image.dev.n = plan_->destination_size;
             image.dev.m = plan_->source_size;
@@ -753,6 +761,8 @@ const DevRefine *DevicePlan::refine() {
         }
         (void)hipSetDevice(current);
         refine_memory_ = memory;
+        refine_bytes_ = total;
+        total_bytes_ += total;
         refine_built_ = true;
         auto *b = static_cast<unsigned char *>(refine_memory_);
         refine_.n = n;

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <map>
 #include <memory>
@@ -118,6 +119,8 @@ public:
     // data.  Built lazily, cached.
     const DevRefine *refine();
     [[nodiscard]] std::size_t device_bytes() const { return bytes_; }
+    // Device bytes of the image and of every table built for it so far.
+    [[nodiscard]] std::size_t total_bytes() const { return total_bytes_.load(std::memory_order_relaxed); }
     // The windowed tables only if an earlier window() / window64() call
     // built them; never builds or allocates (nullptr otherwise).
     const DevWindow *built_window(bool rows);
@@ -127,6 +130,7 @@ private:
     struct ChunkImage {
         DevChunks dev{};
         void *memory = nullptr;
+        std::size_t bytes = 0;
         bool ok = false;
         bool rows_feasible = false;
     };
@@ -141,22 +145,26 @@ private:
     struct WindowImage {
         DevWindow dev{};
         void *memory = nullptr;
+        std::size_t bytes = 0;
         bool built = false;
         bool ok = false;
         bool rows_feasible = false;
     };
-    void *upload_window(const WindowPlan &wp, DevWindow &dev);
+    void *upload_window(const WindowPlan &wp, DevWindow &dev, std::size_t &bytes);
     WindowImage window_image_;
     WindowImage solve_image_;
     struct Window64Image {
         DevWindow64 dev{};
         void *memory = nullptr;
+        std::size_t bytes = 0;
         bool built = false;
         bool ok = false;
     };
     Window64Image window64_image_;
     DevRefine refine_{};
     void *refine_memory_ = nullptr;
+    std::size_t refine_bytes_ = 0;
+    std::atomic<std::size_t> total_bytes_{0};
     bool refine_built_ = false;
 };
 

@@ -109,6 +109,11 @@ struct PlannerCacheStats {
 // with the reference's messages.
 [[nodiscard]] AxisPlan build_axis_plan(const AxisRequest &request);
 [[nodiscard]] std::shared_ptr<const AxisPlan> get_or_build_axis_plan(const AxisRequest &request);
+// A plan from caller-supplied arrays (for example the reference planner's
+// AxisPlan), checked by valid(); throws std::invalid_argument when invalid.
+// The returned plan has a fresh serial.
+[[nodiscard]] std::shared_ptr<const AxisPlan> import_axis_plan(AxisPlan plan);
+[[nodiscard]] std::uint64_t next_plan_serial() noexcept;
 // The cached plan for `request` when it is built, else nullptr; never builds.
 [[nodiscard]] std::shared_ptr<const AxisPlan> find_axis_plan(const AxisRequest &request);
 [[nodiscard]] PlannerCacheStats planner_cache_stats();

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <limits>
 #include <stdexcept>
 #include <string>
@@ -102,6 +103,10 @@ HipExecutor::HipExecutor(int device, SolveMode mode) : device_(device), mode_(mo
     if (hipGetDeviceProperties(&prop, device_) == hipSuccess && prop.multiProcessorCount > 0)
         compute_units_ = prop.multiProcessorCount;
     if (const char *env = std::getenv("DSMVC_HIP_AUTOTUNE")) autotune_ = std::string(env) != "0";
+    if (const char *env = std::getenv("DSMVC_HIP_PLAN_CACHE_MB")) {
+        const long mb = std::strtol(env, nullptr, 10);
+        if (mb > 0) plan_cache_limit_ = static_cast<std::size_t>(mb) << 20;
+    }
     noalloc_scratch_ = noalloc_setup(device_);
 }
 
@@ -115,6 +120,12 @@ HipExecutor::~HipExecutor() {
     for (auto &entry : scratch_) {
         if (entry.second.buffer) (void)hipFree(entry.second.buffer);
         if (entry.second.flag) (void)hipFree(entry.second.flag);
+    }
+    scratch_.clear();
+    for (Slot &slot : slots_) {
+        if (slot.stream) (void)hipStreamDestroy(slot.stream);
+        if (slot.pinned) (void)hipHostFree(slot.pinned);
+        if (slot.device) (void)hipFree(slot.device);
     }
     plans_.clear();
 }
@@ -135,7 +146,8 @@ void HipExecutor::prepare(const std::shared_ptr<const AxisPlan> &plan) {
     if (!plan || !plan->valid())
         throw std::invalid_argument("dsmvc: cannot prepare an invalid HIP axis plan");
     std::lock_guard lock(mutex_);
-    hip::DevicePlan &dp = device_plan(*plan);
+    const auto image = device_plan(*plan);
+    hip::DevicePlan &dp = *image;
     // The windowed tables too, so that a prepared plan runs without any
     // further device allocation (inverse_2d_batch_noalloc relies on this).
     if (mode_ == SolveMode::automatic || mode_ == SolveMode::windowed) {
@@ -149,27 +161,182 @@ void HipExecutor::seal() {
     sealed_ = true;
 }
 
-hip::DevicePlan &HipExecutor::device_plan(const AxisPlan &plan) {
-    // Keyed by address; an entry whose plan no longer matches (the address
-    // was reused by another plan) is rebuilt.
-    auto it = plans_.find(&plan);
-    if (it != plans_.end() && same_plan(it->second->host(), plan)) return *it->second;
+std::uint64_t HipExecutor::plan_key(const AxisPlan &plan) {
+    // Built and imported plans carry a unique serial; other plans are keyed
+    // by address (top bit set) and checked by content on every hit.
+    if (plan.serial != 0) return plan.serial & ~(std::uint64_t{1} << 63);
+    return static_cast<std::uint64_t>(reinterpret_cast<std::uintptr_t>(&plan)) | (std::uint64_t{1} << 63);
+}
+
+std::shared_ptr<hip::DevicePlan> HipExecutor::device_plan(const AxisPlan &plan) {
+    const std::uint64_t key = plan_key(plan);
+    auto it = plans_.find(key);
+    if (it != plans_.end() && same_plan(it->second.image->host(), plan)) {
+        lru_.splice(lru_.begin(), lru_, it->second.lru);
+        ++plan_stats_.hits;
+        return it->second.image;
+    }
     if (it != plans_.end()) {
-        const hip::DevicePlan *stale = it->second.get();
+        // The address was reused by another plan: the old image goes (once
+        // the device has finished with it) and is rebuilt.
+        (void)hipDeviceSynchronize();
+        const hip::DevicePlan *stale = it->second.image.get();
         std::erase_if(tuned_, [stale](const auto &entry) { return entry.first.plan == stale; });
+        lru_.erase(it->second.lru);
+        plans_.erase(it);
     }
     check(hipSetDevice(device_), "hipSetDevice failed");
-    auto image = std::make_unique<hip::DevicePlan>(plan, device_);
-    auto &ref = *image;
-    plans_[&plan] = std::move(image);
-    return ref;
+    auto image = std::make_shared<hip::DevicePlan>(plan, device_);
+    lru_.push_front(key);
+    plans_[key] = PlanEntry{image, lru_.begin()};
+    ++plan_stats_.misses;
+    evict_plans();
+    return image;
+}
+
+// Drops least recently used images until the cache fits its limit, skipping
+// images someone holds (a call in progress, a batch, a served request).  The
+// device is synchronized first, so no queued kernel still reads an image
+// that is freed.
+void HipExecutor::evict_plans() {
+    std::size_t total = 0;
+    for (auto &entry : plans_) total += entry.second.image->total_bytes();
+    if (total <= plan_cache_limit_) return;
+    std::vector<std::uint64_t> victims;
+    for (auto k = lru_.rbegin(); k != lru_.rend() && total > plan_cache_limit_; ++k) {
+        const PlanEntry &e = plans_.at(*k);
+        if (e.image.use_count() > 1) continue;
+        total -= e.image->total_bytes();
+        victims.push_back(*k);
+    }
+    if (victims.empty()) return;
+    (void)hipDeviceSynchronize();
+    for (const std::uint64_t key : victims) {
+        auto it = plans_.find(key);
+        const hip::DevicePlan *gone = it->second.image.get();
+        std::erase_if(tuned_, [gone](const auto &entry) { return entry.first.plan == gone; });
+        lru_.erase(it->second.lru);
+        plans_.erase(it);
+        ++plan_stats_.evictions;
+    }
+}
+
+HipExecutor::PlanCacheStats HipExecutor::plan_cache_stats() {
+    std::lock_guard lock(mutex_);
+    PlanCacheStats out = plan_stats_;
+    out.entries = plans_.size();
+    out.bytes = 0;
+    for (auto &entry : plans_) out.bytes += entry.second.image->total_bytes();
+    out.limit = plan_cache_limit_;
+    return out;
+}
+
+void HipExecutor::set_plan_cache_limit(std::size_t bytes) {
+    if (bytes == 0) throw std::invalid_argument("dsmvc: the plan cache limit must be positive");
+    std::lock_guard lock(mutex_);
+    check(hipSetDevice(device_), "hipSetDevice failed");
+    plan_cache_limit_ = bytes;
+    evict_plans();
+}
+
+// ---------------------------------------------------------------------------
+// Execution slots
+// ---------------------------------------------------------------------------
+HipExecutor::Slot *HipExecutor::acquire_slot() {
+    std::unique_lock lock(slot_mutex_);
+    for (;;) {
+        for (Slot &slot : slots_) {
+            if (slot.busy) continue;
+            if (!slot.stream) {
+                // Default priority and flags: ordered with the null stream,
+                // as the synchronous calls it stands in for were.
+                check(hipSetDevice(device_), "hipSetDevice failed");
+                check(hipStreamCreate(&slot.stream), "slot stream creation failed");
+            }
+            slot.busy = true;
+            return &slot;
+        }
+        slot_cv_.wait(lock);
+    }
+}
+
+void HipExecutor::release_slot(Slot *slot) {
+    {
+        std::lock_guard lock(slot_mutex_);
+        slot->busy = false;
+    }
+    slot_cv_.notify_one();
+}
+
+void HipExecutor::slot_buffers(Slot &slot, std::size_t pinned, std::size_t device) {
+    // The slot is idle here (its last call waited for its stream).
+    check(hipSetDevice(device_), "hipSetDevice failed");
+    if (slot.pinned_bytes < pinned) {
+        if (slot.pinned) check(hipHostFree(slot.pinned), "staging release failed");
+        slot.pinned = nullptr;
+        slot.pinned_bytes = 0;
+        const std::size_t grown = align256(std::max(pinned, slot.pinned_bytes * 3 / 2));
+        check(hipHostMalloc(&slot.pinned, grown), "staging allocation failed");
+        slot.pinned_bytes = grown;
+    }
+    if (slot.device_bytes < device) {
+        if (slot.device) check(hipFree(slot.device), "staging release failed");
+        slot.device = nullptr;
+        slot.device_bytes = 0;
+        const std::size_t grown = align256(std::max(device, slot.device_bytes * 3 / 2));
+        check(hipMalloc(&slot.device, grown), "staging allocation failed");
+        slot.device_bytes = grown;
+    }
+}
+
+namespace {
+struct SlotGuard {
+    std::function<void()> release;
+    ~SlotGuard() {
+        if (release) release();
+    }
+};
+} // namespace
+
+template <class Enqueue>
+void HipExecutor::run_call(hipStream_t stream, bool async, const char *what, Enqueue &&enqueue) {
+    Slot *slot = (!stream && !async) ? acquire_slot() : nullptr;
+    SlotGuard guard;
+    if (slot) guard.release = [this, slot] { release_slot(slot); };
+    const hipStream_t s = slot ? slot->stream : stream;
+    Scratch *used = nullptr;
+    {
+        std::lock_guard lock(mutex_);
+        check(hipSetDevice(device_), "hipSetDevice failed");
+        used = &enqueue(s);
+        check(hipGetLastError(), "kernel launch failed");
+        // Tables built by this call count against the cache's limit too.
+        evict_plans();
+    }
+    if (async) return;
+    complete(s, *used, what);
+}
+
+// Waits for `stream` without the executor's lock and reports a Float64
+// non-finite result.  The scratch entry belongs to the stream; a stream is
+// used by one call at a time (slots are leased, caller streams are the
+// caller's to order).
+void HipExecutor::complete(hipStream_t stream, Scratch &s, const char *what) {
+    check(stream ? hipStreamSynchronize(stream) : hipDeviceSynchronize(), "execution failed");
+    if (s.flag_armed) {
+        int flag = 0;
+        check(hipMemcpy(&flag, s.flag, sizeof flag, hipMemcpyDeviceToHost), "flag readback failed");
+        s.flag_armed = false;
+        if (flag) throw std::runtime_error(what);
+    }
 }
 
 hip::DevicePlan *HipExecutor::prepared_plan(const AxisPlan &plan) {
     // Looks the plan's device image up without building one.
-    auto it = plans_.find(&plan);
-    if (it == plans_.end() || !same_plan(it->second->host(), plan)) return nullptr;
-    return it->second.get();
+    auto it = plans_.find(plan_key(plan));
+    if (it == plans_.end() || !same_plan(it->second.image->host(), plan)) return nullptr;
+    lru_.splice(lru_.begin(), lru_, it->second.lru);
+    return it->second.image.get();
 }
 
 WindowTables &HipExecutor::stream_tables(hipStream_t stream) {
@@ -178,7 +345,7 @@ WindowTables &HipExecutor::stream_tables(hipStream_t stream) {
     return *slot;
 }
 
-hip::DevicePlan &HipExecutor::plan_image(const AxisPlan &plan) {
+std::shared_ptr<hip::DevicePlan> HipExecutor::plan_image(const AxisPlan &plan) {
     if (!plan.valid()) throw std::invalid_argument("dsmvc: invalid HIP axis plan");
     std::lock_guard lock(mutex_);
     check(hipSetDevice(device_), "hipSetDevice failed");
@@ -319,9 +486,9 @@ int HipExecutor::choose_partition(hip::DevicePlan &dp, bool rows, const hip::Chu
 
 int HipExecutor::tuned_chunks(const AxisPlan &plan, bool rows, int kind) {
     std::lock_guard lock(mutex_);
-    auto it = plans_.find(&plan);
-    if (it == plans_.end() || !same_plan(it->second->host(), plan)) return -2;
-    auto t = tuned_.find(TuneKey{it->second.get(), rows, kind});
+    auto it = plans_.find(plan_key(plan));
+    if (it == plans_.end() || !same_plan(it->second.image->host(), plan)) return -2;
+    auto t = tuned_.find(TuneKey{it->second.image.get(), rows, kind});
     return t == tuned_.end() ? 0 : t->second;
 }
 
@@ -340,7 +507,8 @@ void HipExecutor::axis_pass(const AxisPlan &plan, bool rows, const void *input, 
     if (f64 || mode_ != SolveMode::refined || !plan.requires_float64()) return;
     if (in_kind != Sample::f32 || out_kind != Sample::f32)
         throw std::invalid_argument("dsmvc: refined solves need Float32 planes");
-    hip::DevicePlan &dp = device_plan(plan);
+    const auto image = device_plan(plan);
+    hip::DevicePlan &dp = *image;
     const hip::DevRefine *rt = dp.refine();
     if (!rt) throw std::runtime_error("dsmvc: refinement tables are missing");
     hip::RefineArgs a{};
@@ -418,7 +586,8 @@ void HipExecutor::solve_pass(const AxisPlan &plan, bool rows, const void *input,
         if (a0 < b1 && b0 < a1)
             throw std::invalid_argument("dsmvc: input and output planes must not overlap");
     }
-    hip::DevicePlan &dp = device_plan(plan);
+    const auto image = device_plan(plan);
+    hip::DevicePlan &dp = *image;
     // Float64 route of a 2D solve (FP64 intermediate): the windowed FP64
     // kernels when the plan's FP64 recurrence decays within their warm-up.
     const bool f64_pair = rows ? in_kind != Sample::f64 && out_kind == Sample::f64
@@ -505,21 +674,22 @@ void HipExecutor::inverse_rows(const AxisPlan &plan, const float *input,
         || output_row_stride < plan.destination_size || row_count < 0)
         throw std::invalid_argument("dsmvc: invalid row executor arguments");
     if (row_count == 0) return;
-    std::lock_guard lock(mutex_);
-    check(hipSetDevice(device_), "hipSetDevice failed");
-    const bool f64 = float64_route(plan);
-    const std::size_t elem = f64 ? 8 : 4;
-    Scratch &s = scratch(stream, refine_regions() * align256(static_cast<std::size_t>(row_count)
+    run_call(stream, async, "dsmvc: HIP Float64 row execution produced NaN or infinity",
+             [&](hipStream_t st) -> Scratch & {
+        const bool f64 = float64_route(plan);
+        const std::size_t elem = f64 ? 8 : 4;
+        Scratch &s = scratch(st, refine_regions() * align256(static_cast<std::size_t>(row_count)
                                                              * plan.destination_size * elem));
-    const DevConv conv = make_conv(nullptr);
-    if (f64) {
-        arm_flag(s, stream);
-        check(hip::launch_check_finite(input, Sample::f32, input_row_stride, 0, row_count,
-                                       plan.source_size, 1, s.flag, stream), "check launch failed");
-    }
-    axis_pass(plan, true, input, Sample::f32, input_row_stride, 0, output, Sample::f32,
-              output_row_stride, 0, row_count, 1, conv, f64, s, 0, stream);
-    finish(stream, s, async, "dsmvc: HIP Float64 row execution produced NaN or infinity");
+        const DevConv conv = make_conv(nullptr);
+        if (f64) {
+            arm_flag(s, st);
+            check(hip::launch_check_finite(input, Sample::f32, input_row_stride, 0, row_count,
+                                           plan.source_size, 1, s.flag, st), "check launch failed");
+        }
+        axis_pass(plan, true, input, Sample::f32, input_row_stride, 0, output, Sample::f32,
+                  output_row_stride, 0, row_count, 1, conv, f64, s, 0, st);
+        return s;
+    });
 }
 
 void HipExecutor::inverse_columns(const AxisPlan &plan, const float *input,
@@ -530,21 +700,23 @@ void HipExecutor::inverse_columns(const AxisPlan &plan, const float *input,
         || input_row_stride < column_count || output_row_stride < column_count)
         throw std::invalid_argument("dsmvc: invalid column executor arguments");
     if (column_count == 0) return;
-    std::lock_guard lock(mutex_);
-    check(hipSetDevice(device_), "hipSetDevice failed");
-    const bool f64 = float64_route(plan);
-    const std::size_t elem = f64 ? 8 : 4;
-    Scratch &s = scratch(stream, refine_regions() * align256(static_cast<std::size_t>(column_count)
+    run_call(stream, async, "dsmvc: HIP Float64 column execution produced NaN or infinity",
+             [&](hipStream_t st) -> Scratch & {
+        const bool f64 = float64_route(plan);
+        const std::size_t elem = f64 ? 8 : 4;
+        Scratch &s = scratch(st, refine_regions() * align256(static_cast<std::size_t>(column_count)
                                                              * plan.destination_size * elem));
-    const DevConv conv = make_conv(nullptr);
-    if (f64) {
-        arm_flag(s, stream);
-        check(hip::launch_check_finite(input, Sample::f32, input_row_stride, 0, plan.source_size,
-                                       column_count, 1, s.flag, stream), "check launch failed");
-    }
-    axis_pass(plan, false, input, Sample::f32, input_row_stride, 0, output, Sample::f32,
-              output_row_stride, 0, column_count, 1, conv, f64, s, 0, stream);
-    finish(stream, s, async, "dsmvc: HIP Float64 column execution produced NaN or infinity");
+        const DevConv conv = make_conv(nullptr);
+        if (f64) {
+            arm_flag(s, st);
+            check(hip::launch_check_finite(input, Sample::f32, input_row_stride, 0,
+                                           plan.source_size, column_count, 1, s.flag, st),
+                  "check launch failed");
+        }
+        axis_pass(plan, false, input, Sample::f32, input_row_stride, 0, output, Sample::f32,
+                  output_row_stride, 0, column_count, 1, conv, f64, s, 0, st);
+        return s;
+    });
 }
 
 void HipExecutor::inverse_2d_planes(const PlaneJob *planes, std::int32_t count, SampleKind kind,
@@ -569,9 +741,17 @@ void HipExecutor::inverse_2d_planes(const PlaneJob *planes, std::int32_t count, 
                                 "dsmvc: invalid plane arguments");
         }
     }
-    std::lock_guard lock(mutex_);
-    check(hipSetDevice(device_), "hipSetDevice failed");
+    run_call(stream, async, "dsmvc: HIP Float64 2D execution produced NaN or infinity",
+             [&](hipStream_t st) -> Scratch & { return enqueue_planes(planes, count, kind, st); });
+}
+
+// Queues a planes call on `stream` (caller holds the lock); returns the
+// scratch entry whose flag a Float64 plane armed.
+HipExecutor::Scratch &HipExecutor::enqueue_planes(const PlaneJob *planes, std::int32_t count,
+                                                  SampleKind kind, hipStream_t stream) {
     const bool windows = mode_ == SolveMode::automatic || mode_ == SolveMode::windowed;
+    // The device images stay pinned until every launch of this call is queued.
+    std::vector<std::shared_ptr<hip::DevicePlan>> pins;
     std::vector<const hip::DevWindow *> wh(static_cast<std::size_t>(count), nullptr);
     std::vector<const hip::DevWindow *> wv(static_cast<std::size_t>(count), nullptr);
     std::vector<std::size_t> offset(static_cast<std::size_t>(count), 0);
@@ -579,8 +759,10 @@ void HipExecutor::inverse_2d_planes(const PlaneJob *planes, std::int32_t count, 
     for (std::int32_t k = 0; k < count; ++k) {
         const AxisPlan &h = *planes[k].horizontal, &v = *planes[k].vertical;
         if (!windows || h.requires_float64() || v.requires_float64()) continue;
-        wh[static_cast<std::size_t>(k)] = device_plan(h).window(true);
-        wv[static_cast<std::size_t>(k)] = device_plan(v).window(false);
+        pins.push_back(device_plan(h));
+        wh[static_cast<std::size_t>(k)] = pins.back()->window(true);
+        pins.push_back(device_plan(v));
+        wv[static_cast<std::size_t>(k)] = pins.back()->window(false);
         if (!wh[static_cast<std::size_t>(k)] || !wv[static_cast<std::size_t>(k)]) {
             wh[static_cast<std::size_t>(k)] = wv[static_cast<std::size_t>(k)] = nullptr;
             continue;
@@ -639,7 +821,8 @@ void HipExecutor::inverse_2d_planes(const PlaneJob *planes, std::int32_t count, 
         const PlaneJob &p = planes[k];
         last = &solve_2d(*p.horizontal, *p.vertical, kind, p.batch, p.conversion, stream);
     }
-    if (last) finish(stream, *last, async, "dsmvc: HIP Float64 2D execution produced NaN or infinity");
+    if (!last) last = &scratch(stream, 0);
+    return *last;
 }
 
 void HipExecutor::inverse_2d_batch(const AxisPlan &h, const AxisPlan &v, SampleKind kind,
@@ -658,10 +841,8 @@ void HipExecutor::inverse_2d_batch(const AxisPlan &h, const AxisPlan &v, SampleK
         validate_conversion(*conversion, kind == SampleKind::u8 ? 255U : 65535U,
                             "dsmvc: invalid integer 2D executor arguments");
     }
-    std::lock_guard lock(mutex_);
-    check(hipSetDevice(device_), "hipSetDevice failed");
-    Scratch &s = solve_2d(h, v, kind, b, conversion, stream);
-    finish(stream, s, async, "dsmvc: HIP Float64 2D execution produced NaN or infinity");
+    run_call(stream, async, "dsmvc: HIP Float64 2D execution produced NaN or infinity",
+             [&](hipStream_t st) -> Scratch & { return solve_2d(h, v, kind, b, conversion, st); });
 }
 
 // Queues one plane batch's two passes on `stream` (caller holds the lock).
@@ -722,14 +903,14 @@ void HipExecutor::inverse_rows_batch(const AxisPlan &plan, SampleKind in_kind,
         validate_conversion(*conversion, in_kind == SampleKind::u8 ? 255U : 65535U,
                             "dsmvc: invalid row executor arguments");
     }
-    std::lock_guard lock(mutex_);
-    check(hipSetDevice(device_), "hipSetDevice failed");
-    Scratch &s = scratch(stream, refine_regions() * align256(static_cast<std::size_t>(rows)
+    run_call(stream, async, "dsmvc: HIP row execution failed", [&](hipStream_t st) -> Scratch & {
+        Scratch &s = scratch(st, refine_regions() * align256(static_cast<std::size_t>(rows)
                                                              * plan.destination_size * b.frames * 4));
-    axis_pass(plan, true, b.input, sample_of(in_kind), b.input_row_stride, b.input_frame_stride,
-              b.output, Sample::f32, b.output_row_stride, b.output_frame_stride, rows, b.frames,
-              make_conv(conversion), false, s, 0, stream);
-    finish(stream, s, async, "dsmvc: HIP row execution failed");
+        axis_pass(plan, true, b.input, sample_of(in_kind), b.input_row_stride,
+                  b.input_frame_stride, b.output, Sample::f32, b.output_row_stride,
+                  b.output_frame_stride, rows, b.frames, make_conv(conversion), false, s, 0, st);
+        return s;
+    });
 }
 
 void HipExecutor::inverse_columns_batch(const AxisPlan &plan, SampleKind out_kind,
@@ -749,14 +930,168 @@ void HipExecutor::inverse_columns_batch(const AxisPlan &plan, SampleKind out_kin
         validate_conversion(*conversion, out_kind == SampleKind::u8 ? 255U : 65535U,
                             "dsmvc: invalid column executor arguments");
     }
-    std::lock_guard lock(mutex_);
-    check(hipSetDevice(device_), "hipSetDevice failed");
-    Scratch &s = scratch(stream, refine_regions() * align256(static_cast<std::size_t>(columns)
+    run_call(stream, async, "dsmvc: HIP column execution failed", [&](hipStream_t st) -> Scratch & {
+        Scratch &s = scratch(st, refine_regions() * align256(static_cast<std::size_t>(columns)
                                                              * plan.destination_size * b.frames * 4));
-    axis_pass(plan, false, b.input, Sample::f32, b.input_row_stride, b.input_frame_stride,
-              b.output, sample_of(out_kind), b.output_row_stride, b.output_frame_stride, columns,
-              b.frames, make_conv(conversion), false, s, 0, stream);
-    finish(stream, s, async, "dsmvc: HIP column execution failed");
+        axis_pass(plan, false, b.input, Sample::f32, b.input_row_stride, b.input_frame_stride,
+                  b.output, sample_of(out_kind), b.output_row_stride, b.output_frame_stride,
+                  columns, b.frames, make_conv(conversion), false, s, 0, st);
+        return s;
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Host frames
+// ---------------------------------------------------------------------------
+namespace {
+// Copies `rows` rows of `width` samples between strided and packed layouts.
+void pack_rows(void *dst, std::ptrdiff_t dst_stride, const void *src, std::ptrdiff_t src_stride,
+               std::int64_t rows, std::size_t width, std::size_t sample) {
+    auto *d = static_cast<unsigned char *>(dst);
+    const auto *q = static_cast<const unsigned char *>(src);
+    for (std::int64_t r = 0; r < rows; ++r)
+        std::memcpy(d + r * dst_stride * static_cast<std::ptrdiff_t>(sample),
+                    q + r * src_stride * static_cast<std::ptrdiff_t>(sample), width * sample);
+}
+} // namespace
+
+void HipExecutor::inverse_2d_host(const AxisPlan &h, const AxisPlan &v, SampleKind kind,
+                                  const PlaneBatch &b, const IntegerConversion *conversion) {
+    const bool integer = kind != SampleKind::float32;
+    if (!h.valid() || !v.valid() || !b.input || !b.output || b.frames < 1
+        || b.input_row_stride < h.source_size || b.output_row_stride < h.destination_size
+        || (b.frames > 1 && (b.input_frame_stride < b.input_row_stride * v.source_size
+                             || b.output_frame_stride < b.output_row_stride * v.destination_size)))
+        throw std::invalid_argument(integer ? "dsmvc: invalid integer 2D executor arguments"
+                                            : "dsmvc: invalid 2D executor arguments");
+    if (integer) {
+        if (!conversion) throw std::invalid_argument("dsmvc: integer 2D execution needs a conversion");
+        validate_conversion(*conversion, kind == SampleKind::u8 ? 255U : 65535U,
+                            "dsmvc: invalid integer 2D executor arguments");
+    }
+    const std::size_t sample = static_cast<std::size_t>(sample_size(sample_of(kind)));
+    const std::size_t in_plane = static_cast<std::size_t>(h.source_size) * v.source_size;
+    const std::size_t out_plane = static_cast<std::size_t>(h.destination_size) * v.destination_size;
+    const std::size_t in_bytes = align256(in_plane * b.frames * sample);
+    const std::size_t out_bytes = align256(out_plane * b.frames * sample);
+    Slot *slot = acquire_slot();
+    SlotGuard guard{[this, slot] { release_slot(slot); }};
+    slot_buffers(*slot, std::max(in_bytes, out_bytes), in_bytes + out_bytes);
+    for (std::int32_t f = 0; f < b.frames; ++f)
+        pack_rows(static_cast<unsigned char *>(slot->pinned) + f * in_plane * sample, h.source_size,
+                  static_cast<const unsigned char *>(b.input)
+                      + f * b.input_frame_stride * static_cast<std::ptrdiff_t>(sample),
+                  b.input_row_stride, v.source_size, static_cast<std::size_t>(h.source_size), sample);
+    void *dev_in = slot->device;
+    void *dev_out = static_cast<unsigned char *>(slot->device) + in_bytes;
+    const hipStream_t st = slot->stream;
+    Scratch *used = nullptr;
+    {
+        std::lock_guard lock(mutex_);
+        check(hipSetDevice(device_), "hipSetDevice failed");
+        check(hipMemcpyAsync(dev_in, slot->pinned, in_plane * b.frames * sample,
+                             hipMemcpyHostToDevice, st), "host frame upload failed");
+        const PlaneBatch dev{dev_in, h.source_size, static_cast<std::ptrdiff_t>(in_plane), dev_out,
+                             h.destination_size, static_cast<std::ptrdiff_t>(out_plane), b.frames};
+        used = &solve_2d(h, v, kind, dev, conversion, st);
+        check(hipMemcpyAsync(slot->pinned, dev_out, out_plane * b.frames * sample,
+                             hipMemcpyDeviceToHost, st), "host frame download failed");
+        check(hipGetLastError(), "kernel launch failed");
+    }
+    complete(st, *used, "dsmvc: HIP Float64 2D execution produced NaN or infinity");
+    for (std::int32_t f = 0; f < b.frames; ++f)
+        pack_rows(static_cast<unsigned char *>(b.output)
+                      + f * b.output_frame_stride * static_cast<std::ptrdiff_t>(sample),
+                  b.output_row_stride,
+                  static_cast<const unsigned char *>(slot->pinned) + f * out_plane * sample,
+                  h.destination_size, v.destination_size,
+                  static_cast<std::size_t>(h.destination_size), sample);
+}
+
+void HipExecutor::inverse_rows_host(const AxisPlan &plan, const float *input,
+                                    std::ptrdiff_t input_row_stride, float *output,
+                                    std::ptrdiff_t output_row_stride, std::int32_t row_count) {
+    if (!plan.valid() || !input || !output || input_row_stride < plan.source_size
+        || output_row_stride < plan.destination_size || row_count < 0)
+        throw std::invalid_argument("dsmvc: invalid row executor arguments");
+    if (row_count == 0) return;
+    const std::size_t m = plan.source_size, n = plan.destination_size;
+    const std::size_t in_bytes = align256(m * row_count * 4), out_bytes = align256(n * row_count * 4);
+    Slot *slot = acquire_slot();
+    SlotGuard guard{[this, slot] { release_slot(slot); }};
+    slot_buffers(*slot, std::max(in_bytes, out_bytes), in_bytes + out_bytes);
+    pack_rows(slot->pinned, static_cast<std::ptrdiff_t>(m), input, input_row_stride, row_count, m, 4);
+    auto *dev_in = static_cast<float *>(slot->device);
+    auto *dev_out = reinterpret_cast<float *>(static_cast<unsigned char *>(slot->device) + in_bytes);
+    const hipStream_t st = slot->stream;
+    Scratch *used = nullptr;
+    {
+        std::lock_guard lock(mutex_);
+        check(hipSetDevice(device_), "hipSetDevice failed");
+        check(hipMemcpyAsync(dev_in, slot->pinned, m * row_count * 4, hipMemcpyHostToDevice, st),
+              "host frame upload failed");
+        const bool f64 = float64_route(plan);
+        Scratch &s = scratch(st, refine_regions() * align256(static_cast<std::size_t>(row_count) * n
+                                                             * (f64 ? 8 : 4)));
+        if (f64) {
+            arm_flag(s, st);
+            check(hip::launch_check_finite(dev_in, Sample::f32, static_cast<long>(m), 0, row_count,
+                                           plan.source_size, 1, s.flag, st), "check launch failed");
+        }
+        axis_pass(plan, true, dev_in, Sample::f32, static_cast<std::ptrdiff_t>(m), 0, dev_out,
+                  Sample::f32, static_cast<std::ptrdiff_t>(n), 0, row_count, 1, make_conv(nullptr),
+                  f64, s, 0, st);
+        check(hipMemcpyAsync(slot->pinned, dev_out, n * row_count * 4, hipMemcpyDeviceToHost, st),
+              "host frame download failed");
+        check(hipGetLastError(), "kernel launch failed");
+        used = &s;
+    }
+    complete(st, *used, "dsmvc: HIP Float64 row execution produced NaN or infinity");
+    pack_rows(output, output_row_stride, slot->pinned, static_cast<std::ptrdiff_t>(n), row_count, n, 4);
+}
+
+void HipExecutor::inverse_columns_host(const AxisPlan &plan, const float *input,
+                                       std::ptrdiff_t input_row_stride, float *output,
+                                       std::ptrdiff_t output_row_stride, std::int32_t column_count) {
+    if (!plan.valid() || !input || !output || column_count < 0
+        || input_row_stride < column_count || output_row_stride < column_count)
+        throw std::invalid_argument("dsmvc: invalid column executor arguments");
+    if (column_count == 0) return;
+    const std::size_t m = plan.source_size, n = plan.destination_size, c = column_count;
+    const std::size_t in_bytes = align256(m * c * 4), out_bytes = align256(n * c * 4);
+    Slot *slot = acquire_slot();
+    SlotGuard guard{[this, slot] { release_slot(slot); }};
+    slot_buffers(*slot, std::max(in_bytes, out_bytes), in_bytes + out_bytes);
+    pack_rows(slot->pinned, static_cast<std::ptrdiff_t>(c), input, input_row_stride,
+              static_cast<std::int64_t>(m), c, 4);
+    auto *dev_in = static_cast<float *>(slot->device);
+    auto *dev_out = reinterpret_cast<float *>(static_cast<unsigned char *>(slot->device) + in_bytes);
+    const hipStream_t st = slot->stream;
+    Scratch *used = nullptr;
+    {
+        std::lock_guard lock(mutex_);
+        check(hipSetDevice(device_), "hipSetDevice failed");
+        check(hipMemcpyAsync(dev_in, slot->pinned, m * c * 4, hipMemcpyHostToDevice, st),
+              "host frame upload failed");
+        const bool f64 = float64_route(plan);
+        Scratch &s = scratch(st, refine_regions() * align256(c * n * (f64 ? 8 : 4)));
+        if (f64) {
+            arm_flag(s, st);
+            check(hip::launch_check_finite(dev_in, Sample::f32, static_cast<long>(c), 0,
+                                           plan.source_size, column_count, 1, s.flag, st),
+                  "check launch failed");
+        }
+        axis_pass(plan, false, dev_in, Sample::f32, static_cast<std::ptrdiff_t>(c), 0, dev_out,
+                  Sample::f32, static_cast<std::ptrdiff_t>(c), 0, column_count, 1,
+                  make_conv(nullptr), f64, s, 0, st);
+        check(hipMemcpyAsync(slot->pinned, dev_out, n * c * 4, hipMemcpyDeviceToHost, st),
+              "host frame download failed");
+        check(hipGetLastError(), "kernel launch failed");
+        used = &s;
+    }
+    complete(st, *used, "dsmvc: HIP Float64 column execution produced NaN or infinity");
+    pack_rows(output, output_row_stride, slot->pinned, static_cast<std::ptrdiff_t>(c),
+              static_cast<std::int64_t>(n), c, 4);
 }
 
 void HipExecutor::inverse_2d(const AxisPlan &h, const AxisPlan &v, const float *input,

@@ -12,7 +12,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
+#include <condition_variable>
 #include <cstdint>
+#include <list>
 #include <memory>
 #include <mutex>
 #include <unordered_map>
@@ -168,18 +171,41 @@ public:
     // executor keeps per-stream state until then, or until it is destroyed).
     void release_stream(hipStream_t stream);
 
+    // Host-frame entries: the reference's executors take host buffers
+    // (include/dsmvc/engine.hpp:455-490) and stage them through pinned
+    // memory on a stream of their own (src/cuda/cuda_executor.cpp:2798-2822).
+    // These do the same on an execution slot: the input is packed into the
+    // slot's pinned buffer, copied to the device, solved, copied back and
+    // unpacked into the caller's rows (logical width only).  Synchronous.
+    void inverse_2d_host(const AxisPlan &horizontal, const AxisPlan &vertical, SampleKind kind,
+                         const PlaneBatch &host, const IntegerConversion *conversion);
+    void inverse_rows_host(const AxisPlan &plan, const float *input,
+                           std::ptrdiff_t input_row_stride, float *output,
+                           std::ptrdiff_t output_row_stride, std::int32_t row_count);
+    void inverse_columns_host(const AxisPlan &plan, const float *input,
+                              std::ptrdiff_t input_row_stride, float *output,
+                              std::ptrdiff_t output_row_stride, std::int32_t column_count);
+
     // The device image of a plan (uploaded on first use), for callers that
     // launch the kernels themselves (multi-plan batches, the request loop).
-    // It lives as long as the executor and the plan object.
-    hip::DevicePlan &plan_image(const AxisPlan &plan);
+    // The cache never evicts an image while a returned pointer to it lives.
+    std::shared_ptr<hip::DevicePlan> plan_image(const AxisPlan &plan);
+
+    // Device plan cache: images keyed by the plan's build serial (one per
+    // planner request key), least recently used first out once their bytes
+    // (device image plus every table built for it) exceed the limit --
+    // DSMVC_HIP_PLAN_CACHE_MB, default 1024 MiB (the reference bounds its
+    // packed-plan LRU the same way, src/cuda/cuda_executor.cpp:219-235,
+    // 1936-2020).  Images in use (held by a call, a batch or a served
+    // request) are never evicted.
+    struct PlanCacheStats {
+        std::uint64_t entries = 0, bytes = 0, limit = 0;
+        std::uint64_t hits = 0, misses = 0, evictions = 0;
+    };
+    PlanCacheStats plan_cache_stats();
+    void set_plan_cache_limit(std::size_t bytes);
 
 private:
-    // The device address of this GPU's no-allocation scratch array,
-    // resolved once per device when an executor is created (noalloc.cpp).
-    static void *noalloc_setup(int device);
-    hip::DevicePlan *prepared_plan(const AxisPlan &plan);
-    void *noalloc_scratch_ = nullptr;
-
     struct Scratch {
         void *buffer = nullptr;
         std::size_t bytes = 0;
@@ -187,7 +213,43 @@ private:
         bool flag_armed = false;
     };
 
-    dsmvc::hip::DevicePlan &device_plan(const AxisPlan &plan);
+    // The device address of this GPU's no-allocation scratch array,
+    // resolved once per device when an executor is created (noalloc.cpp).
+    static void *noalloc_setup(int device);
+    hip::DevicePlan *prepared_plan(const AxisPlan &plan);
+    void *noalloc_scratch_ = nullptr;
+
+    // Execution slots: synchronous calls made without a stream each take a
+    // slot -- a stream of the executor's own (blocking with respect to the
+    // null stream, default priority) with its host staging buffers -- so that
+    // concurrent callers (fmParallel getFrame threads) overlap on the device
+    // and wait for their own work without holding the executor's lock
+    // (the reference bounds concurrency the same way,
+    // src/cuda/cuda_executor.cpp:1102-1157, 1462-1503).
+    struct Slot {
+        hipStream_t stream = nullptr;
+        bool busy = false;
+        void *pinned = nullptr;     // host staging (input, then output)
+        std::size_t pinned_bytes = 0;
+        void *device = nullptr;     // device copies of host frames
+        std::size_t device_bytes = 0;
+    };
+    static constexpr int kMaxSlots = 8;
+    std::array<Slot, kMaxSlots> slots_{};
+    std::mutex slot_mutex_;
+    std::condition_variable slot_cv_;
+    Slot *acquire_slot();
+    void release_slot(Slot *slot);
+    void slot_buffers(Slot &slot, std::size_t pinned, std::size_t device);
+    // Runs `enqueue(stream) -> Scratch &` under the lock on the caller's
+    // stream, or on a slot for a synchronous call without one, then waits
+    // without the lock and reports a Float64 failure.
+    template <class Enqueue>
+    void run_call(hipStream_t stream, bool async, const char *what, Enqueue &&enqueue);
+    void complete(hipStream_t stream, Scratch &s, const char *what);
+
+    std::shared_ptr<hip::DevicePlan> device_plan(const AxisPlan &plan);
+    void evict_plans();
     Scratch &scratch(hipStream_t stream, std::size_t bytes);
     void finish(hipStream_t stream, Scratch &s, bool async, const char *what);
     // Refined solves keep the residual and its correction side by side in
@@ -203,6 +265,8 @@ private:
                     Scratch &s, std::size_t fwd_offset, hipStream_t stream);
     Scratch &solve_2d(const AxisPlan &h, const AxisPlan &v, SampleKind kind, const PlaneBatch &b,
                       const IntegerConversion *conversion, hipStream_t stream);
+    Scratch &enqueue_planes(const PlaneJob *planes, std::int32_t count, SampleKind kind,
+                            hipStream_t stream);
     void arm_flag(Scratch &s, hipStream_t stream);
     WindowTables &stream_tables(hipStream_t stream);
     void axis_pass(const AxisPlan &plan, bool rows, const void *input, hip::Sample in_kind,
@@ -241,7 +305,15 @@ private:
     std::unordered_map<TuneKey, int, TuneKeyHash> tuned_;
     std::unordered_map<hipStream_t, std::unique_ptr<WindowTables>> plane_tables_;
     std::mutex mutex_;
-    std::unordered_map<const AxisPlan *, std::unique_ptr<dsmvc::hip::DevicePlan>> plans_;
+    struct PlanEntry {
+        std::shared_ptr<hip::DevicePlan> image;
+        std::list<std::uint64_t>::iterator lru;
+    };
+    static std::uint64_t plan_key(const AxisPlan &plan);
+    std::unordered_map<std::uint64_t, PlanEntry> plans_;
+    std::list<std::uint64_t> lru_;   // most recent first
+    std::size_t plan_cache_limit_ = std::size_t{1024} << 20;
+    PlanCacheStats plan_stats_{};
     std::unordered_map<hipStream_t, Scratch> scratch_;
     bool sealed_ = false;
 };

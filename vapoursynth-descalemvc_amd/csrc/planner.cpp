@@ -533,9 +533,23 @@ AxisPlan build_axis_plan(const AxisRequest &r) {
         }
     }
     if (!p.valid()) throw std::runtime_error("failed to build inverse axis plan");
-    static std::atomic<std::uint64_t> next_serial{1};
-    p.serial = next_serial.fetch_add(1, std::memory_order_relaxed);
+    p.serial = next_plan_serial();
     return p;
+}
+
+std::uint64_t next_plan_serial() noexcept {
+    static std::atomic<std::uint64_t> next_serial{1};
+    return next_serial.fetch_add(1, std::memory_order_relaxed);
+}
+
+std::shared_ptr<const AxisPlan> import_axis_plan(AxisPlan plan) {
+    // A plan made elsewhere (the reference planner, a caller's own arrays):
+    // held to the same validity rules as this planner's output, then given
+    // a serial of its own so device images can key on it.
+    plan.serial = 0;
+    if (!plan.valid()) throw std::invalid_argument("dsmvc: imported axis plan is invalid");
+    plan.serial = next_plan_serial();
+    return std::make_shared<const AxisPlan>(std::move(plan));
 }
 
 // ---------------------------------------------------------------------------

@@ -150,11 +150,12 @@ std::uint64_t HipServer::submit(const ServeRequest &r) {
                           || executor_.mode() == SolveMode::windowed;
     // Device image plus the windowed tables the round (or, for a Float64
     // pair, the executor) will use.
-    auto prepare = [&](const AxisPlan &plan, bool rows, bool f64_pair) -> hip::DevicePlan & {
-        hip::DevicePlan &image = executor_.plan_image(plan);
+    auto prepare = [&](const AxisPlan &plan, bool rows,
+                       bool f64_pair) -> std::shared_ptr<hip::DevicePlan> {
+        std::shared_ptr<hip::DevicePlan> image = executor_.plan_image(plan);
         if (parallel) {
-            if (f64_pair) (void)image.window64();
-            else (void)image.window(rows);
+            if (f64_pair) (void)image->window64();
+            else (void)image->window(rows);
         }
         return image;
     };
@@ -209,11 +210,11 @@ std::uint64_t HipServer::submit(const ServeRequest &r) {
         // Device images are uploaded here, on the caller's thread, so the
         // loop only builds tables and launches.
         const bool f64_pair = plane.h->requires_float64() || plane.v->requires_float64();
-        hip::DevicePlan &ih = prepare(*plane.h, true, f64_pair);
-        hip::DevicePlan &iv = prepare(*plane.v, false, f64_pair);
+        plane.ih = prepare(*plane.h, true, f64_pair);
+        plane.iv = prepare(*plane.v, false, f64_pair);
         if (parallel && !f64_pair) {
-            plane.wh = ih.window(true);
-            plane.wv = iv.window(false);
+            plane.wh = plane.ih->window(true);
+            plane.wv = plane.iv->window(false);
         }
         plane.table = plane.wh && plane.wv;
         if (plane.table)

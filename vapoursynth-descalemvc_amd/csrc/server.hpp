@@ -39,6 +39,10 @@
 #include "device_types.hpp"
 #include "engine.hpp"
 
+namespace dsmvc::hip {
+class DevicePlan;
+}
+
 namespace dsmvc {
 
 class HipExecutor;
@@ -106,8 +110,9 @@ private:
         std::ptrdiff_t dst_stride = 0;
         bool table = false;          // windowed table launch (else executor route)
         std::size_t inter_bytes = 0;
-        // The plans' window tables (table planes).  The executor keeps them
-        // while the plans live, which `h` and `v` guarantee.
+        // The plans' device images, held so that the executor's plan cache
+        // keeps them (and their window tables) until the request is gone.
+        std::shared_ptr<hip::DevicePlan> ih, iv;
         const hip::DevWindow *wh = nullptr;
         const hip::DevWindow *wv = nullptr;
     };

@@ -25,7 +25,7 @@ from ._lib import ASYNC, AxisRequestC, ConversionC, DsmvcError, PlanInfoC, check
 
 __all__ = [
     "AxisRequest", "Plan", "Executor", "Clip", "IntegerConversion", "build_plan", "build_plans",
-    "filter_requests",
+    "filter_requests", "import_plan",
     "integer_conversion", "Descale", "Debilinear", "Debicubic", "Delanczos", "Despline16",
     "Despline36", "Despline64", "parse_backend", "resolve_backend", "device_count",
     "padding_index", "planner_cache_stats", "clear_planner_caches", "DsmvcError",
@@ -151,6 +151,36 @@ class Plan:
         ptrs = [ctypes.c_void_p(a.ctypes.data) if a.size else None for a in out.values()]
         check(load().dsmvc_hip_plan_arrays(self._handle, *ptrs))
         return out
+
+
+def import_plan(fields, arrays) -> Plan:
+    """A plan made by another planner -- for example the reference engine's
+    AxisPlan -- from its scalar fields and arrays (dsmvc_hip_plan_import; the
+    arrays are copied and checked by the reference's AxisPlan::valid() rules).
+    `fields`: an object or dict with source_size, destination_size, support,
+    half_bandwidth, active_length, shift, normal_rcond, normal_inf_norm and
+    requires_float64; `arrays`: the AxisPlan arrays by name (Plan.arrays()
+    names), the four FP64 arrays only when requires_float64."""
+    get = (lambda k: fields[k]) if isinstance(fields, dict) else (lambda k: getattr(fields, k))
+    f64 = bool(get("requires_float64"))
+    kinds = {"offsets": np.uint32, "indices": np.int32, "weights_f64": np.float64,
+             "normal_bands_f64": np.float64, "ldlt_bands_f64": np.float64,
+             "inverse_diagonal_f64": np.float64}
+    keep = {}
+    for name in _lib.PLAN_ARRAYS:
+        if name.endswith("_f64") and not f64:
+            continue
+        keep[name] = np.ascontiguousarray(arrays[name], dtype=kinds.get(name, np.float32))
+    info = PlanInfoC(get("source_size"), get("destination_size"), get("support"),
+                     get("half_bandwidth"), float(get("active_length")), float(get("shift")),
+                     float(get("normal_rcond")), float(get("normal_inf_norm")), int(f64),
+                     int(keep["indices"].size))
+    data = _lib.PlanDataC(*[ctypes.c_void_p(keep[n].ctypes.data) if n in keep and keep[n].size
+                            else None for n in _lib.PLAN_ARRAYS])
+    handle = ctypes.c_void_p()
+    check(load().dsmvc_hip_plan_import(ctypes.byref(info), ctypes.byref(data),
+                                       ctypes.byref(handle)))
+    return Plan(None, _handle=handle)
 
 
 def build_plan(request: AxisRequest) -> Plan:
@@ -301,6 +331,66 @@ class Executor:
 
     def synchronize(self, stream=None):
         check(load().dsmvc_hip_synchronize(self._handle, _stream_handle(stream)))
+
+    # -- host frames (numpy arrays; staged through the executor's slots) -------
+    def inverse_2d_host(self, horizontal: Plan, vertical: Plan, x, out=None,
+                        conversion: Optional[IntegerConversion] = None):
+        """x: host array [H, W] or [frames, H, W] of float32, uint8 or uint16.
+        Returns a new (or fills `out`) host array [frames,] h x w."""
+        kinds = {np.dtype(np.float32): 0, np.dtype(np.uint8): 1, np.dtype(np.uint16): 2}
+        x = np.asarray(x)
+        if x.dtype not in kinds:
+            raise ValueError("dsmvc: samples must be float32, uint8 or uint16")
+        xb = x if x.ndim == 3 else x[None]
+        if xb.ndim != 3 or xb.shape[1] != vertical.source_size or xb.shape[2] < horizontal.source_size:
+            raise ValueError("dsmvc: input plane does not match the plans")
+        if xb.strides[2] != xb.itemsize:
+            raise ValueError("dsmvc: rows must be contiguous")
+        if kinds[x.dtype] and conversion is None:
+            raise ValueError("dsmvc: integer planes need a conversion")
+        shape = (xb.shape[0], vertical.destination_size, horizontal.destination_size)
+        ob = np.empty(shape, x.dtype) if out is None else (out if out.ndim == 3 else out[None])
+        if ob.shape != shape or ob.dtype != x.dtype or ob.strides[2] != ob.itemsize:
+            raise ValueError("dsmvc: output does not match the plans")
+        item = x.itemsize
+        conv = ctypes.byref(conversion.to_c()) if conversion is not None else None
+        check(load().dsmvc_hip_inverse_2d_host(
+            self._handle, horizontal.handle, vertical.handle, kinds[x.dtype],
+            ctypes.c_void_p(xb.ctypes.data), xb.strides[1] // item, xb.strides[0] // item,
+            ctypes.c_void_p(ob.ctypes.data), ob.strides[1] // item, ob.strides[0] // item,
+            xb.shape[0], conv))
+        if out is not None:
+            return out
+        return ob if x.ndim == 3 else ob[0]
+
+    def inverse_rows_host(self, plan: Plan, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        if x.ndim != 2 or x.shape[1] < plan.source_size:
+            raise ValueError("dsmvc: input does not match the plan")
+        out = np.empty((x.shape[0], plan.destination_size), np.float32)
+        check(load().dsmvc_hip_inverse_rows_host(
+            self._handle, plan.handle, ctypes.c_void_p(x.ctypes.data), x.shape[1],
+            ctypes.c_void_p(out.ctypes.data), out.shape[1], x.shape[0]))
+        return out
+
+    def inverse_columns_host(self, plan: Plan, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        if x.ndim != 2 or x.shape[0] < plan.source_size:
+            raise ValueError("dsmvc: input does not match the plan")
+        out = np.empty((plan.destination_size, x.shape[1]), np.float32)
+        check(load().dsmvc_hip_inverse_columns_host(
+            self._handle, plan.handle, ctypes.c_void_p(x.ctypes.data), x.shape[1],
+            ctypes.c_void_p(out.ctypes.data), out.shape[1], x.shape[1]))
+        return out
+
+    # -- device plan cache ------------------------------------------------------
+    def set_plan_cache(self, limit_bytes: int):
+        check(load().dsmvc_hip_executor_set_plan_cache(self._handle, int(limit_bytes)))
+
+    def plan_cache_stats(self) -> dict:
+        st = _lib.PlanCacheStatsC()
+        check(load().dsmvc_hip_executor_plan_cache_stats(self._handle, ctypes.byref(st)))
+        return {name: getattr(st, name) for name, _ in _lib.PlanCacheStatsC._fields_}
 
     # -- single axis, float32 ------------------------------------------------
     def inverse_rows(self, plan: Plan, x, out=None, stream=None, asynchronous=False):

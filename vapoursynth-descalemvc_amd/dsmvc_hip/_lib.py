@@ -33,7 +33,9 @@ EXPORTS = (
     "dsmvc_hip_server_submit", "dsmvc_hip_server_wait", "dsmvc_hip_server_poll",
     "dsmvc_hip_server_release", "dsmvc_hip_server_stats_get", "dsmvc_hip_server_destroy",
     "dsmvc_hip_executor_release_stream", "dsmvc_hip_synchronize",
-    "dsmvc_hip_inverse_2d_batch_noalloc",
+    "dsmvc_hip_inverse_2d_batch_noalloc", "dsmvc_hip_plan_import",
+    "dsmvc_hip_inverse_rows_host", "dsmvc_hip_inverse_columns_host", "dsmvc_hip_inverse_2d_host",
+    "dsmvc_hip_executor_set_plan_cache", "dsmvc_hip_executor_plan_cache_stats",
 )
 BATCH_GRAPH = 2
 
@@ -182,6 +184,19 @@ class PlanInfoC(ctypes.Structure):
     ]
 
 
+PLAN_ARRAYS = ("offsets", "indices", "weights", "lower_ld", "upper_l", "inverse_diagonal",
+               "weights_f64", "normal_bands_f64", "ldlt_bands_f64", "inverse_diagonal_f64")
+
+
+class PlanDataC(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_void_p) for name in PLAN_ARRAYS]
+
+
+class PlanCacheStatsC(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_uint64)
+                for name in ("entries", "bytes", "limit", "hits", "misses", "evictions")]
+
+
 class PlannerStatsC(ctypes.Structure):
     _fields_ = [
         ("plan_hits", ctypes.c_uint64),
@@ -239,6 +254,14 @@ def load():
                                        vp, u32]
     lib.dsmvc_hip_inverse_2d_batch.argtypes = [vp, vp, vp, i32, vp, pd, pd, vp, pd, pd, i32,
                                                ctypes.POINTER(ConversionC), vp, u32]
+    lib.dsmvc_hip_inverse_rows_host.argtypes = [vp, vp, vp, pd, vp, pd, i32]
+    lib.dsmvc_hip_inverse_columns_host.argtypes = [vp, vp, vp, pd, vp, pd, i32]
+    lib.dsmvc_hip_inverse_2d_host.argtypes = [vp, vp, vp, i32, vp, pd, pd, vp, pd, pd, i32,
+                                              ctypes.POINTER(ConversionC)]
+    lib.dsmvc_hip_executor_set_plan_cache.argtypes = [vp, ctypes.c_int64]
+    lib.dsmvc_hip_executor_plan_cache_stats.argtypes = [vp, ctypes.POINTER(PlanCacheStatsC)]
+    lib.dsmvc_hip_plan_import.argtypes = [ctypes.POINTER(PlanInfoC), ctypes.POINTER(PlanDataC),
+                                          ctypes.POINTER(vp)]
     lib.dsmvc_hip_inverse_2d_batch_noalloc.argtypes = [vp, vp, vp, i32, vp, pd, pd, vp, pd, pd,
                                                        i32, ctypes.POINTER(ConversionC), vp, u32]
     lib.dsmvc_hip_inverse_rows_batch.argtypes = [vp, vp, i32, vp, pd, pd, vp, pd, pd, i32, i32,
