@@ -39,7 +39,8 @@ extern "C" {
 /* Kernel kinds, border (padding) modes, f64mode values and backends keep the
  * reference numbering (include/dsmvc/engine.hpp:16-58); backend 5 is hip. */
 enum { DSMVC_HIP_BILINEAR = 0, DSMVC_HIP_BICUBIC = 1, DSMVC_HIP_LANCZOS = 2,
-       DSMVC_HIP_SPLINE16 = 3, DSMVC_HIP_SPLINE36 = 4, DSMVC_HIP_SPLINE64 = 5 };
+       DSMVC_HIP_SPLINE16 = 3, DSMVC_HIP_SPLINE36 = 4, DSMVC_HIP_SPLINE64 = 5,
+       DSMVC_HIP_CUSTOM = 6 };
 enum { DSMVC_HIP_SOLVE_AUTO = 0, DSMVC_HIP_SOLVE_EXACT = 1, DSMVC_HIP_SOLVE_CHUNKED = 2,
        DSMVC_HIP_SOLVE_WINDOWED = 3, DSMVC_HIP_SOLVE_REFINED = 4 };
 enum { DSMVC_HIP_SAMPLE_F32 = 0, DSMVC_HIP_SAMPLE_U8 = 1, DSMVC_HIP_SAMPLE_U16 = 2 };
@@ -121,6 +122,17 @@ int dsmvc_hip_plan_build(const dsmvc_hip_axis_request *request, dsmvc_hip_plan *
  * what dsmvc_hip_plan_build returns for its request. */
 int dsmvc_hip_plan_build_many(const dsmvc_hip_axis_request *requests, int32_t count,
                               int32_t threads, dsmvc_hip_plan **plans);
+/* dsmvc::get_or_build_axis_plan(request, custom_kernel) with a custom kernel
+ * (include/dsmvc/engine.hpp:78, 177-178; src/axis_plan.cpp:535-599): request
+ * kernel DSMVC_HIP_CUSTOM, support = request taps (times blur, rounded up).
+ * `kernel(x, user)` is called on the host, during this call only, at each
+ * tap's |distance| (blur applied); a non-finite value fails the build, as a
+ * zero weight sum does.  Plans of custom kernels are built on every call and
+ * never cached, as in the reference (src/axis_plan.cpp:1367-1372); the plugin
+ * passes a VapourSynth function here (src/vs_plugin.cpp:628-653). */
+typedef double (*dsmvc_hip_kernel_fn)(double x, void *user);
+int dsmvc_hip_plan_build_custom(const dsmvc_hip_axis_request *request, dsmvc_hip_kernel_fn kernel,
+                                void *user, dsmvc_hip_plan **plan);
 void dsmvc_hip_plan_release(dsmvc_hip_plan *plan);
 /* AxisPlan scalar fields. */
 int dsmvc_hip_plan_info_get(const dsmvc_hip_plan *plan, dsmvc_hip_plan_info *info);

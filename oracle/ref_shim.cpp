@@ -101,6 +101,31 @@ int ref_plan_build(const ref_request *r, void **out) {
     });
 }
 
+// dsmvc::build_axis_plan(request, custom_kernel) (include/dsmvc/engine.hpp:175)
+// with the callback called at each tap's |distance|.
+int ref_plan_build_custom(const ref_request *r, double (*kernel)(double, void *), void *user,
+                          void **out) {
+    return guarded([&] {
+        dsmvc::AxisRequest request;
+        request.source_size = r->source_size;
+        request.destination_size = r->destination_size;
+        request.active_length = r->active_length;
+        request.shift = r->shift;
+        request.kernel.kind = dsmvc::KernelKind::custom;
+        request.kernel.taps = r->taps;
+        request.kernel.b = r->b;
+        request.kernel.c = r->c;
+        request.kernel.blur = r->blur;
+        request.border = static_cast<dsmvc::BorderMode>(r->border);
+        request.f64_mode = static_cast<dsmvc::F64Mode>(r->f64_mode);
+        auto holder = std::make_unique<RefPlan>();
+        holder->plan = dsmvc::build_axis_plan(request, [kernel, user](double x) {
+            return kernel(x, user);
+        });
+        *out = holder.release();
+    });
+}
+
 void ref_plan_free(void *p) { delete static_cast<RefPlan *>(p); }
 
 void ref_plan_info_get(const void *p, ref_plan_info *info) {

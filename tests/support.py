@@ -285,6 +285,9 @@ class Reference:
         lib.ref_last_error.restype = ctypes.c_char_p
         lib.ref_plan_build.argtypes = [ctypes.POINTER(Request), ctypes.POINTER(vp)]
         lib.ref_plan_free.argtypes = [vp]
+        self.kernel_fn = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_double, ctypes.c_void_p)
+        lib.ref_plan_build_custom.argtypes = [ctypes.POINTER(Request), self.kernel_fn, vp,
+                                              ctypes.POINTER(vp)]
         lib.ref_plan_info_get.argtypes = [vp, ctypes.POINTER(RefPlanInfo)]
         lib.ref_plan_arrays.argtypes = [vp] + [vp] * 10
         for name in ("ref_inverse_axis_f32", "ref_inverse_axis_f32_ordered",
@@ -311,6 +314,14 @@ class Reference:
     def build(self, request: Request) -> "RefPlan":
         handle = ctypes.c_void_p()
         self._check(self.lib.ref_plan_build(ctypes.byref(request), ctypes.byref(handle)))
+        return RefPlan(self, handle)
+
+    def build_custom(self, request: Request, fn) -> "RefPlan":
+        """The reference planner's custom-kernel path; fn(x) at |distance|."""
+        callback = self.kernel_fn(lambda x, _user: fn(x))
+        handle = ctypes.c_void_p()
+        self._check(self.lib.ref_plan_build_custom(ctypes.byref(request), callback, None,
+                                                   ctypes.byref(handle)))
         return RefPlan(self, handle)
 
     def executor(self, path):

@@ -73,13 +73,15 @@ void require(bool ok, const char *what) {
     if (!ok) throw std::invalid_argument(what);
 }
 
-dsmvc::AxisRequest to_request(const dsmvc_hip_axis_request &r) {
+dsmvc::AxisRequest to_request(const dsmvc_hip_axis_request &r, bool custom = false) {
     dsmvc::AxisRequest q;
     q.source_size = r.source_size;
     q.destination_size = r.destination_size;
     q.active_length = r.active_length;
     q.shift = r.shift;
-    require(r.kernel >= 0 && r.kernel <= 5, "kernel is invalid");
+    if (r.kernel == DSMVC_HIP_CUSTOM && !custom)
+        throw std::invalid_argument("custom kernel callback is missing");
+    require(r.kernel >= 0 && r.kernel <= DSMVC_HIP_CUSTOM, "kernel is invalid");
     q.kernel.kind = static_cast<dsmvc::KernelKind>(r.kernel);
     q.kernel.taps = r.taps;
     q.kernel.b = r.b;
@@ -175,6 +177,19 @@ int dsmvc_hip_plan_build_many(const dsmvc_hip_axis_request *requests, int32_t co
             handles[k]->plan = std::move(built[k]);
         }
         for (std::size_t k = 0; k < n; ++k) plans[k] = handles[k].release();
+    });
+}
+
+int dsmvc_hip_plan_build_custom(const dsmvc_hip_axis_request *request, dsmvc_hip_kernel_fn kernel,
+                                void *user, dsmvc_hip_plan **plan) {
+    return guarded([&] {
+        require(request && plan, "dsmvc: null argument");
+        require(request->kernel == DSMVC_HIP_CUSTOM, "kernel is invalid");
+        if (!kernel) throw std::invalid_argument("custom kernel callback is missing");
+        auto handle = std::make_unique<dsmvc_hip_plan>();
+        handle->plan = dsmvc::get_or_build_axis_plan(
+            to_request(*request, true), [kernel, user](double x) { return kernel(x, user); });
+        *plan = handle.release();
     });
 }
 

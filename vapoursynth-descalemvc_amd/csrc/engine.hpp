@@ -9,6 +9,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -107,8 +108,15 @@ struct PlannerCacheStats {
 // Planner (host, FP64).  Throws std::invalid_argument for a malformed request
 // and std::runtime_error / std::length_error for a plan that cannot be built,
 // with the reference's messages.
-[[nodiscard]] AxisPlan build_axis_plan(const AxisRequest &request);
-[[nodiscard]] std::shared_ptr<const AxisPlan> get_or_build_axis_plan(const AxisRequest &request);
+// Custom kernels (KernelKind::custom, support = kernel.taps) are evaluated
+// through the callback at |distance| on the host; their plans are never
+// cached (reference include/dsmvc/engine.hpp:78, src/axis_plan.cpp:535-599,
+// 1367-1372).
+using CustomKernel = std::function<double(double)>;
+[[nodiscard]] AxisPlan build_axis_plan(const AxisRequest &request,
+                                       const CustomKernel &custom_kernel = {});
+[[nodiscard]] std::shared_ptr<const AxisPlan> get_or_build_axis_plan(
+    const AxisRequest &request, const CustomKernel &custom_kernel = {});
 // A plan from caller-supplied arrays (for example the reference planner's
 // AxisPlan), checked by valid(); throws std::invalid_argument when invalid.
 // The returned plan has a fresh serial.

@@ -117,9 +117,8 @@ std::int32_t effective_support(const AxisRequest &r) {
     case KernelKind::spline16: base = 2; break;
     case KernelKind::spline36: base = 3; break;
     case KernelKind::spline64: base = 4; break;
-    case KernelKind::lanczos: base = r.kernel.taps; break;
-    case KernelKind::custom:
-        throw std::invalid_argument("custom kernels are not supported by the HIP planner");
+    case KernelKind::lanczos:
+    case KernelKind::custom: base = r.kernel.taps; break;
     default: throw std::invalid_argument("kernel is invalid");
     }
     constexpr auto max_support = (std::numeric_limits<std::int32_t>::max() - 1) / 2;
@@ -146,7 +145,7 @@ struct SparseRows {
 // rule onto destination indices, weights normalised by their sum and
 // coalesced per destination index in ascending index order
 // (reference build_geometry + make_descale_matrix, axis_plan.cpp:456-599).
-SparseRows sample_operator(const AxisRequest &r, std::int32_t support) {
+SparseRows sample_operator(const AxisRequest &r, std::int32_t support, const CustomKernel &custom) {
     const std::int32_t taps = 2 * support;
     const auto m = static_cast<std::size_t>(r.source_size);
     if (m > std::numeric_limits<std::size_t>::max() / static_cast<std::size_t>(taps))
@@ -188,7 +187,10 @@ SparseRows sample_operator(const AxisRequest &r, std::int32_t support) {
         for (std::int32_t t = 0; t < taps; ++t) {
             Tap &tp = tap[static_cast<std::size_t>(t)];
             const double d = unit_blur ? tp.offset : tp.offset / r.kernel.blur;
-            tp.weight = kernel_weight(r.kernel, d);
+            // A custom kernel is evaluated at |d| (reference make_descale_matrix,
+            // axis_plan.cpp:560-563).
+            tp.weight = r.kernel.kind == KernelKind::custom ? custom(std::abs(d))
+                                                            : kernel_weight(r.kernel, d);
             if (!std::isfinite(tp.weight))
                 throw std::runtime_error("filter returned a non-finite value");
             total += tp.weight;
@@ -481,9 +483,11 @@ std::size_t AxisPlan::storage_bytes() const noexcept {
         + inverse_diagonal_f64.capacity() * 8;
 }
 
-AxisPlan build_axis_plan(const AxisRequest &r) {
+AxisPlan build_axis_plan(const AxisRequest &r, const CustomKernel &custom) {
     const std::int32_t support = effective_support(r);
-    const SparseRows a = sample_operator(r, support);
+    if (r.kernel.kind == KernelKind::custom && !custom)
+        throw std::invalid_argument("custom kernel callback is missing");
+    const SparseRows a = sample_operator(r, support, custom);
     SparseRows t = transpose(a, r.source_size, r.destination_size);
     const std::int32_t n = r.destination_size;
     const std::int32_t w = std::min(2 * support - 1, n - 1);
@@ -702,7 +706,13 @@ PlanCache &plan_cache() {
 
 } // namespace
 
-std::shared_ptr<const AxisPlan> get_or_build_axis_plan(const AxisRequest &r) {
+std::shared_ptr<const AxisPlan> get_or_build_axis_plan(const AxisRequest &r,
+                                                       const CustomKernel &custom) {
+    // Custom kernels cannot be keyed (the callback has no identity the
+    // cache could compare), so their plans are built on every request, as
+    // the reference does (axis_plan.cpp:1367-1372).
+    if (r.kernel.kind == KernelKind::custom)
+        return std::make_shared<const AxisPlan>(build_axis_plan(r, custom));
     return plan_cache().get(r);
 }
 

@@ -25,7 +25,7 @@ from ._lib import ASYNC, AxisRequestC, ConversionC, DsmvcError, PlanInfoC, check
 
 __all__ = [
     "AxisRequest", "Plan", "Executor", "Clip", "IntegerConversion", "build_plan", "build_plans",
-    "filter_requests", "import_plan",
+    "filter_requests", "import_plan", "build_custom_plan", "CUSTOM",
     "integer_conversion", "Descale", "Debilinear", "Debicubic", "Delanczos", "Despline16",
     "Despline36", "Despline64", "parse_backend", "resolve_backend", "device_count",
     "padding_index", "planner_cache_stats", "clear_planner_caches", "DsmvcError",
@@ -151,6 +151,36 @@ class Plan:
         ptrs = [ctypes.c_void_p(a.ctypes.data) if a.size else None for a in out.values()]
         check(load().dsmvc_hip_plan_arrays(self._handle, *ptrs))
         return out
+
+
+CUSTOM = 6
+
+
+def build_custom_plan(request: AxisRequest, kernel) -> Plan:
+    """A plan for a custom kernel (dsmvc_hip_plan_build_custom): kernel(x) is
+    called on the host at each tap's |distance| (blur applied) while the plan
+    is built; support = request.taps.  Custom plans are never cached
+    (reference src/axis_plan.cpp:535-599, 1367-1372)."""
+    failure = []
+
+    def call(x, _user):
+        try:
+            return float(kernel(x))
+        except Exception as exc:   # reported as a non-finite value below
+            failure.append(exc)
+            return float("nan")
+
+    fn = _lib.KERNEL_FN(call)
+    req = AxisRequest(request.source_size, request.destination_size, request.active_length,
+                      request.shift, CUSTOM, request.taps, request.b, request.c, request.blur,
+                      request.border, request.f64_mode)
+    handle = ctypes.c_void_p()
+    status = load().dsmvc_hip_plan_build_custom(ctypes.byref(req.to_c()), fn, None,
+                                                ctypes.byref(handle))
+    if failure:
+        raise failure[0]
+    check(status)
+    return Plan(req, _handle=handle)
 
 
 def import_plan(fields, arrays) -> Plan:
@@ -721,28 +751,47 @@ def _to_integer(plane, conv: IntegerConversion, dtype):
     return torch.round(v).to(torch.int32).to(dtype)
 
 
-def Descale(src, width: int, height: int, kernel: str = "bicubic", taps: int = 3,
-            b: float = 0.0, c: float = 0.5, src_left: float = 0.0, src_top: float = 0.0,
-            src_width: Optional[float] = None, src_height: Optional[float] = None,
-            border_handling: Optional[int] = None, padding: Optional[int] = None,
-            f64mode: int = 0, blur: float = 1.0, force: bool = False,
-            force_h: Optional[bool] = None, force_v: Optional[bool] = None,
+def Descale(src, width: int, height: int, kernel: Optional[str] = None,
+            taps: Optional[int] = None, b: float = 0.0, c: float = 0.5, src_left: float = 0.0,
+            src_top: float = 0.0, src_width: Optional[float] = None,
+            src_height: Optional[float] = None, border_handling: Optional[int] = None,
+            padding: Optional[int] = None, f64mode: int = 0, blur: float = 1.0,
+            force: bool = False, force_h: Optional[bool] = None, force_v: Optional[bool] = None,
+            custom=None, support: Optional[int] = None, custom_kernel=None,
             backend: str = "hip", solve: str = "auto"):
     """core.dsmvc.Descale with backend="hip" (reference src/vs_plugin.cpp:362-509,
     656-705, 1774-1987).  `src` is a Clip or a single float32 plane tensor.
     Integer clips descaled on both axes take the fused integer route; on one
     axis they are converted to float32, descaled and converted back, as the
     reference does through its resize calls (convert_and_invoke,
-    vs_plugin.cpp:570-626)."""
+    vs_plugin.cpp:570-626).  A custom kernel (`custom_kernel`, or the legacy
+    `custom`, a callable of the distance) needs `taps` or `support` and is
+    planned on the host through it (vs_plugin.cpp:430-486, 628-653)."""
     if resolve_backend(backend) != "hip":
         raise ValueError("dsmvc: this build provides backend=\"hip\" only")
     clip = src if isinstance(src, Clip) else Clip([src])
-    if kernel not in KERNELS:
+    custom_fn = custom if custom is not None else custom_kernel
+    if kernel is not None and custom_fn is not None:
+        raise ValueError("specify either kernel or custom kernel, not both")
+    if kernel is None and custom_fn is None:
+        raise ValueError("kernel or custom kernel is required")
+    if custom_fn is None and kernel not in KERNELS:
         raise ValueError("kernel is invalid")
+    taps_given = taps is not None or support is not None
+    taps = taps if taps is not None else (support if support is not None else 3)
+    if (kernel == "lanczos" or custom_fn is not None) and taps < 1:
+        raise ValueError("taps must be greater than zero")
+    if custom_fn is not None and not taps_given:
+        raise ValueError("custom kernels require taps or support")
     if width < 1:
         raise ValueError("width must be greater than zero")
     if height < 8:
         raise ValueError("height must be at least 8")
+    if width > clip.width or height > clip.height:
+        raise ValueError("output dimensions must not exceed input dimensions")
+    if (src_width is not None and not src_width > 0.0) or \
+            (src_height is not None and not src_height > 0.0):
+        raise ValueError("src_width and src_height must be positive")
     if padding is not None and border_handling is not None:
         raise ValueError("specify either padding or border_handling, not both")
     if padding is not None:
@@ -763,8 +812,6 @@ def Descale(src, width: int, height: int, kernel: str = "bicubic", taps: int = 3
         extent = min(extent, sw >> clip.subsampling_w, sh >> clip.subsampling_h)
     if blur >= float(extent):
         raise ValueError("blur exceeds the supported source-plane extent")
-    if kernel == "lanczos" and taps < 1:
-        raise ValueError("taps must be greater than zero")
     if clip.color_family == "yuv" and clip.subsampling_w | clip.subsampling_h:
         if width % (1 << clip.subsampling_w) or height % (1 << clip.subsampling_h):
             raise ValueError("output width is incompatible with subsampling")
@@ -778,8 +825,9 @@ def Descale(src, width: int, height: int, kernel: str = "bicubic", taps: int = 3
         return clip if isinstance(src, Clip) else src
     hreq, vreq = filter_requests(sw, sh, width, height, len(clip.planes), clip.subsampling_w,
                                  clip.subsampling_h, src_left, src_top, sw_act, sh_act,
-                                 kernel=KERNELS[kernel], taps=taps, b=b, c=c, blur=blur,
-                                 border=border, f64_mode=f64mode)
+                                 kernel=CUSTOM if custom_fn is not None else KERNELS[kernel],
+                                 taps=taps, b=b, c=c, blur=blur, border=border, f64_mode=f64mode)
+    make_plan = (lambda r: build_custom_plan(r, custom_fn)) if custom_fn is not None else Plan
     ex = _executor(solve)
     torch = _torch()
 
@@ -791,8 +839,8 @@ def Descale(src, width: int, height: int, kernel: str = "bicubic", taps: int = 3
 
     def plans_of(index):
         chroma = index > 0
-        return (Plan(hreq[1 if chroma and len(hreq) > 1 else 0]),
-                Plan(vreq[1 if chroma and len(vreq) > 1 else 0]))
+        return (make_plan(hreq[1 if chroma and len(hreq) > 1 else 0]),
+                make_plan(vreq[1 if chroma and len(vreq) > 1 else 0]))
 
     if process_h and process_v:
         # Every plane of the frame batch in one set of launches.

@@ -36,7 +36,9 @@ EXPORTS = (
     "dsmvc_hip_inverse_2d_batch_noalloc", "dsmvc_hip_plan_import",
     "dsmvc_hip_inverse_rows_host", "dsmvc_hip_inverse_columns_host", "dsmvc_hip_inverse_2d_host",
     "dsmvc_hip_executor_set_plan_cache", "dsmvc_hip_executor_plan_cache_stats",
+    "dsmvc_hip_plan_build_custom",
 )
+KERNEL_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_double, ctypes.c_void_p)
 BATCH_GRAPH = 2
 
 
@@ -254,6 +256,8 @@ def load():
                                        vp, u32]
     lib.dsmvc_hip_inverse_2d_batch.argtypes = [vp, vp, vp, i32, vp, pd, pd, vp, pd, pd, i32,
                                                ctypes.POINTER(ConversionC), vp, u32]
+    lib.dsmvc_hip_plan_build_custom.argtypes = [ctypes.POINTER(AxisRequestC), KERNEL_FN, vp,
+                                                ctypes.POINTER(vp)]
     lib.dsmvc_hip_inverse_rows_host.argtypes = [vp, vp, vp, pd, vp, pd, i32]
     lib.dsmvc_hip_inverse_columns_host.argtypes = [vp, vp, vp, pd, vp, pd, i32]
     lib.dsmvc_hip_inverse_2d_host.argtypes = [vp, vp, vp, i32, vp, pd, pd, vp, pd, pd, i32,
