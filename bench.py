@@ -365,6 +365,7 @@ def served_stream(torch, dh, dev, seconds, rate=400.0, seed=5):
     server = dh.Server(ex, memory_budget=8 << 30, slots=4)
     ids = [None] * len(reqs)
     lag = np.zeros(len(reqs))
+    submit_s = np.zeros(len(reqs))
     start = time.perf_counter()
 
     def client():
@@ -372,8 +373,10 @@ def served_stream(torch, dh, dev, seconds, rate=400.0, seed=5):
             delay = t - (time.perf_counter() - start)
             if delay > 0:
                 time.sleep(delay)
-            lag[k] = (time.perf_counter() - start) - t
+            t_call = time.perf_counter()
+            lag[k] = (t_call - start) - t
             ids[k] = server.submit(ins, outs, kernel=kernel, taps=taps, b=b, c=c, f64_mode=mode)
+            submit_s[k] = time.perf_counter() - t_call
 
     thread = threading.Thread(target=client)
     thread.start()
@@ -385,6 +388,7 @@ def served_stream(torch, dh, dev, seconds, rate=400.0, seed=5):
     elapsed = time.perf_counter() - start
     stats = server.stats()
     server.close()
+    cache = ex.plan_cache_stats()
     lat = np.array(lat) * 1e3
     planes = sum(len(r[1]) for r in reqs)
 
@@ -454,6 +458,10 @@ def served_stream(torch, dh, dev, seconds, rate=400.0, seed=5):
         "latency_ms_float32_requests": pct(lat[~f64]),
         "latency_ms_float64_requests": pct(lat[f64]),
         "client_submit_lag_ms_p99": round(float(np.percentile(lag, 99)) * 1e3, 3),
+        "submit_call_ms": {"p50": round(float(np.percentile(submit_s, 50)) * 1e3, 3),
+                           "p99": round(float(np.percentile(submit_s, 99)) * 1e3, 3)},
+        "plan_cache": {k: int(cache[k]) for k in ("entries", "bytes", "hits", "misses",
+                                                 "evictions")},
         "rounds": stats["rounds"], "largest_round": stats["largest_round"],
         "table_planes": stats["table_planes"], "fallback_planes": stats["fallback_planes"],
         "memory_high_water_bytes": stats["memory_high_water"],

@@ -336,7 +336,7 @@ int dsmvc_hip_inverse_2d_host(dsmvc_hip_executor *executor, const dsmvc_hip_plan
 /* ---- device plan cache ----------------------------------------------------
  * The executor keeps each plan's device image (and the tables built for it)
  * in an LRU keyed by the plan's build serial -- one per planner request key
- * (kernel, geometry, padding, precision) -- bounded by bytes: 1024 MiB by
+ * (kernel, geometry, padding, precision) -- bounded by bytes: 4096 MiB by
  * default, DSMVC_HIP_PLAN_CACHE_MB, or this call.  The reference bounds its
  * packed-plan cache the same way (src/cuda/cuda_executor.cpp:219-235,
  * 1936-2020).  Images in use by a call, a batch or a served request are

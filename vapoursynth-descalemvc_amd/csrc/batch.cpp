@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cstring>
 #include <cstdlib>
 #include <exception>
 #include <stdexcept>
@@ -60,35 +61,64 @@ WindowTables::~WindowTables() {
 
 void WindowTables::clear() { groups_.clear(); }
 
-void WindowTables::add(const hip::WinJob &job, int band, bool rows, hip::Sample kind) {
-    const int width = job.plan.ell_width;
-    const bool block16 = job.plan.block16 != 0;
+WindowTables::Group &WindowTables::group_for(Kind type, int band, int width, bool rows,
+                                             bool block16, hip::Sample kind) {
     auto it = std::find_if(groups_.begin(), groups_.end(), [&](const Group &g) {
-        return g.rows == rows && g.band == band && g.width == width && g.block16 == block16
-            && g.kind == kind;
+        return g.type == type && g.rows == rows && g.band == band && g.width == width
+            && g.block16 == block16 && g.kind == kind;
     });
-    if (it == groups_.end()) {
-        groups_.push_back(Group{band, width, rows, block16, kind, {}, 0, 0});
-        it = groups_.end() - 1;
-    }
-    hip::WinJob j = job;
-    j.first_item = it->items;
-    it->items += items_of(j);
-    it->jobs.push_back(j);
+    if (it != groups_.end()) return *it;
+    Group g;
+    g.type = type;
+    g.band = band;
+    g.width = width;
+    g.rows = rows;
+    g.block16 = block16;
+    g.kind = kind;
+    groups_.push_back(std::move(g));
+    return groups_.back();
 }
+
+void WindowTables::add(const hip::WinJob &job, int band, bool rows, hip::Sample kind) {
+    Group &g = group_for(Kind::f32, band, job.plan.ell_width, rows, job.plan.block16 != 0, kind);
+    hip::WinJob j = job;
+    j.first_item = g.items;
+    g.items += items_of(j);
+    g.jobs.push_back(j);
+}
+
+void WindowTables::add64(const hip::Win64Job &job, int band, bool rows, hip::Sample kind) {
+    Group &g = group_for(Kind::f64, band, job.plan.ell_width, rows, false, kind);
+    hip::Win64Job j = job;
+    j.first_item = g.items;
+    g.items += static_cast<long>(j.frames) * j.plan.chunks * j.groups;
+    g.jobs64.push_back(j);
+}
+
+void WindowTables::add_check(const hip::CheckJob &job) {
+    Group &g = group_for(Kind::check, 0, 0, true, false, hip::Sample::f32);
+    hip::CheckJob j = job;
+    j.first_item = g.items;
+    g.items += static_cast<long>(j.frames) * j.rows;
+    g.checks.push_back(j);
+}
+
 
 // Tables go through a ring of slots, each a pinned host buffer, a device
 // buffer and an event recorded after the launches that read it: an upload
 // reuses the oldest slot once its event has passed, so uploads and launches
 // stay asynchronous (a copy from pageable memory would wait for the stream).
 int WindowTables::upload(hipStream_t stream) {
-    // Horizontal groups first: every vertical pass reads an intermediate
-    // some horizontal group writes.
-    std::stable_partition(groups_.begin(), groups_.end(), [](const Group &g) { return g.rows; });
-    std::size_t count = 0;
+    // Checks first, then horizontal groups: every vertical pass reads an
+    // intermediate some horizontal group writes.
+    std::stable_sort(groups_.begin(), groups_.end(), [](const Group &a, const Group &b) {
+        auto rank = [](const Group &g) { return g.type == Kind::check ? 0 : (g.rows ? 1 : 2); };
+        return rank(a) < rank(b);
+    });
+    std::size_t bytes = 0;
     for (Group &g : groups_) {
-        g.offset = count;
-        count += g.jobs.size();
+        g.offset = bytes;
+        bytes += (g.bytes() + 15) & ~std::size_t{15};
     }
     check(hipSetDevice(device_), "hipSetDevice failed");
     current_ = (current_ + 1) % kSlots;
@@ -98,22 +128,24 @@ int WindowTables::upload(hipStream_t stream) {
         slot.pending = false;
     }
     if (!slot.done) check(hipEventCreateWithFlags(&slot.done, hipEventDisableTiming), "event creation failed");
-    if (count > slot.capacity) {
+    if (bytes > slot.capacity) {
+        const std::size_t want = std::max<std::size_t>(bytes, slot.capacity * 2);
         if (slot.device) check(hipFree(slot.device), "job table release failed");
         if (slot.host) check(hipHostFree(slot.host), "job table release failed");
         slot.device = nullptr;
         slot.host = nullptr;
-        const std::size_t want = std::max<std::size_t>(count, slot.capacity * 2);
-        check(hipHostMalloc(reinterpret_cast<void **>(&slot.host), want * sizeof(hip::WinJob)),
-              "job table allocation failed");
-        check(hipMalloc(reinterpret_cast<void **>(&slot.device), want * sizeof(hip::WinJob)),
-              "job table allocation failed");
+        slot.capacity = 0;
+        check(hipHostMalloc(reinterpret_cast<void **>(&slot.host), want), "job table allocation failed");
+        check(hipMalloc(reinterpret_cast<void **>(&slot.device), want), "job table allocation failed");
         slot.capacity = want;
     }
-    for (const Group &g : groups_)
-        std::copy(g.jobs.begin(), g.jobs.end(), slot.host + g.offset);
-    if (count) {
-        const std::size_t bytes = count * sizeof(hip::WinJob);
+    for (const Group &g : groups_) {
+        unsigned char *at = slot.host + g.offset;
+        if (!g.jobs.empty()) std::memcpy(at, g.jobs.data(), g.jobs.size() * sizeof(hip::WinJob));
+        if (!g.jobs64.empty()) std::memcpy(at, g.jobs64.data(), g.jobs64.size() * sizeof(hip::Win64Job));
+        if (!g.checks.empty()) std::memcpy(at, g.checks.data(), g.checks.size() * sizeof(hip::CheckJob));
+    }
+    if (bytes) {
         if (stream)
             check(hipMemcpyAsync(slot.device, slot.host, bytes, hipMemcpyHostToDevice, stream),
                   "job table upload failed");
@@ -129,10 +161,24 @@ void WindowTables::launch(hipStream_t stream) {
     Slot &slot = slots_[current_];
     for (const Group &g : groups_) {
         if (g.items == 0) continue;
-        check(hip::launch_window_table(slot.device + g.offset, static_cast<int>(g.jobs.size()),
-                                       g.items, g.band, g.width, g.rows, g.block16, g.kind, order_,
-                                       stream),
-              "table launch failed");
+        unsigned char *at = slot.device + g.offset;
+        const int count = static_cast<int>(g.count());
+        hipError_t e = hipSuccess;
+        switch (g.type) {
+        case Kind::check:
+            e = hip::launch_check_finite_table(reinterpret_cast<const hip::CheckJob *>(at), count,
+                                               g.items, stream);
+            break;
+        case Kind::f32:
+            e = hip::launch_window_table(reinterpret_cast<const hip::WinJob *>(at), count, g.items,
+                                         g.band, g.width, g.rows, g.block16, g.kind, order_, stream);
+            break;
+        case Kind::f64:
+            e = hip::launch_window64_table(reinterpret_cast<const hip::Win64Job *>(at), count,
+                                           g.items, g.band, g.width, g.rows, g.kind, stream);
+            break;
+        }
+        check(e, "table launch failed");
     }
     if (record_) {
         check(hipEventRecord(slot.done, stream), "event record failed");

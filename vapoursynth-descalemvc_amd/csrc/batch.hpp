@@ -53,10 +53,13 @@ struct BatchStats {
 };
 
 // Groups windowed axis jobs by (direction, band class, tap width, staging
-// block size, sample kind) into one
-// device buffer of job tables and launches them: all horizontal groups
-// first, then all vertical groups, on one stream.  Reusable: `build` lays
-// out a new set of jobs (growing the device buffer), `launch` enqueues.
+// block size, sample kind) into one device buffer of job tables and launches
+// them: the Float64 route's input checks first, then all horizontal groups,
+// then all vertical groups, on one stream.  F32 jobs (WinJob) and Float64
+// jobs (Win64Job) form separate groups; Float64 jobs and checks report a
+// non-finite value through their own flag pointers.  Reusable:
+// `clear` then `add*` lay out a new set of jobs, `upload` (growing the
+// buffers) and `launch` enqueue them.
 class WindowTables {
 public:
     explicit WindowTables(int device);
@@ -67,6 +70,8 @@ public:
     void clear();
     // `kind`: input samples of a horizontal job, output samples of a vertical one.
     void add(const hip::WinJob &job, int band, bool rows, hip::Sample kind);
+    void add64(const hip::Win64Job &job, int band, bool rows, hip::Sample kind);
+    void add_check(const hip::CheckJob &job);
     // Uploads the tables (stream-ordered when `stream` is given, else
     // synchronously) and returns the number of launches `launch` makes.
     int upload(hipStream_t stream);
@@ -77,22 +82,32 @@ public:
     [[nodiscard]] std::int64_t items() const;
 
 private:
+    enum class Kind : std::uint8_t { check = 0, f32 = 1, f64 = 2 };
     struct Group {
+        Kind type = Kind::f32;
         int band = 0, width = 0;
         bool rows = false;
-        bool block16 = false;       // 16-row staging blocks
+        bool block16 = false;       // 16-row staging blocks (F32)
         hip::Sample kind = hip::Sample::f32;
         std::vector<hip::WinJob> jobs;
-        std::size_t offset = 0;     // first job's index in the device buffer
+        std::vector<hip::Win64Job> jobs64;
+        std::vector<hip::CheckJob> checks;
+        std::size_t offset = 0;     // byte offset of the first record in the buffers
         long items = 0;
+        [[nodiscard]] std::size_t count() const { return jobs.size() + jobs64.size() + checks.size(); }
+        [[nodiscard]] std::size_t bytes() const {
+            return jobs.size() * sizeof(hip::WinJob) + jobs64.size() * sizeof(hip::Win64Job)
+                   + checks.size() * sizeof(hip::CheckJob);
+        }
     };
     struct Slot {
-        hip::WinJob *host = nullptr;     // pinned
-        hip::WinJob *device = nullptr;
-        std::size_t capacity = 0;
+        unsigned char *host = nullptr;     // pinned
+        unsigned char *device = nullptr;
+        std::size_t capacity = 0;          // bytes
         hipEvent_t done = nullptr;
         bool pending = false;
     };
+    Group &group_for(Kind type, int band, int width, bool rows, bool block16, hip::Sample kind);
     static constexpr int kSlots = 3;
     int device_;
     std::vector<Group> groups_;

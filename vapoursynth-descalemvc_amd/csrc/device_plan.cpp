@@ -7,6 +7,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -21,6 +23,35 @@ void check(hipError_t e, const char *what) {
 }
 
 std::size_t align256(std::size_t v) { return (v + 255U) & ~std::size_t{255}; }
+
+// Host-to-device copy of a table, on a non-blocking stream of the current
+// device, waited for before returning.  A plain hipMemcpy runs on the null
+// stream, which first waits for all work queued on blocking streams; a plan
+// built while the executor's streams are busy (a served request with a new
+// geometry) would stall behind every queued kernel.
+hipError_t upload(void *dst, const void *src, std::size_t bytes) {
+    static std::mutex lock;
+    static std::map<int, hipStream_t> streams;   // one per device, kept until exit
+    int device = 0;
+    if (const hipError_t e = hipGetDevice(&device); e != hipSuccess) return e;
+    hipStream_t stream = nullptr;
+    {
+        std::lock_guard guard(lock);
+        auto it = streams.find(device);
+        if (it == streams.end()) {
+            if (const hipError_t e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
+                e != hipSuccess)
+                return e;
+            streams.emplace(device, stream);
+        } else {
+            stream = it->second;
+        }
+    }
+    if (const hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream);
+        e != hipSuccess)
+        return e;
+    return hipStreamSynchronize(stream);
+}
 
 } // namespace
 
@@ -432,8 +463,9 @@ bool build_window_plan64(const AxisPlan &p, const PackedPlan &packed, WindowPlan
                                 packed.zform != 0, kWindowTolerance64, out);
 }
 
-DevicePlan::DevicePlan(const AxisPlan &plan, int device)
-    : keep_(std::make_shared<const AxisPlan>(plan)), plan_(keep_.get()), device_(device) {
+DevicePlan::DevicePlan(const AxisPlan &plan, int device, ByteAccount account)
+    : keep_(std::make_shared<const AxisPlan>(plan)), plan_(keep_.get()), device_(device),
+      account_(std::move(account)) {
     const PackedPlan packed = pack_plan(*plan_);
     const auto n = static_cast<std::size_t>(packed.n);
     const auto nnz = static_cast<std::size_t>(packed.nnz);
@@ -452,13 +484,13 @@ DevicePlan::DevicePlan(const AxisPlan &plan, int device)
     std::memcpy(host.data() + o_wt64, packed.wt64.data(), nnz * 8);
     std::memcpy(host.data() + o_coef64, packed.coef64.data(), packed.coef64.size() * 8);
     check(hipMalloc(&memory_, bytes_), "device plan allocation failed");
-    const hipError_t copy = hipMemcpy(memory_, host.data(), bytes_, hipMemcpyHostToDevice);
+    const hipError_t copy = upload(memory_, host.data(), bytes_);
     if (copy != hipSuccess) {
         (void)hipFree(memory_);
         memory_ = nullptr;
         check(copy, "device plan upload failed");
     }
-    total_bytes_ = bytes_;
+    grow(bytes_);
     auto *base = static_cast<unsigned char *>(memory_);
     axis_.n = packed.n;
     axis_.m = packed.m;
@@ -473,7 +505,14 @@ DevicePlan::DevicePlan(const AxisPlan &plan, int device)
     axis_.zform = packed.zform ? 1 : 0;
 }
 
+void DevicePlan::grow(std::size_t bytes) {
+    total_bytes_.fetch_add(bytes, std::memory_order_relaxed);
+    if (account_) account_->fetch_add(bytes, std::memory_order_relaxed);
+}
+
 DevicePlan::~DevicePlan() {
+    if (account_) account_->fetch_sub(total_bytes_.load(std::memory_order_relaxed),
+                                      std::memory_order_relaxed);
     for (auto &entry : chunk_images_)
         if (entry.second.memory) (void)hipFree(entry.second.memory);
     if (window_image_.memory) (void)hipFree(window_image_.memory);
@@ -509,9 +548,7 @@ const DevChunks *DevicePlan::chunks(int max_chunks, bool rows) {
             check(hipSetDevice(device_), "hipSetDevice failed");
             check(hipMalloc(&image.memory, total), "chunk table allocation failed");
             image.bytes = total;
-            if (const hipError_t e = hipMemcpy(image.memory, host.data(), total,
-                                               hipMemcpyHostToDevice);
-                e != hipSuccess) {
+            if (const hipError_t e = upload(image.memory, host.data(), total); e != hipSuccess) {
                 (void)hipFree(image.memory);
                 check(e, "chunk table upload failed");
             }
@@ -530,7 +567,7 @@ const DevChunks *DevicePlan::chunks(int max_chunks, bool rows) {
             image.dev.blk_lo = reinterpret_cast<const std::int32_t *>(b + o_blk);
             image.ok = true;
             image.rows_feasible = cp.rows_feasible;
-            total_bytes_ += image.bytes;
+            grow(image.bytes);
         }
         it = chunk_images_.emplace(max_chunks, image).first;
     }
@@ -558,7 +595,7 @@ void *DevicePlan::upload_window(const WindowPlan &wp, DevWindow &dev, std::size_
     check(hipSetDevice(device_), "hipSetDevice failed");
     void *memory = nullptr;
     check(hipMalloc(&memory, total), "window table allocation failed");
-    if (const hipError_t e = hipMemcpy(memory, host.data(), total, hipMemcpyHostToDevice);
+    if (const hipError_t e = upload(memory, host.data(), total);
         e != hipSuccess) {
         (void)hipFree(memory);
         check(e, "window table upload failed");
@@ -590,7 +627,7 @@ const DevWindow *DevicePlan::window(bool rows) {
         WindowPlan wp;
         if (build_window_plan(*plan_, wp)) {
             image.memory = upload_window(wp, image.dev, image.bytes);
-            total_bytes_ += image.bytes;
+            grow(image.bytes);
             image.ok = true;
             image.rows_feasible = wp.rows_feasible;
         }
@@ -619,7 +656,7 @@ const DevWindow *DevicePlan::solve_window(bool rows) {
         WindowPlan wp;
         if (build_solve_window_plan(*plan_, wp)) {
             image.memory = upload_window(wp, image.dev, image.bytes);
-            total_bytes_ += image.bytes;
+            grow(image.bytes);
             image.dev.m = plan_->destination_size;
             image.ok = true;
             image.rows_feasible = wp.rows_feasible;
@@ -651,7 +688,7 @@ const DevWindow64 *DevicePlan::window64() {
             check(hipSetDevice(device_), "hipSetDevice failed");
             void *memory = nullptr;
             check(hipMalloc(&memory, total), "window table allocation failed");
-            if (const hipError_t e = hipMemcpy(memory, host.data(), total, hipMemcpyHostToDevice);
+            if (const hipError_t e = upload(memory, host.data(), total);
                 e != hipSuccess) {
                 (void)hipFree(memory);
                 check(e, "window table upload failed");
@@ -659,7 +696,7 @@ const DevWindow64 *DevicePlan::window64() {
             (void)hipSetDevice(current);
             image.memory = memory;
             image.bytes = total;
-            total_bytes_ += total;
+            grow(total);
             auto *b = static_cast<unsigned char *>(image.memory);
             image.dev.n = plan_->destination_size;
             image.dev.m = plan_->source_size;
@@ -754,7 +791,7 @@ const DevRefine *DevicePlan::refine() {
         check(hipSetDevice(device_), "hipSetDevice failed");
         void *memory = nullptr;
         check(hipMalloc(&memory, total), "refinement table allocation failed");
-        if (const hipError_t e = hipMemcpy(memory, host.data(), total, hipMemcpyHostToDevice);
+        if (const hipError_t e = upload(memory, host.data(), total);
             e != hipSuccess) {
             (void)hipFree(memory);
             check(e, "refinement table upload failed");
@@ -762,7 +799,7 @@ const DevRefine *DevicePlan::refine() {
         (void)hipSetDevice(current);
         refine_memory_ = memory;
         refine_bytes_ = total;
-        total_bytes_ += total;
+        grow(total);
         refine_built_ = true;
         auto *b = static_cast<unsigned char *>(refine_memory_);
         refine_.n = n;

@@ -90,10 +90,15 @@ bool build_window_plan64(const AxisPlan &plan, const PackedPlan &packed, WindowP
 // given right-hand side of n elements per vector.
 bool build_solve_window_plan(const AxisPlan &plan, WindowPlan &out);
 
+// A running total of device bytes, shared by the images that report to it.
+using ByteAccount = std::shared_ptr<std::atomic<std::size_t>>;
+
 // One device allocation holding a plan's image (and its chunk partitions).
+// Every byte it allocates, now or in a later table build, is added to
+// `account` (when given) and taken off again when the image is destroyed.
 class DevicePlan {
 public:
-    DevicePlan(const AxisPlan &plan, int device);
+    DevicePlan(const AxisPlan &plan, int device, ByteAccount account = {});
     ~DevicePlan();
     DevicePlan(const DevicePlan &) = delete;
     DevicePlan &operator=(const DevicePlan &) = delete;
@@ -165,6 +170,8 @@ private:
     void *refine_memory_ = nullptr;
     std::size_t refine_bytes_ = 0;
     std::atomic<std::size_t> total_bytes_{0};
+    ByteAccount account_;
+    void grow(std::size_t bytes);
     bool refine_built_ = false;
 };
 

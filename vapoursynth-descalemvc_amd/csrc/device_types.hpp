@@ -199,6 +199,37 @@ struct DevWindow64 {
     const std::int32_t *blk_lo;   // [rows / 8]
 };
 
+// One windowed FP64 axis pass of one plan over `frames` planes, as a unit of
+// a (possibly multi-plan) FP64 launch; the FP64 counterpart of WinJob.  Items
+// are (frame, chunk, group of 64 vectors), the group varying fastest.  Rows
+// pass: `src` F32 or integer samples, `dst` the FP64 intermediate; columns
+// pass: `src` FP64, `dst` samples of the launch's kind.
+struct Win64Job {
+    DevWindow64 plan;
+    const void *src;
+    long s_row, s_frame;
+    void *dst;
+    long d_row, d_frame;
+    std::int32_t nvec;
+    std::int32_t frames;
+    std::int32_t groups;      // ceil(nvec / 64)
+    std::int32_t pad_;
+    long first_item;          // items of the jobs before this one in its table
+    int *flag;                // set to 1 when a stored value is not finite
+    DevConv conv;
+};
+
+// The Float64 route's input check of one F32 plane batch: items are
+// (frame, row); a non-finite sample sets the job's flag.
+struct CheckJob {
+    const float *src;
+    long s_row, s_frame;
+    std::int32_t rows, cols;
+    std::int32_t frames, pad_;
+    long first_item;
+    int *flag;
+};
+
 // One windowed FP64 axis pass over `frames` planes: rows pass F32 or integer
 // source to an FP64 intermediate, columns pass FP64 intermediate to F32 or
 // integer samples (`conv`).  `flag` is set when a stored value is not finite.

@@ -28,6 +28,20 @@ void check(hipError_t e, const char *what) {
 
 std::size_t align256(std::size_t v) { return (v + 255U) & ~std::size_t{255}; }
 
+// Reads a Float64 flag (into pinned `host`) once its stream has finished.
+// On the stream itself: a null-stream copy would first wait for the work of
+// every other blocking stream (the other execution slots).
+int read_flag(hipStream_t stream, const int *flag, int *host) {
+    if (stream) {
+        check(hipMemcpyAsync(host, flag, sizeof *host, hipMemcpyDeviceToHost, stream),
+              "flag readback failed");
+        check(hipStreamSynchronize(stream), "flag readback failed");
+    } else {
+        check(hipMemcpy(host, flag, sizeof *host, hipMemcpyDeviceToHost), "flag readback failed");
+    }
+    return *host;
+}
+
 std::ptrdiff_t sample_size(hip::Sample kind) {
     switch (kind) {
     case hip::Sample::u8: return 1;
@@ -120,6 +134,7 @@ HipExecutor::~HipExecutor() {
     for (auto &entry : scratch_) {
         if (entry.second.buffer) (void)hipFree(entry.second.buffer);
         if (entry.second.flag) (void)hipFree(entry.second.flag);
+        if (entry.second.flag_host) (void)hipHostFree(entry.second.flag_host);
     }
     scratch_.clear();
     for (Slot &slot : slots_) {
@@ -138,6 +153,7 @@ void HipExecutor::release_stream(hipStream_t stream) {
     if (auto it = scratch_.find(stream); it != scratch_.end()) {
         if (it->second.buffer) (void)hipFree(it->second.buffer);
         if (it->second.flag) (void)hipFree(it->second.flag);
+        if (it->second.flag_host) (void)hipHostFree(it->second.flag_host);
         scratch_.erase(it);
     }
 }
@@ -186,7 +202,7 @@ std::shared_ptr<hip::DevicePlan> HipExecutor::device_plan(const AxisPlan &plan) 
         plans_.erase(it);
     }
     check(hipSetDevice(device_), "hipSetDevice failed");
-    auto image = std::make_shared<hip::DevicePlan>(plan, device_);
+    auto image = std::make_shared<hip::DevicePlan>(plan, device_, cache_bytes_);
     lru_.push_front(key);
     plans_[key] = PlanEntry{image, lru_.begin()};
     ++plan_stats_.misses;
@@ -194,16 +210,19 @@ std::shared_ptr<hip::DevicePlan> HipExecutor::device_plan(const AxisPlan &plan) 
     return image;
 }
 
-// Drops least recently used images until the cache fits its limit, skipping
-// images someone holds (a call in progress, a batch, a served request).  The
-// device is synchronized first, so no queued kernel still reads an image
-// that is freed.
+// Once the executor's images exceed the cache limit, drops least recently
+// used images until they are down to three quarters of it, skipping images
+// someone holds (a call in progress, a batch, a served request).  The device
+// is synchronized first, so no queued kernel still reads an image that is
+// freed; evicting a quarter of the limit at a time keeps those
+// synchronizations rare.  The byte total is kept by the images themselves,
+// so the check costs the same however many images are cached.
 void HipExecutor::evict_plans() {
-    std::size_t total = 0;
-    for (auto &entry : plans_) total += entry.second.image->total_bytes();
+    std::size_t total = cache_bytes_->load(std::memory_order_relaxed);
     if (total <= plan_cache_limit_) return;
+    const std::size_t target = plan_cache_limit_ / 4 * 3;
     std::vector<std::uint64_t> victims;
-    for (auto k = lru_.rbegin(); k != lru_.rend() && total > plan_cache_limit_; ++k) {
+    for (auto k = lru_.rbegin(); k != lru_.rend() && total > target; ++k) {
         const PlanEntry &e = plans_.at(*k);
         if (e.image.use_count() > 1) continue;
         total -= e.image->total_bytes();
@@ -324,10 +343,8 @@ void HipExecutor::run_call(hipStream_t stream, bool async, const char *what, Enq
 void HipExecutor::complete(hipStream_t stream, Scratch &s, const char *what) {
     check(stream ? hipStreamSynchronize(stream) : hipDeviceSynchronize(), "execution failed");
     if (s.flag_armed) {
-        int flag = 0;
-        check(hipMemcpy(&flag, s.flag, sizeof flag, hipMemcpyDeviceToHost), "flag readback failed");
         s.flag_armed = false;
-        if (flag) throw std::runtime_error(what);
+        if (read_flag(stream, s.flag, s.flag_host)) throw std::runtime_error(what);
     }
 }
 
@@ -355,6 +372,9 @@ std::shared_ptr<hip::DevicePlan> HipExecutor::plan_image(const AxisPlan &plan) {
 HipExecutor::Scratch &HipExecutor::scratch(hipStream_t stream, std::size_t bytes) {
     Scratch &s = scratch_[stream];
     if (!s.flag) check(hipMalloc(reinterpret_cast<void **>(&s.flag), 256), "scratch allocation failed");
+    if (!s.flag_host)
+        check(hipHostMalloc(reinterpret_cast<void **>(&s.flag_host), sizeof(int)),
+              "scratch allocation failed");
     if (s.bytes < bytes) {
         const std::size_t grown = align256(std::max(bytes, s.bytes + s.bytes / 2));
         if (s.buffer) {
@@ -374,10 +394,8 @@ void HipExecutor::finish(hipStream_t stream, Scratch &s, bool async, const char 
     if (async) return;
     check(stream ? hipStreamSynchronize(stream) : hipDeviceSynchronize(), "execution failed");
     if (s.flag_armed) {
-        int flag = 0;
-        check(hipMemcpy(&flag, s.flag, sizeof flag, hipMemcpyDeviceToHost), "flag readback failed");
         s.flag_armed = false;
-        if (flag) throw std::runtime_error(what);
+        if (read_flag(stream, s.flag, s.flag_host)) throw std::runtime_error(what);
     }
 }
 
@@ -752,24 +770,41 @@ HipExecutor::Scratch &HipExecutor::enqueue_planes(const PlaneJob *planes, std::i
     const bool windows = mode_ == SolveMode::automatic || mode_ == SolveMode::windowed;
     // The device images stay pinned until every launch of this call is queued.
     std::vector<std::shared_ptr<hip::DevicePlan>> pins;
-    std::vector<const hip::DevWindow *> wh(static_cast<std::size_t>(count), nullptr);
-    std::vector<const hip::DevWindow *> wv(static_cast<std::size_t>(count), nullptr);
-    std::vector<std::size_t> offset(static_cast<std::size_t>(count), 0);
+    const auto n = static_cast<std::size_t>(count);
+    std::vector<const hip::DevWindow *> wh(n, nullptr), wv(n, nullptr);
+    std::vector<const hip::DevWindow64 *> wh64(n, nullptr), wv64(n, nullptr);
+    std::vector<std::size_t> offset(n, 0);
     std::size_t total = 0;
-    for (std::int32_t k = 0; k < count; ++k) {
+    for (std::size_t k = 0; k < n; ++k) {
         const AxisPlan &h = *planes[k].horizontal, &v = *planes[k].vertical;
-        if (!windows || h.requires_float64() || v.requires_float64()) continue;
-        pins.push_back(device_plan(h));
-        wh[static_cast<std::size_t>(k)] = pins.back()->window(true);
-        pins.push_back(device_plan(v));
-        wv[static_cast<std::size_t>(k)] = pins.back()->window(false);
-        if (!wh[static_cast<std::size_t>(k)] || !wv[static_cast<std::size_t>(k)]) {
-            wh[static_cast<std::size_t>(k)] = wv[static_cast<std::size_t>(k)] = nullptr;
-            continue;
+        if (!windows) continue;
+        const bool f64 = float64_route(h) || float64_route(v);
+        // Held by value: a reference into `pins` would dangle once it grows.
+        const std::shared_ptr<hip::DevicePlan> ih = device_plan(h), iv = device_plan(v);
+        pins.push_back(ih);
+        pins.push_back(iv);
+        std::size_t elem = 4;
+        if (!f64) {
+            wh[k] = ih->window(true);
+            wv[k] = iv->window(false);
+            if (!wh[k] || !wv[k]) {
+                wh[k] = wv[k] = nullptr;
+                continue;
+            }
+        } else {
+            // The Float64 route on the windowed FP64 kernels, in the same
+            // table set, when both plans' FP64 recurrences decay in time.
+            wh64[k] = ih->window64();
+            wv64[k] = iv->window64();
+            if (!wh64[k] || !wv64[k]) {
+                wh64[k] = wv64[k] = nullptr;
+                continue;
+            }
+            elem = 8;
         }
-        offset[static_cast<std::size_t>(k)] = total;
+        offset[k] = total;
         total += align256(static_cast<std::size_t>(padded_width(h.destination_size)) * v.source_size
-                          * static_cast<std::size_t>(planes[k].batch.frames) * sizeof(float));
+                          * static_cast<std::size_t>(planes[k].batch.frames) * elem);
     }
     Scratch *last = nullptr;
     if (total) {
@@ -778,16 +813,61 @@ HipExecutor::Scratch &HipExecutor::enqueue_planes(const PlaneJob *planes, std::i
         WindowTables &tables = stream_tables(stream);
         tables.clear();
         const Sample sk = sample_of(kind);
-        for (std::int32_t k = 0; k < count; ++k) {
-            if (!wh[static_cast<std::size_t>(k)]) continue;
+        bool armed = false;
+        for (std::size_t k = 0; k < n; ++k) {
+            if (!wh[k] && !wh64[k]) continue;
             const PlaneJob &p = planes[k];
             const AxisPlan &h = *p.horizontal, &v = *p.vertical;
             const std::ptrdiff_t row = padded_width(h.destination_size);
             const std::ptrdiff_t frame = row * v.source_size;
-            float *inter = reinterpret_cast<float *>(static_cast<unsigned char *>(s.buffer)
-                                                     + offset[static_cast<std::size_t>(k)]);
-            hip::WinJob a{};
-            a.plan = *wh[static_cast<std::size_t>(k)];
+            void *inter = static_cast<unsigned char *>(s.buffer) + offset[k];
+            const DevConv conv = make_conv(p.conversion);
+            if (wh[k]) {
+                hip::WinJob a{};
+                a.plan = *wh[k];
+                a.src = p.batch.input;
+                a.s_row = p.batch.input_row_stride;
+                a.s_frame = p.batch.input_frame_stride;
+                a.dst = inter;
+                a.d_row = row;
+                a.d_frame = frame;
+                a.nvec = v.source_size;
+                a.frames = p.batch.frames;
+                a.groups = (a.nvec + 63) / 64;
+                a.conv = conv;
+                tables.add(a, hip::window_band_class(h.half_bandwidth), true, sk);
+                hip::WinJob b{};
+                b.plan = *wv[k];
+                b.src = inter;
+                b.s_row = row;
+                b.s_frame = frame;
+                b.dst = p.batch.output;
+                b.d_row = p.batch.output_row_stride;
+                b.d_frame = p.batch.output_frame_stride;
+                b.nvec = h.destination_size;
+                b.frames = p.batch.frames;
+                b.groups = (b.nvec + 63) / 64;
+                b.conv = conv;
+                tables.add(b, hip::window_band_class(v.half_bandwidth), false, sk);
+                continue;
+            }
+            if (!armed) {
+                arm_flag(s, stream);
+                armed = true;
+            }
+            if (sk == Sample::f32) {
+                hip::CheckJob c{};
+                c.src = static_cast<const float *>(p.batch.input);
+                c.s_row = p.batch.input_row_stride;
+                c.s_frame = p.batch.input_frame_stride;
+                c.rows = v.source_size;
+                c.cols = h.source_size;
+                c.frames = p.batch.frames;
+                c.flag = s.flag;
+                tables.add_check(c);
+            }
+            hip::Win64Job a{};
+            a.plan = *wh64[k];
             a.src = p.batch.input;
             a.s_row = p.batch.input_row_stride;
             a.s_frame = p.batch.input_frame_stride;
@@ -797,10 +877,11 @@ HipExecutor::Scratch &HipExecutor::enqueue_planes(const PlaneJob *planes, std::i
             a.nvec = v.source_size;
             a.frames = p.batch.frames;
             a.groups = (a.nvec + 63) / 64;
-            a.conv = make_conv(p.conversion);
-            tables.add(a, hip::window_band_class(h.half_bandwidth), true, sk);
-            hip::WinJob b{};
-            b.plan = *wv[static_cast<std::size_t>(k)];
+            a.flag = s.flag;
+            a.conv = conv;
+            tables.add64(a, hip::window_band_class(h.half_bandwidth), true, sk);
+            hip::Win64Job b{};
+            b.plan = *wv64[k];
             b.src = inter;
             b.s_row = row;
             b.s_frame = frame;
@@ -810,19 +891,28 @@ HipExecutor::Scratch &HipExecutor::enqueue_planes(const PlaneJob *planes, std::i
             b.nvec = h.destination_size;
             b.frames = p.batch.frames;
             b.groups = (b.nvec + 63) / 64;
-            b.conv = a.conv;
-            tables.add(b, hip::window_band_class(v.half_bandwidth), false, sk);
+            b.flag = s.flag;
+            b.conv = conv;
+            tables.add64(b, hip::window_band_class(v.half_bandwidth), false, sk);
         }
         tables.upload(stream);
         tables.launch(stream);
     }
-    for (std::int32_t k = 0; k < count; ++k) {
-        if (wh[static_cast<std::size_t>(k)]) continue;
+    for (std::size_t k = 0; k < n; ++k) {
+        if (wh[k] || wh64[k]) continue;
         const PlaneJob &p = planes[k];
         last = &solve_2d(*p.horizontal, *p.vertical, kind, p.batch, p.conversion, stream);
     }
     if (!last) last = &scratch(stream, 0);
     return *last;
+}
+
+int *HipExecutor::float64_flag(hipStream_t stream) {
+    std::lock_guard lock(mutex_);
+    check(hipSetDevice(device_), "hipSetDevice failed");
+    Scratch &s = scratch(stream, 0);
+    arm_flag(s, stream);
+    return s.flag;
 }
 
 void HipExecutor::inverse_2d_batch(const AxisPlan &h, const AxisPlan &v, SampleKind kind,

@@ -166,6 +166,11 @@ public:
 
     // Waits for `stream` and reports a deferred Float64 non-finite failure.
     void synchronize(hipStream_t stream);
+    // The device flag of `stream`'s Float64 failures, armed: cleared on the
+    // stream unless an unreported failure is pending.  Table launches of
+    // Float64 jobs made by other components on that stream may point their
+    // jobs at it; synchronize(stream) reports it.
+    int *float64_flag(hipStream_t stream);
     // Waits for `stream` and frees the scratch and job tables kept for it.
     // Call before destroying a stream this executor has run work on (the
     // executor keeps per-stream state until then, or until it is destroyed).
@@ -194,7 +199,7 @@ public:
     // Device plan cache: images keyed by the plan's build serial (one per
     // planner request key), least recently used first out once their bytes
     // (device image plus every table built for it) exceed the limit --
-    // DSMVC_HIP_PLAN_CACHE_MB, default 1024 MiB (the reference bounds its
+    // DSMVC_HIP_PLAN_CACHE_MB, default 4096 MiB (the reference bounds its
     // packed-plan LRU the same way, src/cuda/cuda_executor.cpp:219-235,
     // 1936-2020).  Images in use (held by a call, a batch or a served
     // request) are never evicted.
@@ -210,6 +215,7 @@ private:
         void *buffer = nullptr;
         std::size_t bytes = 0;
         int *flag = nullptr;
+        int *flag_host = nullptr;   // pinned: the flag's readback lands here
         bool flag_armed = false;
     };
 
@@ -312,7 +318,10 @@ private:
     static std::uint64_t plan_key(const AxisPlan &plan);
     std::unordered_map<std::uint64_t, PlanEntry> plans_;
     std::list<std::uint64_t> lru_;   // most recent first
-    std::size_t plan_cache_limit_ = std::size_t{1024} << 20;
+    std::size_t plan_cache_limit_ = std::size_t{4096} << 20;
+    // Device bytes of every image this executor built that is still alive
+    // (cached, or evicted but held by a call, batch or served request).
+    hip::ByteAccount cache_bytes_ = std::make_shared<std::atomic<std::size_t>>(0);
     PlanCacheStats plan_stats_{};
     std::unordered_map<hipStream_t, Scratch> scratch_;
     bool sealed_ = false;

@@ -88,18 +88,31 @@ DSMVC_WINDOW_EXTERN(7, 16)
 
 // Windowed FP64 pass of the Float64 route (kernels_window64.hpp): rows
 // F32/u8/u16 source -> FP64 intermediate, columns FP64 intermediate ->
-// F32/u8/u16; one launch per pass, instantiated per band class in
-// kernels_window64_w{1,3,5,7}.hip.
+// F32/u8/u16, instantiated per band class in kernels_window64_w{1,3,5,7}.hip.
 // `kind`: sample type of the rows pass's source or the columns pass's output.
+// launch_window64 runs one pass (its job passed by value; a non-finite
+// stored value sets a.flag); the table form runs `count` jobs from a device
+// table, `items` items in all, every job of one band class, tap width,
+// direction and sample kind (a non-finite stored value sets its job's flag).
 hipError_t launch_window64(const Win64Args &a, int band, int width, bool rows, Sample kind,
                            hipStream_t s);
+hipError_t launch_window64_table(const Win64Job *jobs, int count, long items, int band, int width,
+                                 bool rows, Sample kind, hipStream_t s);
 template <int W>
-hipError_t launch_window64_band(const Win64Args &a, int width, bool rows, Sample kind,
-                                hipStream_t s);
-extern template hipError_t launch_window64_band<1>(const Win64Args &, int, bool, Sample, hipStream_t);
-extern template hipError_t launch_window64_band<3>(const Win64Args &, int, bool, Sample, hipStream_t);
-extern template hipError_t launch_window64_band<5>(const Win64Args &, int, bool, Sample, hipStream_t);
-extern template hipError_t launch_window64_band<7>(const Win64Args &, int, bool, Sample, hipStream_t);
+hipError_t launch_window64_table_band(const Win64Job *jobs, int count, long items,
+                                      const Win64Job &one, int width, bool rows, Sample kind,
+                                      hipStream_t s);
+#define DSMVC_WINDOW64_EXTERN(W)                                                                 \
+    extern template hipError_t launch_window64_table_band<W>(                                     \
+        const Win64Job *, int, long, const Win64Job &, int, bool, Sample, hipStream_t);
+DSMVC_WINDOW64_EXTERN(1)
+DSMVC_WINDOW64_EXTERN(3)
+DSMVC_WINDOW64_EXTERN(5)
+DSMVC_WINDOW64_EXTERN(7)
+#undef DSMVC_WINDOW64_EXTERN
+// The Float64 route's input check over `count` F32 plane batches (items =
+// frames x rows in all): a non-finite sample sets its job's flag.
+hipError_t launch_check_finite_table(const CheckJob *jobs, int count, long items, hipStream_t s);
 
 // Refinement of an F32 axis solve x (in place): `steps` times,
 //   r = A^T y - G x   in compensated F32 (two-product and two-sum, with the

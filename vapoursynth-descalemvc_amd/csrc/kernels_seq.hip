@@ -111,6 +111,25 @@ check_finite(const T *__restrict__ src, long s_row, long s_frame, int rows, int 
     if (!ok) atomicOr(flag, 1);
 }
 
+// One workgroup per (plan batch, frame, row) of a table of F32 plane batches.
+__global__ void __launch_bounds__(256)
+check_finite_table(const CheckJob *__restrict__ jobs, int count) {
+    const long item = blockIdx.x;
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (constant(jobs)[mid].first_item <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    const CheckJob jb = jobs[uniform(lo)];
+    const long local = item - jb.first_item;
+    const long frame = local / jb.rows, row = local - frame * jb.rows;
+    const float *r = jb.src + frame * jb.s_frame + row * jb.s_row;
+    bool ok = true;
+    for (int c = static_cast<int>(threadIdx.x); c < jb.cols; c += 256) ok = ok && isfinite(r[c]);
+    if (!ok) atomicOr(jb.flag, 1);
+}
+
 } // namespace
 } // namespace dev
 
@@ -175,15 +194,50 @@ int window_band_class(int w) {
     return 0;
 }
 
-hipError_t launch_window64(const Win64Args &a, int band, int width, bool rows, Sample kind,
-                           hipStream_t s) {
+namespace {
+hipError_t window64_dispatch(const Win64Job *jobs, int count, long items, const Win64Job &one,
+                             int band, int width, bool rows, Sample kind, hipStream_t s) {
     switch (band) {
-    case 1: return launch_window64_band<1>(a, width, rows, kind, s);
-    case 3: return launch_window64_band<3>(a, width, rows, kind, s);
-    case 5: return launch_window64_band<5>(a, width, rows, kind, s);
-    case 7: return launch_window64_band<7>(a, width, rows, kind, s);
+    case 1: return launch_window64_table_band<1>(jobs, count, items, one, width, rows, kind, s);
+    case 3: return launch_window64_table_band<3>(jobs, count, items, one, width, rows, kind, s);
+    case 5: return launch_window64_table_band<5>(jobs, count, items, one, width, rows, kind, s);
+    case 7: return launch_window64_table_band<7>(jobs, count, items, one, width, rows, kind, s);
     default: return hipErrorInvalidValue;
     }
+}
+} // namespace
+
+hipError_t launch_window64(const Win64Args &a, int band, int width, bool rows, Sample kind,
+                           hipStream_t s) {
+    Win64Job one{};
+    one.plan = a.plan;
+    one.src = a.src;
+    one.s_row = a.s_row;
+    one.s_frame = a.s_frame;
+    one.dst = a.dst;
+    one.d_row = a.d_row;
+    one.d_frame = a.d_frame;
+    one.nvec = a.nvec;
+    one.frames = a.frames;
+    one.groups = a.groups;
+    one.first_item = 0;
+    one.flag = a.flag;
+    one.conv = a.conv;
+    const long items = static_cast<long>(a.frames) * a.plan.chunks * a.groups;
+    return window64_dispatch(nullptr, 0, items, one, band, width, rows, kind, s);
+}
+
+hipError_t launch_window64_table(const Win64Job *jobs, int count, long items, int band, int width,
+                                 bool rows, Sample kind, hipStream_t s) {
+    if (!jobs || count <= 0) return items > 0 ? hipErrorInvalidValue : hipSuccess;
+    return window64_dispatch(jobs, count, items, Win64Job{}, band, width, rows, kind, s);
+}
+
+hipError_t launch_check_finite_table(const CheckJob *jobs, int count, long items, hipStream_t s) {
+    if (items <= 0 || count <= 0) return hipSuccess;
+    if (items > 0x7FFFFFFFL) return hipErrorInvalidValue;
+    check_finite_table<<<dim3(static_cast<unsigned>(items)), 256, 0, s>>>(jobs, count);
+    return hipGetLastError();
 }
 
 hipError_t launch_window_table(const WinJob *jobs, int count, long items, int band, int width,

@@ -37,7 +37,7 @@ struct Pos64 {
 };
 
 // Items run vector group fastest, then chunk, then frame.
-__device__ __forceinline__ Pos64 decode64(const Win64Args &a, long item) {
+__device__ __forceinline__ Pos64 decode64(const Win64Job &a, long item) {
     const long per_frame = static_cast<long>(a.plan.chunks) * a.groups;
     Pos64 p;
     p.frame = static_cast<int>(item / per_frame);
@@ -108,7 +108,7 @@ __device__ __forceinline__ void backward_sweep64(ConstPtr<double> bwd, double (&
 // staged exactly as F32 and normalised per tap in FP64, as the Float64
 // route's load does: (x - offset) * scale.
 template <int W, int K, class In>
-__device__ __forceinline__ bool rows64_item(const Win64Args &a, const Pos64 &at,
+__device__ __forceinline__ bool rows64_item(const Win64Job &a, const Pos64 &at,
                                             std::uint32_t *ring, int lane) {
     constexpr int L = window_length64(W);
     constexpr int P = window_warmup64(W);
@@ -221,7 +221,7 @@ __device__ __forceinline__ bool rows64_item(const Win64Args &a, const Pos64 &at,
 }
 
 template <int W, int K, class Out>
-__device__ __forceinline__ bool columns64_item(const Win64Args &a, const Pos64 &at, int lane) {
+__device__ __forceinline__ bool columns64_item(const Win64Job &a, const Pos64 &at, int lane) {
     constexpr int L = window_length64(W);
     constexpr int P = window_warmup64(W);
     constexpr int R = L + 2 * P;
@@ -289,24 +289,57 @@ __device__ __forceinline__ bool columns64_item(const Win64Args &a, const Pos64 &
     return finite;
 }
 
+// A job record from the device table, read with scalar loads.
+__device__ __forceinline__ Win64Job load_job64(const Win64Job *p) {
+    static_assert(sizeof(Win64Job) % 4 == 0, "job record is whole words");
+    struct Words {
+        std::uint32_t v[sizeof(Win64Job) / 4];
+    } out;
+    const auto words = constant(reinterpret_cast<const std::uint32_t *>(p));
+#pragma unroll
+    for (unsigned i = 0; i < sizeof(Win64Job) / 4; ++i) out.v[i] = words[i];
+    return __builtin_bit_cast(Win64Job, out);
+}
+
+__device__ __forceinline__ int find_job64(const Win64Job *__restrict__ jobs, int count, long item) {
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (constant(jobs)[mid].first_item <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    return uniform(lo);
+}
+
+// One launch over a device table of jobs (many plans of one band class, tap
+// width, direction and sample kind) or, when `jobs` is null, over the one
+// job passed by value: wave w of workgroup b takes item b * kWaves64 + w and
+// finds its job by a binary search over the table.  A non-finite stored
+// value sets the job's flag.
 template <int W, int K, class In>
-__global__ void __launch_bounds__(64 * kWaves64) rows_window64(Win64Args a, long items) {
+__global__ void __launch_bounds__(64 * kWaves64)
+rows_window64(const Win64Job *__restrict__ jobs, int count, long items, Win64Job one) {
     __shared__ std::uint32_t lds[kWaves64 * kRing64];
     const int lane = threadIdx.x & 63;
     const int wave = uniform(threadIdx.x >> 6);
     const long item = static_cast<long>(blockIdx.x) * kWaves64 + wave;
     if (item >= items) return;
-    if (!rows64_item<W, K, In>(a, decode64(a, item), lds + wave * kRing64, lane) && a.flag)
-        atomicOr(a.flag, 1);
+    const Win64Job jb = jobs ? load_job64(jobs + find_job64(jobs, count, item)) : one;
+    if (!rows64_item<W, K, In>(jb, decode64(jb, item - jb.first_item), lds + wave * kRing64, lane)
+        && jb.flag)
+        atomicOr(jb.flag, 1);
 }
 
 template <int W, int K, class Out>
-__global__ void __launch_bounds__(64 * kWaves64) columns_window64(Win64Args a, long items) {
+__global__ void __launch_bounds__(64 * kWaves64)
+columns_window64(const Win64Job *__restrict__ jobs, int count, long items, Win64Job one) {
     const int lane = threadIdx.x & 63;
     const int wave = uniform(threadIdx.x >> 6);
     const long item = static_cast<long>(blockIdx.x) * kWaves64 + wave;
     if (item >= items) return;
-    if (!columns64_item<W, K, Out>(a, decode64(a, item), lane) && a.flag) atomicOr(a.flag, 1);
+    const Win64Job jb = jobs ? load_job64(jobs + find_job64(jobs, count, item)) : one;
+    if (!columns64_item<W, K, Out>(jb, decode64(jb, item - jb.first_item), lane) && jb.flag)
+        atomicOr(jb.flag, 1);
 }
 
 } // namespace
@@ -315,36 +348,38 @@ __global__ void __launch_bounds__(64 * kWaves64) columns_window64(Win64Args a, l
 namespace dev {
 namespace {
 template <int W, int K, class T>
-hipError_t launch64(const Win64Args &a, long items, bool rows, hipStream_t s) {
+hipError_t launch64(const Win64Job *jobs, int count, long items, const Win64Job &one, bool rows,
+                    hipStream_t s) {
     const long blocks = (items + kWaves64 - 1) / kWaves64;
     if (blocks > 0x7FFFFFFFL) return hipErrorInvalidValue;
     const dim3 grid(static_cast<unsigned>(blocks));
     const dim3 block(64 * kWaves64);
-    if (rows) rows_window64<W, K, T><<<grid, block, 0, s>>>(a, items);
-    else columns_window64<W, K, T><<<grid, block, 0, s>>>(a, items);
+    if (rows) rows_window64<W, K, T><<<grid, block, 0, s>>>(jobs, count, items, one);
+    else columns_window64<W, K, T><<<grid, block, 0, s>>>(jobs, count, items, one);
     return hipGetLastError();
 }
 } // namespace
 } // namespace dev
 
 // `kind`: the sample type of the rows pass's source or the columns pass's
-// destination (the intermediate is FP64).
+// destination (the intermediate is FP64).  `jobs` null: the single job `one`.
 template <int W>
-hipError_t launch_window64_band(const Win64Args &a, int width, bool rows, Sample kind,
-                                hipStream_t s) {
-    const long items = static_cast<long>(a.frames) * a.plan.chunks * a.groups;
+hipError_t launch_window64_table_band(const Win64Job *jobs, int count, long items,
+                                      const Win64Job &one, int width, bool rows, Sample kind,
+                                      hipStream_t s) {
     if (items <= 0) return hipSuccess;
     if (width != 8 && width != 16) return hipErrorInvalidValue;
     const bool wide = width == 16;
     switch (kind) {
     case Sample::f32:
-        return wide ? dev::launch64<W, 16, float>(a, items, rows, s) : dev::launch64<W, 8, float>(a, items, rows, s);
+        return wide ? dev::launch64<W, 16, float>(jobs, count, items, one, rows, s)
+                    : dev::launch64<W, 8, float>(jobs, count, items, one, rows, s);
     case Sample::u8:
-        return wide ? dev::launch64<W, 16, std::uint8_t>(a, items, rows, s)
-                    : dev::launch64<W, 8, std::uint8_t>(a, items, rows, s);
+        return wide ? dev::launch64<W, 16, std::uint8_t>(jobs, count, items, one, rows, s)
+                    : dev::launch64<W, 8, std::uint8_t>(jobs, count, items, one, rows, s);
     case Sample::u16:
-        return wide ? dev::launch64<W, 16, std::uint16_t>(a, items, rows, s)
-                    : dev::launch64<W, 8, std::uint16_t>(a, items, rows, s);
+        return wide ? dev::launch64<W, 16, std::uint16_t>(jobs, count, items, one, rows, s)
+                    : dev::launch64<W, 8, std::uint16_t>(jobs, count, items, one, rows, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -3,5 +3,6 @@
 #include "kernels_window64.hpp"
 
 namespace dsmvc::hip {
-template hipError_t launch_window64_band<1>(const Win64Args &, int, bool, Sample, hipStream_t);
+template hipError_t launch_window64_table_band<1>(const Win64Job *, int, long, const Win64Job &,
+                                                     int, bool, Sample, hipStream_t);
 } // namespace dsmvc::hip

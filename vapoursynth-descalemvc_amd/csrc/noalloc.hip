@@ -293,8 +293,16 @@ void HipExecutor::inverse_2d_batch_noalloc(const AxisPlan &h, const AxisPlan &v,
     if (async) return;
     check(stream ? hipStreamSynchronize(stream) : hipDeviceSynchronize(), "execution failed");
     if (f64) {
+        // On the call's stream: a null-stream copy would wait for every other
+        // blocking stream's work too.
         int bad = 0;
-        check(hipMemcpy(&bad, flag, sizeof bad, hipMemcpyDeviceToHost), "flag readback failed");
+        if (stream) {
+            check(hipMemcpyAsync(&bad, flag, sizeof bad, hipMemcpyDeviceToHost, stream),
+                  "flag readback failed");
+            check(hipStreamSynchronize(stream), "flag readback failed");
+        } else {
+            check(hipMemcpy(&bad, flag, sizeof bad, hipMemcpyDeviceToHost), "flag readback failed");
+        }
         if (bad) throw std::runtime_error("dsmvc: HIP Float64 2D execution produced NaN or infinity");
     }
 }

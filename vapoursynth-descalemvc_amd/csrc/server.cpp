@@ -43,7 +43,12 @@ struct HipServer::Slot {
     std::unique_ptr<WindowTables> tables;
     std::vector<std::shared_ptr<Entry>> round;
     bool fallback = false;   // the round solved planes through the executor
+    bool float64 = false;    // the round ran Float64 table jobs (per-request flags)
     bool busy = false;
+    // One Float64 failure flag per request of a round: set by its table
+    // jobs, copied back with the round, read when it completes.
+    int *flags = nullptr;         // device
+    int *flags_host = nullptr;    // pinned
 };
 
 HipServer::HipServer(HipExecutor &executor, const ServerConfig &config)
@@ -60,6 +65,10 @@ HipServer::HipServer(HipExecutor &executor, const ServerConfig &config)
         check(hipStreamCreateWithFlags(&slot->stream, hipStreamNonBlocking), "stream creation failed");
         check(hipEventCreateWithFlags(&slot->done, hipEventDisableTiming), "event creation failed");
         slot->tables = std::make_unique<WindowTables>(executor_.device());
+        const std::size_t flag_bytes = static_cast<std::size_t>(config_.max_round_requests) * sizeof(int);
+        check(hipMalloc(reinterpret_cast<void **>(&slot->flags), flag_bytes), "flag allocation failed");
+        check(hipHostMalloc(reinterpret_cast<void **>(&slot->flags_host), flag_bytes),
+              "flag allocation failed");
         slots_.push_back(std::move(slot));
     }
     worker_ = std::thread([this] { loop(); });
@@ -82,6 +91,8 @@ HipServer::~HipServer() {
             }
         }
         slot->tables.reset();
+        if (slot->flags) (void)hipFree(slot->flags);
+        if (slot->flags_host) (void)hipHostFree(slot->flags_host);
         if (slot->done) (void)hipEventDestroy(slot->done);
         if (slot->stream) (void)hipStreamDestroy(slot->stream);
     }
@@ -215,10 +226,15 @@ std::uint64_t HipServer::submit(const ServeRequest &r) {
         if (parallel && !f64_pair) {
             plane.wh = plane.ih->window(true);
             plane.wv = plane.iv->window(false);
+        } else if (parallel) {
+            plane.wh64 = plane.ih->window64();
+            plane.wv64 = plane.iv->window64();
         }
         plane.table = plane.wh && plane.wv;
-        if (plane.table)
-            plane.inter_bytes = align256(static_cast<std::size_t>(pitch_of(dw)) * sh * sizeof(float));
+        plane.table64 = plane.wh64 && plane.wv64;
+        if (plane.table || plane.table64)
+            plane.inter_bytes = align256(static_cast<std::size_t>(pitch_of(dw)) * sh
+                                         * (plane.table64 ? sizeof(double) : sizeof(float)));
         entry->bytes += plane.inter_bytes;
         entry->planes.push_back(std::move(plane));
     }
@@ -321,21 +337,63 @@ bool HipServer::admit_round(Slot &slot, std::unique_lock<std::mutex> &lock) {
     ++stats_.rounds;
     stats_.largest_round = std::max<std::uint64_t>(stats_.largest_round, round.size());
     lock.unlock();
-    bool fallback = false;
+    bool fallback = false, float64_round = false;
     std::string failure;
     std::uint64_t table_planes = 0, fallback_planes = 0;
     try {
         check(hipSetDevice(executor_.device()), "hipSetDevice failed");
         WindowTables &tables = *slot.tables;
         tables.clear();
-        for (const auto &e : round) {
+        bool float64 = false;
+        for (std::size_t r = 0; r < round.size(); ++r) {
+            const auto &e = round[r];
             std::size_t at = e->offset;
             for (const Plane &p : e->planes) {
-                if (!p.table) continue;
+                if (!p.table && !p.table64) continue;
                 const AxisPlan &h = *p.h, &v = *p.v;
                 const std::ptrdiff_t row = pitch_of(h.destination_size);
-                float *inter = reinterpret_cast<float *>(static_cast<unsigned char *>(arena_) + at);
+                void *inter = static_cast<unsigned char *>(arena_) + at;
                 at += p.inter_bytes;
+                if (p.table64) {
+                    // The Float64 route in the round's tables: the input
+                    // check and both passes report into the request's flag.
+                    int *flag = slot.flags + r;
+                    hip::CheckJob c{};
+                    c.src = p.src;
+                    c.s_row = p.src_stride;
+                    c.rows = v.source_size;
+                    c.cols = h.source_size;
+                    c.frames = 1;
+                    c.flag = flag;
+                    tables.add_check(c);
+                    hip::Win64Job a{};
+                    a.plan = *p.wh64;
+                    a.src = p.src;
+                    a.s_row = p.src_stride;
+                    a.dst = inter;
+                    a.d_row = row;
+                    a.nvec = v.source_size;
+                    a.frames = 1;
+                    a.groups = (a.nvec + 63) / 64;
+                    a.flag = flag;
+                    a.conv = identity_conv();
+                    tables.add64(a, hip::window_band_class(h.half_bandwidth), true, hip::Sample::f32);
+                    hip::Win64Job b{};
+                    b.plan = *p.wv64;
+                    b.src = inter;
+                    b.s_row = row;
+                    b.dst = p.dst;
+                    b.d_row = p.dst_stride;
+                    b.nvec = h.destination_size;
+                    b.frames = 1;
+                    b.groups = (b.nvec + 63) / 64;
+                    b.flag = flag;
+                    b.conv = identity_conv();
+                    tables.add64(b, hip::window_band_class(v.half_bandwidth), false, hip::Sample::f32);
+                    float64 = true;
+                    ++table_planes;
+                    continue;
+                }
                 hip::WinJob a{};
                 a.plan = *p.wh;
                 a.src = p.src;
@@ -361,23 +419,32 @@ bool HipServer::admit_round(Slot &slot, std::unique_lock<std::mutex> &lock) {
                 ++table_planes;
             }
         }
+        if (float64)
+            check(hipMemsetAsync(slot.flags, 0, round.size() * sizeof(int), slot.stream),
+                  "flag reset failed");
         tables.upload(slot.stream);
         tables.launch(slot.stream);
+        if (float64)
+            check(hipMemcpyAsync(slot.flags_host, slot.flags, round.size() * sizeof(int),
+                                 hipMemcpyDeviceToHost, slot.stream),
+                  "flag readback failed");
         for (const auto &e : round)
             for (const Plane &p : e->planes) {
-                if (p.table) continue;
+                if (p.table || p.table64) continue;
                 executor_.inverse_2d_batch(*p.h, *p.v, SampleKind::float32,
                                            PlaneBatch{p.src, p.src_stride, 0, p.dst, p.dst_stride, 0, 1},
                                            nullptr, slot.stream, true);
                 fallback = true;
                 ++fallback_planes;
             }
+        float64_round = float64;
         check(hipEventRecord(slot.done, slot.stream), "event record failed");
     } catch (const std::exception &ex) {
         failure = ex.what();
     }
     lock.lock();
     slot.fallback = fallback;
+    slot.float64 = float64_round;
     stats_.table_planes += table_planes;
     stats_.fallback_planes += fallback_planes;
     if (!failure.empty()) {
@@ -413,13 +480,21 @@ void HipServer::finish_round(Slot &slot, std::unique_lock<std::mutex> &lock, std
         lock.lock();
     }
     const double t = now();
-    for (const auto &e : slot.round) {
+    for (std::size_t r = 0; r < slot.round.size(); ++r) {
+        const auto &e = slot.round[r];
         arena_free(e->offset, e->bytes);
-        bool uses_fallback = false;
-        for (const Plane &p : e->planes) uses_fallback |= !p.table;
-        if (!device_error.empty() || (!failure.empty() && uses_fallback)) {
+        bool uses_fallback = false, uses_table64 = false;
+        for (const Plane &p : e->planes) {
+            uses_fallback |= !p.table && !p.table64;
+            uses_table64 |= p.table64;
+        }
+        // The request's own Float64 flag (copied back with the round).
+        const bool nonfinite = slot.float64 && uses_table64 && slot.flags_host[r] != 0;
+        if (!device_error.empty() || (!failure.empty() && uses_fallback) || nonfinite) {
             e->state = RequestState::failed;
-            e->error = device_error.empty() ? failure : device_error;
+            e->error = !device_error.empty() ? device_error
+                       : nonfinite ? "dsmvc: HIP Float64 2D execution produced NaN or infinity"
+                                   : failure;
             ++stats_.failed;
         } else {
             e->state = RequestState::done;
@@ -430,6 +505,7 @@ void HipServer::finish_round(Slot &slot, std::unique_lock<std::mutex> &lock, std
     slot.round.clear();
     slot.busy = false;
     slot.fallback = false;
+    slot.float64 = false;
     done_cv_.notify_all();
 }
 

@@ -108,11 +108,14 @@ private:
         std::ptrdiff_t src_stride = 0;
         float *dst = nullptr;
         std::ptrdiff_t dst_stride = 0;
-        bool table = false;          // windowed table launch (else executor route)
+        bool table = false;          // windowed F32 table launch
+        bool table64 = false;        // windowed Float64 table launch (else executor route)
         std::size_t inter_bytes = 0;
         // The plans' device images, held so that the executor's plan cache
         // keeps them (and their window tables) until the request is gone.
         std::shared_ptr<hip::DevicePlan> ih, iv;
+        const hip::DevWindow64 *wh64 = nullptr;
+        const hip::DevWindow64 *wv64 = nullptr;
         const hip::DevWindow *wh = nullptr;
         const hip::DevWindow *wv = nullptr;
     };
