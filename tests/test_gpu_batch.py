@@ -56,8 +56,11 @@ def test_batch_of_uneven_plans_matches_oracle(executor, oracle):
     info = batch.info()
     print(info, flush=True)
     assert info["jobs"] == len(heights) and info["table_jobs"] == len(heights)
-    # One launch per (direction, tap width, staging block size) group.
-    assert info["fallback_jobs"] == 0 and info["launches"] <= 6
+    # One launch per (direction, band class, tap width, staging block size)
+    # group: at most two block sizes per (direction, band class, tap width).
+    classes = {(rows, p.window_info()["band"], p.window_info()["width"])
+               for _, _, hp, vp, _ in jobs for rows, p in ((True, hp), (False, vp))}
+    assert info["fallback_jobs"] == 0 and info["launches"] <= 2 * len(classes)
     batch.run()
     for (hr, vr, _, _, out), h in zip(jobs, heights):
         got = out.cpu().numpy()

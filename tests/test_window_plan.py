@@ -123,6 +123,49 @@ def test_windowed_solve_matches_sequential(name, req):
     assert float(np.abs(short - exact).max()) > 8 * TOLERANCE * scale
 
 
+def _streamed(b, lld, ul, inv, w, length, warmup, span):
+    """The kernels' segments: each wavefront solves `span` consecutive chunks.
+    The forward recurrence starts from a zero state at least 2P rows before
+    the segment's first chunk ends its warm-up (whole chunks of L rows, the
+    smallest count covering 2P: padded row (q0 - ceil(2P / L)) L + 2P, plan
+    row minus P, clamped to 0) and runs on without a restart; each chunk's
+    backward recurrence starts P rows past the chunk from a zero state."""
+    n = b.shape[1]
+    chunks = -(-n // length)
+    warm = -(-2 * warmup // length)
+    out = np.zeros_like(b)
+    for q0 in range(0, chunks, span):
+        q1 = min(q0 + span, chunks)
+        f0 = max(0, (q0 - warm) * length + warmup)
+        f1 = min(n, q1 * length + warmup)
+        y = _forward(b, lld, inv, f0, f1, w)
+        for q in range(q0, q1):
+            a, e = q * length, min(n, (q + 1) * length)
+            x = _backward(y, ul, a, min(n, e + warmup), w, f0)
+            out[:, a:e] = x[:, : e - a]
+    return out
+
+
+@pytest.mark.parametrize("name,req", CASES[:4], ids=[c[0] for c in CASES[:4]])
+@pytest.mark.parametrize("span", [1, 2, 3, 5])
+def test_streamed_segments_match_sequential(name, req, span):
+    """Segments of several chunks, as the kernels run them: the forward sweep
+    carries from chunk to chunk, so only the segment's start and each chunk's
+    backward end are truncated, and the same bound holds."""
+    plan = _plan(req)
+    info = plan.window_info()
+    _, lld, ul, inv = _coefficients(plan)
+    rng = np.random.default_rng(plan.destination_size + span)
+    b = _rhs(plan, rng.uniform(0.0, 1.0, (2, plan.source_size)))
+    w = plan.half_bandwidth
+    exact = _sequential(b, lld, ul, inv, w)
+    scale = float(np.abs(exact).max())
+    got = _streamed(b, lld, ul, inv, w, info["length"], info["warmup"], span)
+    assert float(np.abs(got - exact).max()) <= 8 * TOLERANCE * scale
+    short = _streamed(b, lld, ul, inv, w, info["length"], info["warmup"] // 4, span)
+    assert float(np.abs(short - exact).max()) > 8 * TOLERANCE * scale
+
+
 @pytest.mark.parametrize("workload", ["debilinear_1080p_720p", "yuv420p16_debicubic_1080p_720p",
                                       "mitchell_21_heights", "served_mix_sample"])
 def test_workload_plans_are_windowed(workload):

@@ -27,8 +27,6 @@ std::size_t align256(std::size_t v) { return (v + 255U) & ~std::size_t{255}; }
 
 std::ptrdiff_t pitch_of(std::int32_t n) { return (static_cast<std::ptrdiff_t>(n) + 63) & ~std::ptrdiff_t{63}; }
 
-long items_of(const hip::WinJob &j) { return static_cast<long>(j.frames) * j.plan.chunks * j.groups; }
-
 hip::DevConv identity_conv() {
     hip::DevConv c{};
     c.in_scale = 1.0F;
@@ -83,7 +81,7 @@ void WindowTables::add(const hip::WinJob &job, int band, bool rows, hip::Sample 
     Group &g = group_for(Kind::f32, band, job.plan.ell_width, rows, job.plan.block16 != 0, kind);
     hip::WinJob j = job;
     j.first_item = g.items;
-    g.items += items_of(j);
+    g.items += hip::window_items(j);   // until upload() sets the launch's spans
     g.jobs.push_back(j);
 }
 
@@ -115,12 +113,19 @@ int WindowTables::upload(hipStream_t stream) {
         auto rank = [](const Group &g) { return g.type == Kind::check ? 0 : (g.rows ? 1 : 2); };
         return rank(a) < rank(b);
     });
+    check(hipSetDevice(device_), "hipSetDevice failed");
+    // Each F32 launch's span (chunks per wavefront), from its kernel's
+    // occupancy; the jobs' first items follow.
+    for (Group &g : groups_)
+        if (g.type == Kind::f32 && !g.jobs.empty())
+            g.items = hip::assign_window_spans(
+                g.jobs.data(), static_cast<int>(g.jobs.size()),
+                hip::window_table_slots(g.band, g.width, g.rows, g.block16, g.kind));
     std::size_t bytes = 0;
     for (Group &g : groups_) {
         g.offset = bytes;
         bytes += (g.bytes() + 15) & ~std::size_t{15};
     }
-    check(hipSetDevice(device_), "hipSetDevice failed");
     current_ = (current_ + 1) % kSlots;
     Slot &slot = slots_[current_];
     if (slot.pending) {

@@ -11,6 +11,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 
 #include "kernels.hpp"
 
@@ -246,7 +247,9 @@ bool build_window(const AxisPlan &p, int length, int warm, Coef coef_of, Weight 
     for (int i = 0; i < n; ++i)
         taps = std::max(taps, static_cast<int>(p.transpose_offsets[static_cast<std::size_t>(i) + 1]
                                                - p.transpose_offsets[static_cast<std::size_t>(i)]));
-    const int width = chunked_width_for(std::max(taps, 1));
+    // The FP64 kernels take tap widths 8 and 16, the F32 kernels also 6.
+    const int width = std::is_same_v<T, float> ? window_width_for(band, std::max(taps, 1))
+                                               : chunked_width_for(std::max(taps, 1));
     if (width == 0) return false;
     const int chunks = (n + length - 1) / length;
     const int rows = chunks * length + 2 * warm;
@@ -286,7 +289,8 @@ bool build_window(const AxisPlan &p, int length, int warm, Coef coef_of, Weight 
                 hi = std::max(hi, p.transpose_indices[k1 - 1]);
             }
             if (hi < 0) lo = previous;
-            else if (hi - lo >= kStageCols) fits = false;
+            lo &= ~1;   // even: integer samples are staged in aligned pairs
+            if (hi >= 0 && hi - lo >= kStageCols) fits = false;
             table[static_cast<std::size_t>(b)] = lo;
             previous = lo;
         }

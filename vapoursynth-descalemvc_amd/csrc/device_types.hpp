@@ -88,7 +88,8 @@ constexpr int kStageCols = 32;     // source columns staged per block
 //             the K consecutive samples [start, start + K), weight 0 where the
 //             row has no tap, inside its staging window (both block sizes)
 //   blk_lo    [rows / 8]  first staged source sample of each 8-row block (its
-//             taps lie in [lo, lo + 32))
+//             taps lie in [lo, lo + 32)); even, so that integer samples can
+//             be staged two at a time with aligned loads
 //   blk_lo16  [rows / 16] the same for 16-row blocks, valid when block16 = 1
 //             (every 16-row block's taps fit 32 samples)
 // The records are read a few rows at a time with wide scalar loads, so the
@@ -123,10 +124,13 @@ struct DevConv {
 enum class Sample : std::int32_t { f32 = 0, u8 = 1, u16 = 2, f64 = 3 };
 
 // One axis pass of one plan over `frames` planes, as a unit of a
-// (possibly multi-plan) windowed launch.  Items are (frame, chunk, group of
-// 64 vectors), the group varying fastest.  Rows pass: vectors are
-// rows, `src` holds samples of the launch's kind, `dst` F32.  Columns pass:
-// vectors are columns, `src` F32, `dst` samples of the launch's kind.
+// (possibly multi-plan) windowed launch.  Items are (frame, segment, group
+// of 64 vectors), the group varying fastest; a segment is `span` consecutive
+// chunks, which one wavefront solves in sequence (the forward recurrence runs
+// on from one chunk into the next, so only the segment's first chunk pays the
+// forward warm-up).  Rows pass: vectors are rows, `src` holds samples of the
+// launch's kind, `dst` F32.  Columns pass: vectors are columns, `src` F32,
+// `dst` samples of the launch's kind.
 struct WinJob {
     DevWindow plan;
     const void *src;
@@ -136,10 +140,18 @@ struct WinJob {
     std::int32_t nvec;
     std::int32_t frames;
     std::int32_t groups;      // ceil(nvec / 64)
-    std::int32_t pad_;
+    std::int32_t span;        // chunks per segment; 0 or 1: one
     long first_item;          // items of the jobs before this one in its table
     DevConv conv;
 };
+
+constexpr int window_span(const WinJob &j) { return j.span > 1 ? j.span : 1; }
+constexpr int window_segments(const WinJob &j) {
+    return (j.plan.chunks + window_span(j) - 1) / window_span(j);
+}
+constexpr long window_items(const WinJob &j) {
+    return static_cast<long>(j.frames) * window_segments(j) * j.groups;
+}
 
 // Refinement tables of a plan that retains FP64 data: the FP64 weights of
 // A^T (same CSR as DevAxis) and the FP64 normal matrix G = A^T A as rows of
@@ -164,11 +176,16 @@ struct DevRefine {
 };
 
 // Windowed-kernel shape per band class: chunk length L and warm-up P.
-constexpr int window_length(int w) { return w <= 3 ? 64 : (w <= 5 ? 48 : 32); }
+constexpr int window_length(int) { return 32; }
 constexpr int window_warmup(int w) { return w <= 3 ? 24 : (w <= 5 ? 32 : 64); }
 // Record sizes of the windowed tables (DevWindow) in words, and the rows one
 // batch of scalar loads covers (at most 64 words, a divisor of 8).
 constexpr int window_fwd_words(int w, int k) { return (k + w + 1 + 3) & ~3; }
+// Tap widths of the windowed F32 kernels: 6 for band classes 1 and 3, whose
+// kernels at moderate ratios read at most 6 samples a row, then 8 and 16.
+constexpr int window_width_for(int band, int taps) {
+    return taps <= 6 && band <= 3 ? 6 : (taps <= 8 ? 8 : (taps <= 16 ? 16 : 0));
+}
 constexpr int window_bwd_words(int w) { return w <= 3 ? 4 : 8; }
 constexpr int window_fwd_batch(int w, int k) { return window_fwd_words(w, k) <= 16 ? 4 : 2; }
 constexpr int window_bwd_batch(int w) { return 64 / window_bwd_words(w); }

@@ -60,22 +60,31 @@ extern template hipError_t launch_chunked_band<5>(const ChunkArgs &, bool, Sampl
 extern template hipError_t launch_chunked_band<7>(const ChunkArgs &, bool, Sample, hipStream_t);
 // Windowed (independent-chunk) kernels.  The band class is 1, 3, 5 or 7
 // (smaller bands run in the next class with zero coefficients); the tap
-// width 8 or 16.  `launch_window_table` runs `count` jobs from a
-// device-resident table (`items` items in all) in one launch, all jobs
-// sharing the band class, tap width, direction, staging block size and
-// sample kind (the input kind of a horizontal pass, the output kind of a
-// vertical pass).  A single pass is a one-job table.
+// width 6 (band classes 1 and 3), 8 or 16.  `launch_window_table` runs
+// `count` jobs from a device-resident table (`items` items in all) in one
+// launch, all jobs sharing the band class, tap width, direction, staging
+// block size and sample kind (the input kind of a horizontal pass, the
+// output kind of a vertical pass).  A single pass is a one-job table.
 int window_band_class(int w);   // 1, 3, 5, 7, or 0 when unsupported
 constexpr int kWindowWaves = 4;   // wavefronts (one item each) per workgroup
-// `order`: bit 0 chunk-fastest item order, bit 1 XCD-contiguous workgroups.
+// `order`: bit 0 segment-fastest item order, bit 1 XCD-contiguous workgroups.
 hipError_t launch_window_table(const WinJob *jobs, int count, long items, int band, int width,
                                bool rows, bool block16, Sample kind, int order, hipStream_t s);
+// Wavefronts of that launch's kernel the GPU holds at once.
+long window_table_slots(int band, int width, bool rows, bool block16, Sample kind);
+// Sets the span (chunks per wavefront) and first item of a launch's `count`
+// jobs for `slots` resident wavefronts; returns the launch's item count.
+long assign_window_spans(WinJob *jobs, int count, long slots);
 template <int W, int K>
 hipError_t launch_window_table_band(const WinJob *jobs, int count, long items, bool rows,
                                     bool block16, Sample kind, int order, hipStream_t s);
+template <int W, int K> long window_table_slots_band(bool rows, bool block16, Sample kind);
 #define DSMVC_WINDOW_EXTERN(W, K)                                                                 \
     extern template hipError_t launch_window_table_band<W, K>(const WinJob *, int, long, bool,    \
-                                                              bool, Sample, int, hipStream_t);
+                                                              bool, Sample, int, hipStream_t);    \
+    extern template long window_table_slots_band<W, K>(bool, bool, Sample);
+DSMVC_WINDOW_EXTERN(1, 6)
+DSMVC_WINDOW_EXTERN(3, 6)
 DSMVC_WINDOW_EXTERN(1, 8)
 DSMVC_WINDOW_EXTERN(1, 16)
 DSMVC_WINDOW_EXTERN(3, 8)

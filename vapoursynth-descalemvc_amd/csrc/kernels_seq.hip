@@ -30,6 +30,10 @@
 //                    Source columns are staged through a per-wave LDS ring
 //                    (transposed, pitch 65) and results leave through an LDS
 //                    transpose, so HBM traffic stays coalesced.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+
 #include "kernels_common.hpp"
 
 namespace dsmvc::hip {
@@ -242,15 +246,80 @@ hipError_t launch_check_finite_table(const CheckJob *jobs, int count, long items
 
 hipError_t launch_window_table(const WinJob *jobs, int count, long items, int band, int width,
                                bool rows, bool block16, Sample kind, int order, hipStream_t s) {
-    if (width != 8 && width != 16) return hipErrorInvalidValue;
-    const bool wide = width == 16;
-    switch (band) {
-    case 1: return wide ? launch_window_table_band<1, 16>(jobs, count, items, rows, block16, kind, order, s) : launch_window_table_band<1, 8>(jobs, count, items, rows, block16, kind, order, s);
-    case 3: return wide ? launch_window_table_band<3, 16>(jobs, count, items, rows, block16, kind, order, s) : launch_window_table_band<3, 8>(jobs, count, items, rows, block16, kind, order, s);
-    case 5: return wide ? launch_window_table_band<5, 16>(jobs, count, items, rows, block16, kind, order, s) : launch_window_table_band<5, 8>(jobs, count, items, rows, block16, kind, order, s);
-    case 7: return wide ? launch_window_table_band<7, 16>(jobs, count, items, rows, block16, kind, order, s) : launch_window_table_band<7, 8>(jobs, count, items, rows, block16, kind, order, s);
+    switch (band * 100 + width) {
+    case 106: return launch_window_table_band<1, 6>(jobs, count, items, rows, block16, kind, order, s);
+    case 108: return launch_window_table_band<1, 8>(jobs, count, items, rows, block16, kind, order, s);
+    case 116: return launch_window_table_band<1, 16>(jobs, count, items, rows, block16, kind, order, s);
+    case 306: return launch_window_table_band<3, 6>(jobs, count, items, rows, block16, kind, order, s);
+    case 308: return launch_window_table_band<3, 8>(jobs, count, items, rows, block16, kind, order, s);
+    case 316: return launch_window_table_band<3, 16>(jobs, count, items, rows, block16, kind, order, s);
+    case 508: return launch_window_table_band<5, 8>(jobs, count, items, rows, block16, kind, order, s);
+    case 516: return launch_window_table_band<5, 16>(jobs, count, items, rows, block16, kind, order, s);
+    case 708: return launch_window_table_band<7, 8>(jobs, count, items, rows, block16, kind, order, s);
+    case 716: return launch_window_table_band<7, 16>(jobs, count, items, rows, block16, kind, order, s);
     default: return hipErrorInvalidValue;
     }
+}
+
+long window_table_slots(int band, int width, bool rows, bool block16, Sample kind) {
+    switch (band * 100 + width) {
+    case 106: return window_table_slots_band<1, 6>(rows, block16, kind);
+    case 108: return window_table_slots_band<1, 8>(rows, block16, kind);
+    case 116: return window_table_slots_band<1, 16>(rows, block16, kind);
+    case 306: return window_table_slots_band<3, 6>(rows, block16, kind);
+    case 308: return window_table_slots_band<3, 8>(rows, block16, kind);
+    case 316: return window_table_slots_band<3, 16>(rows, block16, kind);
+    case 508: return window_table_slots_band<5, 8>(rows, block16, kind);
+    case 516: return window_table_slots_band<5, 16>(rows, block16, kind);
+    case 708: return window_table_slots_band<7, 8>(rows, block16, kind);
+    case 716: return window_table_slots_band<7, 16>(rows, block16, kind);
+    default: return 0;
+    }
+}
+
+// A longer span saves the forward warm-up (2P rows per segment) but leaves
+// fewer wavefronts.  The span minimises a model of the launch: the rounds of
+// `slots` resident wavefronts it takes, times the mean wavefront cost in
+// forward rows (a backward row counts 0.3 of one).  DSMVC_HIP_WINDOW_SPAN
+// (> 0) fixes the span instead.
+long assign_window_spans(WinJob *jobs, int count, long slots) {
+    static const int fixed = [] {
+        const char *env = std::getenv("DSMVC_HIP_WINDOW_SPAN");
+        return env ? std::atoi(env) : 0;
+    }();
+    int best = fixed > 0 ? fixed : 1;
+    if (fixed <= 0 && count > 0) {
+        int top = 1;
+        for (int i = 0; i < count; ++i) top = std::max(top, jobs[i].plan.chunks);
+        top = std::min(top, 64);
+        const double resident = static_cast<double>(std::max(slots, 1L));
+        double best_time = 0.0;
+        for (int span = 1; span <= top; ++span) {
+            double waves = 0.0, work = 0.0;
+            for (int i = 0; i < count; ++i) {
+                const DevWindow &p = jobs[i].plan;
+                const int segments = (p.chunks + span - 1) / span;
+                const double per = static_cast<double>(jobs[i].frames) * jobs[i].groups * segments;
+                const double chunks = static_cast<double>(p.chunks) / segments;   // mean per segment
+                waves += per;
+                work += per * (chunks * p.length + 2.0 * p.warmup
+                               + 0.3 * chunks * (p.length + p.warmup));
+            }
+            if (waves <= 0.0) break;
+            const double time = std::ceil(waves / resident) * (work / waves);
+            if (span == 1 || time < best_time * 0.999) {
+                best_time = time;
+                best = span;
+            }
+        }
+    }
+    long items = 0;
+    for (int i = 0; i < count; ++i) {
+        jobs[i].span = best;
+        jobs[i].first_item = items;
+        items += window_items(jobs[i]);
+    }
+    return items;
 }
 
 hipError_t launch_chunked(const ChunkArgs &a, bool rows, Sample kind, hipStream_t s) {

@@ -1,27 +1,34 @@
 // Windowed (independent-chunk) axis solves for gfx950, templated on the band
 // class W, the tap width K and the sample type.  The chunk length L and the
 // warm-up P are fixed per band class (window_length / window_warmup).
-// Included by kernels_window_w{1,3,5,7}_k{8,16}.hip, one (band class, tap
+// Included by kernels_window_w{1,3,5,7}_k{6,8,16}.hip, one (band class, tap
 // width) pair each, so the build compiles them in parallel.
 //
-// Every wavefront solves one item: 64 vectors (lanes) x one chunk of L
-// destination rows of one plan.  It runs the forward recurrence over the
-// L + 2P padded rows [qL - P, qL + L + P) from a zero state, keeps the last
-// L + P values in registers, runs the backward recurrence down from the end
-// of the window from a zero state, and stores the chunk's L rows.  Chunks
-// never exchange carries, so a workgroup's wavefronts are independent (no
-// workgroup barrier) and one launch can hold items of many plans.  The four
-// wavefronts of a workgroup take consecutive items (decode_item, xcd_block).
+// Every wavefront solves one item: 64 vectors (lanes) x one segment of
+// `span` consecutive chunks of L destination rows of one plan.  It runs the
+// forward recurrence from a zero state P rows before the segment, and on
+// through the segment's chunks without a restart; after the forward sweep
+// passes P rows beyond a chunk it runs the backward recurrence down from
+// there from a zero state (the last L + P forward values are in registers),
+// stores the chunk's L rows and carries the P values beyond the chunk into
+// the next.  Chunks never exchange carries, so a workgroup's wavefronts are
+// independent (no workgroup barrier) and one launch can hold items of many
+// plans.  The four wavefronts of a workgroup take consecutive items
+// (decode_item, xcd_block).
 //
 // Vertical pass (columns_window): lanes are 64 adjacent columns; each block
 // of G rows stages the 32 source rows its taps need (256-byte runs) into the
 // wavefront's LDS ring, the next block's loads in flight meanwhile.
 // Horizontal pass (rows_window): lanes are 64 rows; each block of G (16 when
 // the plan's windows allow it, else 8) rows stages the 32 source columns its
-// taps need through the wavefront's LDS ring (transposed, pitch 65), the next
-// block's loads in flight meanwhile, and results leave through the same ring.
+// taps need through the wavefront's LDS ring (transposed, pitch 33), the
+// next block's loads in flight meanwhile, and results leave through the same
+// ring.  Integer samples are staged two to a register and enter the solve as
+// (float)x - offset, which is exact; the scale multiplies each result once
+// instead of each sample (the solve is linear).
 #pragma once
 
+#include <cstdint>
 #include <type_traits>
 
 #include "kernels_common.hpp"
@@ -36,29 +43,41 @@ namespace {
 constexpr int kRowPitch = kStageCols + 1;
 constexpr int kWinRing = 64 * kRowPitch;
 
+// A wave-uniform pointer passed through an empty asm that also reads
+// `after`: loads through the result cannot be moved above the computation
+// of `after`.
+template <class T>
+__device__ __forceinline__ ConstPtr<T> pinned(ConstPtr<T> p, float after) {
+    auto v = reinterpret_cast<std::uintptr_t>(p);
+    asm volatile("" : "+s"(v) : "v"(after));
+    return reinterpret_cast<ConstPtr<T>>(v);
+}
+
 struct ItemPos {
-    int frame, chunk, group;
+    int frame, chunk, group;   // chunk: the first chunk of the item's segment
 };
 
 // Items run vector group fastest: a workgroup's four wavefronts usually
-// solve the same chunk of one plan for neighbouring vectors, so they read the
-// same records through the scalar cache.  With bit 0 of `order` the chunk
-// varies fastest instead.
+// solve the same segment of one plan for neighbouring vectors, so they read
+// the same records through the scalar cache.  With bit 0 of `order` the
+// segment varies fastest instead.
 __device__ __forceinline__ ItemPos decode_item(const WinJob &jb, int local, int order) {
-    const int chunks = jb.plan.chunks, groups = jb.groups;
+    const int segments = window_segments(jb), groups = jb.groups;
     ItemPos p;
+    int segment;
     if (order & 1) {
-        const int rest = local / chunks;
-        p.chunk = local - rest * chunks;
+        const int rest = local / segments;
+        segment = local - rest * segments;
         p.frame = rest / groups;
         p.group = rest - p.frame * groups;
     } else {
-        const int per_frame = chunks * groups;
+        const int per_frame = segments * groups;
         p.frame = local / per_frame;
         const int r = local - p.frame * per_frame;
-        p.chunk = r / groups;
-        p.group = r - p.chunk * groups;
+        segment = r / groups;
+        p.group = r - segment * groups;
     }
+    p.chunk = segment * window_span(jb);
     return p;
 }
 
@@ -104,6 +123,8 @@ __device__ __forceinline__ void forward_batch(ConstPtr<float> rec, const float (
 // Backward sweep over y[0, L + P) from the end, a batch of rows' backward
 // records per group of scalar loads; record row of y[k] is k + P.  W == 3
 // sums d ascending, other bands descending, as the reference solver does.
+// The chunk's rows y[0, L) receive the solution; y[L, L + P) keep their
+// forward values, from which the segment's next chunk continues.
 template <int W, int L, int P>
 __device__ __forceinline__ void backward_sweep(ConstPtr<float> bwd, float (&y)[L + P]) {
     constexpr int BW = window_bwd_words(W);
@@ -114,12 +135,16 @@ __device__ __forceinline__ void backward_sweep(ConstPtr<float> bwd, float (&y)[L
 #pragma unroll
     for (int top = L + P; top > 0; top -= BB) {
         const int count = top < BB ? top : BB;
+        // The batch's records through a pointer pinned here, so that their
+        // scalar loads stay with the batch: the records are invariant loads,
+        // and hoisted all together they would exhaust the scalar registers.
+        const auto rec = pinned(bwd, st[0]);
         float c[BB][W];
 #pragma unroll
         for (int u = 0; u < BB; ++u)
             if (u < count)
 #pragma unroll
-                for (int d = 0; d < W; ++d) c[u][d] = bwd[(top - 1 - u + P) * BW + d];
+                for (int d = 0; d < W; ++d) c[u][d] = rec[(top - 1 - u + P) * BW + d];
 #pragma unroll
         for (int u = 0; u < BB; ++u) {
             if (u >= count) break;
@@ -133,76 +158,182 @@ __device__ __forceinline__ void backward_sweep(ConstPtr<float> bwd, float (&y)[L
                 for (int d = W; d >= 1; --d) v = __builtin_fmaf(-c[u][d - 1], st[d - 1], v);
             }
             push(st, v);
-            y[k] = v;
+            if (k < L) y[k] = v;
         }
     }
 }
+
+// Integer samples staged two to a 32-bit register: 16-bit halves for u16,
+// the low two bytes for u8.  `both` loads an aligned pair, `one` a single
+// sample, `join` packs two singles as `both` would.
+template <class In> struct Pair;
+template <> struct Pair<std::uint16_t> {
+    __device__ static std::uint32_t both(__amdgpu_buffer_rsrc_t r, int v) {
+        return __builtin_amdgcn_raw_buffer_load_b32(r, v, 0, 0);
+    }
+    __device__ static std::uint32_t one(__amdgpu_buffer_rsrc_t r, int v) {
+        return __builtin_amdgcn_raw_buffer_load_b16(r, v, 0, 0);
+    }
+    __device__ static std::uint32_t join(std::uint32_t a, std::uint32_t b) { return a | (b << 16); }
+    __device__ static float first(std::uint32_t w) { return static_cast<float>(w & 0xFFFFU); }
+    __device__ static float second(std::uint32_t w) { return static_cast<float>(w >> 16); }
+};
+template <> struct Pair<std::uint8_t> {
+    __device__ static std::uint32_t both(__amdgpu_buffer_rsrc_t r, int v) {
+        return __builtin_amdgcn_raw_buffer_load_b16(r, v, 0, 0);
+    }
+    __device__ static std::uint32_t one(__amdgpu_buffer_rsrc_t r, int v) {
+        return __builtin_amdgcn_raw_buffer_load_b8(r, v, 0, 0);
+    }
+    __device__ static std::uint32_t join(std::uint32_t a, std::uint32_t b) { return a | (b << 8); }
+    __device__ static float first(std::uint32_t w) { return static_cast<float>(w & 0xFFU); }
+    __device__ static float second(std::uint32_t w) { return static_cast<float>((w >> 8) & 0xFFU); }
+};
+
+// One converted sample into a plane through a buffer: `v` bytes of lane
+// offset plus `s` bytes of wave-uniform offset.
+template <class Out> struct BufferStore;
+template <> struct BufferStore<float> {
+    __device__ static void at(__amdgpu_buffer_rsrc_t r, int v, int s, float x, const DevConv &) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(std::uint32_t, x), r, v, s, 0);
+    }
+};
+template <class T> __device__ __forceinline__ T to_sample(float v, const DevConv &c) {
+    // Output conversion of Store<T>::f32 (kernels_common.hpp).
+    float s = __fadd_rn(__fmul_rn(v, c.out_scale), c.out_offset);
+    s = s < 0.0F ? 0.0F : (c.out_max < s ? c.out_max : s);
+    return static_cast<T>(rintf(s));
+}
+template <> struct BufferStore<std::uint16_t> {
+    __device__ static void at(__amdgpu_buffer_rsrc_t r, int v, int s, float x, const DevConv &c) {
+        __builtin_amdgcn_raw_buffer_store_b16(to_sample<std::uint16_t>(x, c), r, v, s, 0);
+    }
+};
+template <> struct BufferStore<std::uint8_t> {
+    __device__ static void at(__amdgpu_buffer_rsrc_t r, int v, int s, float x, const DevConv &c) {
+        __builtin_amdgcn_raw_buffer_store_b8(to_sample<std::uint8_t>(x, c), r, v, s, 0);
+    }
+};
 
 template <int W, int K, int G, class In>
 __device__ __forceinline__ void rows_item(const WinJob &jb, const ItemPos &at, float *ring,
                                           int lane) {
     constexpr int L = window_length(W);
     constexpr int P = window_warmup(W);
-    constexpr int NB = (L + 2 * P) / G;
+    constexpr int NBL = L / G;         // staging blocks per chunk
+    constexpr int NBP = 2 * P / G;     // blocks before a chunk's own rows in its window
     constexpr int RW = window_fwd_words(W, K);
-    constexpr int B = window_fwd_batch(W, K);
-    static_assert((L + 2 * P) % G == 0 && L % G == 0 && G % B == 0, "blocks tile the window");
+    constexpr bool kInt = !std::is_same_v<In, float>;
+    // Rows per batch of scalar record loads; integer samples use 2 to make
+    // room for their second staging buffer.
+    constexpr int B = kInt ? 2 : window_fwd_batch(W, K);
+    // Blocks in flight: integer samples take half the staging registers, so
+    // two blocks are loaded ahead of the one being solved.
+    constexpr int D = kInt ? 2 : 1;
+    static_assert(L % G == 0 && (2 * P) % G == 0 && G % B == 0 && NBL % D == 0,
+                  "blocks tile the window");
     const DevWindow &pl = jb.plan;
     const int nrows = jb.nvec;
     const int row0 = at.group * 64;
-    const int r0 = at.chunk * L;
+    const int q0 = at.chunk;
+    const int q1 = min(q0 + window_span(jb), pl.chunks);
     const In *src = static_cast<const In *>(jb.src) + static_cast<long>(at.frame) * jb.s_frame;
     const auto rs = plane_rsrc(src, static_cast<long>(nrows) * jb.s_row * static_cast<long>(sizeof(In)));
-    const int m1 = pl.m - 1;
-    const int scol = lane & (kStageCols - 1);
-    const int srow = lane >> 5;
+    const int m = pl.m;
+    const int s_row = static_cast<int>(jb.s_row);
     // Plan tables through the constant address space: the loads stay scalar
     // across the inline-asm pins below (which count as memory side effects).
-    const auto ts = constant(pl.tap_start + r0);
-    const auto fr = constant(pl.fwd + static_cast<long>(r0) * RW);
-    const auto blk = constant((G == 16 ? pl.blk_lo16 : pl.blk_lo) + r0 / G);
+    // Indexed by padded row (records, tap runs) and by block (staging starts).
+    const auto ts = constant(pl.tap_start);
+    const auto fwd = constant(pl.fwd);
+    const auto blk = constant(G == 16 ? pl.blk_lo16 : pl.blk_lo);
     const DevConv cv = jb.conv;
     const float *mine = ring + lane * kRowPitch;   // this lane's row of the ring
 
-    // The next block's 32 source columns are loaded into registers while the
-    // current block is solved, so the loads' latency overlaps the solve.
-    // Held as 32-bit words so they can be pinned in registers below.
-    using Word = std::conditional_t<std::is_same_v<In, float>, float, std::uint32_t>;
-    // Rows past the plane (last vector group) read 0 through the buffer's
-    // range check; their results are not stored.  The lane's address is
-    // rebuilt per block (pinned, so the 32 row offsets are not kept live).
-    Word staged[32];
-    const int row_step = 2 * static_cast<int>(jb.s_row) * static_cast<int>(sizeof(In));
-    auto fetch = [&](int b) {
-        const int column = min(blk[b] + scol, m1);
-        int at = ((row0 + srow) * static_cast<int>(jb.s_row) + column) * static_cast<int>(sizeof(In));
-        asm volatile("" : "+v"(at));
+    // Staging of a block's 32 source columns for the 64 rows: F32 samples
+    // one per register (lane: column lane & 31 of rows 2t + lane / 32),
+    // integer samples two per register (lane: columns 2 (lane & 15) + {0, 1}
+    // of rows 4t + lane / 16).  Pairs are loaded whole when every pair is
+    // aligned and inside its row (even width, stride and frame stride,
+    // aligned base), else sample by sample.  Rows past the plane (last
+    // vector group) read 0 through the buffer's range check; their results
+    // are not stored.  The lane's address is rebuilt per block (pinned, so
+    // the row offsets are not kept live).
+    constexpr int NS = kInt ? 16 : 32;
+    constexpr int kLoadRows = kInt ? 4 : 2;
+    using Word = std::conditional_t<kInt, std::uint32_t, float>;
+    Word staged[D][NS];
+    const bool paired =
+        kInt && uniform((m % 2) == 0 && (s_row % 2) == 0 && (jb.s_frame % 2) == 0
+                        && (reinterpret_cast<std::uintptr_t>(jb.src) % (2 * sizeof(In))) == 0) != 0;
+    const int scol = kInt ? 2 * (lane & 15) : (lane & 31);
+    const int srow = kInt ? (lane >> 4) : (lane >> 5);
+    const int row_step = kLoadRows * s_row * static_cast<int>(sizeof(In));
+    // Blocks before padded row 0 hold zero rows: they load block 0's samples
+    // (finite, never used) and skip the solve.
+    auto fetch = [&](int gb, Word (&to)[NS]) {
+        const int lo = blk[max(gb, 0)];
+        const int rbase = (row0 + srow) * s_row;
+        if constexpr (kInt) {
+            if (paired) {
+                int a = (rbase + min(lo + scol, m - 2)) * static_cast<int>(sizeof(In));
+                asm volatile("" : "+v"(a));
 #pragma unroll
-        for (int t = 0; t < 32; ++t)
-            staged[t] = static_cast<Word>(BufferLoad<In>::at(rs, at + t * row_step, 0));
+                for (int t = 0; t < NS; ++t) to[t] = Pair<In>::both(rs, a + t * row_step);
+            } else {
+                int a0 = (rbase + min(lo + scol, m - 1)) * static_cast<int>(sizeof(In));
+                int a1 = (rbase + min(lo + scol + 1, m - 1)) * static_cast<int>(sizeof(In));
+                asm volatile("" : "+v"(a0), "+v"(a1));
+#pragma unroll
+                for (int t = 0; t < NS; ++t)
+                    to[t] = Pair<In>::join(Pair<In>::one(rs, a0 + t * row_step),
+                                           Pair<In>::one(rs, a1 + t * row_step));
+            }
+        } else {
+            int a = (rbase + min(lo + scol, m - 1)) * 4;
+            asm volatile("" : "+v"(a));
+#pragma unroll
+            for (int t = 0; t < NS; ++t) to[t] = BufferLoad<float>::at(rs, a + t * row_step, 0);
+        }
     };
-    fetch(0);
+    // A staged block into the ring.  Pinned here, so the conversion (and
+    // the wait for the loads) stays at this point instead of moving up to
+    // the fetch.
+    auto stage = [&](Word (&from)[NS]) {
+#pragma unroll
+        for (int t = 0; t < NS; ++t) {
+            asm volatile("" : "+v"(from[t]));
+            float *to = ring + (srow + kLoadRows * t) * kRowPitch + scol;
+            if constexpr (kInt) {
+                to[0] = __fsub_rn(Pair<In>::first(from[t]), cv.in_offset);
+                to[1] = __fsub_rn(Pair<In>::second(from[t]), cv.in_offset);
+            } else {
+                to[0] = from[t];
+            }
+        }
+    };
 
     float st[W];
 #pragma unroll
     for (int d = 0; d < W; ++d) st[d] = 0.0F;
     float y[L + P];
 #pragma unroll
-    for (int b = 0; b < NB; ++b) {
-        const int lo = blk[b];
-        int start[G];   // the block's tap runs, one scalar load
-#pragma unroll
-        for (int t = 0; t < G; ++t) start[t] = ts[b * G + t] - lo;
+    for (int k = 0; k < P; ++k) y[k] = 0.0F;
+    // The segment's blocks run from `first` to `last`; block gb is staged
+    // from buffer gb % D (the loop body holds a whole number of D blocks),
+    // and requests block gb + D into that buffer once staged.  Row t of the
+    // block is y[base + t] (rows before y[0] are warm-up, not kept).
+    constexpr int WARM = (2 * P + L - 1) / L;
+    const int first = (q0 - WARM) * NBL + NBP;
+    const int last = (q1 - 1) * NBL + NBP + NBL - 1;
+    auto block = [&](int gb, int base, Word (&buf)[NS]) {
         wave_sync();
-#pragma unroll
-        for (int t = 0; t < 32; ++t) {
-            // Pinned here, so the conversion (and the wait for the load)
-            // stays at this point instead of moving up to the fetch.
-            asm volatile("" : "+v"(staged[t]));
-            ring[(srow + 2 * t) * kRowPitch + scol] = Load<In>::norm(static_cast<In>(staged[t]), cv);
-        }
+        stage(buf);
         wave_sync();
-        if (b + 1 < NB) fetch(b + 1);
+        if (gb + D <= last) fetch(gb + D, buf);
+        if (gb < 0) return;
+        const int lo = blk[gb];
+        const auto fr = fwd + static_cast<long>(gb) * G * RW;
 #pragma unroll
         for (int j = 0; j < G; j += B) {
             // Each row's K taps are consecutive columns of this lane's ring
@@ -211,40 +342,68 @@ __device__ __forceinline__ void rows_item(const WinJob &jb, const ItemPos &at, f
             float tap[B][K];
 #pragma unroll
             for (int u = 0; u < B; ++u) {
-                const float *a = mine + start[j + u];
+                const float *a = mine + (ts[gb * G + j + u] - lo);
 #pragma unroll
                 for (int e = 0; e < K; ++e) tap[u][e] = a[e];
             }
             float out[B];
-            forward_batch<W, K, B>(fr + (b * G + j) * RW, tap, st, out);
+            forward_batch<W, K, B>(pinned(fr + j * RW, st[0]), tap, st, out);
 #pragma unroll
             for (int u = 0; u < B; ++u)
-                if (b * G + j + u >= P) y[b * G + j + u - P] = out[u];
+                if (base + j + u >= 0) y[base + j + u] = out[u];
         }
-    }
-    backward_sweep<W, L, P>(constant(pl.bwd + static_cast<long>(r0) * window_bwd_words(W)), y);
+    };
 
-    // Results leave through the ring, 32 destination columns at a time:
-    // lanes 0-31 write row t, lanes 32-63 row t + 1 (128-byte runs).
-    float *out = static_cast<float *>(jb.dst) + static_cast<long>(at.frame) * jb.d_frame + r0;
-    const int oc = lane & 31;
-    const int orow = lane >> 5;
     const int n = pl.n;
+    float *dst = static_cast<float *>(jb.dst) + static_cast<long>(at.frame) * jb.d_frame;
+    const auto ws = plane_rsrc(dst, static_cast<long>(nrows) * jb.d_row * 4);
+    const int d_row = static_cast<int>(jb.d_row);
+
+    // Iteration q runs the forward sweep over padded rows [qL + 2P,
+    // (q + 1)L + 2P), i.e. y[P, L + P) of chunk q, then (for q >= q0) the
+    // chunk's backward sweep and stores.  The WARM iterations before q0 only
+    // run the forward sweep: they cover the segment's warm-up (at least 2P
+    // rows; blocks before padded row 0 are skipped).  One loop body for all
+    // iterations keeps the register allocation of a single chunk.
 #pragma unroll
-    for (int kb = 0; kb < L; kb += 32) {
-        wave_sync();
+    for (int d = 0; d < D; ++d) fetch(first + d, staged[d]);
+    int gb = first;
+    for (int q = q0 - WARM; q < q1; ++q) {
 #pragma unroll
-        for (int c = 0; c < 32; ++c)
-            if (kb + c < L) ring[lane * kRowPitch + c] = y[kb + c];
-        wave_sync();
-        const bool col_ok = kb + oc < L && r0 + kb + oc < n;
+        for (int b = NBP; b < NBP + NBL; ++b, ++gb) block(gb, b * G - P, staged[(b - NBP) % D]);
+        if (q >= q0) {
+            backward_sweep<W, L, P>(constant(pl.bwd + static_cast<long>(q) * L * window_bwd_words(W)), y);
+
+            // Results leave through the ring, 32 destination columns at a
+            // time: lanes 0-31 write row t, lanes 32-63 row t + 1 (128-byte
+            // runs).  Rows past the plane fall outside the store buffer's
+            // range.  The lane's addresses are derived here from a pinned
+            // copy of the lane index, so that they are not kept live (or
+            // spilled) across the chunk loop.
+            const int r0 = q * L;
+            int ln = lane;
+            asm volatile("" : "+v"(ln));
+            const int oc = ln & 31;
+            const int orow = ln >> 5;
+            float *const my_row = ring + ln * kRowPitch;
+#pragma unroll
+            for (int kb = 0; kb < L; kb += 32) {
+                wave_sync();
+#pragma unroll
+                for (int c = 0; c < 32; ++c)
+                    if (kb + c < L) my_row[c] = kInt ? __fmul_rn(y[kb + c], cv.in_scale) : y[kb + c];
+                wave_sync();
+                if (kb + oc < L && r0 + kb + oc < n) {
+                    const int a = ((row0 + orow) * d_row + r0 + kb + oc) * 4;
+                    const float *from = ring + orow * kRowPitch + oc;
 #pragma unroll 4
-        for (int t = 0; t < 64; t += 2) {
-            const int rr = t + orow;
-            const int row = row0 + rr;
-            if (col_ok && row < nrows)
-                out[static_cast<long>(row) * jb.d_row + kb + oc] = ring[rr * kRowPitch + oc];
+                    for (int t = 0; t < 64; t += 2)
+                        BufferStore<float>::at(ws, a + t * d_row * 4, 0, from[t * kRowPitch], cv);
+                }
+            }
         }
+#pragma unroll
+        for (int k = 0; k < P; ++k) y[k] = y[L + k];
     }
 }
 
@@ -253,50 +412,52 @@ __device__ __forceinline__ void columns_item(const WinJob &jb, const ItemPos &at
                                              int lane) {
     constexpr int L = window_length(W);
     constexpr int P = window_warmup(W);
-    constexpr int NB = (L + 2 * P) / G;
+    constexpr int NBL = L / G;
+    constexpr int NBP = 2 * P / G;
     constexpr int RW = window_fwd_words(W, K);
     constexpr int B = window_fwd_batch(W, K);
-    static_assert((L + 2 * P) % G == 0 && L % G == 0 && G % B == 0, "blocks tile the window");
+    static_assert(L % G == 0 && (2 * P) % G == 0 && G % B == 0, "blocks tile the window");
     static_assert(kStageCols * 64 <= kWinRing, "ring holds a block");
     const DevWindow &pl = jb.plan;
     const int ncols = jb.nvec;
     const int col = at.group * 64 + lane;
     const bool live = col < ncols;
     const int cc = live ? col : ncols - 1;
-    const int r0 = at.chunk * L;
+    const int q0 = at.chunk;
+    const int q1 = min(q0 + window_span(jb), pl.chunks);
     const float *src = static_cast<const float *>(jb.src) + static_cast<long>(at.frame) * jb.s_frame;
     const auto rs = plane_rsrc(src, static_cast<long>(pl.m) * jb.s_row * 4);
     const int voff = cc * 4;
     const int row_bytes = static_cast<int>(jb.s_row * 4);
     const int m1 = pl.m - 1;
-    const auto ts = constant(pl.tap_start + r0);
-    const auto fr = constant(pl.fwd + static_cast<long>(r0) * RW);
-    const auto blk = constant((G == 16 ? pl.blk_lo16 : pl.blk_lo) + r0 / G);
+    const auto ts = constant(pl.tap_start);
+    const auto fwd = constant(pl.fwd);
+    const auto blk = constant(G == 16 ? pl.blk_lo16 : pl.blk_lo);
     const float *mine = ring + lane;   // this lane's column of the ring
 
     // Each block stages the 32 source rows its taps need (one 256-byte run
     // per row across the wavefront) into the ring, the next block's rows in
     // flight meanwhile; the taps then read the ring, each source row loaded
-    // from memory once per block instead of once per tap.
+    // from memory once per block instead of once per tap.  Blocks before
+    // padded row 0 load block 0's rows and skip the solve.
     float staged[32];
-    auto fetch = [&](int b) {
-        const int lo = blk[b];
+    auto fetch = [&](int gb) {
+        const int lo = blk[max(gb, 0)];
 #pragma unroll
         for (int t = 0; t < 32; ++t)
             staged[t] = BufferLoad<float>::at(rs, voff, min(lo + t, m1) * row_bytes);
     };
-    fetch(0);
 
     float st[W];
 #pragma unroll
     for (int d = 0; d < W; ++d) st[d] = 0.0F;
     float y[L + P];
 #pragma unroll
-    for (int b = 0; b < NB; ++b) {
-        const int lo = blk[b];
-        int start[G];
-#pragma unroll
-        for (int t = 0; t < G; ++t) start[t] = ts[b * G + t] - lo;
+    for (int k = 0; k < P; ++k) y[k] = 0.0F;
+    constexpr int WARM = (2 * P + L - 1) / L;
+    const int first = (q0 - WARM) * NBL + NBP;
+    const int last = (q1 - 1) * NBL + NBP + NBL - 1;
+    auto block = [&](int gb, int base) {
         wave_sync();
 #pragma unroll
         for (int t = 0; t < 32; ++t) {
@@ -304,31 +465,65 @@ __device__ __forceinline__ void columns_item(const WinJob &jb, const ItemPos &at
             ring[t * 64 + lane] = staged[t];
         }
         wave_sync();
-        if (b + 1 < NB) fetch(b + 1);
+        if (gb < last) fetch(gb + 1);
+        if (gb < 0) return;
+        const int lo = blk[gb];
+        const auto fr = fwd + static_cast<long>(gb) * G * RW;
 #pragma unroll
         for (int j = 0; j < G; j += B) {
             float tap[B][K];
 #pragma unroll
             for (int u = 0; u < B; ++u) {
-                const float *row = mine + start[j + u] * 64;
+                const float *row = mine + (ts[gb * G + j + u] - lo) * 64;
 #pragma unroll
                 for (int e = 0; e < K; ++e) tap[u][e] = row[e * 64];
             }
             float out[B];
-            forward_batch<W, K, B>(fr + (b * G + j) * RW, tap, st, out);
+            forward_batch<W, K, B>(pinned(fr + j * RW, st[0]), tap, st, out);
 #pragma unroll
             for (int u = 0; u < B; ++u)
-                if (b * G + j + u >= P) y[b * G + j + u - P] = out[u];
+                if (base + j + u >= 0) y[base + j + u] = out[u];
         }
-    }
-    backward_sweep<W, L, P>(constant(pl.bwd + static_cast<long>(r0) * window_bwd_words(W)), y);
+    };
 
-    Out *o = static_cast<Out *>(jb.dst) + static_cast<long>(at.frame) * jb.d_frame + cc;
+    // Results: row r0 + k of column cc, through a buffer over the output
+    // plane (the row as the wave-uniform offset; rows past the plane are
+    // skipped, columns past it masked).
     const int n = pl.n;
+    Out *dst = static_cast<Out *>(jb.dst) + static_cast<long>(at.frame) * jb.d_frame;
+    const auto ws = plane_rsrc(dst, static_cast<long>(n) * jb.d_row * static_cast<long>(sizeof(Out)));
+    const int d_row_bytes = static_cast<int>(jb.d_row) * static_cast<int>(sizeof(Out));
+    const int ooff = cc * static_cast<int>(sizeof(Out));
     const DevConv cv = jb.conv;
+
+    // The iterations of rows_item: WARM forward-only iterations, then one
+    // per chunk; the next block's rows are always requested before a
+    // chunk's stores.
+    fetch(first);
+    int gb = first;
+    for (int q = q0 - WARM; q < q1; ++q) {
 #pragma unroll
-    for (int k = 0; k < L; ++k)
-        if (r0 + k < n && live) Store<Out>::f32(o + static_cast<long>(r0 + k) * jb.d_row, y[k], cv);
+        for (int b = NBP; b < NBP + NBL; ++b, ++gb) block(gb, b * G - P);
+        if (q >= q0) {
+            backward_sweep<W, L, P>(constant(pl.bwd + static_cast<long>(q) * L * window_bwd_words(W)), y);
+            // Row offsets advance one row at a time through a pinned
+            // register (precomputed for all L rows they would exhaust the
+            // scalar registers).
+            const int r0 = q * L;
+            if (live) {
+                const int rows = min(L, n - r0);
+                int so = r0 * d_row_bytes;
+#pragma unroll
+                for (int k = 0; k < L; ++k) {
+                    if (k < rows) BufferStore<Out>::at(ws, ooff, so, y[k], cv);
+                    so += d_row_bytes;
+                    asm volatile("" : "+s"(so));
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < P; ++k) y[k] = y[L + k];
+    }
 }
 
 // A job record, read with scalar loads.
@@ -426,6 +621,42 @@ hipError_t launch_table(const WinJob *jobs, int count, long items, bool rows, bo
     return hipGetLastError();
 }
 
+// Wavefronts of the launch's kernel the GPU holds at once (occupancy per CU
+// times CUs), computed once per kernel.
+template <class F> long resident_waves(F kernel) {
+    int blocks = 0, device = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, 64 * kWindowWaves, 0) != hipSuccess
+        || blocks <= 0)
+        blocks = 1;
+    if (hipGetDevice(&device) != hipSuccess
+        || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess
+        || cus <= 0)
+        cus = 256;
+    return static_cast<long>(blocks) * cus * kWindowWaves;
+}
+
+template <int W, int K, class T> long table_slots(bool rows, bool block16) {
+    if (!rows && block16) {
+        static const long v = resident_waves(columns_window_table<W, K, 16, T>);
+        return v;
+    }
+    if (!rows) {
+        static const long v = resident_waves(columns_window_table<W, K, 8, T>);
+        return v;
+    }
+    if (block16) {
+        static const long v = resident_waves(rows_window_table4<W, K, 16, T>);
+        return v;
+    }
+    if constexpr (rows_four_waves<W, T, 8>) {
+        static const long v = resident_waves(rows_window_table4<W, K, 8, T>);
+        return v;
+    } else {
+        static const long v = resident_waves(rows_window_table<W, K, 8, T>);
+        return v;
+    }
+}
+
 } // namespace
 } // namespace dev
 
@@ -437,6 +668,16 @@ hipError_t launch_window_table_band(const WinJob *jobs, int count, long items, b
     case Sample::u8: return dev::launch_table<W, K, std::uint8_t>(jobs, count, items, rows, block16, order, s);
     case Sample::u16: return dev::launch_table<W, K, std::uint16_t>(jobs, count, items, rows, block16, order, s);
     default: return hipErrorInvalidValue;
+    }
+}
+
+template <int W, int K>
+long window_table_slots_band(bool rows, bool block16, Sample kind) {
+    switch (kind) {
+    case Sample::f32: return dev::table_slots<W, K, float>(rows, block16);
+    case Sample::u8: return dev::table_slots<W, K, std::uint8_t>(rows, block16);
+    case Sample::u16: return dev::table_slots<W, K, std::uint16_t>(rows, block16);
+    default: return 0;
     }
 }
 

@@ -4,4 +4,5 @@
 namespace dsmvc::hip {
 template hipError_t launch_window_table_band<1, 8>(const WinJob *, int, long, bool, bool,
                                                         Sample, int, hipStream_t);
+template long window_table_slots_band<1, 8>(bool, bool, Sample);
 } // namespace dsmvc::hip

@@ -4,4 +4,5 @@
 namespace dsmvc::hip {
 template hipError_t launch_window_table_band<7, 16>(const WinJob *, int, long, bool, bool,
                                                         Sample, int, hipStream_t);
+template long window_table_slots_band<7, 16>(bool, bool, Sample);
 } // namespace dsmvc::hip

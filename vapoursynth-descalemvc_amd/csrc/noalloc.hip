@@ -209,18 +209,20 @@ void HipExecutor::inverse_2d_batch_noalloc(const AxisPlan &h, const AxisPlan &v,
         c.frames = b.frames;
         c.groups = (c.nvec + 63) / 64;
         c.conv = conv;
+        const int band_h = hip::window_band_class(h.half_bandwidth);
+        const int band_v = hip::window_band_class(v.half_bandwidth);
+        const long items_h = hip::assign_window_spans(
+            &a, 1, hip::window_table_slots(band_h, a.plan.ell_width, true, a.plan.block16 != 0, in_kind));
+        const long items_v = hip::assign_window_spans(
+            &c, 1, hip::window_table_slots(band_v, c.plan.ell_width, false, c.plan.block16 != 0, in_kind));
         store_jobs<<<1, 64, 0, stream>>>(a, c, jobs);
         check(hipGetLastError(), "job record launch failed");
         constexpr int order = 2;   // XCD-contiguous workgroups, vector group fastest
-        const long items_h = static_cast<long>(a.frames) * a.plan.chunks * a.groups;
-        const long items_v = static_cast<long>(c.frames) * c.plan.chunks * c.groups;
-        check(hip::launch_window_table(jobs, 1, items_h, hip::window_band_class(h.half_bandwidth),
-                                       a.plan.ell_width, true, a.plan.block16 != 0, in_kind, order,
-                                       stream),
+        check(hip::launch_window_table(jobs, 1, items_h, band_h, a.plan.ell_width, true,
+                                       a.plan.block16 != 0, in_kind, order, stream),
               "windowed launch failed");
-        check(hip::launch_window_table(jobs + 1, 1, items_v, hip::window_band_class(v.half_bandwidth),
-                                       c.plan.ell_width, false, c.plan.block16 != 0, in_kind, order,
-                                       stream),
+        check(hip::launch_window_table(jobs + 1, 1, items_v, band_v, c.plan.ell_width, false,
+                                       c.plan.block16 != 0, in_kind, order, stream),
               "windowed launch failed");
     } else {
         // Sequential kernels (the exact route; the Float64 route's
