@@ -102,6 +102,10 @@ typedef struct dsmvc_hip_planner_stats {
     uint64_t plan_builds;
     uint64_t plan_entries;
     uint64_t plan_resident_bytes;
+    uint64_t geometry_hits;      /* sampling geometries reused (any kernel or blur of equal support) */
+    uint64_t geometry_builds;
+    uint64_t geometry_entries;
+    uint64_t geometry_resident_bytes;
 } dsmvc_hip_planner_stats;
 
 typedef struct dsmvc_hip_plan dsmvc_hip_plan;         /* immutable axis plan */
@@ -170,10 +174,12 @@ typedef struct dsmvc_hip_plan_data {
  * (src/axis_plan.cpp:1252-1341), else DSMVC_HIP_INVALID_ARGUMENT. */
 int dsmvc_hip_plan_import(const dsmvc_hip_plan_info *info, const dsmvc_hip_plan_data *data,
                           dsmvc_hip_plan **plan);
-/* dsmvc::planner_cache_stats / clear_planner_caches (engine.hpp:179-180). */
 /* No counterpart in the reference: reports how the windowed (independent
  * chunk) kernels would partition this plan, computed on the host. */
 int dsmvc_hip_plan_window_info(const dsmvc_hip_plan *plan, dsmvc_hip_window_info *info);
+/* dsmvc::planner_cache_stats / clear_planner_caches (engine.hpp:140-149,
+ * 179-180): the plan LRU and the sampling-geometry LRU beneath it
+ * (src/axis_plan.cpp:520-533). */
 int dsmvc_hip_planner_cache_stats(dsmvc_hip_planner_stats *stats);
 void dsmvc_hip_clear_planner_caches(void);
 /* dsmvc::detail::padding_index (src/axis_plan.cpp:1024-1066). */

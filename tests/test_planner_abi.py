@@ -147,6 +147,40 @@ def test_planner_cache_single_flight():
     assert dh.planner_cache_stats()["plan_builds"] == 2
 
 
+def test_geometry_reused_across_kernels_blur_and_precision(oracle):
+    """The sampling geometry is keyed by sizes, active length, shift, support
+    and border only (reference axis_plan.cpp:444-454, 520-533): lanczos3 and
+    spline36 (support 3), bicubic at blur 1.0 and 0.9 (support ceil(1.8) = 2)
+    and every f64mode share one geometry, and each plan built on a reused
+    geometry equals the oracle's bit for bit."""
+    dh.clear_planner_caches()
+    shared = [make_request(1920, 1136, kernel=dh.LANCZOS, taps=3),
+              make_request(1920, 1136, kernel=dh.SPLINE36),
+              make_request(1920, 1136, kernel=dh.LANCZOS, taps=3, f64_mode=2),
+              make_request(1920, 1136, kernel=dh.SPLINE36, f64_mode=1)]
+    for req in shared:
+        _equal(_product_plan(req), oracle.build(req).arrays())
+    stats = dh.planner_cache_stats()
+    assert stats["plan_builds"] == 4
+    assert stats["geometry_builds"] == 1 and stats["geometry_hits"] == 3
+    assert stats["geometry_entries"] == 1 and stats["geometry_resident_bytes"] > 0
+    blur = [make_request(1920, 1136, kernel=dh.BICUBIC),
+            make_request(1920, 1136, kernel=dh.BICUBIC, blur=0.9),
+            make_request(1920, 1136, kernel=dh.BICUBIC, b=1 / 3, c=1 / 3, blur=0.95)]
+    for req in blur:
+        _equal(_product_plan(req), oracle.build(req).arrays())
+    stats = dh.planner_cache_stats()
+    assert stats["geometry_builds"] == 2 and stats["geometry_hits"] == 5
+    # A different shift or border is a different geometry.
+    for req in (make_request(1920, 1136, 1136.0, 0.25, dh.BICUBIC),
+                make_request(1920, 1136, kernel=dh.BICUBIC, border=dh.ZERO)):
+        _equal(_product_plan(req), oracle.build(req).arrays())
+    assert dh.planner_cache_stats()["geometry_builds"] == 4
+    dh.clear_planner_caches()
+    stats = dh.planner_cache_stats()
+    assert stats["geometry_entries"] == 0 and stats["geometry_builds"] == 0
+
+
 def test_padding_index_table():
     expect = [2, 1, 0, 1, 2, 3, 2, 1, 0, 1, 2, 3, 2, 1, 0, 1, 2, 3, 2, 1]
     assert [dh.padding_index(-8 + k, 4, dh.REFLECT101) for k in range(20)] == expect

@@ -312,6 +312,10 @@ int dsmvc_hip_planner_cache_stats(dsmvc_hip_planner_stats *stats) {
         stats->plan_builds = s.plan_builds;
         stats->plan_entries = s.plan_entries;
         stats->plan_resident_bytes = s.plan_resident_bytes;
+        stats->geometry_hits = s.geometry_hits;
+        stats->geometry_builds = s.geometry_builds;
+        stats->geometry_entries = s.geometry_entries;
+        stats->geometry_resident_bytes = s.geometry_resident_bytes;
     });
 }
 

@@ -103,6 +103,10 @@ struct PlannerCacheStats {
     std::uint64_t plan_builds = 0;
     std::size_t plan_entries = 0;
     std::size_t plan_resident_bytes = 0;
+    std::uint64_t geometry_hits = 0;
+    std::uint64_t geometry_builds = 0;
+    std::size_t geometry_entries = 0;
+    std::size_t geometry_resident_bytes = 0;
 };
 
 // Planner (host, FP64).  Throws std::invalid_argument for a malformed request

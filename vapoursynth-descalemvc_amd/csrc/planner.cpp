@@ -140,82 +140,206 @@ struct SparseRows {
     std::vector<double> value;
 };
 
-// The forward resampling operator A (source samples x destination samples):
-// one row per source sample, its 2*support taps mapped through the border
-// rule onto destination indices, weights normalised by their sum and
-// coalesced per destination index in ascending index order
-// (reference build_geometry + make_descale_matrix, axis_plan.cpp:456-599).
-SparseRows sample_operator(const AxisRequest &r, std::int32_t support, const CustomKernel &custom) {
+// Sampling geometry of an axis: for every source sample, its 2*support tap
+// offsets (tap centre - sample position) and the destination index of each
+// tap through the border rule (-1: none), plus the sample's distinct
+// destination indices in ascending order.  It depends only on the sizes,
+// active length, shift, support and border, so every kernel and blur of the
+// same support reuses it (reference build_geometry, axis_plan.cpp:456-518,
+// cached by get_geometry, 520-533).
+struct Geometry {
+    std::int32_t taps = 0;
+    std::vector<double> offset;          // [sample][tap]
+    std::vector<std::int32_t> dest;      // [sample][tap]
+    std::vector<std::uint32_t> start;    // [sample + 1] into `unique`
+    std::vector<std::int32_t> unique;
+
+    [[nodiscard]] std::size_t storage_bytes() const noexcept {
+        return sizeof(*this) + offset.capacity() * sizeof(double) + dest.capacity() * 4
+            + start.capacity() * 4 + unique.capacity() * 4;
+    }
+};
+
+Geometry build_geometry(const AxisRequest &r, std::int32_t support) {
     const std::int32_t taps = 2 * support;
     const auto m = static_cast<std::size_t>(r.source_size);
-    if (m > std::numeric_limits<std::size_t>::max() / static_cast<std::size_t>(taps))
+    if (m > std::numeric_limits<std::size_t>::max() / 2 / static_cast<std::size_t>(taps))
         throw std::length_error("inverse geometry is too large");
-    SparseRows a;
-    a.start.reserve(m + 1);
-    a.column.reserve(m * static_cast<std::size_t>(taps));
-    a.value.reserve(m * static_cast<std::size_t>(taps));
-    a.start.push_back(0U);
-
-    struct Tap { double offset; double weight; std::int32_t dest; };
-    std::vector<Tap> tap(static_cast<std::size_t>(taps));
+    Geometry g;
+    g.taps = taps;
+    g.offset.resize(m * static_cast<std::size_t>(taps));
+    g.dest.resize(m * static_cast<std::size_t>(taps));
+    g.start.reserve(m + 1);
+    g.unique.reserve(m * static_cast<std::size_t>(taps));
+    g.start.push_back(0U);
     std::vector<std::int32_t> dests;
-    std::vector<double> merged;
-    std::vector<char> used;
-    dests.reserve(tap.size());
-
+    dests.reserve(static_cast<std::size_t>(taps));
     const double ratio = static_cast<double>(r.source_size) / r.active_length;
-    const bool unit_blur = r.kernel.blur == 1.0;
     for (std::int32_t s = 0; s < r.source_size; ++s) {
         const double position = (static_cast<double>(s) + 0.5) / ratio + r.shift;
         const double lifted = position - static_cast<double>(support);
         const double first = (lifted < 0.0 ? std::floor(lifted + 0.5)
                                            : std::floor(lifted + 0.49999999999999994)) + 0.5;
+        const std::size_t base = static_cast<std::size_t>(s) * static_cast<std::size_t>(taps);
         dests.clear();
-        double total = 0.0;
         for (std::int32_t t = 0; t < taps; ++t) {
-            Tap &tp = tap[static_cast<std::size_t>(t)];
             const double center = first + static_cast<double>(t);
-            tp.offset = center - position;
+            g.offset[base + static_cast<std::size_t>(t)] = center - position;
             const double cell = std::floor(center);
             if (!std::isfinite(cell) || cell < -0x1p63 || cell >= 0x1p63)
                 throw std::out_of_range(
                     "shift places filter support outside the 64-bit pixel grid");
-            tp.dest = padding_index(static_cast<std::int64_t>(cell), r.destination_size, r.border);
-            if (tp.dest >= 0 && std::find(dests.begin(), dests.end(), tp.dest) == dests.end())
-                dests.push_back(tp.dest);
+            const std::int32_t d =
+                padding_index(static_cast<std::int64_t>(cell), r.destination_size, r.border);
+            g.dest[base + static_cast<std::size_t>(t)] = d;
+            if (d >= 0 && std::find(dests.begin(), dests.end(), d) == dests.end()) dests.push_back(d);
         }
+        std::sort(dests.begin(), dests.end());
+        g.unique.insert(g.unique.end(), dests.begin(), dests.end());
+        if (g.unique.size() > std::numeric_limits<std::uint32_t>::max())
+            throw std::length_error("inverse geometry coefficient table is too large");
+        g.start.push_back(static_cast<std::uint32_t>(g.unique.size()));
+    }
+    return g;
+}
+
+// Bounded LRU of geometries keyed by the exact bits of what they depend on
+// (reference geometry_key, axis_plan.cpp:444-454).  A geometry is built
+// outside the lock; two threads that miss the same key at once both build
+// it and the first insertion wins (the results are identical).
+class GeometryCache {
+public:
+    std::shared_ptr<const Geometry> get(const AxisRequest &r, std::int32_t support) {
+        const Key key{static_cast<std::uint32_t>(r.source_size),
+                      static_cast<std::uint32_t>(r.destination_size),
+                      std::bit_cast<std::uint64_t>(r.active_length),
+                      std::bit_cast<std::uint64_t>(r.shift), static_cast<std::uint32_t>(support),
+                      static_cast<std::uint32_t>(r.border)};
+        {
+            std::lock_guard lock(mutex_);
+            if (auto it = map_.find(key); it != map_.end()) {
+                ++hits_;
+                lru_.splice(lru_.begin(), lru_, it->second.where);
+                return it->second.value;
+            }
+        }
+        auto built = std::make_shared<const Geometry>(build_geometry(r, support));
+        std::lock_guard lock(mutex_);
+        ++builds_;
+        if (auto it = map_.find(key); it != map_.end()) return it->second.value;
+        lru_.push_front(key);
+        bytes_ += built->storage_bytes();
+        map_.emplace(key, Entry{built, lru_.begin()});
+        while ((map_.size() > kEntries || bytes_ > kBytes) && map_.size() > 1) {
+            auto victim = map_.find(lru_.back());
+            bytes_ -= victim->second.value->storage_bytes();
+            map_.erase(victim);
+            lru_.pop_back();
+        }
+        return built;
+    }
+
+    void stats(PlannerCacheStats &s) {
+        std::lock_guard lock(mutex_);
+        s.geometry_hits = hits_;
+        s.geometry_builds = builds_;
+        s.geometry_entries = map_.size();
+        s.geometry_resident_bytes = bytes_;
+    }
+
+    void clear() {
+        std::lock_guard lock(mutex_);
+        map_.clear();
+        lru_.clear();
+        bytes_ = 0;
+        hits_ = builds_ = 0;
+    }
+
+private:
+    struct Key {
+        std::uint32_t source, destination;
+        std::uint64_t active, shift;
+        std::uint32_t support, border;
+        bool operator==(const Key &) const = default;
+    };
+    struct KeyHash {
+        std::size_t operator()(const Key &k) const noexcept {
+            std::uint64_t h = 0x9E3779B97F4A7C15ULL;
+            for (const std::uint64_t w : {std::uint64_t{k.source} << 32 | k.destination, k.active,
+                                          k.shift, std::uint64_t{k.support} << 32 | k.border})
+                h = (h ^ w) * 0xBF58476D1CE4E5B9ULL + (h >> 31);
+            return static_cast<std::size_t>(h);
+        }
+    };
+    struct Entry {
+        std::shared_ptr<const Geometry> value;
+        std::list<Key>::iterator where;
+    };
+    static constexpr std::size_t kEntries = 1024;
+    static constexpr std::size_t kBytes = 128U << 20;
+    std::mutex mutex_;
+    std::unordered_map<Key, Entry, KeyHash> map_;
+    std::list<Key> lru_;
+    std::size_t bytes_ = 0;
+    std::uint64_t hits_ = 0, builds_ = 0;
+};
+
+GeometryCache &geometry_cache() {
+    static GeometryCache cache;
+    return cache;
+}
+
+// The forward resampling operator A (source samples x destination samples)
+// on a geometry: one row per source sample, its taps' kernel weights
+// normalised by their sum and coalesced per destination index in ascending
+// index order (reference make_descale_matrix, axis_plan.cpp:535-599).
+SparseRows sample_operator(const AxisRequest &r, const Geometry &g, const CustomKernel &custom) {
+    const std::int32_t taps = g.taps;
+    const auto m = static_cast<std::size_t>(r.source_size);
+    SparseRows a;
+    a.start.reserve(m + 1);
+    a.column.reserve(g.unique.size());
+    a.value.reserve(g.unique.size());
+    a.start.push_back(0U);
+
+    std::vector<double> weight(static_cast<std::size_t>(taps));
+    std::vector<double> merged;
+    std::vector<char> used;
+    const bool unit_blur = r.kernel.blur == 1.0;
+    for (std::size_t s = 0; s < m; ++s) {
+        const std::size_t base = s * static_cast<std::size_t>(taps);
+        double total = 0.0;
         for (std::int32_t t = 0; t < taps; ++t) {
-            Tap &tp = tap[static_cast<std::size_t>(t)];
-            const double d = unit_blur ? tp.offset : tp.offset / r.kernel.blur;
+            const double offset = g.offset[base + static_cast<std::size_t>(t)];
+            const double d = unit_blur ? offset : offset / r.kernel.blur;
             // A custom kernel is evaluated at |d| (reference make_descale_matrix,
             // axis_plan.cpp:560-563).
-            tp.weight = r.kernel.kind == KernelKind::custom ? custom(std::abs(d))
-                                                            : kernel_weight(r.kernel, d);
-            if (!std::isfinite(tp.weight))
-                throw std::runtime_error("filter returned a non-finite value");
-            total += tp.weight;
+            const double w = r.kernel.kind == KernelKind::custom ? custom(std::abs(d))
+                                                                 : kernel_weight(r.kernel, d);
+            if (!std::isfinite(w)) throw std::runtime_error("filter returned a non-finite value");
+            weight[static_cast<std::size_t>(t)] = w;
+            total += w;
         }
         if (!std::isfinite(total) || total == 0.0)
             throw std::runtime_error("filter produced a zero or non-finite weight sum");
-        std::sort(dests.begin(), dests.end());
-        merged.assign(dests.size(), 0.0);
-        used.assign(dests.size(), 0);
-        for (const Tap &tp : tap) {
-            if (tp.dest < 0) continue;
-            const auto slot = static_cast<std::size_t>(
-                std::lower_bound(dests.begin(), dests.end(), tp.dest) - dests.begin());
-            const double w = tp.weight / total;
+        const auto ub = g.unique.begin() + g.start[s], ue = g.unique.begin() + g.start[s + 1];
+        const auto count = static_cast<std::size_t>(ue - ub);
+        merged.assign(count, 0.0);
+        used.assign(count, 0);
+        for (std::int32_t t = 0; t < taps; ++t) {
+            const std::int32_t dest = g.dest[base + static_cast<std::size_t>(t)];
+            if (dest < 0) continue;
+            const auto slot = static_cast<std::size_t>(std::lower_bound(ub, ue, dest) - ub);
+            const double w = weight[static_cast<std::size_t>(t)] / total;
             if (w == 0.0) continue;
             merged[slot] += w;
             used[slot] = 1;
         }
-        for (std::size_t k = 0; k < dests.size(); ++k) {
+        for (std::size_t k = 0; k < count; ++k) {
             if (!used[k]) continue;
-            a.column.push_back(dests[k]);
+            a.column.push_back(ub[static_cast<std::ptrdiff_t>(k)]);
             a.value.push_back(merged[k]);
         }
-        if (a.column.size() > std::numeric_limits<std::uint32_t>::max())
-            throw std::length_error("inverse coefficient table is too large");
         a.start.push_back(static_cast<std::uint32_t>(a.column.size()));
     }
     return a;
@@ -487,7 +611,8 @@ AxisPlan build_axis_plan(const AxisRequest &r, const CustomKernel &custom) {
     const std::int32_t support = effective_support(r);
     if (r.kernel.kind == KernelKind::custom && !custom)
         throw std::invalid_argument("custom kernel callback is missing");
-    const SparseRows a = sample_operator(r, support, custom);
+    const std::shared_ptr<const Geometry> geometry = geometry_cache().get(r, support);
+    const SparseRows a = sample_operator(r, *geometry, custom);
     SparseRows t = transpose(a, r.source_size, r.destination_size);
     const std::int32_t n = r.destination_size;
     const std::int32_t w = std::min(2 * support - 1, n - 1);
@@ -660,7 +785,12 @@ public:
 
     PlannerCacheStats stats() {
         std::lock_guard lock(mutex_);
-        return {hits_, builds_, map_.size(), bytes_};
+        PlannerCacheStats s;
+        s.plan_hits = hits_;
+        s.plan_builds = builds_;
+        s.plan_entries = map_.size();
+        s.plan_resident_bytes = bytes_;
+        return s;
     }
 
     void clear() {
@@ -718,8 +848,15 @@ std::shared_ptr<const AxisPlan> get_or_build_axis_plan(const AxisRequest &r,
 
 std::shared_ptr<const AxisPlan> find_axis_plan(const AxisRequest &r) { return plan_cache().find(r); }
 
-PlannerCacheStats planner_cache_stats() { return plan_cache().stats(); }
+PlannerCacheStats planner_cache_stats() {
+    PlannerCacheStats s = plan_cache().stats();
+    geometry_cache().stats(s);
+    return s;
+}
 
-void clear_planner_caches() { plan_cache().clear(); }
+void clear_planner_caches() {
+    plan_cache().clear();
+    geometry_cache().clear();
+}
 
 } // namespace dsmvc

@@ -234,8 +234,7 @@ def build_plans(requests: Sequence[AxisRequest], threads: int = 0) -> List[Plan]
 def planner_cache_stats() -> dict:
     s = _lib.PlannerStatsC()
     check(load().dsmvc_hip_planner_cache_stats(ctypes.byref(s)))
-    return {"plan_hits": s.plan_hits, "plan_builds": s.plan_builds,
-            "plan_entries": s.plan_entries, "plan_resident_bytes": s.plan_resident_bytes}
+    return {name: getattr(s, name) for name, _ in _lib.PlannerStatsC._fields_}
 
 
 def clear_planner_caches():

@@ -205,6 +205,10 @@ class PlannerStatsC(ctypes.Structure):
         ("plan_builds", ctypes.c_uint64),
         ("plan_entries", ctypes.c_uint64),
         ("plan_resident_bytes", ctypes.c_uint64),
+        ("geometry_hits", ctypes.c_uint64),
+        ("geometry_builds", ctypes.c_uint64),
+        ("geometry_entries", ctypes.c_uint64),
+        ("geometry_resident_bytes", ctypes.c_uint64),
     ]
 
 
