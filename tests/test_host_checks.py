@@ -6,6 +6,8 @@ a kernel read or write outside them.  These tests build objects without a
 device handle and pass CPU tensors: the checks fire before any pointer is
 taken, so a ValueError naming the problem is raised and nothing else runs.
 """
+import threading
+
 import pytest
 
 torch = pytest.importorskip("torch")
@@ -83,6 +85,8 @@ def test_server_checks_derived_chroma_sizes():
     server = object.__new__(dh.Server)
     server._handle = None
     server._keep = {}
+    server._local = threading.local()
+    server._rid_local = threading.local()
     luma_in, luma_out = torch.zeros((108, 192)), torch.zeros((72, 128))
     good_in, good_out = torch.zeros((54, 96)), torch.zeros((36, 64))
     with pytest.raises(ValueError, match="input plane 1"):

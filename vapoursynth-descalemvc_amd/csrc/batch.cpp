@@ -1,5 +1,6 @@
 // Multi-plan batches over windowed job tables (see batch.hpp).
 #include "batch.hpp"
+#include "trace.hpp"
 
 #include <algorithm>
 #include <atomic>
@@ -89,7 +90,7 @@ void WindowTables::add64(const hip::Win64Job &job, int band, bool rows, hip::Sam
     Group &g = group_for(Kind::f64, band, job.plan.ell_width, rows, false, kind);
     hip::Win64Job j = job;
     j.first_item = g.items;
-    g.items += static_cast<long>(j.frames) * j.plan.chunks * j.groups;
+    g.items += hip::window64_items(j);   // until upload() sets the launch's spans
     g.jobs64.push_back(j);
 }
 
@@ -107,6 +108,7 @@ void WindowTables::add_check(const hip::CheckJob &job) {
 // reuses the oldest slot once its event has passed, so uploads and launches
 // stay asynchronous (a copy from pageable memory would wait for the stream).
 int WindowTables::upload(hipStream_t stream) {
+    const trace::Range range(trace::Label::table_upload);
     // Checks first, then horizontal groups: every vertical pass reads an
     // intermediate some horizontal group writes.
     std::stable_sort(groups_.begin(), groups_.end(), [](const Group &a, const Group &b) {
@@ -114,13 +116,18 @@ int WindowTables::upload(hipStream_t stream) {
         return rank(a) < rank(b);
     });
     check(hipSetDevice(device_), "hipSetDevice failed");
-    // Each F32 launch's span (chunks per wavefront), from its kernel's
+    // Each launch's span (chunks per wavefront), from its kernel's
     // occupancy; the jobs' first items follow.
-    for (Group &g : groups_)
+    for (Group &g : groups_) {
         if (g.type == Kind::f32 && !g.jobs.empty())
             g.items = hip::assign_window_spans(
                 g.jobs.data(), static_cast<int>(g.jobs.size()),
                 hip::window_table_slots(g.band, g.width, g.rows, g.block16, g.kind));
+        if (g.type == Kind::f64 && !g.jobs64.empty())
+            g.items = hip::assign_window64_spans(
+                g.jobs64.data(), static_cast<int>(g.jobs64.size()),
+                hip::window64_table_slots(g.band, g.width, g.rows, g.kind));
+    }
     std::size_t bytes = 0;
     for (Group &g : groups_) {
         g.offset = bytes;
@@ -166,6 +173,9 @@ void WindowTables::launch(hipStream_t stream) {
     Slot &slot = slots_[current_];
     for (const Group &g : groups_) {
         if (g.items == 0) continue;
+        // Checks and horizontal groups come first (upload() sorts them).
+        const trace::Range range(g.rows || g.type == Kind::check ? trace::Label::horizontal_stage
+                                                                 : trace::Label::vertical_stage);
         unsigned char *at = slot.device + g.offset;
         const int count = static_cast<int>(g.count());
         hipError_t e = hipSuccess;
@@ -202,6 +212,7 @@ std::int64_t WindowTables::items() const {
 // ---------------------------------------------------------------------------
 HipBatch::HipBatch(HipExecutor &executor, std::vector<BatchJob> jobs, bool graph)
     : executor_(executor), jobs_(std::move(jobs)), tables_(executor.device()) {
+    const trace::Range range(trace::Label::batch_setup);
     for (const BatchJob &j : jobs_) {
         const AxisPlan *h = j.horizontal.get(), *v = j.vertical.get();
         if (!h || !v || !h->valid() || !v->valid() || !j.input || !j.output || j.frames < 1
@@ -346,6 +357,7 @@ HipBatch::~HipBatch() {
 }
 
 void HipBatch::run(hipStream_t stream, bool async) {
+    const trace::Range range(trace::Label::batch_run);
     check(hipSetDevice(executor_.device()), "hipSetDevice failed");
     if (graph_exec_) check(hipGraphLaunch(graph_exec_, stream), "batch graph launch failed");
     else tables_.launch(stream);

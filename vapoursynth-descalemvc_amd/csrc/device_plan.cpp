@@ -1,5 +1,6 @@
 // Packing of axis plans into device images and chunk partitions.
 #include "device_plan.hpp"
+#include "trace.hpp"
 
 #include <algorithm>
 #include <array>
@@ -626,6 +627,7 @@ const DevWindow *DevicePlan::window(bool rows) {
     std::lock_guard lock(mutex_);
     WindowImage &image = window_image_;
     if (!image.built) {
+        const trace::Range range(trace::Label::window_build);
         // Marked built only once the tables are built and uploaded: a failed
         // upload throws here and the next call tries again.
         WindowPlan wp;
@@ -657,6 +659,7 @@ const DevWindow *DevicePlan::solve_window(bool rows) {
     std::lock_guard lock(mutex_);
     WindowImage &image = solve_image_;
     if (!image.built) {
+        const trace::Range range(trace::Label::window_build);
         WindowPlan wp;
         if (build_solve_window_plan(*plan_, wp)) {
             image.memory = upload_window(wp, image.dev, image.bytes);
@@ -675,6 +678,7 @@ const DevWindow64 *DevicePlan::window64() {
     std::lock_guard lock(mutex_);
     Window64Image &image = window64_image_;
     if (!image.built) {
+        const trace::Range range(trace::Label::window_build);
         WindowPlan64 wp;
         if (build_window_plan64(*plan_, pack_plan(*plan_), wp)) {
             const std::size_t o_fwd = 0;

@@ -218,9 +218,10 @@ struct DevWindow64 {
 
 // One windowed FP64 axis pass of one plan over `frames` planes, as a unit of
 // a (possibly multi-plan) FP64 launch; the FP64 counterpart of WinJob.  Items
-// are (frame, chunk, group of 64 vectors), the group varying fastest.  Rows
-// pass: `src` F32 or integer samples, `dst` the FP64 intermediate; columns
-// pass: `src` FP64, `dst` samples of the launch's kind.
+// are (frame, segment, group of 64 vectors), the group varying fastest; a
+// segment is `span` consecutive chunks solved by one wavefront in sequence,
+// as in WinJob.  Rows pass: `src` F32 or integer samples, `dst` the FP64
+// intermediate; columns pass: `src` FP64, `dst` samples of the launch's kind.
 struct Win64Job {
     DevWindow64 plan;
     const void *src;
@@ -230,11 +231,19 @@ struct Win64Job {
     std::int32_t nvec;
     std::int32_t frames;
     std::int32_t groups;      // ceil(nvec / 64)
-    std::int32_t pad_;
+    std::int32_t span;        // chunks per segment; 0 or 1: one
     long first_item;          // items of the jobs before this one in its table
     int *flag;                // set to 1 when a stored value is not finite
     DevConv conv;
 };
+
+constexpr int window64_span(const Win64Job &j) { return j.span > 1 ? j.span : 1; }
+constexpr int window64_segments(const Win64Job &j) {
+    return (j.plan.chunks + window64_span(j) - 1) / window64_span(j);
+}
+constexpr long window64_items(const Win64Job &j) {
+    return static_cast<long>(j.frames) * window64_segments(j) * j.groups;
+}
 
 // The Float64 route's input check of one F32 plane batch: items are
 // (frame, row); a non-finite sample sets the job's flag.

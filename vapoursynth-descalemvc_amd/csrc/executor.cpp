@@ -1,5 +1,6 @@
 // HIP executor: route selection, scratch management and launches.
 #include "executor.hpp"
+#include "trace.hpp"
 
 #include <algorithm>
 #include <bit>
@@ -185,11 +186,13 @@ std::uint64_t HipExecutor::plan_key(const AxisPlan &plan) {
 }
 
 std::shared_ptr<hip::DevicePlan> HipExecutor::device_plan(const AxisPlan &plan) {
+    const trace::Range range(trace::Label::plan_cache_lookup);
     const std::uint64_t key = plan_key(plan);
     auto it = plans_.find(key);
     if (it != plans_.end() && same_plan(it->second.image->host(), plan)) {
         lru_.splice(lru_.begin(), lru_, it->second.lru);
         ++plan_stats_.hits;
+        trace::mark("dsmvc.hip: plan cache hit");
         return it->second.image;
     }
     if (it != plans_.end()) {
@@ -202,6 +205,8 @@ std::shared_ptr<hip::DevicePlan> HipExecutor::device_plan(const AxisPlan &plan) 
         plans_.erase(it);
     }
     check(hipSetDevice(device_), "hipSetDevice failed");
+    trace::mark("dsmvc.hip: plan cache miss");
+    const trace::Range upload(trace::Label::plan_upload);
     auto image = std::make_shared<hip::DevicePlan>(plan, device_, cache_bytes_);
     lru_.push_front(key);
     plans_[key] = PlanEntry{image, lru_.begin()};
@@ -229,6 +234,7 @@ void HipExecutor::evict_plans() {
         victims.push_back(*k);
     }
     if (victims.empty()) return;
+    const trace::Range range(trace::Label::plan_evict);
     (void)hipDeviceSynchronize();
     for (const std::uint64_t key : victims) {
         auto it = plans_.find(key);
@@ -262,6 +268,7 @@ void HipExecutor::set_plan_cache_limit(std::size_t bytes) {
 // Execution slots
 // ---------------------------------------------------------------------------
 HipExecutor::Slot *HipExecutor::acquire_slot() {
+    const trace::Range range(trace::Label::slot_wait);
     std::unique_lock lock(slot_mutex_);
     for (;;) {
         for (Slot &slot : slots_) {
@@ -341,6 +348,7 @@ void HipExecutor::run_call(hipStream_t stream, bool async, const char *what, Enq
 // used by one call at a time (slots are leased, caller streams are the
 // caller's to order).
 void HipExecutor::complete(hipStream_t stream, Scratch &s, const char *what) {
+    const trace::Range range(trace::Label::stream_synchronize);
     check(stream ? hipStreamSynchronize(stream) : hipDeviceSynchronize(), "execution failed");
     if (s.flag_armed) {
         s.flag_armed = false;
@@ -376,6 +384,7 @@ HipExecutor::Scratch &HipExecutor::scratch(hipStream_t stream, std::size_t bytes
         check(hipHostMalloc(reinterpret_cast<void **>(&s.flag_host), sizeof(int)),
               "scratch allocation failed");
     if (s.bytes < bytes) {
+        const trace::Range range(trace::Label::buffer_reserve);
         const std::size_t grown = align256(std::max(bytes, s.bytes + s.bytes / 2));
         if (s.buffer) {
             check(stream ? hipStreamSynchronize(stream) : hipDeviceSynchronize(), "stream synchronize failed");
@@ -392,6 +401,7 @@ HipExecutor::Scratch &HipExecutor::scratch(hipStream_t stream, std::size_t bytes
 void HipExecutor::finish(hipStream_t stream, Scratch &s, bool async, const char *what) {
     check(hipGetLastError(), "kernel launch failed");
     if (async) return;
+    const trace::Range range(trace::Label::stream_synchronize);
     check(stream ? hipStreamSynchronize(stream) : hipDeviceSynchronize(), "execution failed");
     if (s.flag_armed) {
         s.flag_armed = false;
@@ -688,6 +698,7 @@ void HipExecutor::inverse_rows(const AxisPlan &plan, const float *input,
                                std::ptrdiff_t input_row_stride, float *output,
                                std::ptrdiff_t output_row_stride, std::int32_t row_count,
                                hipStream_t stream, bool async) {
+    const trace::Range range(trace::Label::execute_rows);
     if (!plan.valid() || !input || !output || input_row_stride < plan.source_size
         || output_row_stride < plan.destination_size || row_count < 0)
         throw std::invalid_argument("dsmvc: invalid row executor arguments");
@@ -714,6 +725,7 @@ void HipExecutor::inverse_columns(const AxisPlan &plan, const float *input,
                                   std::ptrdiff_t input_row_stride, float *output,
                                   std::ptrdiff_t output_row_stride, std::int32_t column_count,
                                   hipStream_t stream, bool async) {
+    const trace::Range range(trace::Label::execute_columns);
     if (!plan.valid() || !input || !output || column_count < 0
         || input_row_stride < column_count || output_row_stride < column_count)
         throw std::invalid_argument("dsmvc: invalid column executor arguments");
@@ -739,6 +751,7 @@ void HipExecutor::inverse_columns(const AxisPlan &plan, const float *input,
 
 void HipExecutor::inverse_2d_planes(const PlaneJob *planes, std::int32_t count, SampleKind kind,
                                     hipStream_t stream, bool async) {
+    const trace::Range range(trace::Label::execute_planes);
     const bool integer = kind != SampleKind::float32;
     if (count < 0 || (count > 0 && !planes))
         throw std::invalid_argument("dsmvc: invalid plane list");
@@ -918,6 +931,7 @@ int *HipExecutor::float64_flag(hipStream_t stream) {
 void HipExecutor::inverse_2d_batch(const AxisPlan &h, const AxisPlan &v, SampleKind kind,
                                    const PlaneBatch &b, const IntegerConversion *conversion,
                                    hipStream_t stream, bool async) {
+    const trace::Range range(trace::Label::execute_2d);
     const bool integer = kind != SampleKind::float32;
     if (!h.valid() || !v.valid() || !b.input || !b.output || b.frames < 1
         || b.input_row_stride < h.source_size || b.output_row_stride < h.destination_size
@@ -980,6 +994,7 @@ void HipExecutor::inverse_rows_batch(const AxisPlan &plan, SampleKind in_kind,
                                      const PlaneBatch &b, std::int32_t rows,
                                      const IntegerConversion *conversion, hipStream_t stream,
                                      bool async) {
+    const trace::Range range(trace::Label::execute_rows);
     if (!plan.valid() || !b.input || !b.output || b.frames < 1 || rows < 1
         || b.input_row_stride < plan.source_size || b.output_row_stride < plan.destination_size
         || (b.frames > 1 && (b.input_frame_stride < b.input_row_stride * rows
@@ -1007,6 +1022,7 @@ void HipExecutor::inverse_columns_batch(const AxisPlan &plan, SampleKind out_kin
                                         const PlaneBatch &b, std::int32_t columns,
                                         const IntegerConversion *conversion, hipStream_t stream,
                                         bool async) {
+    const trace::Range range(trace::Label::execute_columns);
     if (!plan.valid() || !b.input || !b.output || b.frames < 1 || columns < 1
         || b.input_row_stride < columns || b.output_row_stride < columns
         || (b.frames > 1 && (b.input_frame_stride < b.input_row_stride * plan.source_size
@@ -1047,6 +1063,7 @@ void pack_rows(void *dst, std::ptrdiff_t dst_stride, const void *src, std::ptrdi
 
 void HipExecutor::inverse_2d_host(const AxisPlan &h, const AxisPlan &v, SampleKind kind,
                                   const PlaneBatch &b, const IntegerConversion *conversion) {
+    const trace::Range range(trace::Label::execute_host_frame);
     const bool integer = kind != SampleKind::float32;
     if (!h.valid() || !v.valid() || !b.input || !b.output || b.frames < 1
         || b.input_row_stride < h.source_size || b.output_row_stride < h.destination_size
@@ -1067,11 +1084,16 @@ void HipExecutor::inverse_2d_host(const AxisPlan &h, const AxisPlan &v, SampleKi
     Slot *slot = acquire_slot();
     SlotGuard guard{[this, slot] { release_slot(slot); }};
     slot_buffers(*slot, std::max(in_bytes, out_bytes), in_bytes + out_bytes);
-    for (std::int32_t f = 0; f < b.frames; ++f)
-        pack_rows(static_cast<unsigned char *>(slot->pinned) + f * in_plane * sample, h.source_size,
-                  static_cast<const unsigned char *>(b.input)
-                      + f * b.input_frame_stride * static_cast<std::ptrdiff_t>(sample),
-                  b.input_row_stride, v.source_size, static_cast<std::size_t>(h.source_size), sample);
+    {
+        const trace::Range pack(trace::Label::host_pack);
+        for (std::int32_t f = 0; f < b.frames; ++f)
+            pack_rows(static_cast<unsigned char *>(slot->pinned) + f * in_plane * sample,
+                      h.source_size,
+                      static_cast<const unsigned char *>(b.input)
+                          + f * b.input_frame_stride * static_cast<std::ptrdiff_t>(sample),
+                      b.input_row_stride, v.source_size, static_cast<std::size_t>(h.source_size),
+                      sample);
+    }
     void *dev_in = slot->device;
     void *dev_out = static_cast<unsigned char *>(slot->device) + in_bytes;
     const hipStream_t st = slot->stream;
@@ -1089,6 +1111,7 @@ void HipExecutor::inverse_2d_host(const AxisPlan &h, const AxisPlan &v, SampleKi
         check(hipGetLastError(), "kernel launch failed");
     }
     complete(st, *used, "dsmvc: HIP Float64 2D execution produced NaN or infinity");
+    const trace::Range unpack(trace::Label::host_unpack);
     for (std::int32_t f = 0; f < b.frames; ++f)
         pack_rows(static_cast<unsigned char *>(b.output)
                       + f * b.output_frame_stride * static_cast<std::ptrdiff_t>(sample),
@@ -1101,6 +1124,7 @@ void HipExecutor::inverse_2d_host(const AxisPlan &h, const AxisPlan &v, SampleKi
 void HipExecutor::inverse_rows_host(const AxisPlan &plan, const float *input,
                                     std::ptrdiff_t input_row_stride, float *output,
                                     std::ptrdiff_t output_row_stride, std::int32_t row_count) {
+    const trace::Range range(trace::Label::execute_host_frame);
     if (!plan.valid() || !input || !output || input_row_stride < plan.source_size
         || output_row_stride < plan.destination_size || row_count < 0)
         throw std::invalid_argument("dsmvc: invalid row executor arguments");
@@ -1143,6 +1167,7 @@ void HipExecutor::inverse_rows_host(const AxisPlan &plan, const float *input,
 void HipExecutor::inverse_columns_host(const AxisPlan &plan, const float *input,
                                        std::ptrdiff_t input_row_stride, float *output,
                                        std::ptrdiff_t output_row_stride, std::int32_t column_count) {
+    const trace::Range range(trace::Label::execute_host_frame);
     if (!plan.valid() || !input || !output || column_count < 0
         || input_row_stride < column_count || output_row_stride < column_count)
         throw std::invalid_argument("dsmvc: invalid column executor arguments");

@@ -107,13 +107,20 @@ hipError_t launch_window64(const Win64Args &a, int band, int width, bool rows, S
                            hipStream_t s);
 hipError_t launch_window64_table(const Win64Job *jobs, int count, long items, int band, int width,
                                  bool rows, Sample kind, hipStream_t s);
+// Wavefronts of an FP64 launch's kernel the GPU holds at once, and the span
+// (chunks per wavefront) and first items of a launch's jobs for that many
+// resident wavefronts; returns the launch's item count.
+long window64_table_slots(int band, int width, bool rows, Sample kind);
+long assign_window64_spans(Win64Job *jobs, int count, long slots);
 template <int W>
 hipError_t launch_window64_table_band(const Win64Job *jobs, int count, long items,
                                       const Win64Job &one, int width, bool rows, Sample kind,
                                       hipStream_t s);
+template <int W> long window64_table_slots_band(int width, bool rows, Sample kind);
 #define DSMVC_WINDOW64_EXTERN(W)                                                                 \
     extern template hipError_t launch_window64_table_band<W>(                                     \
-        const Win64Job *, int, long, const Win64Job &, int, bool, Sample, hipStream_t);
+        const Win64Job *, int, long, const Win64Job &, int, bool, Sample, hipStream_t);           \
+    extern template long window64_table_slots_band<W>(int, bool, Sample);
 DSMVC_WINDOW64_EXTERN(1)
 DSMVC_WINDOW64_EXTERN(3)
 DSMVC_WINDOW64_EXTERN(5)

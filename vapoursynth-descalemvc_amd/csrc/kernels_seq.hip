@@ -224,10 +224,9 @@ hipError_t launch_window64(const Win64Args &a, int band, int width, bool rows, S
     one.nvec = a.nvec;
     one.frames = a.frames;
     one.groups = a.groups;
-    one.first_item = 0;
     one.flag = a.flag;
     one.conv = a.conv;
-    const long items = static_cast<long>(a.frames) * a.plan.chunks * a.groups;
+    const long items = assign_window64_spans(&one, 1, window64_table_slots(band, width, rows, kind));
     return window64_dispatch(nullptr, 0, items, one, band, width, rows, kind, s);
 }
 
@@ -282,11 +281,13 @@ long window_table_slots(int band, int width, bool rows, bool block16, Sample kin
 // `slots` resident wavefronts it takes, times the mean wavefront cost in
 // forward rows (a backward row counts 0.3 of one).  DSMVC_HIP_WINDOW_SPAN
 // (> 0) fixes the span instead.
-long assign_window_spans(WinJob *jobs, int count, long slots) {
-    static const int fixed = [] {
-        const char *env = std::getenv("DSMVC_HIP_WINDOW_SPAN");
-        return env ? std::atoi(env) : 0;
-    }();
+namespace {
+// The span (chunks per wavefront) shared by a launch's jobs: the one that
+// minimises waves-in-flight rounds x mean work per wave, with each
+// segment's forward warm-up (2P rows) and backward warm-ups (P per chunk)
+// counted as work.  `fixed` > 0 overrides the choice.
+template <class Job, class Items>
+long assign_spans(Job *jobs, int count, long slots, int fixed, Items items_of) {
     int best = fixed > 0 ? fixed : 1;
     if (fixed <= 0 && count > 0) {
         int top = 1;
@@ -297,7 +298,7 @@ long assign_window_spans(WinJob *jobs, int count, long slots) {
         for (int span = 1; span <= top; ++span) {
             double waves = 0.0, work = 0.0;
             for (int i = 0; i < count; ++i) {
-                const DevWindow &p = jobs[i].plan;
+                const auto &p = jobs[i].plan;
                 const int segments = (p.chunks + span - 1) / span;
                 const double per = static_cast<double>(jobs[i].frames) * jobs[i].groups * segments;
                 const double chunks = static_cast<double>(p.chunks) / segments;   // mean per segment
@@ -317,9 +318,35 @@ long assign_window_spans(WinJob *jobs, int count, long slots) {
     for (int i = 0; i < count; ++i) {
         jobs[i].span = best;
         jobs[i].first_item = items;
-        items += window_items(jobs[i]);
+        items += items_of(jobs[i]);
     }
     return items;
+}
+
+int span_override(const char *name) {
+    const char *env = std::getenv(name);
+    return env ? std::atoi(env) : 0;
+}
+} // namespace
+
+long assign_window_spans(WinJob *jobs, int count, long slots) {
+    static const int fixed = span_override("DSMVC_HIP_WINDOW_SPAN");
+    return assign_spans(jobs, count, slots, fixed, [](const WinJob &j) { return window_items(j); });
+}
+
+long assign_window64_spans(Win64Job *jobs, int count, long slots) {
+    static const int fixed = span_override("DSMVC_HIP_WINDOW64_SPAN");
+    return assign_spans(jobs, count, slots, fixed, [](const Win64Job &j) { return window64_items(j); });
+}
+
+long window64_table_slots(int band, int width, bool rows, Sample kind) {
+    switch (band) {
+    case 1: return window64_table_slots_band<1>(width, rows, kind);
+    case 3: return window64_table_slots_band<3>(width, rows, kind);
+    case 5: return window64_table_slots_band<5>(width, rows, kind);
+    case 7: return window64_table_slots_band<7>(width, rows, kind);
+    default: return 0;
+    }
 }
 
 hipError_t launch_chunked(const ChunkArgs &a, bool rows, Sample kind, hipStream_t s) {

@@ -5,4 +5,5 @@
 namespace dsmvc::hip {
 template hipError_t launch_window64_table_band<3>(const Win64Job *, int, long, const Win64Job &,
                                                      int, bool, Sample, hipStream_t);
+template long window64_table_slots_band<3>(int, bool, Sample);
 } // namespace dsmvc::hip

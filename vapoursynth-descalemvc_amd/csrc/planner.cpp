@@ -9,6 +9,7 @@
 // CSR, and a BandSystem object owns formation, factorisation and the
 // conditioning test.  Compile with -ffp-contract=off.
 #include "engine.hpp"
+#include "trace.hpp"
 
 #include <algorithm>
 #include <atomic>
@@ -161,6 +162,7 @@ struct Geometry {
 };
 
 Geometry build_geometry(const AxisRequest &r, std::int32_t support) {
+    const trace::Range range(trace::Label::geometry_build);
     const std::int32_t taps = 2 * support;
     const auto m = static_cast<std::size_t>(r.source_size);
     if (m > std::numeric_limits<std::size_t>::max() / 2 / static_cast<std::size_t>(taps))
@@ -608,6 +610,7 @@ std::size_t AxisPlan::storage_bytes() const noexcept {
 }
 
 AxisPlan build_axis_plan(const AxisRequest &r, const CustomKernel &custom) {
+    const trace::Range range(trace::Label::plan_build);
     const std::int32_t support = effective_support(r);
     if (r.kernel.kind == KernelKind::custom && !custom)
         throw std::invalid_argument("custom kernel callback is missing");

@@ -1,5 +1,6 @@
 // Continuous-batching request loop (see server.hpp).
 #include "server.hpp"
+#include "trace.hpp"
 
 #include <algorithm>
 #include <array>
@@ -105,6 +106,7 @@ double HipServer::now() const {
 
 std::uint64_t HipServer::submit(const ServeRequest &r) {
     const double t0 = now();
+    const trace::Range range(trace::Label::server_submit);
     require(r.planes == 1 || r.planes == 3, "dsmvc: a request has 1 or 3 planes");
     require(r.src_width > 0 && r.src_height > 0 && r.dst_width > 0 && r.dst_height > 0,
             "dsmvc: invalid request geometry");
@@ -332,6 +334,7 @@ bool HipServer::admit_round(Slot &slot, std::unique_lock<std::mutex> &lock) {
         queue_.pop_front();
     }
     if (round.empty()) return false;
+    const trace::Range range(trace::Label::server_round);
     slot.busy = true;
     slot.round = round;
     ++stats_.rounds;

@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import threading
 from dataclasses import dataclass, field
 from typing import List, Optional, Sequence
 
@@ -315,6 +316,19 @@ def _check_planes(t, what: str, rows: int, cols: int, dtype=None, frames: Option
     return tb
 
 
+def _check_served(h: int, w: int, row_stride: int, col_stride: int, side: str, k: int,
+                  rows: int, cols: int):
+    """_check_planes for served plane k (input or output side) from its shape
+    and strides."""
+    if h < rows or w < cols:
+        raise ValueError(f"dsmvc: {side} plane {k} is {h}x{w}, needs at least {rows}x{cols} "
+                         f"samples")
+    if col_stride != 1:
+        raise ValueError(f"dsmvc: {side} plane {k} rows must be contiguous")
+    if rows > 1 and row_stride < cols:
+        raise ValueError(f"dsmvc: {side} plane {k} strides overlap its rows or planes")
+
+
 class Executor:
     """dsmvc::Executor(backend="hip") on one GPU (reference include/dsmvc/engine.hpp:363-504).
 
@@ -596,6 +610,9 @@ class Server:
                                              ctypes.byref(handle)))
         self._handle = handle
         self._keep = {}
+        self._local = threading.local()
+        self._submit = load().dsmvc_hip_server_submit
+        self._rid_local = threading.local()
 
     def close(self):
         handle = getattr(self, "_handle", None)
@@ -610,36 +627,53 @@ class Server:
     def submit(self, planes_in, planes_out, kernel: int = BICUBIC, taps: int = 3, b: float = 0.0,
                c: float = 0.5, blur: float = 1.0, border: int = SYMMETRIC, f64_mode: int = 0,
                subsampling_w: int = 1, subsampling_h: int = 1) -> int:
-        """planes_in / planes_out: 1 (gray) or 3 (YUV) 2D float32 tensors."""
+        """planes_in / planes_out: 1 (gray) or 3 (YUV) 2D float32 tensors.
+
+        Every plane is checked before the library sees it (float32, 2-D,
+        on the device, contiguous rows, at least the size the library
+        derives from luma and the subsampling); the request record is
+        reused per calling thread, since the library copies it."""
         n = len(planes_in)
         if n not in (1, 3) or len(planes_out) != n:
             raise ValueError("dsmvc: a request has 1 or 3 planes")
-        r = _lib.ServeRequestC()
+        r = getattr(self._local, "request", None)
+        if r is None:
+            r = self._local.request = _lib.ServeRequestC()
+        f32 = _torch().float32
+        sub_w = subsampling_w if n == 3 else 0
+        sub_h = subsampling_h if n == 3 else 0
+        for k in range(n):
+            x, o = planes_in[k], planes_out[k]
+            if x.dtype != f32 or o.dtype != f32:
+                raise ValueError("dsmvc: served planes are float32")
+            if x.dim() != 2 or o.dim() != 2:
+                raise ValueError("dsmvc: served planes are 2-D")
+            (xh, xw), (xs, xc) = x.shape, x.stride()
+            (oh, ow), (os_, oc) = o.shape, o.stride()
+            if k == 0:
+                src_h, src_w, dst_h, dst_w = xh, xw, oh, ow
+            # The library derives chroma sizes from luma by the subsampling.
+            sw_, sh_ = (sub_w, sub_h) if k else (0, 0)
+            _check_served(xh, xw, xs, xc, "input", k, src_h >> sh_, src_w >> sw_)
+            _check_served(oh, ow, os_, oc, "output", k, dst_h >> sh_, dst_w >> sw_)
+            r.src_stride[k] = xs
+            r.dst_stride[k] = os_
+        for k in range(n):
+            x, o = planes_in[k], planes_out[k]
+            if not x.is_cuda or not o.is_cuda:
+                raise ValueError("dsmvc: HIP executor buffers must be device tensors")
+            r.src[k] = x.data_ptr()
+            r.dst[k] = o.data_ptr()
         r.planes = n
-        r.subsampling_w = subsampling_w if n == 3 else 0
-        r.subsampling_h = subsampling_h if n == 3 else 0
-        r.src_height, r.src_width = planes_in[0].shape
-        r.dst_height, r.dst_width = planes_out[0].shape
+        r.subsampling_w = sub_w
+        r.subsampling_h = sub_h
+        r.src_height, r.src_width, r.dst_height, r.dst_width = src_h, src_w, dst_h, dst_w
         r.kernel, r.taps, r.border, r.f64_mode = kernel, taps, border, f64_mode
         r.b, r.c, r.blur = b, c, blur
-        for k in range(n):
-            if planes_in[k].dtype != _torch().float32 or planes_out[k].dtype != _torch().float32:
-                raise ValueError("dsmvc: served planes are float32")
-            if planes_in[k].dim() != 2 or planes_out[k].dim() != 2:
-                raise ValueError("dsmvc: served planes are 2-D")
-            # The library derives chroma sizes from luma by the subsampling.
-            sw_, sh_ = r.subsampling_w if k else 0, r.subsampling_h if k else 0
-            _check_planes(planes_in[k], f"input plane {k}", r.src_height >> sh_,
-                          r.src_width >> sw_)
-            _check_planes(planes_out[k], f"output plane {k}", r.dst_height >> sh_,
-                          r.dst_width >> sw_)
-        for k in range(n):
-            r.src[k] = _dev_ptr(planes_in[k]).value
-            r.src_stride[k] = planes_in[k].stride(0)
-            r.dst[k] = _dev_ptr(planes_out[k]).value
-            r.dst_stride[k] = planes_out[k].stride(0)
-        rid = ctypes.c_uint64()
-        check(load().dsmvc_hip_server_submit(self._handle, ctypes.byref(r), ctypes.byref(rid)))
+        rid = getattr(self._rid_local, "rid", None)
+        if rid is None:
+            rid = self._rid_local.rid = ctypes.c_uint64()
+        check(self._submit(self._handle, ctypes.byref(r), ctypes.byref(rid)))
         self._keep[rid.value] = (planes_in, planes_out)
         return rid.value
 
