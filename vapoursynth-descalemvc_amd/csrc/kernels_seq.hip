@@ -282,12 +282,20 @@ long window_table_slots(int band, int width, bool rows, bool block16, Sample kin
 // forward rows (a backward row counts 0.3 of one).  DSMVC_HIP_WINDOW_SPAN
 // (> 0) fixes the span instead.
 namespace {
-// The span (chunks per wavefront) shared by a launch's jobs: the one that
-// minimises waves-in-flight rounds x mean work per wave, with each
-// segment's forward warm-up (2P rows) and backward warm-ups (P per chunk)
-// counted as work.  `fixed` > 0 overrides the choice.
+// Spans (chunks per wavefront) of a launch's jobs.  For a common target
+// segment length of t chunks, each job splits its chunks into
+// ceil(chunks / t) segments of near-equal length, so that a job with fewer
+// chunks than another in the same launch is not left with a short last
+// segment that still pays a full warm-up.  The target chosen minimises
+// max(rounds of resident waves x mean work per wave, longest wave), with
+// each segment's forward warm-up (2P rows) and backward warm-ups (P per
+// chunk) counted as work.  `fixed` > 0 sets every job's span instead.
 template <class Job, class Items>
 long assign_spans(Job *jobs, int count, long slots, int fixed, Items items_of) {
+    auto span_for = [](int chunks, int target) {
+        const int segments = (chunks + target - 1) / target;
+        return (chunks + segments - 1) / segments;
+    };
     int best = fixed > 0 ? fixed : 1;
     if (fixed <= 0 && count > 0) {
         int top = 1;
@@ -295,28 +303,32 @@ long assign_spans(Job *jobs, int count, long slots, int fixed, Items items_of) {
         top = std::min(top, 64);
         const double resident = static_cast<double>(std::max(slots, 1L));
         double best_time = 0.0;
-        for (int span = 1; span <= top; ++span) {
-            double waves = 0.0, work = 0.0;
+        for (int target = 1; target <= top; ++target) {
+            double waves = 0.0, work = 0.0, longest = 0.0;
             for (int i = 0; i < count; ++i) {
                 const auto &p = jobs[i].plan;
+                const int span = span_for(p.chunks, target);
                 const int segments = (p.chunks + span - 1) / span;
                 const double per = static_cast<double>(jobs[i].frames) * jobs[i].groups * segments;
                 const double chunks = static_cast<double>(p.chunks) / segments;   // mean per segment
+                const double each = chunks * p.length + 2.0 * p.warmup
+                                    + 0.3 * chunks * (p.length + p.warmup);
                 waves += per;
-                work += per * (chunks * p.length + 2.0 * p.warmup
-                               + 0.3 * chunks * (p.length + p.warmup));
+                work += per * each;
+                longest = std::max(longest, span * p.length + 2.0 * p.warmup
+                                                + 0.3 * span * (p.length + p.warmup));
             }
             if (waves <= 0.0) break;
-            const double time = std::ceil(waves / resident) * (work / waves);
-            if (span == 1 || time < best_time * 0.999) {
+            const double time = std::max(std::ceil(waves / resident) * (work / waves), longest);
+            if (target == 1 || time < best_time * 0.999) {
                 best_time = time;
-                best = span;
+                best = target;
             }
         }
     }
     long items = 0;
     for (int i = 0; i < count; ++i) {
-        jobs[i].span = best;
+        jobs[i].span = fixed > 0 ? fixed : span_for(jobs[i].plan.chunks, best);
         jobs[i].first_item = items;
         items += items_of(jobs[i]);
     }
