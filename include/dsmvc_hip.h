@@ -339,6 +339,72 @@ int dsmvc_hip_inverse_2d_host(dsmvc_hip_executor *executor, const dsmvc_hip_plan
                               ptrdiff_t output_row_stride, ptrdiff_t output_frame_stride,
                               int32_t frames, const dsmvc_hip_integer_conversion *conversion);
 
+/* ---- GPU frames exported by another API ----------------------------------
+ * One plane of a GPU frame exported by its producer (VapourSynth's GPU core)
+ * for a foreign API (reference include/dsmvc/engine.hpp:306-323,
+ * ExportedPlane).  handle_type 0x1 (VkExternalMemoryHandleTypeFlagBits
+ * OPAQUE_FD): `handle` is a POSIX file descriptor of the whole allocation,
+ * imported with hipImportExternalMemory once per memory_id (the producer
+ * never reuses one) and kept in a bounded LRU of the executor; the plane
+ * lives at `offset`, rows `pitch_bytes` apart.  Every call consumes both
+ * planes' handles, whatever the outcome.  ready_semaphore / ready_value: the
+ * input's producer pair, waited for on the device when that semaphore was
+ * imported (dsmvc_hip_import_exported_semaphore), else 0 or already waited
+ * for by the caller. */
+typedef struct dsmvc_hip_exported_plane {
+    uint64_t memory_id;
+    uint64_t memory_size;
+    uint64_t offset;
+    uint64_t plane_bytes;    /* pitch_bytes * height */
+    int32_t handle_type;
+    int32_t reserved;
+    int64_t handle;
+    int64_t pitch_bytes;
+    uint64_t ready_semaphore;
+    uint64_t ready_value;
+} dsmvc_hip_exported_plane;
+
+typedef struct dsmvc_hip_import_stats {
+    uint64_t entries;        /* allocations currently imported */
+    uint64_t imports;
+    uint64_t hits;
+    uint64_t evictions;
+} dsmvc_hip_import_stats;
+
+/* Executor::inverse_2d_exported / inverse_2d_u8_exported /
+ * inverse_2d_u16_exported (engine.hpp:432-442) by sample kind.  Synchronous:
+ * the output plane is written when the call returns.  A plane outside its
+ * allocation, a pitch that is not a whole number of samples or shorter than
+ * a row: DSMVC_HIP_INVALID_ARGUMENT; a handle HIP cannot import:
+ * DSMVC_HIP_RUNTIME_ERROR. */
+int dsmvc_hip_inverse_2d_exported(dsmvc_hip_executor *executor, const dsmvc_hip_plan *horizontal,
+                                  const dsmvc_hip_plan *vertical, int32_t sample_kind,
+                                  const dsmvc_hip_exported_plane *input,
+                                  const dsmvc_hip_exported_plane *output,
+                                  const dsmvc_hip_integer_conversion *conversion);
+/* Executor::inverse_rows_exported / inverse_columns_exported
+ * (engine.hpp:426-431): Float32 planes. */
+int dsmvc_hip_inverse_rows_exported(dsmvc_hip_executor *executor, const dsmvc_hip_plan *plan,
+                                    const dsmvc_hip_exported_plane *input,
+                                    const dsmvc_hip_exported_plane *output, int32_t row_count);
+int dsmvc_hip_inverse_columns_exported(dsmvc_hip_executor *executor, const dsmvc_hip_plan *plan,
+                                       const dsmvc_hip_exported_plane *input,
+                                       const dsmvc_hip_exported_plane *output,
+                                       int32_t column_count);
+/* Executor::import_exported_semaphore / has_exported_semaphore
+ * (engine.hpp:423-425): a timeline semaphore of the producer, keyed by its
+ * producer-side handle value `semaphore`; *imported = 1 when HIP imported it
+ * (hipImportExternalSemaphore).  The handle is consumed either way. */
+int dsmvc_hip_import_exported_semaphore(dsmvc_hip_executor *executor, uint64_t semaphore,
+                                        int32_t handle_type, int64_t handle, int32_t *imported);
+int dsmvc_hip_has_exported_semaphore(dsmvc_hip_executor *executor, uint64_t semaphore,
+                                     int32_t *imported);
+/* dsmvc::close_exported_plane (engine.hpp:325-326): closes the handle of a
+ * plane that was never passed to an executor. */
+void dsmvc_hip_close_exported_plane(const dsmvc_hip_exported_plane *plane);
+/* No counterpart in the reference: the executor's import cache counters. */
+int dsmvc_hip_executor_import_stats(dsmvc_hip_executor *executor, dsmvc_hip_import_stats *stats);
+
 /* ---- device plan cache ----------------------------------------------------
  * The executor keeps each plan's device image (and the tables built for it)
  * in an LRU keyed by the plan's build serial -- one per planner request key

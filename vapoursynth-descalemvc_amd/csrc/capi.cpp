@@ -104,6 +104,38 @@ dsmvc::IntegerConversion to_conversion(const dsmvc_hip_integer_conversion &c) {
     return r;
 }
 
+dsmvc::ExportedPlane to_exported(const dsmvc_hip_exported_plane &p) {
+    dsmvc::ExportedPlane e;
+    e.memory_id = p.memory_id;
+    e.memory_size = p.memory_size;
+    e.offset = p.offset;
+    e.plane_bytes = p.plane_bytes;
+    e.handle_type = p.handle_type;
+    e.handle = static_cast<std::intptr_t>(p.handle);
+    e.pitch_bytes = static_cast<std::ptrdiff_t>(p.pitch_bytes);
+    e.ready_semaphore = p.ready_semaphore;
+    e.ready_value = p.ready_value;
+    return e;
+}
+
+// Runs an exported call; the planes' handles are consumed on every path,
+// including arguments rejected before the executor sees them.
+template <class F>
+int exported_call(const dsmvc_hip_exported_plane *input, const dsmvc_hip_exported_plane *output,
+                  F &&f) {
+    return guarded([&] {
+        if (!input || !output) {
+            if (input) dsmvc::close_exported_plane(to_exported(*input));
+            if (output) dsmvc::close_exported_plane(to_exported(*output));
+            throw std::invalid_argument("dsmvc: null argument");
+        }
+        f(to_exported(*input), to_exported(*output));
+    });
+}
+
+bool usable(const dsmvc_hip_executor *e) { return e && e->executor; }
+bool usable(const dsmvc_hip_plan *p) { return p && p->plan; }
+
 dsmvc::HipExecutor &exec_of(dsmvc_hip_executor *e) {
     require(e && e->executor, "dsmvc: executor is null");
     return *e->executor;
@@ -539,6 +571,96 @@ int dsmvc_hip_inverse_2d_host(dsmvc_hip_executor *executor, const dsmvc_hip_plan
         exec_of(executor).inverse_2d_host(plan_of(horizontal), plan_of(vertical),
                                           static_cast<dsmvc::SampleKind>(sample_kind), batch,
                                           conversion ? &conv : nullptr);
+    });
+}
+
+int dsmvc_hip_inverse_2d_exported(dsmvc_hip_executor *executor, const dsmvc_hip_plan *horizontal,
+                                  const dsmvc_hip_plan *vertical, int32_t sample_kind,
+                                  const dsmvc_hip_exported_plane *input,
+                                  const dsmvc_hip_exported_plane *output,
+                                  const dsmvc_hip_integer_conversion *conversion) {
+    return exported_call(input, output, [&](const dsmvc::ExportedPlane &in,
+                                            const dsmvc::ExportedPlane &out) {
+        if (!usable(executor) || !usable(horizontal) || !usable(vertical) || sample_kind < 0
+            || sample_kind > 2) {
+            dsmvc::close_exported_plane(in);
+            dsmvc::close_exported_plane(out);
+            require(sample_kind >= 0 && sample_kind <= 2, "dsmvc: sample kind is invalid");
+            throw std::invalid_argument("dsmvc: null argument");
+        }
+        dsmvc::IntegerConversion conv;
+        if (conversion) conv = to_conversion(*conversion);
+        exec_of(executor).inverse_2d_exported(plan_of(horizontal), plan_of(vertical),
+                                              static_cast<dsmvc::SampleKind>(sample_kind), in, out,
+                                              conversion ? &conv : nullptr);
+    });
+}
+
+int dsmvc_hip_inverse_rows_exported(dsmvc_hip_executor *executor, const dsmvc_hip_plan *plan,
+                                    const dsmvc_hip_exported_plane *input,
+                                    const dsmvc_hip_exported_plane *output, int32_t row_count) {
+    return exported_call(input, output, [&](const dsmvc::ExportedPlane &in,
+                                            const dsmvc::ExportedPlane &out) {
+        if (!usable(executor) || !usable(plan)) {
+            dsmvc::close_exported_plane(in);
+            dsmvc::close_exported_plane(out);
+            throw std::invalid_argument("dsmvc: null argument");
+        }
+        exec_of(executor).inverse_rows_exported(plan_of(plan), in, out, row_count);
+    });
+}
+
+int dsmvc_hip_inverse_columns_exported(dsmvc_hip_executor *executor, const dsmvc_hip_plan *plan,
+                                       const dsmvc_hip_exported_plane *input,
+                                       const dsmvc_hip_exported_plane *output,
+                                       int32_t column_count) {
+    return exported_call(input, output, [&](const dsmvc::ExportedPlane &in,
+                                            const dsmvc::ExportedPlane &out) {
+        if (!usable(executor) || !usable(plan)) {
+            dsmvc::close_exported_plane(in);
+            dsmvc::close_exported_plane(out);
+            throw std::invalid_argument("dsmvc: null argument");
+        }
+        exec_of(executor).inverse_columns_exported(plan_of(plan), in, out, column_count);
+    });
+}
+
+int dsmvc_hip_import_exported_semaphore(dsmvc_hip_executor *executor, uint64_t semaphore,
+                                        int32_t handle_type, int64_t handle, int32_t *imported) {
+    return guarded([&] {
+        if (!usable(executor) || !imported) {
+            dsmvc::ExportedPlane p;
+            p.handle_type = handle_type;
+            p.handle = static_cast<std::intptr_t>(handle);
+            dsmvc::close_exported_plane(p);
+            throw std::invalid_argument("dsmvc: null argument");
+        }
+        *imported = exec_of(executor).import_exported_semaphore(
+                        semaphore, handle_type, static_cast<std::intptr_t>(handle))
+                        ? 1 : 0;
+    });
+}
+
+int dsmvc_hip_has_exported_semaphore(dsmvc_hip_executor *executor, uint64_t semaphore,
+                                     int32_t *imported) {
+    return guarded([&] {
+        require(imported, "dsmvc: null argument");
+        *imported = exec_of(executor).has_exported_semaphore(semaphore) ? 1 : 0;
+    });
+}
+
+void dsmvc_hip_close_exported_plane(const dsmvc_hip_exported_plane *plane) {
+    if (plane) dsmvc::close_exported_plane(to_exported(*plane));
+}
+
+int dsmvc_hip_executor_import_stats(dsmvc_hip_executor *executor, dsmvc_hip_import_stats *stats) {
+    return guarded([&] {
+        require(stats, "dsmvc: null argument");
+        const auto s = exec_of(executor).import_stats();
+        stats->entries = s.entries;
+        stats->imports = s.imports;
+        stats->hits = s.hits;
+        stats->evictions = s.evictions;
     });
 }
 

@@ -129,6 +129,23 @@ using CustomKernel = std::function<double(double)>;
 // The cached plan for `request` when it is built, else nullptr; never builds.
 [[nodiscard]] std::shared_ptr<const AxisPlan> find_axis_plan(const AxisRequest &request);
 [[nodiscard]] PlannerCacheStats planner_cache_stats();
+
+// One plane of a GPU frame exported by its producer for a foreign API
+// (reference include/dsmvc/engine.hpp:306-326): the whole allocation is
+// imported once (keyed by memory_id) and the plane lives at `offset`.
+struct ExportedPlane {
+    std::uint64_t memory_id = 0;     // stable for the allocation's lifetime
+    std::uint64_t memory_size = 0;
+    std::uint64_t offset = 0;
+    std::uint64_t plane_bytes = 0;   // pitch * height
+    int handle_type = 0;             // VkExternalMemoryHandleTypeFlagBits (0x1: opaque fd)
+    std::intptr_t handle = -1;
+    std::ptrdiff_t pitch_bytes = 0;
+    std::uint64_t ready_semaphore = 0;   // producer's timeline semaphore, 0: ready now
+    std::uint64_t ready_value = 0;
+};
+// Closes the handle of an exported plane that was never passed to an executor.
+void close_exported_plane(const ExportedPlane &plane) noexcept;
 void clear_planner_caches();
 
 [[nodiscard]] std::int32_t padding_index(std::int64_t index, std::int32_t size, BorderMode mode);

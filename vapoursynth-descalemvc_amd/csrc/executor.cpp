@@ -1,5 +1,6 @@
 // HIP executor: route selection, scratch management and launches.
 #include "executor.hpp"
+#include "exported.hpp"
 #include "trace.hpp"
 
 #include <algorithm>
@@ -131,6 +132,7 @@ HipExecutor::~HipExecutor() {
     // as a whole is waited for.
     (void)hipSetDevice(device_);
     (void)hipDeviceSynchronize();
+    imports_.reset();
     plane_tables_.clear();
     for (auto &entry : scratch_) {
         if (entry.second.buffer) (void)hipFree(entry.second.buffer);

@@ -191,6 +191,36 @@ public:
                               std::ptrdiff_t input_row_stride, float *output,
                               std::ptrdiff_t output_row_stride, std::int32_t column_count);
 
+    // GPU frames exported by another API (reference include/dsmvc/engine.hpp:
+    // 306-326, 414-442: VapourSynth's GPU core exports a frame's memory as a
+    // POSIX file descriptor, src/cuda/cuda_executor.cpp:1180-1300,
+    // 3640-3900).  The allocation is imported once per memory_id
+    // (hipImportExternalMemory, then mapped) and kept in a bounded LRU; the
+    // plane lives at `offset` with `pitch_bytes` between rows.  The handle is
+    // consumed by every call, whatever the outcome: a successful import owns
+    // it, any other path closes it.  An input whose producer pair names a
+    // semaphore imported with import_exported_semaphore is waited for on the
+    // device (hipWaitExternalSemaphoresAsync); otherwise the caller has
+    // waited.  Synchronous: the output is written when the call returns.
+    void inverse_2d_exported(const AxisPlan &horizontal, const AxisPlan &vertical, SampleKind kind,
+                             const ExportedPlane &input, const ExportedPlane &output,
+                             const IntegerConversion *conversion);
+    void inverse_rows_exported(const AxisPlan &plan, const ExportedPlane &input,
+                               const ExportedPlane &output, std::int32_t row_count);
+    void inverse_columns_exported(const AxisPlan &plan, const ExportedPlane &input,
+                                  const ExportedPlane &output, std::int32_t column_count);
+    // A timeline semaphore exported by the frame producer, keyed by its
+    // producer-side handle value; returns whether it could be imported (the
+    // handle is consumed either way).
+    bool import_exported_semaphore(std::uint64_t semaphore, int handle_type, std::intptr_t handle);
+    [[nodiscard]] bool has_exported_semaphore(std::uint64_t semaphore);
+    struct ImportStats {
+        std::uint64_t entries = 0, imports = 0, hits = 0, evictions = 0;
+    };
+    ImportStats import_stats();
+    // Imported exported allocations and semaphores (exported.hpp).
+    class Imports;
+
     // The device image of a plan (uploaded on first use), for callers that
     // launch the kernels themselves (multi-plan batches, the request loop).
     // The cache never evicts an image while a returned pointer to it lives.
@@ -325,6 +355,10 @@ private:
     PlanCacheStats plan_stats_{};
     std::unordered_map<hipStream_t, Scratch> scratch_;
     bool sealed_ = false;
+
+    std::unique_ptr<Imports> imports_;
+    Imports &imports();
+    std::mutex imports_mutex_;
 };
 
 } // namespace dsmvc

@@ -20,6 +20,7 @@ enum class Label : std::uint8_t {
     execute_rows,
     execute_columns,
     execute_host_frame,
+    execute_exported,
     plan_build,
     geometry_build,
     plan_cache_lookup,
@@ -49,6 +50,7 @@ inline constexpr std::array<const char *, static_cast<std::size_t>(Label::count)
     "dsmvc.hip: execute rows",
     "dsmvc.hip: execute columns",
     "dsmvc.hip: execute host frame",
+    "dsmvc.hip: execute exported frame",
     "dsmvc.hip: plan build",
     "dsmvc.hip: geometry build",
     "dsmvc.hip: plan cache lookup",

@@ -427,6 +427,41 @@ class Executor:
         return out
 
     # -- device plan cache ------------------------------------------------------
+    def inverse_2d_exported(self, horizontal: Plan, vertical: Plan, input_plane, output_plane,
+                            sample_kind: int = 0, conversion: Optional[IntegerConversion] = None):
+        """dsmvc_hip_inverse_2d_exported: planes of GPU frames exported by
+        another API (`_lib.ExportedPlaneC`, e.g. a POSIX descriptor from
+        VapourSynth's GPU core); both planes' handles are consumed."""
+        conv = ctypes.byref(conversion.to_c()) if conversion is not None else None
+        check(load().dsmvc_hip_inverse_2d_exported(
+            self._handle, horizontal.handle, vertical.handle, sample_kind,
+            ctypes.byref(input_plane), ctypes.byref(output_plane), conv))
+
+    def inverse_rows_exported(self, plan: Plan, input_plane, output_plane, rows: int):
+        check(load().dsmvc_hip_inverse_rows_exported(
+            self._handle, plan.handle, ctypes.byref(input_plane), ctypes.byref(output_plane), rows))
+
+    def inverse_columns_exported(self, plan: Plan, input_plane, output_plane, columns: int):
+        check(load().dsmvc_hip_inverse_columns_exported(
+            self._handle, plan.handle, ctypes.byref(input_plane), ctypes.byref(output_plane),
+            columns))
+
+    def import_exported_semaphore(self, semaphore: int, handle_type: int, handle: int) -> bool:
+        out = ctypes.c_int32()
+        check(load().dsmvc_hip_import_exported_semaphore(self._handle, semaphore, handle_type,
+                                                         handle, ctypes.byref(out)))
+        return bool(out.value)
+
+    def has_exported_semaphore(self, semaphore: int) -> bool:
+        out = ctypes.c_int32()
+        check(load().dsmvc_hip_has_exported_semaphore(self._handle, semaphore, ctypes.byref(out)))
+        return bool(out.value)
+
+    def import_stats(self) -> dict:
+        s = _lib.ImportStatsC()
+        check(load().dsmvc_hip_executor_import_stats(self._handle, ctypes.byref(s)))
+        return {name: getattr(s, name) for name, _ in _lib.ImportStatsC._fields_}
+
     def set_plan_cache(self, limit_bytes: int):
         check(load().dsmvc_hip_executor_set_plan_cache(self._handle, int(limit_bytes)))
 

@@ -36,7 +36,10 @@ EXPORTS = (
     "dsmvc_hip_inverse_2d_batch_noalloc", "dsmvc_hip_plan_import",
     "dsmvc_hip_inverse_rows_host", "dsmvc_hip_inverse_columns_host", "dsmvc_hip_inverse_2d_host",
     "dsmvc_hip_executor_set_plan_cache", "dsmvc_hip_executor_plan_cache_stats",
-    "dsmvc_hip_plan_build_custom",
+    "dsmvc_hip_plan_build_custom", "dsmvc_hip_inverse_2d_exported",
+    "dsmvc_hip_inverse_rows_exported", "dsmvc_hip_inverse_columns_exported",
+    "dsmvc_hip_import_exported_semaphore", "dsmvc_hip_has_exported_semaphore",
+    "dsmvc_hip_close_exported_plane", "dsmvc_hip_executor_import_stats",
 )
 KERNEL_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_double, ctypes.c_void_p)
 BATCH_GRAPH = 2
@@ -219,6 +222,30 @@ class DsmvcError(RuntimeError):
 _lib = None
 
 
+class ExportedPlaneC(ctypes.Structure):
+    _fields_ = [
+        ("memory_id", ctypes.c_uint64),
+        ("memory_size", ctypes.c_uint64),
+        ("offset", ctypes.c_uint64),
+        ("plane_bytes", ctypes.c_uint64),
+        ("handle_type", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("handle", ctypes.c_int64),
+        ("pitch_bytes", ctypes.c_int64),
+        ("ready_semaphore", ctypes.c_uint64),
+        ("ready_value", ctypes.c_uint64),
+    ]
+
+
+class ImportStatsC(ctypes.Structure):
+    _fields_ = [
+        ("entries", ctypes.c_uint64),
+        ("imports", ctypes.c_uint64),
+        ("hits", ctypes.c_uint64),
+        ("evictions", ctypes.c_uint64),
+    ]
+
+
 def load():
     """Loads libdsmvc_hip.so (built by `make -C vapoursynth-descalemvc_amd`)."""
     global _lib
@@ -293,6 +320,17 @@ def load():
     lib.dsmvc_hip_server_stats_get.argtypes = [vp, ctypes.POINTER(ServerStatsC)]
     lib.dsmvc_hip_server_destroy.argtypes = [vp]
     lib.dsmvc_hip_server_destroy.restype = None
+    xp = ctypes.POINTER(ExportedPlaneC)
+    lib.dsmvc_hip_inverse_2d_exported.argtypes = [vp, vp, vp, i32, xp, xp,
+                                                  ctypes.POINTER(ConversionC)]
+    lib.dsmvc_hip_inverse_rows_exported.argtypes = [vp, vp, xp, xp, i32]
+    lib.dsmvc_hip_inverse_columns_exported.argtypes = [vp, vp, xp, xp, i32]
+    lib.dsmvc_hip_import_exported_semaphore.argtypes = [vp, ctypes.c_uint64, i32, ctypes.c_int64,
+                                                        ctypes.POINTER(i32)]
+    lib.dsmvc_hip_has_exported_semaphore.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(i32)]
+    lib.dsmvc_hip_close_exported_plane.argtypes = [xp]
+    lib.dsmvc_hip_close_exported_plane.restype = None
+    lib.dsmvc_hip_executor_import_stats.argtypes = [vp, ctypes.POINTER(ImportStatsC)]
     lib.dsmvc_hip_executor_release_stream.argtypes = [vp, vp]
     lib.dsmvc_hip_synchronize.argtypes = [vp, vp]
     _lib = lib
