@@ -138,6 +138,54 @@ int bnd_inverse_2d(void *ex, const bnd_request *h, const bnd_request *v, int32_t
     });
 }
 
+// Exported GPU frames (Executor::inverse_2d_exported and its integer forms):
+// the planes' fields as dsmvc::ExportedPlane holds them.
+struct bnd_exported {
+    uint64_t memory_id;
+    uint64_t memory_size;
+    uint64_t offset;
+    uint64_t plane_bytes;
+    int32_t handle_type;
+    int32_t reserved;
+    int64_t handle;
+    int64_t pitch_bytes;
+    uint64_t ready_semaphore;
+    uint64_t ready_value;
+};
+
+int bnd_inverse_2d_exported(void *ex, const bnd_request *h, const bnd_request *v, int32_t kind,
+                            const bnd_exported *in, const bnd_exported *out,
+                            const bnd_conversion *conversion) {
+    auto plane = [](const bnd_exported *p) {
+        dsmvc::ExportedPlane e;
+        e.memory_id = p->memory_id;
+        e.memory_size = p->memory_size;
+        e.offset = p->offset;
+        e.plane_bytes = p->plane_bytes;
+        e.handle_type = p->handle_type;
+        e.handle = static_cast<std::intptr_t>(p->handle);
+        e.pitch_bytes = static_cast<std::ptrdiff_t>(p->pitch_bytes);
+        e.ready_semaphore = p->ready_semaphore;
+        e.ready_value = p->ready_value;
+        return e;
+    };
+    return guarded([&] {
+        const auto hp = reference_plan(h), vp = reference_plan(v);
+        dsmvc::IntegerConversion c;
+        if (conversion) {
+            c.input_offset = conversion->input_offset;
+            c.input_scale = conversion->input_scale;
+            c.output_scale = conversion->output_scale;
+            c.output_offset = conversion->output_offset;
+            c.output_maximum = conversion->output_maximum;
+        }
+        auto &e = executor(ex);
+        if (kind == 0) e.inverse_2d_exported(*hp, *vp, plane(in), plane(out));
+        else if (kind == 1) e.inverse_2d_u8_exported(*hp, *vp, plane(in), plane(out), c);
+        else e.inverse_2d_u16_exported(*hp, *vp, plane(in), plane(out), c);
+    });
+}
+
 int bnd_inverse_rows(void *ex, const bnd_request *r, const float *in, ptrdiff_t in_stride,
                      float *out, ptrdiff_t out_stride, int32_t rows) {
     return guarded([&] {

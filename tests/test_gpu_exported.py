@@ -11,7 +11,10 @@ memory API and exported anew for every call.
     hit the cache, whose handles the library closes;
   * a plane outside its allocation is rejected and its handle closed;
   * a semaphore handle HIP cannot import is reported as not imported and
-    closed.
+    closed;
+  * the reference-side binding (reference_binding/hip_executor.cpp) takes
+    the reference planner's plans and exported planes and equals its own
+    host-buffer entry bit for bit.
 """
 import ctypes
 import os
@@ -24,7 +27,9 @@ pytestmark = pytest.mark.gpu
 
 import dsmvc_hip as dh  # noqa: E402
 from dsmvc_hip import _lib  # noqa: E402
-from support import ROOT  # noqa: E402
+from support import (BICUBIC, ROOT, Conversion, make_request,
+                     reference_available)  # noqa: E402
+from workloads import integer_conversion  # noqa: E402
 
 DEV = torch.device("cuda:0")
 HELPER = ROOT / "tests" / "native" / "libexportable.so"
@@ -175,3 +180,41 @@ def test_rejected_planes_and_semaphores_consume_their_handles(memory):
     fd = os.open(os.devnull, os.O_RDONLY)
     assert not ex.import_exported_semaphore(77, OPAQUE_FD, fd)
     assert _closed(fd) and not ex.has_exported_semaphore(77)
+
+
+BINDING_SO = ROOT / "oracle" / "_ref" / "libdsmvc_binding.so"
+
+
+def test_reference_binding_exported_equals_its_host_entry(memory):
+    if not reference_available() or not BINDING_SO.exists():
+        pytest.skip("oracle/_ref (reference engine and binding) not built")
+    from support import Request
+    lib = ctypes.CDLL(str(BINDING_SO))
+    vp, pd, i32 = ctypes.c_void_p, ctypes.c_ssize_t, ctypes.c_int32
+    lib.bnd_last_error.restype = ctypes.c_char_p
+    lib.bnd_create.argtypes = [i32, ctypes.POINTER(vp)]
+    lib.bnd_free.argtypes = [vp]
+    lib.bnd_inverse_2d.argtypes = [vp, ctypes.POINTER(Request), ctypes.POINTER(Request), i32,
+                                   vp, pd, vp, pd, ctypes.POINTER(Conversion)]
+    lib.bnd_inverse_2d_exported.argtypes = [vp, ctypes.POINTER(Request), ctypes.POINTER(Request),
+                                            i32, ctypes.POINTER(_lib.ExportedPlaneC),
+                                            ctypes.POINTER(_lib.ExportedPlaneC),
+                                            ctypes.POINTER(Conversion)]
+    src, dst = memory
+    ex = vp()
+    assert lib.bnd_create(0, ctypes.byref(ex)) == 0, lib.bnd_last_error()
+    try:
+        h, v = make_request(400, 256, kernel=BICUBIC), make_request(226, 144, kernel=BICUBIC)
+        conv = integer_conversion(16, False, False)
+        rng = np.random.default_rng(11)
+        x = rng.integers(4096, 60000, (226, 400)).astype(np.uint16)
+        want = np.empty((144, 256), np.uint16)
+        assert lib.bnd_inverse_2d(ex, h, v, 2, x.ctypes.data, 400, want.ctypes.data, 256,
+                                  conv) == 0, lib.bnd_last_error()
+        src.write(0, x)
+        pin, pout = src.plane(0, 800, 226), dst.plane(0, 512, 144)
+        assert lib.bnd_inverse_2d_exported(ex, h, v, 2, pin, pout, conv) == 0, \
+            lib.bnd_last_error()
+        assert np.array_equal(dst.read(0, (144, 256), np.uint16), want)
+    finally:
+        lib.bnd_free(ex)

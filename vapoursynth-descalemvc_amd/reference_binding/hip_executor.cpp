@@ -171,4 +171,104 @@ void HipExecutor::inverse_2d_u16(const AxisPlan &horizontal, const AxisPlan &ver
                                     &c));
 }
 
+namespace {
+dsmvc_hip_exported_plane to_c(const ExportedPlane &p) {
+    dsmvc_hip_exported_plane out{};
+    out.memory_id = p.memory_id;
+    out.memory_size = p.memory_size;
+    out.offset = p.offset;
+    out.plane_bytes = p.plane_bytes;
+    out.handle_type = p.handle_type;
+    out.handle = static_cast<std::int64_t>(p.handle);
+    out.pitch_bytes = static_cast<std::int64_t>(p.pitch_bytes);
+    out.ready_semaphore = p.ready_semaphore;
+    out.ready_value = p.ready_value;
+    return out;
+}
+} // namespace
+
+bool HipExecutor::has_exported_semaphore(std::uint64_t semaphore) const {
+    std::int32_t imported = 0;
+    check(dsmvc_hip_has_exported_semaphore(state_->executor, semaphore, &imported));
+    return imported != 0;
+}
+
+bool HipExecutor::import_exported_semaphore(std::uint64_t semaphore, int handle_type,
+                                            std::intptr_t handle) const {
+    std::int32_t imported = 0;
+    check(dsmvc_hip_import_exported_semaphore(state_->executor, semaphore, handle_type,
+                                              static_cast<std::int64_t>(handle), &imported));
+    return imported != 0;
+}
+
+void HipExecutor::inverse_rows_exported(const AxisPlan &plan, const ExportedPlane &input,
+                                        const ExportedPlane &output, std::int32_t row_count) const {
+    const dsmvc_hip_exported_plane in = to_c(input), out = to_c(output);
+    std::shared_ptr<Imported> p;
+    try {
+        p = state_->plan(plan);
+    } catch (...) {
+        dsmvc_hip_close_exported_plane(&in);
+        dsmvc_hip_close_exported_plane(&out);
+        throw;
+    }
+    check(dsmvc_hip_inverse_rows_exported(state_->executor, p->handle, &in, &out, row_count));
+}
+
+void HipExecutor::inverse_columns_exported(const AxisPlan &plan, const ExportedPlane &input,
+                                           const ExportedPlane &output,
+                                           std::int32_t column_count) const {
+    const dsmvc_hip_exported_plane in = to_c(input), out = to_c(output);
+    std::shared_ptr<Imported> p;
+    try {
+        p = state_->plan(plan);
+    } catch (...) {
+        dsmvc_hip_close_exported_plane(&in);
+        dsmvc_hip_close_exported_plane(&out);
+        throw;
+    }
+    check(dsmvc_hip_inverse_columns_exported(state_->executor, p->handle, &in, &out,
+                                             column_count));
+}
+
+namespace {
+template <class State, class Plan>
+void exported_2d(State &state, const Plan &horizontal, const Plan &vertical, std::int32_t kind,
+                 const ExportedPlane &input, const ExportedPlane &output,
+                 const dsmvc_hip_integer_conversion *conversion) {
+    const dsmvc_hip_exported_plane in = to_c(input), out = to_c(output);
+    decltype(state.plan(horizontal)) h, v;
+    try {
+        h = state.plan(horizontal);
+        v = state.plan(vertical);
+    } catch (...) {
+        dsmvc_hip_close_exported_plane(&in);
+        dsmvc_hip_close_exported_plane(&out);
+        throw;
+    }
+    check(dsmvc_hip_inverse_2d_exported(state.executor, h->handle, v->handle, kind, &in, &out,
+                                        conversion));
+}
+} // namespace
+
+void HipExecutor::inverse_2d_exported(const AxisPlan &horizontal, const AxisPlan &vertical,
+                                      const ExportedPlane &input,
+                                      const ExportedPlane &output) const {
+    exported_2d(*state_, horizontal, vertical, DSMVC_HIP_SAMPLE_F32, input, output, nullptr);
+}
+
+void HipExecutor::inverse_2d_u8_exported(const AxisPlan &horizontal, const AxisPlan &vertical,
+                                         const ExportedPlane &input, const ExportedPlane &output,
+                                         const IntegerConversion &conversion) const {
+    const dsmvc_hip_integer_conversion c = to_c(conversion);
+    exported_2d(*state_, horizontal, vertical, DSMVC_HIP_SAMPLE_U8, input, output, &c);
+}
+
+void HipExecutor::inverse_2d_u16_exported(const AxisPlan &horizontal, const AxisPlan &vertical,
+                                          const ExportedPlane &input, const ExportedPlane &output,
+                                          const IntegerConversion &conversion) const {
+    const dsmvc_hip_integer_conversion c = to_c(conversion);
+    exported_2d(*state_, horizontal, vertical, DSMVC_HIP_SAMPLE_U16, input, output, &c);
+}
+
 } // namespace dsmvc::hip_detail

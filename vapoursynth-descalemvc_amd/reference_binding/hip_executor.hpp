@@ -66,6 +66,26 @@ public:
                         const IntegerConversion &conversion,
                         std::shared_ptr<const void> input_lifetime) const;
 
+    // GPU frames exported by the core (the CUDA executor's *_exported
+    // surface, src/cuda/cuda_executor.hpp; include/dsmvc/engine.hpp:414-442):
+    // the planes' descriptors are imported by the HIP library, which consumes
+    // them whatever the outcome.
+    [[nodiscard]] bool has_exported_semaphore(std::uint64_t semaphore) const;
+    [[nodiscard]] bool import_exported_semaphore(std::uint64_t semaphore, int handle_type,
+                                                 std::intptr_t handle) const;
+    void inverse_rows_exported(const AxisPlan &plan, const ExportedPlane &input,
+                               const ExportedPlane &output, std::int32_t row_count) const;
+    void inverse_columns_exported(const AxisPlan &plan, const ExportedPlane &input,
+                                  const ExportedPlane &output, std::int32_t column_count) const;
+    void inverse_2d_exported(const AxisPlan &horizontal, const AxisPlan &vertical,
+                             const ExportedPlane &input, const ExportedPlane &output) const;
+    void inverse_2d_u8_exported(const AxisPlan &horizontal, const AxisPlan &vertical,
+                                const ExportedPlane &input, const ExportedPlane &output,
+                                const IntegerConversion &conversion) const;
+    void inverse_2d_u16_exported(const AxisPlan &horizontal, const AxisPlan &vertical,
+                                 const ExportedPlane &input, const ExportedPlane &output,
+                                 const IntegerConversion &conversion) const;
+
 private:
     struct State;
     std::shared_ptr<State> state_;
