@@ -197,7 +197,7 @@ def test_near_identity_plans_report_their_decay():
 # x_i = y_i - sum_d L(i+d, i) x_{i+d}; reference src/axis_plan.cpp:1124-1191)
 # on chunks of L rows with P warm-up rows per side, and accept a plan when
 # the wrong start has decayed below 2^-40.  Chunk shape per band class:
-F64_SHAPES = {1: (48, 32), 3: (48, 32), 5: (32, 48), 7: (24, 64)}
+F64_SHAPES = {1: (32, 32), 3: (32, 32), 5: (24, 48), 7: (24, 64)}
 TOLERANCE64 = 2.0 ** -40
 
 

@@ -193,11 +193,12 @@ constexpr int window_bwd_batch(int w) { return 64 / window_bwd_words(w); }
 // Windowed FP64 partition (the Float64 route of F32 planes): the same
 // padded-row layout as DevWindow with FP64 records, a warm-up long enough
 // for a 2^-40 decay check and chunks short enough that a lane keeps its
-// L + P values as doubles in registers.  Forward records hold the FP64 tap
+// L + P values as doubles in registers without spilling at two wavefronts
+// per SIMD (L = 32 for band classes up to 3, 24 above).  Forward records hold the FP64 tap
 // weights, forward factors and the diagonal multiplier of the Float64 route
 // (unit-L factors when zform = 1: the recurrence carries z = L^-1 b and the
 // diagonal is applied to the stored value only).  Staging uses 8-row blocks.
-constexpr int window_length64(int w) { return w <= 3 ? 48 : (w <= 5 ? 32 : 24); }
+constexpr int window_length64(int w) { return w <= 3 ? 32 : 24; }
 constexpr int window_warmup64(int w) { return w <= 3 ? 32 : (w <= 5 ? 48 : 64); }
 constexpr int window_fwd_batch64(int w, int k) { return k + w + 1 <= 12 ? 2 : 1; }
 constexpr int window_bwd_batch64(int w) { return 32 / window_bwd_words(w); }
