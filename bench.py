@@ -720,6 +720,41 @@ def main():
         extra["precision_routes_lanczos4_1904x1071"] = prec
         del pin, pout, refined
 
+    if not args.no_gray:
+        # Host frames (PCIe-inclusive, never the headline): the same YUV420P16
+        # workload from pageable host memory through dsmvc_hip_inverse_2d_host
+        # per plane (the library stages each plane through pinned memory on an
+        # execution slot: pack, upload, solve, download, unpack).  Four
+        # threads, as fmParallel drives the plugin, 16 frames per call.
+        import threading
+        host_in = [torch.cat([p, p])[:16].cpu().numpy().astype(np.uint16) for p in unique]
+        host_out = [[np.empty((16, DST_H >> (p > 0), DST_W >> (p > 0)), np.uint16)
+                     for p in range(3)] for _ in range(4)]
+        hconv = [dh.integer_conversion(16, False, False), dh.integer_conversion(16, True, False)]
+
+        def host_frames(t, calls):
+            for _ in range(calls):
+                for p in range(3):
+                    hp, vp = plans[0 if p == 0 else 1]
+                    ex.inverse_2d_host(hp, vp, host_in[p], out=host_out[t][p],
+                                       conversion=hconv[0 if p == 0 else 1])
+        host_frames(0, 1)
+        calls = 6
+        t0 = time.perf_counter()
+        pool = [threading.Thread(target=host_frames, args=(t, calls)) for t in range(4)]
+        for th in pool:
+            th.start()
+        for th in pool:
+            th.join()
+        hel = time.perf_counter() - t0
+        hfps = 4 * calls * 16 / hel
+        extra["host_frames_yuv420p16_debicubic"] = {
+            "frames_per_s": round(hfps, 1),
+            "achieved_GBps": round(FRAME_BYTES_YUV16 * hfps / 1e9, 1),
+            "note": "pageable host planes, 4 threads x 16-frame calls, per plane "
+                    "dsmvc_hip_inverse_2d_host (PCIe-inclusive)"}
+        del host_in, host_out
+
     if not args.no_batch:
         frame = (unique[0][0].to(torch.float32) / 65535.0).contiguous()
         extra["mixed_size_scan"] = batched_scan(torch, dh, dev, frame, repeats=5)
