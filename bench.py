@@ -3,8 +3,11 @@
 
 Workload (one step): Debicubic(b=0, c=0.5) of 64 YUV420P16 frames
 1920x1080 -> 1280x720 (chroma 960x540 -> 640x360, limited range, padding 3),
-F32 internal, 16-bit output -- the horizontal pass (u16 rows -> F32
-intermediate) then the vertical pass (F32 columns -> u16) for each plane.
+16-bit output, all three planes in one dsmvc_hip_inverse_2d_planes call.  In
+the default solve mode that call takes the fused 2D route (one kernel:
+X = M_v Y M_h^T with the planes' explicit operators, F32 accumulation, no
+intermediate in memory); the two-pass route's per-axis kernels (u16 rows ->
+F32 intermediate -> u16 columns) are timed beside it.
 Frames are synthetic: a seeded 1280x720 U[0,65535] image blurred by a
 Gaussian (sigma 1.5 px), upscaled by the forward bicubic operator, plus
 N(0, 64^2) noise, clamped to 16 bits; inputs already in HBM, rotating through
@@ -612,10 +615,24 @@ def main():
     ex.synchronize()
     fps = FRAMES * args.steps / elapsed
 
-    # Per-kernel roofline: the luma plane's two passes on their own (the
-    # same kernels the frame call runs), bracketed by HIP events on the
-    # launch stream.
+    # The frame call's own launches, bracketed by HIP events on its stream:
+    # with the fused route that is one kernel for all three planes.
     rsteps = max(10, args.steps // 4)
+    fevents = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(rsteps)]
+    torch.cuda.synchronize()
+    for s in range(rsteps):
+        fevents[s][0].record(stream)
+        frame_step(s)
+        fevents[s][1].record(stream)
+    torch.cuda.synchronize()
+    ex.synchronize()
+    frame_call_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in fevents]))
+    fused = (args.solve == "auto" and os.environ.get("DSMVC_HIP_FUSED", "1") != "0"
+             and all(p.operator()["rows_feasible"] for p in (plans[0][0], plans[1][0]))
+             and all(p.operator()["columns_feasible"] for p in (plans[0][1], plans[1][1])))
+
+    # The two-pass route's per-axis kernels: the luma plane's two passes on
+    # their own, bracketed by HIP events on the launch stream.
     events = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(rsteps)]
     torch.cuda.synchronize()
     for s in range(rsteps):
@@ -641,7 +658,10 @@ def main():
         name = "window" if route == -1 else ("chunked" if route > 0 else "seq")
         return (f"rows_{name}<u16> (luma horizontal pass, 64 frames)" if rows
                 else f"columns_{name}<u16> (luma vertical pass, 64 frames)")
-    if h_avg >= v_avg:
+    if fused:
+        dom = "operator2d<u16> (fused 2D, YUV420P16 frame call, 64 frames)"
+        dom_bytes, dom_s = FRAMES * FRAME_BYTES_YUV16, frame_call_ms * 1e-3
+    elif h_avg >= v_avg:
         dom, dom_bytes, dom_s = family(True), h_bytes, h_avg
     else:
         dom, dom_bytes, dom_s = family(False), v_bytes, v_avg
@@ -653,6 +673,9 @@ def main():
                 "traffic_source": traffic_source,
                 "kernel": dom, "algorithmic_bytes_per_launch": dom_bytes,
                 "avg_launch_ms": round(dom_s * 1e3, 4),
+                # The frame call (events around it) and, beside it, the
+                # two-pass route's luma passes launched on their own.
+                "frame_call_ms": round(frame_call_ms, 4),
                 "luma_rows_ms": round(h_avg * 1e3, 4), "luma_columns_ms": round(v_avg * 1e3, 4),
                 # The whole frame (all passes of all planes, one call per step):
                 # algorithmic frame bytes x frames/s.
@@ -664,7 +687,8 @@ def main():
                            "frac_of_8TBps": round(frame_gbps / HBM_PEAK_GBPS, 4)},
              "frame_call": "dsmvc_hip_inverse_2d_planes (3 planes per call)",
              "per_plane_calls_frames_per_s": round(per_plane_fps, 1),
-             "routes": {"luma_rows": family(True), "luma_columns": family(False)},
+             "routes": {"frame_call": "fused 2D (operator2d)" if fused else "two passes",
+                        "per_plane_luma_rows": family(True), "per_plane_luma_columns": family(False)},
              "input_rotation_bytes": nbuf * batch_in_bytes, "solve": args.solve}
 
     if not args.no_gray:

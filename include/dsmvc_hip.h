@@ -42,7 +42,7 @@ enum { DSMVC_HIP_BILINEAR = 0, DSMVC_HIP_BICUBIC = 1, DSMVC_HIP_LANCZOS = 2,
        DSMVC_HIP_SPLINE16 = 3, DSMVC_HIP_SPLINE36 = 4, DSMVC_HIP_SPLINE64 = 5,
        DSMVC_HIP_CUSTOM = 6 };
 enum { DSMVC_HIP_SOLVE_AUTO = 0, DSMVC_HIP_SOLVE_EXACT = 1, DSMVC_HIP_SOLVE_CHUNKED = 2,
-       DSMVC_HIP_SOLVE_WINDOWED = 3, DSMVC_HIP_SOLVE_REFINED = 4 };
+       DSMVC_HIP_SOLVE_WINDOWED = 3, DSMVC_HIP_SOLVE_REFINED = 4, DSMVC_HIP_SOLVE_FUSED = 5 };
 enum { DSMVC_HIP_SAMPLE_F32 = 0, DSMVC_HIP_SAMPLE_U8 = 1, DSMVC_HIP_SAMPLE_U16 = 2 };
 
 /* dsmvc::AxisRequest with KernelSpec flattened (include/dsmvc/engine.hpp:60-78). */
@@ -96,6 +96,20 @@ typedef struct dsmvc_hip_window_info {
                                  the Float64 route of F32 planes (2^-40 check) */
     double residual;         /* worst influence of a chunk's zero start, relative */
 } dsmvc_hip_window_info;
+
+/* Explicit operator of a plan for the fused 2D route (see
+ * dsmvc_hip_plan_operator). */
+typedef struct dsmvc_hip_operator_info {
+    int32_t rows_feasible;    /* 1: the fused route can take the plan horizontally */
+    int32_t columns_feasible; /* 1: ... vertically */
+    int32_t blocks;           /* blocks of 16 destination indices (padded) */
+    int32_t steps;            /* K steps of 32 source indices per block */
+    int32_t max_band;         /* longest kept row */
+    int32_t reserved;
+    int64_t values;           /* kept entries over all rows */
+    double dropped;           /* largest L1 share truncation dropped from a row */
+    double max_abs;
+} dsmvc_hip_operator_info;
 
 typedef struct dsmvc_hip_planner_stats {
     uint64_t plan_hits;
@@ -177,6 +191,14 @@ int dsmvc_hip_plan_import(const dsmvc_hip_plan_info *info, const dsmvc_hip_plan_
 /* No counterpart in the reference: reports how the windowed (independent
  * chunk) kernels would partition this plan, computed on the host. */
 int dsmvc_hip_plan_window_info(const dsmvc_hip_plan *plan, dsmvc_hip_window_info *info);
+/* No counterpart in the reference: the plan's explicit operator
+ * M = G^-1 A^T (G = A^T A), built on the host in FP64 and truncated to the
+ * band holding all but 2^-28 of each row's L1 mass at either end, which the
+ * fused 2D route multiplies on MFMA.  `lo` / `count` ([destination_size],
+ * optional) receive each row's first source index and length, `values`
+ * (optional, `capacity` entries) the rows one after another. */
+int dsmvc_hip_plan_operator(const dsmvc_hip_plan *plan, dsmvc_hip_operator_info *info, int32_t *lo,
+                            int32_t *count, double *values, int64_t capacity);
 /* dsmvc::planner_cache_stats / clear_planner_caches (engine.hpp:140-149,
  * 179-180): the plan LRU and the sampling-geometry LRU beneath it
  * (src/axis_plan.cpp:520-533). */
@@ -203,7 +225,13 @@ int dsmvc_hip_device_count(int32_t *count);
  *             F32 solver (and its Float64 route).
  *   CHUNKED / WINDOWED  force that F32 family (plans it cannot take fail).
  *   REFINED   AUTO, with plans that retain FP64 data solved in F32 plus
- *             refinement instead of the Float64 route. */
+ *             refinement instead of the Float64 route.
+ *   FUSED     AUTO, and every 2D call (F32, u8, u16 planes on the F32 route)
+ *             takes the fused route wherever both plans' explicit operators
+ *             fit it (dsmvc_hip_plan_operator): X = M_v Y M_h^T in one launch
+ *             on MFMA with 16-bit split operands, no intermediate in memory.
+ *             AUTO takes it the same way for u8/u16 planes unless
+ *             DSMVC_HIP_FUSED=0. */
 int dsmvc_hip_executor_create(int32_t device, int32_t solve_mode, dsmvc_hip_executor **executor);
 void dsmvc_hip_executor_destroy(dsmvc_hip_executor *executor);
 /* Executor::prepare / seal (src/executor.cpp:313-349): uploads the plan image. */

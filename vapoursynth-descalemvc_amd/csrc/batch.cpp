@@ -102,6 +102,13 @@ void WindowTables::add_check(const hip::CheckJob &job) {
     g.checks.push_back(j);
 }
 
+void WindowTables::add_op(const hip::OpJob &job, int steps, hip::Sample kind) {
+    Group &g = group_for(Kind::op, 0, steps, true, false, kind);
+    hip::OpJob j = job;
+    j.first_item = g.items;
+    g.items += hip::operator_items(j);
+    g.ops.push_back(j);
+}
 
 // Tables go through a ring of slots, each a pinned host buffer, a device
 // buffer and an event recorded after the launches that read it: an upload
@@ -112,7 +119,9 @@ int WindowTables::upload(hipStream_t stream) {
     // Checks first, then horizontal groups: every vertical pass reads an
     // intermediate some horizontal group writes.
     std::stable_sort(groups_.begin(), groups_.end(), [](const Group &a, const Group &b) {
-        auto rank = [](const Group &g) { return g.type == Kind::check ? 0 : (g.rows ? 1 : 2); };
+        auto rank = [](const Group &g) {
+            return g.type == Kind::check || g.type == Kind::op ? 0 : (g.rows ? 1 : 2);
+        };
         return rank(a) < rank(b);
     });
     check(hipSetDevice(device_), "hipSetDevice failed");
@@ -156,6 +165,7 @@ int WindowTables::upload(hipStream_t stream) {
         if (!g.jobs.empty()) std::memcpy(at, g.jobs.data(), g.jobs.size() * sizeof(hip::WinJob));
         if (!g.jobs64.empty()) std::memcpy(at, g.jobs64.data(), g.jobs64.size() * sizeof(hip::Win64Job));
         if (!g.checks.empty()) std::memcpy(at, g.checks.data(), g.checks.size() * sizeof(hip::CheckJob));
+        if (!g.ops.empty()) std::memcpy(at, g.ops.data(), g.ops.size() * sizeof(hip::OpJob));
     }
     if (bytes) {
         if (stream)
@@ -174,8 +184,10 @@ void WindowTables::launch(hipStream_t stream) {
     for (const Group &g : groups_) {
         if (g.items == 0) continue;
         // Checks and horizontal groups come first (upload() sorts them).
-        const trace::Range range(g.rows || g.type == Kind::check ? trace::Label::horizontal_stage
-                                                                 : trace::Label::vertical_stage);
+        const trace::Range range(g.type == Kind::op
+                                     ? trace::Label::fused_stage
+                                     : (g.rows || g.type == Kind::check ? trace::Label::horizontal_stage
+                                                                        : trace::Label::vertical_stage));
         unsigned char *at = slot.device + g.offset;
         const int count = static_cast<int>(g.count());
         hipError_t e = hipSuccess;
@@ -191,6 +203,10 @@ void WindowTables::launch(hipStream_t stream) {
         case Kind::f64:
             e = hip::launch_window64_table(reinterpret_cast<const hip::Win64Job *>(at), count,
                                            g.items, g.band, g.width, g.rows, g.kind, stream);
+            break;
+        case Kind::op:
+            e = hip::launch_operator_table(reinterpret_cast<const hip::OpJob *>(at), count, g.items,
+                                           g.width, g.kind, stream);
             break;
         }
         check(e, "table launch failed");
@@ -222,7 +238,7 @@ HipBatch::HipBatch(HipExecutor &executor, std::vector<BatchJob> jobs, bool graph
             throw std::invalid_argument("dsmvc: invalid batch job");
     }
     check(hipSetDevice(executor_.device()), "hipSetDevice failed");
-    const bool parallel = executor_.mode() == SolveMode::automatic
+    const bool parallel = automatic_like(executor_.mode())
                           || executor_.mode() == SolveMode::windowed;
     // Intermediate layout of the table jobs: [frame][source row][pitch(n_h)].
     std::vector<std::size_t> offset(jobs_.size(), 0);

@@ -72,6 +72,9 @@ public:
     void add(const hip::WinJob &job, int band, bool rows, hip::Sample kind);
     void add64(const hip::Win64Job &job, int band, bool rows, hip::Sample kind);
     void add_check(const hip::CheckJob &job);
+    // A fused 2D job (kernels_operator.hip) whose operator images were
+    // packed at `steps` K steps; `kind`: its input and output samples.
+    void add_op(const hip::OpJob &job, int steps, hip::Sample kind);
     // Uploads the tables (stream-ordered when `stream` is given, else
     // synchronously) and returns the number of launches `launch` makes.
     int upload(hipStream_t stream);
@@ -82,7 +85,7 @@ public:
     [[nodiscard]] std::int64_t items() const;
 
 private:
-    enum class Kind : std::uint8_t { check = 0, f32 = 1, f64 = 2 };
+    enum class Kind : std::uint8_t { check = 0, f32 = 1, f64 = 2, op = 3 };
     struct Group {
         Kind type = Kind::f32;
         int band = 0, width = 0;
@@ -92,12 +95,15 @@ private:
         std::vector<hip::WinJob> jobs;
         std::vector<hip::Win64Job> jobs64;
         std::vector<hip::CheckJob> checks;
+        std::vector<hip::OpJob> ops;
         std::size_t offset = 0;     // byte offset of the first record in the buffers
         long items = 0;
-        [[nodiscard]] std::size_t count() const { return jobs.size() + jobs64.size() + checks.size(); }
+        [[nodiscard]] std::size_t count() const {
+            return jobs.size() + jobs64.size() + checks.size() + ops.size();
+        }
         [[nodiscard]] std::size_t bytes() const {
             return jobs.size() * sizeof(hip::WinJob) + jobs64.size() * sizeof(hip::Win64Job)
-                   + checks.size() * sizeof(hip::CheckJob);
+                   + checks.size() * sizeof(hip::CheckJob) + ops.size() * sizeof(hip::OpJob);
         }
     };
     struct Slot {

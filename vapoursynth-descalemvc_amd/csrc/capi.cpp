@@ -18,6 +18,7 @@
 #include "engine.hpp"
 #include "batch.hpp"
 #include "device_plan.hpp"
+#include "operator_plan.hpp"
 #include "executor.hpp"
 #include "server.hpp"
 
@@ -313,6 +314,31 @@ int dsmvc_hip_plan_window_info(const dsmvc_hip_plan *plan, dsmvc_hip_window_info
     });
 }
 
+int dsmvc_hip_plan_operator(const dsmvc_hip_plan *plan, dsmvc_hip_operator_info *info, int32_t *lo,
+                            int32_t *count, double *values, int64_t capacity) {
+    return guarded([&] {
+        const auto &p = plan_of(plan);
+        require(info, "dsmvc: null argument");
+        dsmvc::hip::OperatorPlan op;
+        dsmvc::hip::build_operator_plan(p, op);
+        *info = dsmvc_hip_operator_info{};
+        info->rows_feasible = op.h_feasible ? 1 : 0;
+        info->columns_feasible = op.v_feasible ? 1 : 0;
+        info->blocks = op.blocks;
+        info->steps = op.steps;
+        for (int c : op.count) info->max_band = std::max(info->max_band, c);
+        info->values = static_cast<int64_t>(op.values.size());
+        info->dropped = op.dropped;
+        info->max_abs = op.max_abs;
+        if (lo) std::memcpy(lo, op.lo.data(), op.lo.size() * sizeof(int32_t));
+        if (count) std::memcpy(count, op.count.data(), op.count.size() * sizeof(int32_t));
+        if (values) {
+            require(capacity >= static_cast<int64_t>(op.values.size()), "dsmvc: operator buffer too small");
+            std::memcpy(values, op.values.data(), op.values.size() * sizeof(double));
+        }
+    });
+}
+
 int dsmvc_hip_plan_arrays(const dsmvc_hip_plan *plan, uint32_t *offsets, int32_t *indices,
                           float *weights, float *lower_ld, float *upper_l,
                           float *inverse_diagonal, double *weights_f64,
@@ -389,7 +415,7 @@ int dsmvc_hip_device_count(int32_t *count) {
 
 int dsmvc_hip_executor_create(int32_t device, int32_t solve_mode, dsmvc_hip_executor **executor) {
     return guarded([&] {
-        require(executor && solve_mode >= 0 && solve_mode <= 4, "dsmvc: invalid executor arguments");
+        require(executor && solve_mode >= 0 && solve_mode <= 5, "dsmvc: invalid executor arguments");
         if (!dsmvc::hip_available()) throw Unavailable("dsmvc: backend hip is unavailable: no HIP device");
         auto handle = std::make_unique<dsmvc_hip_executor>();
         handle->executor = std::make_unique<dsmvc::HipExecutor>(

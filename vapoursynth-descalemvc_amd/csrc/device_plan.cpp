@@ -522,6 +522,8 @@ DevicePlan::~DevicePlan() {
         if (entry.second.memory) (void)hipFree(entry.second.memory);
     if (window_image_.memory) (void)hipFree(window_image_.memory);
     if (window64_image_.memory) (void)hipFree(window64_image_.memory);
+    for (auto &entry : op_images_)
+        if (entry.second.memory) (void)hipFree(entry.second.memory);
     if (solve_image_.memory) (void)hipFree(solve_image_.memory);
     if (refine_memory_) (void)hipFree(refine_memory_);
     if (memory_) (void)hipFree(memory_);
@@ -722,6 +724,58 @@ const DevWindow64 *DevicePlan::window64() {
         image.built = true;
     }
     return image.ok ? &image.dev : nullptr;
+}
+
+const OperatorPlan *DevicePlan::operator_plan_locked() {
+    if (!op_built_) {
+        const trace::Range range(trace::Label::operator_build);
+        auto op = std::make_unique<OperatorPlan>();
+        if (build_operator_plan(*plan_, *op)) op_plan_ = std::move(op);
+        op_built_ = true;
+    }
+    return op_plan_.get();
+}
+
+const OperatorPlan *DevicePlan::operator_plan() {
+    std::lock_guard lock(mutex_);
+    return operator_plan_locked();
+}
+
+const DevOperator *DevicePlan::op(bool rows, bool bf16, int steps) {
+    std::lock_guard lock(mutex_);
+    const OperatorPlan *host = operator_plan_locked();
+    if (!host || !operator_fits(*host, rows, steps)) return nullptr;
+    OperatorImage &image = op_images_[(bf16 ? 8 : 0) + steps];
+    if (!image.ok) {
+        const std::vector<std::uint32_t> data = pack_operator(*host, bf16, steps);
+        const std::size_t o_data = 0;
+        const std::size_t o_k = align256(data.size() * 4);
+        const std::size_t total = align256(o_k + host->kstart.size() * 4);
+        std::vector<unsigned char> staging(total, 0);
+        std::memcpy(staging.data() + o_data, data.data(), data.size() * 4);
+        std::memcpy(staging.data() + o_k, host->kstart.data(), host->kstart.size() * 4);
+        int current = 0;
+        check(hipGetDevice(&current), "hipGetDevice failed");
+        check(hipSetDevice(device_), "hipSetDevice failed");
+        void *memory = nullptr;
+        check(hipMalloc(&memory, total), "operator image allocation failed");
+        if (const hipError_t e = upload(memory, staging.data(), total); e != hipSuccess) {
+            (void)hipFree(memory);
+            check(e, "operator image upload failed");
+        }
+        (void)hipSetDevice(current);
+        grow(total);
+        auto *b = static_cast<unsigned char *>(memory);
+        image.memory = memory;
+        image.dev.data = reinterpret_cast<const std::uint32_t *>(b + o_data);
+        image.dev.kstart = reinterpret_cast<const std::int32_t *>(b + o_k);
+        image.dev.n = host->n;
+        image.dev.m = host->m;
+        image.dev.blocks = host->blocks;
+        image.dev.steps = steps;
+        image.ok = true;
+    }
+    return &image.dev;
 }
 
 const DevRefine *DevicePlan::refine() {

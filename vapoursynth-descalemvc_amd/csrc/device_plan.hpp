@@ -12,6 +12,7 @@
 
 #include "device_types.hpp"
 #include "engine.hpp"
+#include "operator_plan.hpp"
 
 namespace dsmvc::hip {
 
@@ -123,6 +124,13 @@ public:
     // pairs, and tap-major FP64 copies); nullptr for a plan without FP64
     // data.  Built lazily, cached.
     const DevRefine *refine();
+    // The plan's explicit operator for the fused 2D route, on the host;
+    // nullptr when no direction can take the route.  Built lazily, cached.
+    const OperatorPlan *operator_plan();
+    // Its MFMA operand image packed at `steps` K steps, FP16 (bf16 = false)
+    // or bfloat16 parts; nullptr when the plan cannot take the route in the
+    // given direction at that step count.  Built lazily, cached.
+    const DevOperator *op(bool rows, bool bf16, int steps);
     [[nodiscard]] std::size_t device_bytes() const { return bytes_; }
     // Device bytes of the image and of every table built for it so far.
     [[nodiscard]] std::size_t total_bytes() const { return total_bytes_.load(std::memory_order_relaxed); }
@@ -166,6 +174,15 @@ private:
         bool ok = false;
     };
     Window64Image window64_image_;
+    struct OperatorImage {
+        DevOperator dev{};
+        void *memory = nullptr;
+        bool ok = false;
+    };
+    std::unique_ptr<OperatorPlan> op_plan_;
+    bool op_built_ = false;
+    std::map<int, OperatorImage> op_images_;   // key: bf16 * 8 + steps
+    const OperatorPlan *operator_plan_locked();
     DevRefine refine_{};
     void *refine_memory_ = nullptr;
     std::size_t refine_bytes_ = 0;

@@ -123,6 +123,50 @@ struct DevConv {
 
 enum class Sample : std::int32_t { f32 = 0, u8 = 1, u16 = 2, f64 = 3 };
 
+// ---- fused 2D route (explicit operators on MFMA) ----------------------------
+// The operator of an axis, M = G^-1 A^T truncated to its band, in blocks of
+// kOpBlock destination indices; block b covers source indices
+// [kstart[b], kstart[b] + steps * kOpStep).  A workgroup owns kOpTileCols
+// destination columns (kOpTileBlocks horizontal blocks) and a run of units of
+// kOpUnit destination rows (two vertical blocks); it stages kOpSlab source
+// rows at a time (at most kOpStageCols columns), multiplies them by the
+// horizontal operator, keeps the results in a ring of kOpRing source rows in
+// LDS and multiplies the ring by the vertical operator.
+constexpr int kOpBlock = 16;
+constexpr int kOpStep = 32;
+constexpr int kOpMaxSteps = 4;
+constexpr int kOpTileBlocks = 4;
+constexpr int kOpTileCols = kOpBlock * kOpTileBlocks;
+constexpr int kOpUnit = 2 * kOpBlock;
+constexpr int kOpSlab = 32;
+constexpr int kOpStageCols = 208;
+constexpr int kOpRing = 192;
+
+struct DevOperator {
+    const std::uint32_t *data;   // [blocks][steps][parts][64][4] 16-bit operand pairs
+    const std::int32_t *kstart;  // [blocks]
+    std::int32_t n, m, blocks, steps;
+};
+
+struct OpJob {
+    DevOperator h, v;
+    const void *src;
+    void *dst;
+    long s_row, s_frame, d_row, d_frame;
+    std::int32_t frames;
+    std::int32_t tiles;           // ceil(n_h / kOpTileCols)
+    std::int32_t units;           // ceil(n_v / kOpUnit)
+    std::int32_t units_per_item;  // units of one workgroup
+    std::int32_t segments;        // ceil(units / units_per_item)
+    std::int32_t vector_loads;    // source rows and base allow 16-byte loads
+    long first_item;
+    DevConv conv;
+};
+
+constexpr long operator_items(const OpJob &j) {
+    return static_cast<long>(j.frames) * j.tiles * j.segments;
+}
+
 // One axis pass of one plan over `frames` planes, as a unit of a
 // (possibly multi-plan) windowed launch.  Items are (frame, segment, group
 // of 64 vectors), the group varying fastest; a segment is `span` consecutive

@@ -119,6 +119,7 @@ HipExecutor::HipExecutor(int device, SolveMode mode) : device_(device), mode_(mo
     if (hipGetDeviceProperties(&prop, device_) == hipSuccess && prop.multiProcessorCount > 0)
         compute_units_ = prop.multiProcessorCount;
     if (const char *env = std::getenv("DSMVC_HIP_AUTOTUNE")) autotune_ = std::string(env) != "0";
+    if (const char *env = std::getenv("DSMVC_HIP_FUSED")) fused_auto_ = std::string(env) != "0";
     if (const char *env = std::getenv("DSMVC_HIP_PLAN_CACHE_MB")) {
         const long mb = std::strtol(env, nullptr, 10);
         if (mb > 0) plan_cache_limit_ = static_cast<std::size_t>(mb) << 20;
@@ -169,7 +170,7 @@ void HipExecutor::prepare(const std::shared_ptr<const AxisPlan> &plan) {
     hip::DevicePlan &dp = *image;
     // The windowed tables too, so that a prepared plan runs without any
     // further device allocation (inverse_2d_batch_noalloc relies on this).
-    if (mode_ == SolveMode::automatic || mode_ == SolveMode::windowed) {
+    if (automatic_like(mode_) || mode_ == SolveMode::windowed) {
         (void)dp.window(false);
         if (plan->requires_float64()) (void)dp.window64();
     }
@@ -622,7 +623,7 @@ void HipExecutor::solve_pass(const AxisPlan &plan, bool rows, const void *input,
     // kernels when the plan's FP64 recurrence decays within their warm-up.
     const bool f64_pair = rows ? in_kind != Sample::f64 && out_kind == Sample::f64
                                : in_kind == Sample::f64 && out_kind != Sample::f64;
-    if (f64 && f64_pair && (mode_ == SolveMode::automatic || mode_ == SolveMode::windowed)) {
+    if (f64 && f64_pair && (automatic_like(mode_) || mode_ == SolveMode::windowed)) {
         if (const hip::DevWindow64 *win = dp.window64()) {
             hip::Win64Args a{};
             a.plan = *win;
@@ -782,18 +783,27 @@ void HipExecutor::inverse_2d_planes(const PlaneJob *planes, std::int32_t count, 
 // scratch entry whose flag a Float64 plane armed.
 HipExecutor::Scratch &HipExecutor::enqueue_planes(const PlaneJob *planes, std::int32_t count,
                                                   SampleKind kind, hipStream_t stream) {
-    const bool windows = mode_ == SolveMode::automatic || mode_ == SolveMode::windowed;
+    const bool windows = automatic_like(mode_) || mode_ == SolveMode::windowed;
     // The device images stay pinned until every launch of this call is queued.
     std::vector<std::shared_ptr<hip::DevicePlan>> pins;
     const auto n = static_cast<std::size_t>(count);
     std::vector<const hip::DevWindow *> wh(n, nullptr), wv(n, nullptr);
     std::vector<const hip::DevWindow64 *> wh64(n, nullptr), wv64(n, nullptr);
     std::vector<std::size_t> offset(n, 0);
+    std::vector<char> fused(n, 0);
     std::size_t total = 0;
+    WindowTables &tables = stream_tables(stream);
+    tables.clear();
+    const bool try_fused = fused_route(kind);
     for (std::size_t k = 0; k < n; ++k) {
         const AxisPlan &h = *planes[k].horizontal, &v = *planes[k].vertical;
-        if (!windows) continue;
         const bool f64 = float64_route(h) || float64_route(v);
+        if (try_fused && !f64
+            && add_fused(h, v, kind, planes[k].batch, planes[k].conversion, tables, pins)) {
+            fused[k] = 1;
+            continue;
+        }
+        if (!windows) continue;
         // Held by value: a reference into `pins` would dangle once it grows.
         const std::shared_ptr<hip::DevicePlan> ih = device_plan(h), iv = device_plan(v);
         pins.push_back(ih);
@@ -822,11 +832,10 @@ HipExecutor::Scratch &HipExecutor::enqueue_planes(const PlaneJob *planes, std::i
                           * static_cast<std::size_t>(planes[k].batch.frames) * elem);
     }
     Scratch *last = nullptr;
-    if (total) {
+    const bool any_fused = std::find(fused.begin(), fused.end(), 1) != fused.end();
+    if (total || any_fused) {
         Scratch &s = scratch(stream, total);
         last = &s;
-        WindowTables &tables = stream_tables(stream);
-        tables.clear();
         const Sample sk = sample_of(kind);
         bool armed = false;
         for (std::size_t k = 0; k < n; ++k) {
@@ -914,7 +923,7 @@ HipExecutor::Scratch &HipExecutor::enqueue_planes(const PlaneJob *planes, std::i
         tables.launch(stream);
     }
     for (std::size_t k = 0; k < n; ++k) {
-        if (wh[k] || wh64[k]) continue;
+        if (wh[k] || wh64[k] || fused[k]) continue;
         const PlaneJob &p = planes[k];
         last = &solve_2d(*p.horizontal, *p.vertical, kind, p.batch, p.conversion, stream);
     }
@@ -951,11 +960,70 @@ void HipExecutor::inverse_2d_batch(const AxisPlan &h, const AxisPlan &v, SampleK
              [&](hipStream_t st) -> Scratch & { return solve_2d(h, v, kind, b, conversion, st); });
 }
 
+bool HipExecutor::add_fused(const AxisPlan &h, const AxisPlan &v, SampleKind kind, const PlaneBatch &b,
+                            const IntegerConversion *conversion, WindowTables &tables,
+                            std::vector<std::shared_ptr<hip::DevicePlan>> &pins) {
+    const std::shared_ptr<hip::DevicePlan> ih = device_plan(h), iv = device_plan(v);
+    const hip::OperatorPlan *ph = ih->operator_plan(), *pv = iv->operator_plan();
+    if (!ph || !pv) return false;
+    // The kernel addresses one frame's plane with 32-bit byte offsets.
+    const std::size_t bytes = kind == SampleKind::float32 ? 4 : (kind == SampleKind::u16 ? 2 : 1);
+    const auto span = [&](std::ptrdiff_t row, std::int32_t rows, std::int32_t cols) {
+        return (static_cast<std::size_t>(rows - 1) * static_cast<std::size_t>(row) + cols) * bytes;
+    };
+    if (span(b.input_row_stride, v.source_size, h.source_size) >= 0x7FFFFFF0U
+        || span(b.output_row_stride, v.destination_size, h.destination_size) >= 0x7FFFFFF0U)
+        return false;
+    const int steps = std::max({ph->steps, pv->steps, 2});
+    const bool bf16 = kind == SampleKind::float32;
+    const hip::DevOperator *oh = ih->op(true, bf16, steps), *ov = iv->op(false, bf16, steps);
+    if (!oh || !ov) return false;
+    pins.push_back(ih);
+    pins.push_back(iv);
+    hip::OpJob j{};
+    j.h = *oh;
+    j.v = *ov;
+    j.src = b.input;
+    j.dst = b.output;
+    j.s_row = b.input_row_stride;
+    j.s_frame = b.input_frame_stride;
+    j.d_row = b.output_row_stride;
+    j.d_frame = b.output_frame_stride;
+    j.frames = b.frames;
+    j.tiles = (h.destination_size + hip::kOpTileCols - 1) / hip::kOpTileCols;
+    j.units = (v.destination_size + hip::kOpUnit - 1) / hip::kOpUnit;
+    // Whole columns per workgroup unless that leaves the device too few
+    // workgroups; each split repeats about one unit's source rows.
+    const long per_column = static_cast<long>(j.frames) * j.tiles;
+    int upi = j.units;
+    while (upi > 4 && per_column * ((j.units + upi - 1) / upi) < 4L * compute_units_) upi = (upi + 1) / 2;
+    j.units_per_item = upi;
+    j.segments = (j.units + upi - 1) / upi;
+    const std::size_t align = kind == SampleKind::u8 ? 8 : 16;
+    const bool aligned = reinterpret_cast<std::uintptr_t>(b.input) % align == 0
+                         && (static_cast<std::size_t>(b.input_row_stride) * bytes) % align == 0
+                         && (b.frames == 1 || (static_cast<std::size_t>(b.input_frame_stride) * bytes) % align == 0);
+    j.vector_loads = aligned ? 1 : 0;
+    j.conv = make_conv(conversion);
+    tables.add_op(j, steps, sample_of(kind));
+    return true;
+}
+
 // Queues one plane batch's two passes on `stream` (caller holds the lock).
 HipExecutor::Scratch &HipExecutor::solve_2d(const AxisPlan &h, const AxisPlan &v, SampleKind kind,
                                             const PlaneBatch &b, const IntegerConversion *conversion,
                                             hipStream_t stream) {
     const bool f64 = float64_route(h) || float64_route(v);
+    if (!f64 && fused_route(kind)) {
+        std::vector<std::shared_ptr<hip::DevicePlan>> pins;
+        WindowTables &tables = stream_tables(stream);
+        tables.clear();
+        if (add_fused(h, v, kind, b, conversion, tables, pins)) {
+            tables.upload(stream);
+            tables.launch(stream);
+            return scratch(stream, 0);
+        }
+    }
     const std::size_t elem = f64 ? 8 : 4;
     const std::ptrdiff_t inter_row = padded_width(h.destination_size);
     const std::ptrdiff_t inter_frame = inter_row * v.source_size;

@@ -37,7 +37,12 @@ enum class SolveMode : std::uint8_t {
     windowed = 3,    // windowed kernels only; throws when the plan cannot use them
     refined = 4,     // as automatic, but plans that retain FP64 data take the F32 route
                      // plus refinement steps instead of the Float64 route
+    fused = 5,       // as automatic, and 2D calls take the fused route wherever both plans
+                     // fit it (automatic does too for u8/u16 planes unless DSMVC_HIP_FUSED=0)
 };
+
+// Modes whose solves choose among the parallel kernels by themselves.
+constexpr bool automatic_like(SolveMode m) { return m == SolveMode::automatic || m == SolveMode::fused; }
 
 enum class SampleKind : std::uint8_t { float32 = 0, u8 = 1, u16 = 2 };
 
@@ -303,6 +308,19 @@ private:
                       const IntegerConversion *conversion, hipStream_t stream);
     Scratch &enqueue_planes(const PlaneJob *planes, std::int32_t count, SampleKind kind,
                             hipStream_t stream);
+    // True when this executor's 2D calls of `kind` take the fused route where
+    // it fits: always in fused mode; in automatic mode for integer samples,
+    // where it outruns the two passes (F32 planes run its three-part
+    // bfloat16 scheme, slower than the two passes there).
+    bool fused_route(SampleKind kind) const {
+        return mode_ == SolveMode::fused
+               || (mode_ == SolveMode::automatic && fused_auto_ && kind != SampleKind::float32);
+    }
+    // Adds one plane batch to `tables` as a fused 2D job when both plans fit
+    // the fused kernel (and pins their images); false otherwise.
+    bool add_fused(const AxisPlan &h, const AxisPlan &v, SampleKind kind, const PlaneBatch &b,
+                   const IntegerConversion *conversion, WindowTables &tables,
+                   std::vector<std::shared_ptr<hip::DevicePlan>> &pins);
     void arm_flag(Scratch &s, hipStream_t stream);
     WindowTables &stream_tables(hipStream_t stream);
     void axis_pass(const AxisPlan &plan, bool rows, const void *input, hip::Sample in_kind,
@@ -336,6 +354,7 @@ private:
     SolveMode mode_;
     LaunchTuning tuning_{};
     bool autotune_ = true;
+    bool fused_auto_ = true;
     int refine_steps_ = 1;
     int compute_units_ = 256;
     std::unordered_map<TuneKey, int, TuneKeyHash> tuned_;

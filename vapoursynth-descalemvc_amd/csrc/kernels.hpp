@@ -159,4 +159,9 @@ hipError_t launch_refine_add(const RefineArgs &a, const float *d, bool rows, hip
 hipError_t launch_check_finite(const void *src, Sample in, long s_row, long s_frame, int rows,
                                int cols, int frames, int *flag, hipStream_t s);
 
+// Fused 2D route (kernels_operator.hip): one launch over `count` jobs of one
+// sample kind whose operator images were packed at `steps` K steps (2..4).
+hipError_t launch_operator_table(const OpJob *table, int count, long items, int steps, Sample kind,
+                                 hipStream_t stream);
+
 } // namespace dsmvc::hip

@@ -1,0 +1,490 @@
+// Fused 2D route for gfx950: X = M_v Y M_h^T on MFMA, with M = G^-1 A^T the
+// explicit operator of each axis (operator_plan.hpp) and no horizontal result
+// in memory.
+//
+// A workgroup (four wavefronts) owns kOpTileCols destination columns of one
+// frame and a run of units of kOpUnit destination rows.  It walks down the
+// source rows in slabs of kOpSlab:
+//   1. the slab's source columns [kstart_h[first block], ... ) are staged in
+//      LDS, already split into 16-bit parts;
+//   2. wavefront w multiplies the slab by horizontal block w of the tile
+//      (16x16x32 MFMA tiles, the operator held in registers) and writes the
+//      F32 results, split again, into a ring of kOpRing source rows x 64
+//      columns in LDS;
+//   3. once the ring holds every source row a unit's two vertical blocks
+//      read, wavefront w multiplies the vertical operator by the ring's
+//      columns of block w and stores the unit's outputs.
+//
+// Precision.  Each operand is a sum of 16-bit parts and each product the sum
+// of the part products whose order is at most the operand's own rounding:
+//   - integer samples (u8/u16, values y): y = 64 (y >> 6) + (y & 63) exactly
+//     as the FP16 parts (y >> 6) and (y & 63) / 64; operators and the F32
+//     ring values as FP16 + FP16 remainder; three products per term;
+//   - F32 samples: every F32 value split exactly into three bfloat16 parts
+//     by truncation; operators as three rounded bfloat16 parts; six products.
+// MFMA products of 16-bit parts are exact in F32 and accumulate in F32.  The
+// integer route applies the input offset and scale to the result (the
+// operator maps constants to themselves), then the output conversion.
+#include <type_traits>
+
+#include "kernels_common.hpp"
+
+namespace dsmvc::hip {
+namespace dev {
+namespace {
+
+using u32x4 = std::uint32_t __attribute__((ext_vector_type(4)));
+using u32x2 = std::uint32_t __attribute__((ext_vector_type(2)));
+using h8 = _Float16 __attribute__((ext_vector_type(8)));
+using h2 = _Float16 __attribute__((ext_vector_type(2)));
+using bf8 = __bf16 __attribute__((ext_vector_type(8)));
+using f32x4 = float __attribute__((ext_vector_type(4)));
+
+constexpr int kThreads = 256;
+constexpr int kYPitch = kOpStageCols + 8;      // 16-bit values per staged source row
+constexpr int kRPitch = kOpRing + 8;           // 16-bit values per ring column
+constexpr int kChunksPerRow = kOpStageCols / 8;
+constexpr int kChunks = (kOpSlab * kChunksPerRow + kThreads - 1) / kThreads;
+// Pitches of 4 x odd dwords: 16 lanes reading 16 bytes each from 16 rows (or
+// ring columns) hit 16 distinct groups of 4 banks.
+static_assert((kYPitch / 2) % 8 == 4 && (kRPitch / 2) % 8 == 4, "conflict-free LDS pitches");
+static_assert(kOpRing % 8 == 0 && kOpSlab % 8 == 0, "ring rows in groups of 8");
+
+template <bool Bf>
+__device__ __forceinline__ f32x4 mma(u32x4 a, u32x4 b, f32x4 c) {
+    if constexpr (Bf)
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a),
+                                                       __builtin_bit_cast(bf8, b), c, 0, 0, 0);
+    else
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h8, a),
+                                                      __builtin_bit_cast(h8, b), c, 0, 0, 0);
+}
+
+// c + sum of a[i] b[j] over the kept part pairs, smallest first.
+template <bool Bf>
+__device__ __forceinline__ f32x4 products(const u32x4 *a, const u32x4 *b, f32x4 c) {
+    if constexpr (Bf) {
+        c = mma<true>(a[2], b[0], c);
+        c = mma<true>(a[1], b[1], c);
+        c = mma<true>(a[0], b[2], c);
+        c = mma<true>(a[1], b[0], c);
+        c = mma<true>(a[0], b[1], c);
+        c = mma<true>(a[0], b[0], c);
+    } else {
+        c = mma<false>(a[1], b[0], c);
+        c = mma<false>(a[0], b[1], c);
+        c = mma<false>(a[0], b[0], c);
+    }
+    return c;
+}
+
+__device__ __forceinline__ std::uint32_t fbits(float x) { return __builtin_bit_cast(std::uint32_t, x); }
+__device__ __forceinline__ float bitsf(std::uint32_t u) { return __builtin_bit_cast(float, u); }
+
+// Two u16 samples (low and high half of d) -> FP16 pairs of (y >> 6) and
+// (y & 63) / 64, exactly: the integer fields are placed in the mantissa of
+// 1024.0 and 1.0 and the bias subtracted.
+__device__ __forceinline__ void split_u16(std::uint32_t d, std::uint32_t &p0, std::uint32_t &p1) {
+    const std::uint32_t hb = ((d >> 6) & 0x03FF03FFU) | 0x64006400U;
+    const std::uint32_t lb = ((d << 4) & 0x03F003F0U) | 0x3C003C00U;
+    const h2 hv = __builtin_bit_cast(h2, hb) - h2{static_cast<_Float16>(1024.0F), static_cast<_Float16>(1024.0F)};
+    const h2 lv = __builtin_bit_cast(h2, lb) - h2{static_cast<_Float16>(1.0F), static_cast<_Float16>(1.0F)};
+    p0 = __builtin_bit_cast(std::uint32_t, hv);
+    p1 = __builtin_bit_cast(std::uint32_t, lv);
+}
+
+// An F32 value as three bfloat16 parts by truncation; exact (each remainder
+// has at most the bits the next part keeps).
+__device__ __forceinline__ void split_f32(float x, std::uint32_t &a, std::uint32_t &b, std::uint32_t &c) {
+    const std::uint32_t h = fbits(x) & 0xFFFF0000U;
+    const float r1 = __fsub_rn(x, bitsf(h));
+    const std::uint32_t m = fbits(r1) & 0xFFFF0000U;
+    const float r2 = __fsub_rn(r1, bitsf(m));
+    a = h >> 16;
+    b = m >> 16;
+    c = fbits(r2) >> 16;
+}
+
+// An F32 value as FP16 + FP16 remainder (round to nearest).
+__device__ __forceinline__ void split_half(float x, std::uint32_t &a, std::uint32_t &b) {
+    const _Float16 h = static_cast<_Float16>(x);
+    const _Float16 r = static_cast<_Float16>(__fsub_rn(x, static_cast<float>(h)));
+    a = __builtin_bit_cast(std::uint16_t, h);
+    b = __builtin_bit_cast(std::uint16_t, r);
+}
+
+// Staged samples of one chunk (8 consecutive source columns of one row).
+template <class In> struct Chunk;
+template <> struct Chunk<std::uint16_t> { u32x4 v; };
+template <> struct Chunk<std::uint8_t> { u32x2 v; };
+template <> struct Chunk<float> { u32x4 v[2]; };
+
+// Byte offset that every buffer load treats as out of range (reads 0).
+constexpr int kOutside = 0x7FFFFFF0;
+
+// One chunk from the plane's buffer resource at byte offset `off` (kOutside
+// for a row past the plane), columns c .. c + 7 of which those at or past
+// `cols` read as 0.  `vector`: the offset is aligned for one wide load;
+// otherwise one load per sample.  Buffer loads never touch the LDS counter,
+// so the chunk stays in flight across the wavefront's LDS work.
+template <class In>
+__device__ __forceinline__ void fetch_chunk(Chunk<In> &k, __amdgpu_buffer_rsrc_t rs, int off, int c, int cols,
+                                            bool vector) {
+    if constexpr (std::is_same_v<In, std::uint16_t>) {
+        if (vector) {
+            k.v = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const std::uint32_t a = __builtin_amdgcn_raw_buffer_load_b16(rs, off + 4 * q, 0, 0);
+                const std::uint32_t b = __builtin_amdgcn_raw_buffer_load_b16(rs, off + 4 * q + 2, 0, 0);
+                k.v[q] = a | (b << 16);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const std::uint32_t keep = (c + 2 * q < cols ? 0x0000FFFFU : 0U) | (c + 2 * q + 1 < cols ? 0xFFFF0000U : 0U);
+            k.v[q] &= keep;
+        }
+    } else if constexpr (std::is_same_v<In, std::uint8_t>) {
+        if (vector) {
+            k.v = __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0));
+        } else {
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                std::uint32_t w = 0;
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    w |= static_cast<std::uint32_t>(__builtin_amdgcn_raw_buffer_load_b8(rs, off + 4 * q + e, 0, 0)) << (8 * e);
+                k.v[q] = w;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            std::uint32_t keep = 0;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) keep |= (c + 4 * q + e < cols ? 0xFFU : 0U) << (8 * e);
+            k.v[q] &= keep;
+        }
+    } else {
+        if (vector) {
+            k.v[0] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
+            k.v[1] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rs, off + 16, 0, 0));
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                k.v[0][q] = __builtin_amdgcn_raw_buffer_load_b32(rs, off + 4 * q, 0, 0);
+                k.v[1][q] = __builtin_amdgcn_raw_buffer_load_b32(rs, off + 16 + 4 * q, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (c + q >= cols) k.v[0][q] = 0U;
+            if (c + 4 + q >= cols) k.v[1][q] = 0U;
+        }
+    }
+}
+
+// Writes one chunk's parts to the staged rows: part p at
+// ylds[p][row][col .. col + 8).
+template <class In>
+__device__ __forceinline__ void stage_chunk(const Chunk<In> &k, std::uint16_t *ylds, int row, int col) {
+    constexpr int plane = kOpSlab * kYPitch;
+    u32x4 p0, p1;
+    if constexpr (std::is_same_v<In, float>) {
+        u32x4 p2;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            std::uint32_t a0, b0, c0, a1, b1, c1;
+            const float x0 = bitsf(q < 2 ? k.v[0][2 * q] : k.v[1][2 * q - 4]);
+            const float x1 = bitsf(q < 2 ? k.v[0][2 * q + 1] : k.v[1][2 * q - 3]);
+            split_f32(x0, a0, b0, c0);
+            split_f32(x1, a1, b1, c1);
+            p0[q] = a0 | (a1 << 16);
+            p1[q] = b0 | (b1 << 16);
+            p2[q] = c0 | (c1 << 16);
+        }
+        *reinterpret_cast<u32x4 *>(ylds + 2 * plane + row * kYPitch + col) = p2;
+    } else {
+        u32x4 d;
+        if constexpr (std::is_same_v<In, std::uint16_t>) {
+            d = k.v;
+        } else {
+            // Bytes to u16 pairs.
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                d[2 * q] = (k.v[q] & 0xFFU) | ((k.v[q] & 0xFF00U) << 8);
+                d[2 * q + 1] = ((k.v[q] >> 16) & 0xFFU) | ((k.v[q] >> 8) & 0x00FF0000U);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            std::uint32_t a, b;
+            split_u16(d[q], a, b);
+            p0[q] = a;
+            p1[q] = b;
+        }
+    }
+    *reinterpret_cast<u32x4 *>(ylds + row * kYPitch + col) = p0;
+    *reinterpret_cast<u32x4 *>(ylds + plane + row * kYPitch + col) = p1;
+}
+
+// Workgroup b runs on XCD b % 8 (each XCD has its own L2), as the b / 8-th
+// workgroup there.  Every job's items are split into 8 contiguous shares and
+// share x goes to XCD x, one job after another: each XCD then holds the same
+// mix of long (luma) and short (chroma) items, and neighbouring tiles, whose
+// source columns overlap, share an L2.  The launch has 8 * (ceil(items / 8)
+// + jobs) workgroups, enough for the largest share sum; workgroups past
+// their XCD's shares return.  Returns the job, or -1.
+constexpr long kXcds = 8;
+__device__ __forceinline__ int balanced_item(const OpJob *table, int count, long b, long &item) {
+    const long x = b % kXcds, slot = b / kXcds;
+    long before = 0;
+    for (int j = 0; j < count; ++j) {
+        const long n = operator_items(table[j]);
+        const long lo = n * x / kXcds, share = n * (x + 1) / kXcds - lo;
+        if (slot < before + share) {
+            item = table[j].first_item + lo + (slot - before);
+            return j;
+        }
+        before += share;
+    }
+    return -1;
+}
+
+// Output sample from the result x (normalised for integer samples):
+// clamp(x * scale + offset, 0, max) rounded to nearest even, as the
+// library's other integer stores.
+template <class Out> __device__ __forceinline__ void store_sample(__amdgpu_buffer_rsrc_t rs, int off, float x,
+                                                                const DevConv &c) {
+    if constexpr (std::is_same_v<Out, float>) {
+        __builtin_amdgcn_raw_buffer_store_b32(fbits(x), rs, off, 0, 0);
+    } else {
+        float v = __fadd_rn(__fmul_rn(x, c.out_scale), c.out_offset);
+        v = v < 0.0F ? 0.0F : (c.out_max < v ? c.out_max : v);
+        const auto q = static_cast<std::uint32_t>(rintf(v));
+        if constexpr (std::is_same_v<Out, std::uint16_t>)
+            __builtin_amdgcn_raw_buffer_store_b16(static_cast<std::uint16_t>(q), rs, off, 0, 0);
+        else
+            __builtin_amdgcn_raw_buffer_store_b8(static_cast<std::uint8_t>(q), rs, off, 0, 0);
+    }
+}
+
+template <class In, int S>
+__global__ __launch_bounds__(kThreads)
+__attribute__((amdgpu_waves_per_eu(std::is_same_v<In, float> ? 1 : 2)))
+void operator2d(const OpJob *__restrict__ table, int count, long items) {
+    constexpr bool Bf = std::is_same_v<In, float>;
+    constexpr int NP = Bf ? 3 : 2;
+    constexpr int kSize = static_cast<int>(sizeof(In));
+    __shared__ __attribute__((aligned(16))) std::uint16_t ylds[NP * kOpSlab * kYPitch];
+    __shared__ __attribute__((aligned(16))) std::uint16_t ring[NP * kOpTileCols * kRPitch];
+
+    (void)items;
+    long item = 0;
+    const int job = balanced_item(table, count, blockIdx.x, item);
+    if (job < 0) return;
+    // The job's fields, read once (scalar registers).
+    const OpJob jb = table[job];
+    const long local = item - jb.first_item;
+    const int tile = static_cast<int>(local % jb.tiles);
+    const long rest = local / jb.tiles;
+    const int seg = static_cast<int>(rest % jb.segments);
+    const long frame = rest / jb.segments;
+    const int u0 = seg * jb.units_per_item;
+    const int u1 = min(jb.units, u0 + jb.units_per_item);
+
+    const int tid = threadIdx.x, l = tid & 63, lr = l & 15, lk = l >> 4;
+    const int w = uniform(tid >> 6);
+    const ConstPtr<std::int32_t> kh = constant(jb.h.kstart), kv = constant(jb.v.kstart);
+    const int cs0 = kh[4 * tile];
+    const int koff = kh[4 * tile + w] - cs0;
+    const int wc = kh[4 * tile + 3] - cs0 + kOpStep * S;
+    const int mh = jb.h.m, mv = jb.v.m, nh = jb.h.n, nv = jb.v.n;
+
+    // Horizontal block 4 * tile + w, in registers for the whole item.
+    u32x4 bh[S][NP];
+    {
+        const GlobalPtr<u32x4> d = global(reinterpret_cast<const u32x4 *>(jb.h.data))
+                                   + static_cast<long>(4 * tile + w) * S * NP * 64 + l;
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+#pragma unroll
+            for (int p = 0; p < NP; ++p) bh[s][p] = d[(s * NP + p) * 64];
+    }
+
+    // This thread's staging chunks (fixed for the item): slab row and column.
+    int crow[kChunks], ccol[kChunks];
+    bool cact[kChunks];
+#pragma unroll
+    for (int i = 0; i < kChunks; ++i) {
+        const int q = tid + kThreads * i;
+        crow[i] = q / kChunksPerRow;
+        ccol[i] = 8 * (q % kChunksPerRow);
+        cact[i] = crow[i] < kOpSlab && ccol[i] < wc;
+    }
+    const auto rs = plane_rsrc(static_cast<const In *>(jb.src) + frame * jb.s_frame,
+                               (static_cast<long>(mv - 1) * jb.s_row + mh) * kSize);
+    const bool vector = jb.vector_loads != 0;
+    const long s_row = jb.s_row;
+    auto fetch = [&](Chunk<In> (&buf)[kChunks], int r0) {
+#pragma unroll
+        for (int i = 0; i < kChunks; ++i) {
+            const int sr = r0 + crow[i], c = cs0 + ccol[i];
+            const int off = cact[i] && sr < mv ? static_cast<int>((sr * s_row + c) * kSize) : kOutside;
+            fetch_chunk<In>(buf[i], rs, off, c, mh, vector);
+        }
+    };
+
+    // Slabs from the first unit's first source row to the last unit's last.
+    const int first = kv[2 * u0];
+    const int slabs = (kv[2 * (u1 - 1) + 1] + kOpStep * S - first + kOpSlab - 1) / kOpSlab;
+    const int col0 = kOpTileCols * tile + kOpBlock * w + lr;
+    const auto ws = plane_rsrc(static_cast<In *>(jb.dst) + frame * jb.d_frame,
+                               (static_cast<long>(nv - 1) * jb.d_row + nh) * kSize);
+    const GlobalPtr<u32x4> vdata = global(reinterpret_cast<const u32x4 *>(jb.v.data)) + l;
+
+    // Unit u's two vertical blocks (loaded a unit ahead; they complete under the slabs).
+    u32x4 av[2][S][NP];
+    auto load_unit = [&](int u) {
+#pragma unroll
+        for (int vb = 0; vb < 2; ++vb)
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+#pragma unroll
+                for (int p = 0; p < NP; ++p)
+                    av[vb][s][p] = vdata[((static_cast<long>(2 * u + vb) * S + s) * NP + p) * 64];
+    };
+    // Stage-1 of slab k from its prefetched chunks, whose buffer then
+    // fetches slab k + 2.
+    auto slab = [&](Chunk<In> (&buf)[kChunks], int k) {
+        const int r0 = first + kOpSlab * k;
+        __syncthreads();   // every wavefront is done with the staged rows and the ring rows below
+#pragma unroll
+        for (int i = 0; i < kChunks; ++i)
+            if (cact[i]) stage_chunk<In>(buf[i], ylds, crow[i], ccol[i]);
+        if (k + 2 < slabs) fetch(buf, r0 + 2 * kOpSlab);
+        __syncthreads();
+        f32x4 acc[2] = {f32x4{0, 0, 0, 0}, f32x4{0, 0, 0, 0}};
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb) {
+                u32x4 a[NP];
+                const std::uint16_t *at = ylds + (kOpBlock * rb + lr) * kYPitch + koff + kOpStep * s + 8 * lk;
+#pragma unroll
+                for (int p = 0; p < NP; ++p)
+                    a[p] = *reinterpret_cast<const u32x4 *>(at + p * kOpSlab * kYPitch);
+                acc[rb] = products<Bf>(a, bh[s], acc[rb]);
+            }
+        // Results (source rows r0 + 16 rb + 4 lk + v, column kOpBlock * w + lr)
+        // into the ring, split into parts.
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb) {
+            const int rr = (r0 + kOpBlock * rb + 4 * lk) % kOpRing;
+            std::uint16_t *at = ring + (kOpBlock * w + lr) * kRPitch + rr;
+            std::uint32_t q[NP][4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                if constexpr (Bf) split_f32(acc[rb][v], q[0][v], q[1][v], q[2][v]);
+                else split_half(acc[rb][v], q[0][v], q[1][v]);
+            }
+#pragma unroll
+            for (int p = 0; p < NP; ++p)
+                *reinterpret_cast<u32x2 *>(at + p * kOpTileCols * kRPitch) =
+                    u32x2{q[p][0] | (q[p][1] << 16), q[p][2] | (q[p][3] << 16)};
+        }
+    };
+    // Stage-2 of unit u from the ring, and its outputs.
+    auto unit = [&](int u) {
+        __syncthreads();   // the ring rows of this unit are written
+        f32x4 out[2] = {f32x4{0, 0, 0, 0}, f32x4{0, 0, 0, 0}};
+#pragma unroll
+        for (int vb = 0; vb < 2; ++vb) {
+            const int k0 = kv[2 * u + vb];
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const int rr = (k0 + kOpStep * s + 8 * lk) % kOpRing;
+                const std::uint16_t *at = ring + (kOpBlock * w + lr) * kRPitch + rr;
+                u32x4 b[NP];
+#pragma unroll
+                for (int p = 0; p < NP; ++p)
+                    b[p] = *reinterpret_cast<const u32x4 *>(at + p * kOpTileCols * kRPitch);
+                out[vb] = products<Bf>(av[vb][s], b, out[vb]);
+            }
+        }
+        if (col0 < nh) {
+#pragma unroll
+            for (int vb = 0; vb < 2; ++vb)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int d = kOpUnit * u + kOpBlock * vb + 4 * lk + v;
+                    if (d >= nv) continue;
+                    const int off = static_cast<int>((d * jb.d_row + col0) * kSize);
+                    float x = out[vb][v];
+                    if constexpr (!Bf)
+                        x = __fmul_rn(__fsub_rn(__fmul_rn(x, 64.0F), jb.conv.in_offset), jb.conv.in_scale);
+                    store_sample<In>(ws, off, x, jb.conv);
+                }
+        }
+    };
+
+    Chunk<In> pa[kChunks], pb[kChunks];
+    fetch(pa, first);
+    if (slabs > 1) fetch(pb, first + kOpSlab);
+    load_unit(u0);
+    int u = u0;
+    // After slab k the ring holds source rows below first + 32 (k + 1);
+    // every unit whose rows are all there is finished before the next slab.
+    auto drain = [&](int k) {
+        const int produced = first + kOpSlab * (k + 1);
+        while (u < u1 && kv[2 * u + 1] + kOpStep * S <= produced) {
+            unit(u);
+            ++u;
+            if (u < u1) load_unit(u);
+        }
+    };
+    for (int k = 0; k < slabs; k += 2) {
+        slab(pa, k);
+        drain(k);
+        if (k + 1 < slabs) {
+            slab(pb, k + 1);
+            drain(k + 1);
+        }
+    }
+}
+
+} // namespace
+} // namespace dev
+
+template <class In, int S>
+static hipError_t launch_op(const OpJob *table, int count, long items, hipStream_t stream) {
+    const long blocks = dev::kXcds * ((items + dev::kXcds - 1) / dev::kXcds + count);
+    hipLaunchKernelGGL((dev::operator2d<In, S>), dim3(static_cast<unsigned>(blocks)), dim3(dev::kThreads), 0,
+                       stream, table, count, items);
+    return hipGetLastError();
+}
+
+template <class In>
+static hipError_t launch_op_steps(const OpJob *table, int count, long items, int steps, hipStream_t stream) {
+    switch (steps) {
+    case 2: return launch_op<In, 2>(table, count, items, stream);
+    case 3: return launch_op<In, 3>(table, count, items, stream);
+    case 4: return launch_op<In, 4>(table, count, items, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_operator_table(const OpJob *table, int count, long items, int steps, Sample kind,
+                                 hipStream_t stream) {
+    if (count < 1 || items < 1) return hipSuccess;
+    if (items + 8L * count > 0x7FFFFFF0L) return hipErrorInvalidValue;
+    switch (kind) {
+    case Sample::f32: return launch_op_steps<float>(table, count, items, steps, stream);
+    case Sample::u8: return launch_op_steps<std::uint8_t>(table, count, items, steps, stream);
+    case Sample::u16: return launch_op_steps<std::uint16_t>(table, count, items, steps, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+} // namespace dsmvc::hip

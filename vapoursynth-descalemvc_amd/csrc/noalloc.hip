@@ -151,7 +151,7 @@ void HipExecutor::inverse_2d_batch_noalloc(const AxisPlan &h, const AxisPlan &v,
     if (f64 && async)
         throw std::invalid_argument("dsmvc: the no-allocation call takes the Float64 route "
                                     "synchronously only (it reports non-finite results on return)");
-    const bool windows = mode_ == SolveMode::automatic || mode_ == SolveMode::windowed;
+    const bool windows = automatic_like(mode_) || mode_ == SolveMode::windowed;
 
     // Sizes of the scratch regions, checked against the array before any
     // launch: the intermediate [frame][v.m][pitch(h.n)] and, for sequential

@@ -159,7 +159,7 @@ std::uint64_t HipServer::submit(const ServeRequest &r) {
                         : axis(sh, dh, dh, 0.0);
         kinds = 4;
     }
-    const bool parallel = executor_.mode() == SolveMode::automatic
+    const bool parallel = automatic_like(executor_.mode())
                           || executor_.mode() == SolveMode::windowed;
     // Device image plus the windowed tables the round (or, for a Float64
     // pair, the executor) will use.

@@ -32,6 +32,8 @@ enum class Label : std::uint8_t {
     table_upload,
     horizontal_stage,
     vertical_stage,
+    fused_stage,
+    operator_build,
     stream_synchronize,
     host_pack,
     source_upload,
@@ -62,6 +64,8 @@ inline constexpr std::array<const char *, static_cast<std::size_t>(Label::count)
     "dsmvc.hip: table upload",
     "dsmvc.hip: horizontal stage",
     "dsmvc.hip: vertical stage",
+    "dsmvc.hip: fused 2d stage",
+    "dsmvc.hip: operator build",
     "dsmvc.hip: stream synchronize",
     "dsmvc.hip: host pack",
     "dsmvc.hip: source upload",

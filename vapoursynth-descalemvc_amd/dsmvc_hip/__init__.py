@@ -40,7 +40,7 @@ ZERO, REPEAT, REFLECT101, SYMMETRIC, MIRROR = range(5)
 KERNELS = {"bilinear": BILINEAR, "bicubic": BICUBIC, "lanczos": LANCZOS,
            "spline16": SPLINE16, "spline36": SPLINE36, "spline64": SPLINE64}
 BACKENDS = {0: "auto", 1: "cpu", 2: "metal", 3: "vulkan", 4: "cuda", 5: "hip"}
-SOLVE_MODES = {"auto": 0, "exact": 1, "chunked": 2, "windowed": 3, "refined": 4}
+SOLVE_MODES = {"auto": 0, "exact": 1, "chunked": 2, "windowed": 3, "refined": 4, "fused": 5}
 SAMPLE_KINDS = {"float32": 0, "uint8": 1, "uint16": 2}
 
 
@@ -136,6 +136,23 @@ class Plan:
         info = _lib.WindowInfoC()
         check(load().dsmvc_hip_plan_window_info(self._handle, ctypes.byref(info)))
         return {name: getattr(info, name) for name, _ in _lib.WindowInfoC._fields_}
+
+    def operator(self) -> dict:
+        """The plan's explicit operator M = G^-1 A^T truncated to its band (the
+        fused 2D route's operand): info fields plus `lo`, `count` and the
+        concatenated row `values`."""
+        info = _lib.OperatorInfoC()
+        check(load().dsmvc_hip_plan_operator(self._handle, ctypes.byref(info), None, None, None, 0))
+        n = self.destination_size
+        lo, count = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        values = np.zeros(max(1, info.values))
+        check(load().dsmvc_hip_plan_operator(self._handle, ctypes.byref(info),
+                                             ctypes.c_void_p(lo.ctypes.data),
+                                             ctypes.c_void_p(count.ctypes.data),
+                                             ctypes.c_void_p(values.ctypes.data), values.size))
+        out = {name: getattr(info, name) for name, _ in _lib.OperatorInfoC._fields_}
+        out.update(lo=lo, count=count, values=values[:info.values])
+        return out
 
     def arrays(self) -> dict:
         """Copies of the AxisPlan arrays (reference include/dsmvc/engine.hpp:90-107)."""

@@ -18,6 +18,7 @@ EXPORTS = (
     "dsmvc_hip_last_error", "dsmvc_hip_plan_build", "dsmvc_hip_plan_build_many",
     "dsmvc_hip_plan_release",
     "dsmvc_hip_plan_info_get", "dsmvc_hip_plan_arrays", "dsmvc_hip_plan_window_info",
+    "dsmvc_hip_plan_operator",
     "dsmvc_hip_planner_cache_stats",
     "dsmvc_hip_clear_planner_caches", "dsmvc_hip_padding_index", "dsmvc_hip_parse_backend",
     "dsmvc_hip_resolve_backend", "dsmvc_hip_device_count", "dsmvc_hip_executor_create",
@@ -82,6 +83,20 @@ class WindowInfoC(ctypes.Structure):
         ("width", ctypes.c_int32),
         ("float64_feasible", ctypes.c_int32),
         ("residual", ctypes.c_double),
+    ]
+
+
+class OperatorInfoC(ctypes.Structure):
+    _fields_ = [
+        ("rows_feasible", ctypes.c_int32),
+        ("columns_feasible", ctypes.c_int32),
+        ("blocks", ctypes.c_int32),
+        ("steps", ctypes.c_int32),
+        ("max_band", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("values", ctypes.c_int64),
+        ("dropped", ctypes.c_double),
+        ("max_abs", ctypes.c_double),
     ]
 
 
@@ -265,6 +280,8 @@ def load():
     lib.dsmvc_hip_plan_info_get.argtypes = [vp, ctypes.POINTER(PlanInfoC)]
     lib.dsmvc_hip_plan_arrays.argtypes = [vp] + [vp] * 10
     lib.dsmvc_hip_plan_window_info.argtypes = [vp, ctypes.POINTER(WindowInfoC)]
+    lib.dsmvc_hip_plan_operator.argtypes = [vp, ctypes.POINTER(OperatorInfoC), vp, vp, vp,
+                                            ctypes.c_int64]
     lib.dsmvc_hip_planner_cache_stats.argtypes = [ctypes.POINTER(PlannerStatsC)]
     lib.dsmvc_hip_clear_planner_caches.restype = None
     lib.dsmvc_hip_padding_index.argtypes = [ctypes.c_int64, i32, i32, ctypes.POINTER(i32)]
