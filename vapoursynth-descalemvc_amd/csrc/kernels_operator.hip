@@ -40,7 +40,7 @@ using h2 = _Float16 __attribute__((ext_vector_type(2)));
 using bf8 = __bf16 __attribute__((ext_vector_type(8)));
 using f32x4 = float __attribute__((ext_vector_type(4)));
 
-constexpr int kThreads = 256;
+constexpr int kThreads = 512;   // eight wavefronts: one (16-row, 16-column) MFMA block each
 constexpr int kYPitch = kOpStageCols + 8;      // 16-bit values per staged source row
 constexpr int kRPitch = kOpRing + 8;           // 16-bit values per ring column
 constexpr int kChunksPerRow = kOpStageCols / 8;
@@ -123,10 +123,13 @@ template <> struct Chunk<float> { u32x4 v[2]; };
 constexpr int kOutside = 0x7FFFFFF0;
 
 // One chunk from the plane's buffer resource at byte offset `off` (kOutside
-// for a row past the plane), columns c .. c + 7 of which those at or past
-// `cols` read as 0.  `vector`: the offset is aligned for one wide load;
-// otherwise one load per sample.  Buffer loads never touch the LDS counter,
-// so the chunk stays in flight across the wavefront's LDS work.
+// for a row past the plane).  `vector`: the offset is aligned for one wide
+// load; otherwise one load per sample.  Columns at or past the plane's width
+// read whatever follows in memory (the row's padding or the next row): the
+// operator's entries there are zero, and an integer sample times zero is
+// zero; F32 samples there are cleared (`cols`), since padding may hold
+// non-finite bits.  Buffer loads never touch the LDS counter, so the chunk
+// stays in flight across the wavefront's LDS work.
 template <class In>
 __device__ __forceinline__ void fetch_chunk(Chunk<In> &k, __amdgpu_buffer_rsrc_t rs, int off, int c, int cols,
                                             bool vector) {
@@ -141,11 +144,6 @@ __device__ __forceinline__ void fetch_chunk(Chunk<In> &k, __amdgpu_buffer_rsrc_t
                 k.v[q] = a | (b << 16);
             }
         }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const std::uint32_t keep = (c + 2 * q < cols ? 0x0000FFFFU : 0U) | (c + 2 * q + 1 < cols ? 0xFFFF0000U : 0U);
-            k.v[q] &= keep;
-        }
     } else if constexpr (std::is_same_v<In, std::uint8_t>) {
         if (vector) {
             k.v = __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0));
@@ -159,13 +157,6 @@ __device__ __forceinline__ void fetch_chunk(Chunk<In> &k, __amdgpu_buffer_rsrc_t
                 k.v[q] = w;
             }
         }
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            std::uint32_t keep = 0;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) keep |= (c + 4 * q + e < cols ? 0xFFU : 0U) << (8 * e);
-            k.v[q] &= keep;
-        }
     } else {
         if (vector) {
             k.v[0] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
@@ -177,10 +168,12 @@ __device__ __forceinline__ void fetch_chunk(Chunk<In> &k, __amdgpu_buffer_rsrc_t
                 k.v[1][q] = __builtin_amdgcn_raw_buffer_load_b32(rs, off + 16 + 4 * q, 0, 0);
             }
         }
+        if (c + 8 > cols) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            if (c + q >= cols) k.v[0][q] = 0U;
-            if (c + 4 + q >= cols) k.v[1][q] = 0U;
+            for (int q = 0; q < 4; ++q) {
+                if (c + q >= cols) k.v[0][q] = 0U;
+                if (c + 4 + q >= cols) k.v[1][q] = 0U;
+            }
         }
     }
 }
@@ -272,15 +265,16 @@ template <class Out> __device__ __forceinline__ void store_sample(__amdgpu_buffe
 
 template <class In, int S>
 __global__ __launch_bounds__(kThreads)
-__attribute__((amdgpu_waves_per_eu(std::is_same_v<In, float> ? 1 : 2)))
-void operator2d(const OpJob *__restrict__ table, int count, long items) {
+__attribute__((amdgpu_waves_per_eu(std::is_same_v<In, float> ? 2 : 4)))
+void operator2d(const OpJob *__restrict__ table, int count) {
     constexpr bool Bf = std::is_same_v<In, float>;
     constexpr int NP = Bf ? 3 : 2;
     constexpr int kSize = static_cast<int>(sizeof(In));
-    __shared__ __attribute__((aligned(16))) std::uint16_t ylds[NP * kOpSlab * kYPitch];
-    __shared__ __attribute__((aligned(16))) std::uint16_t ring[NP * kOpTileCols * kRPitch];
+    constexpr int kYPlane = kOpSlab * kYPitch;          // 16-bit values per staged part
+    constexpr int kRPlane = kOpTileCols * kRPitch;      // 16-bit values per ring part
+    __shared__ __attribute__((aligned(16))) std::uint16_t ylds[NP * kYPlane];
+    __shared__ __attribute__((aligned(16))) std::uint16_t ring[NP * kRPlane];
 
-    (void)items;
     long item = 0;
     const int job = balanced_item(table, count, blockIdx.x, item);
     if (job < 0) return;
@@ -294,144 +288,140 @@ void operator2d(const OpJob *__restrict__ table, int count, long items) {
     const int u0 = seg * jb.units_per_item;
     const int u1 = min(jb.units, u0 + jb.units_per_item);
 
+    // Wavefront w owns horizontal block cb = w % 4 of the tile and the
+    // 16-row half hb = w / 4 of every slab (stage 1) and of every unit
+    // (stage 2).
     const int tid = threadIdx.x, l = tid & 63, lr = l & 15, lk = l >> 4;
-    const int w = uniform(tid >> 6);
+    const int w = uniform(tid >> 6), cb = w & 3, hb = w >> 2;
     const ConstPtr<std::int32_t> kh = constant(jb.h.kstart), kv = constant(jb.v.kstart);
     const int cs0 = kh[4 * tile];
-    const int koff = kh[4 * tile + w] - cs0;
     const int wc = kh[4 * tile + 3] - cs0 + kOpStep * S;
     const int mh = jb.h.m, mv = jb.v.m, nh = jb.h.n, nv = jb.v.n;
 
-    // Horizontal block 4 * tile + w, in registers for the whole item.
+    // Horizontal block 4 * tile + cb, in registers for the whole item.
     u32x4 bh[S][NP];
     {
         const GlobalPtr<u32x4> d = global(reinterpret_cast<const u32x4 *>(jb.h.data))
-                                   + static_cast<long>(4 * tile + w) * S * NP * 64 + l;
+                                   + static_cast<long>(4 * tile + cb) * S * NP * 64 + l;
 #pragma unroll
         for (int s = 0; s < S; ++s)
 #pragma unroll
             for (int p = 0; p < NP; ++p) bh[s][p] = d[(s * NP + p) * 64];
     }
 
-    // This thread's staging chunks (fixed for the item): slab row and column.
-    int crow[kChunks], ccol[kChunks];
+    // Slabs start on multiples of kOpSlab, so a slab's rows never wrap in
+    // the ring; from the first unit's first source row to the last unit's last.
+    const int first = kv[2 * u0] / kOpSlab * kOpSlab;
+    const int slabs = (kv[2 * (u1 - 1) + 1] + kOpStep * S - first + kOpSlab - 1) / kOpSlab;
+
+    // This thread's staging chunks (fixed for the item): slab row, column,
+    // and byte offset at the first slab.
+    int crow[kChunks], ccol[kChunks], coff[kChunks];
     bool cact[kChunks];
+    const long s_row = jb.s_row;
 #pragma unroll
     for (int i = 0; i < kChunks; ++i) {
         const int q = tid + kThreads * i;
         crow[i] = q / kChunksPerRow;
         ccol[i] = 8 * (q % kChunksPerRow);
         cact[i] = crow[i] < kOpSlab && ccol[i] < wc;
+        coff[i] = static_cast<int>(((first + crow[i]) * s_row + cs0 + ccol[i]) * kSize);
     }
     const auto rs = plane_rsrc(static_cast<const In *>(jb.src) + frame * jb.s_frame,
-                               (static_cast<long>(mv - 1) * jb.s_row + mh) * kSize);
+                               (static_cast<long>(mv - 1) * s_row + mh) * kSize);
     const bool vector = jb.vector_loads != 0;
-    const long s_row = jb.s_row;
-    auto fetch = [&](Chunk<In> (&buf)[kChunks], int r0) {
+    const int slab_bytes = static_cast<int>(kOpSlab * s_row * kSize);
+    auto fetch = [&](Chunk<In> (&buf)[kChunks], int k) {
+        const int rows_left = mv - first - kOpSlab * k;   // slab rows inside the plane
 #pragma unroll
         for (int i = 0; i < kChunks; ++i) {
-            const int sr = r0 + crow[i], c = cs0 + ccol[i];
-            const int off = cact[i] && sr < mv ? static_cast<int>((sr * s_row + c) * kSize) : kOutside;
-            fetch_chunk<In>(buf[i], rs, off, c, mh, vector);
+            const int off = cact[i] && crow[i] < rows_left ? coff[i] + k * slab_bytes : kOutside;
+            fetch_chunk<In>(buf[i], rs, off, cs0 + ccol[i], mh, vector);
         }
     };
 
-    // Slabs from the first unit's first source row to the last unit's last.
-    const int first = kv[2 * u0];
-    const int slabs = (kv[2 * (u1 - 1) + 1] + kOpStep * S - first + kOpSlab - 1) / kOpSlab;
-    const int col0 = kOpTileCols * tile + kOpBlock * w + lr;
+    // LDS addresses (16-bit units): the A operand of stage 1, the ring
+    // column this wavefront writes and reads.
+    const std::uint16_t *a_at = ylds + (kOpBlock * hb + lr) * kYPitch + (kh[4 * tile + cb] - cs0) + 8 * lk;
+    std::uint16_t *r_col = ring + (kOpBlock * cb + lr) * kRPitch;
+    const int r_lane = kOpBlock * hb + 4 * lk;          // row of the wavefront's results in a slab
+
+    const int col0 = kOpTileCols * tile + kOpBlock * cb + lr;
     const auto ws = plane_rsrc(static_cast<In *>(jb.dst) + frame * jb.d_frame,
                                (static_cast<long>(nv - 1) * jb.d_row + nh) * kSize);
     const GlobalPtr<u32x4> vdata = global(reinterpret_cast<const u32x4 *>(jb.v.data)) + l;
 
-    // Unit u's two vertical blocks (loaded a unit ahead; they complete under the slabs).
-    u32x4 av[2][S][NP];
+    // Vertical block 2u + hb of unit u (loaded a unit ahead; it completes under the slabs).
+    u32x4 av[S][NP];
     auto load_unit = [&](int u) {
 #pragma unroll
-        for (int vb = 0; vb < 2; ++vb)
+        for (int s = 0; s < S; ++s)
 #pragma unroll
-            for (int s = 0; s < S; ++s)
-#pragma unroll
-                for (int p = 0; p < NP; ++p)
-                    av[vb][s][p] = vdata[((static_cast<long>(2 * u + vb) * S + s) * NP + p) * 64];
+            for (int p = 0; p < NP; ++p)
+                av[s][p] = vdata[((static_cast<long>(2 * u + hb) * S + s) * NP + p) * 64];
     };
-    // Stage-1 of slab k from its prefetched chunks, whose buffer then
-    // fetches slab k + 2.
+    // Slab k from its prefetched chunks, whose buffer then fetches slab k + 2.
     auto slab = [&](Chunk<In> (&buf)[kChunks], int k) {
-        const int r0 = first + kOpSlab * k;
         __syncthreads();   // every wavefront is done with the staged rows and the ring rows below
 #pragma unroll
         for (int i = 0; i < kChunks; ++i)
             if (cact[i]) stage_chunk<In>(buf[i], ylds, crow[i], ccol[i]);
-        if (k + 2 < slabs) fetch(buf, r0 + 2 * kOpSlab);
+        if (k + 2 < slabs) fetch(buf, k + 2);
         __syncthreads();
-        f32x4 acc[2] = {f32x4{0, 0, 0, 0}, f32x4{0, 0, 0, 0}};
+        f32x4 acc = f32x4{0, 0, 0, 0};
 #pragma unroll
-        for (int s = 0; s < S; ++s)
+        for (int s = 0; s < S; ++s) {
+            u32x4 a[NP];
 #pragma unroll
-            for (int rb = 0; rb < 2; ++rb) {
-                u32x4 a[NP];
-                const std::uint16_t *at = ylds + (kOpBlock * rb + lr) * kYPitch + koff + kOpStep * s + 8 * lk;
-#pragma unroll
-                for (int p = 0; p < NP; ++p)
-                    a[p] = *reinterpret_cast<const u32x4 *>(at + p * kOpSlab * kYPitch);
-                acc[rb] = products<Bf>(a, bh[s], acc[rb]);
-            }
-        // Results (source rows r0 + 16 rb + 4 lk + v, column kOpBlock * w + lr)
-        // into the ring, split into parts.
-#pragma unroll
-        for (int rb = 0; rb < 2; ++rb) {
-            const int rr = (r0 + kOpBlock * rb + 4 * lk) % kOpRing;
-            std::uint16_t *at = ring + (kOpBlock * w + lr) * kRPitch + rr;
-            std::uint32_t q[NP][4];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                if constexpr (Bf) split_f32(acc[rb][v], q[0][v], q[1][v], q[2][v]);
-                else split_half(acc[rb][v], q[0][v], q[1][v]);
-            }
-#pragma unroll
-            for (int p = 0; p < NP; ++p)
-                *reinterpret_cast<u32x2 *>(at + p * kOpTileCols * kRPitch) =
-                    u32x2{q[p][0] | (q[p][1] << 16), q[p][2] | (q[p][3] << 16)};
+            for (int p = 0; p < NP; ++p) a[p] = *reinterpret_cast<const u32x4 *>(a_at + p * kYPlane + kOpStep * s);
+            acc = products<Bf>(a, bh[s], acc);
         }
+        // Results (source rows r0 + r_lane + v, column 16 cb + lr) into the
+        // ring, split into parts; r0 % kOpRing is a multiple of kOpSlab.
+        const int rr = (first + kOpSlab * k) % kOpRing + r_lane;
+        std::uint32_t q[NP][4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            if constexpr (Bf) split_f32(acc[v], q[0][v], q[1][v], q[2][v]);
+            else split_half(acc[v], q[0][v], q[1][v]);
+        }
+#pragma unroll
+        for (int p = 0; p < NP; ++p)
+            *reinterpret_cast<u32x2 *>(r_col + p * kRPlane + rr) =
+                u32x2{q[p][0] | (q[p][1] << 16), q[p][2] | (q[p][3] << 16)};
     };
-    // Stage-2 of unit u from the ring, and its outputs.
+    // Unit u: vertical block 2u + hb times the ring's column, and its outputs.
     auto unit = [&](int u) {
         __syncthreads();   // the ring rows of this unit are written
-        f32x4 out[2] = {f32x4{0, 0, 0, 0}, f32x4{0, 0, 0, 0}};
+        f32x4 out = f32x4{0, 0, 0, 0};
+        const int k0 = kv[2 * u + hb] % kOpRing;
 #pragma unroll
-        for (int vb = 0; vb < 2; ++vb) {
-            const int k0 = kv[2 * u + vb];
+        for (int s = 0; s < S; ++s) {
+            int rr = k0 + kOpStep * s + 8 * lk;
+            rr = rr >= kOpRing ? rr - kOpRing : rr;
+            rr = rr >= kOpRing ? rr - kOpRing : rr;
+            u32x4 b[NP];
 #pragma unroll
-            for (int s = 0; s < S; ++s) {
-                const int rr = (k0 + kOpStep * s + 8 * lk) % kOpRing;
-                const std::uint16_t *at = ring + (kOpBlock * w + lr) * kRPitch + rr;
-                u32x4 b[NP];
-#pragma unroll
-                for (int p = 0; p < NP; ++p)
-                    b[p] = *reinterpret_cast<const u32x4 *>(at + p * kOpTileCols * kRPitch);
-                out[vb] = products<Bf>(av[vb][s], b, out[vb]);
-            }
+            for (int p = 0; p < NP; ++p) b[p] = *reinterpret_cast<const u32x4 *>(r_col + p * kRPlane + rr);
+            out = products<Bf>(av[s], b, out);
         }
         if (col0 < nh) {
 #pragma unroll
-            for (int vb = 0; vb < 2; ++vb)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int d = kOpUnit * u + kOpBlock * vb + 4 * lk + v;
-                    if (d >= nv) continue;
-                    const int off = static_cast<int>((d * jb.d_row + col0) * kSize);
-                    float x = out[vb][v];
-                    if constexpr (!Bf)
-                        x = __fmul_rn(__fsub_rn(__fmul_rn(x, 64.0F), jb.conv.in_offset), jb.conv.in_scale);
-                    store_sample<In>(ws, off, x, jb.conv);
-                }
+            for (int v = 0; v < 4; ++v) {
+                const int d = kOpUnit * u + kOpBlock * hb + 4 * lk + v;
+                if (d >= nv) continue;
+                const int off = static_cast<int>((d * jb.d_row + col0) * kSize);
+                float x = out[v];
+                if constexpr (!Bf)
+                    x = __fmul_rn(__fsub_rn(__fmul_rn(x, 64.0F), jb.conv.in_offset), jb.conv.in_scale);
+                store_sample<In>(ws, off, x, jb.conv);
+            }
         }
     };
 
     Chunk<In> pa[kChunks], pb[kChunks];
-    fetch(pa, first);
-    if (slabs > 1) fetch(pb, first + kOpSlab);
+    fetch(pa, 0);
+    if (slabs > 1) fetch(pb, 1);
     load_unit(u0);
     int u = u0;
     // After slab k the ring holds source rows below first + 32 (k + 1);
@@ -461,7 +451,7 @@ template <class In, int S>
 static hipError_t launch_op(const OpJob *table, int count, long items, hipStream_t stream) {
     const long blocks = dev::kXcds * ((items + dev::kXcds - 1) / dev::kXcds + count);
     hipLaunchKernelGGL((dev::operator2d<In, S>), dim3(static_cast<unsigned>(blocks)), dim3(dev::kThreads), 0,
-                       stream, table, count, items);
+                       stream, table, count);
     return hipGetLastError();
 }
 
