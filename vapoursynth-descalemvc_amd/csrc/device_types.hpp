@@ -161,6 +161,10 @@ struct OpJob {
     std::int32_t vector_loads;    // source rows and base allow 16-byte loads
     long first_item;
     DevConv conv;
+    // Integer output: sample = rint(clamp(x * out_mul + out_add, 0, max)) for
+    // the kernel's result x = (M_v Y M_h^T) / 256 in input codes, i.e. the
+    // conversion's ((256 x - in_offset) * in_scale) * out_scale + out_offset.
+    float out_mul, out_add;
 };
 
 constexpr long operator_items(const OpJob &j) {

@@ -1005,6 +1005,11 @@ bool HipExecutor::add_fused(const AxisPlan &h, const AxisPlan &v, SampleKind kin
                          && (b.frames == 1 || (static_cast<std::size_t>(b.input_frame_stride) * bytes) % align == 0);
     j.vector_loads = aligned ? 1 : 0;
     j.conv = make_conv(conversion);
+    if (conversion) {
+        const double scale = static_cast<double>(conversion->input_scale) * conversion->output_scale;
+        j.out_mul = static_cast<float>(256.0 * scale);
+        j.out_add = static_cast<float>(conversion->output_offset - conversion->input_offset * scale);
+    }
     tables.add_op(j, steps, sample_of(kind));
     return true;
 }

@@ -17,9 +17,10 @@
 //
 // Precision.  Each operand is a sum of 16-bit parts and each product the sum
 // of the part products whose order is at most the operand's own rounding:
-//   - integer samples (u8/u16, values y): y = 64 (y >> 6) + (y & 63) exactly
-//     as the FP16 parts (y >> 6) and (y & 63) / 64; operators and the F32
-//     ring values as FP16 + FP16 remainder; three products per term;
+//   - integer samples (u8/u16, values y): y / 256 = (y >> 8) + (y & 255) / 256
+//     exactly as two FP16 parts, operators and the F32 ring values as FP16 +
+//     FP16 remainder; all four part products in the horizontal product, three
+//     in the vertical one;
 //   - F32 samples: every F32 value split exactly into three bfloat16 parts
 //     by truncation; operators as three rounded bfloat16 parts; six products.
 // MFMA products of 16-bit parts are exact in F32 and accumulate in F32.  The
@@ -60,9 +61,11 @@ __device__ __forceinline__ f32x4 mma(u32x4 a, u32x4 b, f32x4 c) {
                                                       __builtin_bit_cast(h8, b), c, 0, 0, 0);
 }
 
-// c + sum of a[i] b[j] over the kept part pairs, smallest first.
-template <bool Bf>
+// c + sum of a[i] b[j] over the kept part pairs, smallest first.  `All`:
+// every pair of two-part operands (exact products of the parts).
+template <bool Bf, bool All = false>
 __device__ __forceinline__ f32x4 products(const u32x4 *a, const u32x4 *b, f32x4 c) {
+    if constexpr (All && !Bf) c = mma<false>(a[1], b[1], c);
     if constexpr (Bf) {
         c = mma<true>(a[2], b[0], c);
         c = mma<true>(a[1], b[1], c);
@@ -81,14 +84,15 @@ __device__ __forceinline__ f32x4 products(const u32x4 *a, const u32x4 *b, f32x4 
 __device__ __forceinline__ std::uint32_t fbits(float x) { return __builtin_bit_cast(std::uint32_t, x); }
 __device__ __forceinline__ float bitsf(std::uint32_t u) { return __builtin_bit_cast(float, u); }
 
-// Two u16 samples (low and high half of d) -> FP16 pairs of (y >> 6) and
-// (y & 63) / 64, exactly: the integer fields are placed in the mantissa of
-// 1024.0 and 1.0 and the bias subtracted.
+// Two u16 samples (low and high half of d) -> FP16 pairs of y >> 8 and
+// (y & 255) / 256, exactly: a byte permute places the high bytes in the
+// mantissa of 1024.0 (0x64xx) and the low bytes in that of 4.0 (0x44xx,
+// spacing 1/256), and the bias is subtracted.
 __device__ __forceinline__ void split_u16(std::uint32_t d, std::uint32_t &p0, std::uint32_t &p1) {
-    const std::uint32_t hb = ((d >> 6) & 0x03FF03FFU) | 0x64006400U;
-    const std::uint32_t lb = ((d << 4) & 0x03F003F0U) | 0x3C003C00U;
+    const std::uint32_t hb = __builtin_amdgcn_perm(0x64646464U, d, 0x04030401U);
+    const std::uint32_t lb = __builtin_amdgcn_perm(0x44444444U, d, 0x04020400U);
     const h2 hv = __builtin_bit_cast(h2, hb) - h2{static_cast<_Float16>(1024.0F), static_cast<_Float16>(1024.0F)};
-    const h2 lv = __builtin_bit_cast(h2, lb) - h2{static_cast<_Float16>(1.0F), static_cast<_Float16>(1.0F)};
+    const h2 lv = __builtin_bit_cast(h2, lb) - h2{static_cast<_Float16>(4.0F), static_cast<_Float16>(4.0F)};
     p0 = __builtin_bit_cast(std::uint32_t, hv);
     p1 = __builtin_bit_cast(std::uint32_t, lv);
 }
@@ -245,17 +249,16 @@ __device__ __forceinline__ int balanced_item(const OpJob *table, int count, long
     return -1;
 }
 
-// Output sample from the result x (normalised for integer samples):
-// clamp(x * scale + offset, 0, max) rounded to nearest even, as the
-// library's other integer stores.
+// Output sample from the result x: F32 as is; integer samples
+// clamp(x * mul + add, 0, max) rounded to nearest even, mul and add folding
+// the input and output conversions.
 template <class Out> __device__ __forceinline__ void store_sample(__amdgpu_buffer_rsrc_t rs, int off, float x,
-                                                                const DevConv &c) {
+                                                                float mul, float add, float max) {
     if constexpr (std::is_same_v<Out, float>) {
         __builtin_amdgcn_raw_buffer_store_b32(fbits(x), rs, off, 0, 0);
     } else {
-        float v = __fadd_rn(__fmul_rn(x, c.out_scale), c.out_offset);
-        v = v < 0.0F ? 0.0F : (c.out_max < v ? c.out_max : v);
-        const auto q = static_cast<std::uint32_t>(rintf(v));
+        const float v = __builtin_amdgcn_fmed3f(__builtin_fmaf(x, mul, add), 0.0F, max);
+        const auto q = static_cast<std::uint32_t>(__builtin_rintf(v));
         if constexpr (std::is_same_v<Out, std::uint16_t>)
             __builtin_amdgcn_raw_buffer_store_b16(static_cast<std::uint16_t>(q), rs, off, 0, 0);
         else
@@ -347,6 +350,9 @@ void operator2d(const OpJob *__restrict__ table, int count) {
     const int r_lane = kOpBlock * hb + 4 * lk;          // row of the wavefront's results in a slab
 
     const int col0 = kOpTileCols * tile + kOpBlock * cb + lr;
+    const int d_row_bytes = static_cast<int>(jb.d_row * kSize);
+    const int lane_out = 4 * lk * d_row_bytes + col0 * kSize;   // this lane's first output, unit row 0
+    const float out_mul = jb.out_mul, out_add = jb.out_add, out_max = jb.conv.out_max;
     const auto ws = plane_rsrc(static_cast<In *>(jb.dst) + frame * jb.d_frame,
                                (static_cast<long>(nv - 1) * jb.d_row + nh) * kSize);
     const GlobalPtr<u32x4> vdata = global(reinterpret_cast<const u32x4 *>(jb.v.data)) + l;
@@ -374,7 +380,7 @@ void operator2d(const OpJob *__restrict__ table, int count) {
             u32x4 a[NP];
 #pragma unroll
             for (int p = 0; p < NP; ++p) a[p] = *reinterpret_cast<const u32x4 *>(a_at + p * kYPlane + kOpStep * s);
-            acc = products<Bf>(a, bh[s], acc);
+            acc = products<Bf, true>(a, bh[s], acc);
         }
         // Results (source rows r0 + r_lane + v, column 16 cb + lr) into the
         // ring, split into parts; r0 % kOpRing is a multiple of kOpSlab.
@@ -406,16 +412,12 @@ void operator2d(const OpJob *__restrict__ table, int count) {
             out = products<Bf>(av[s], b, out);
         }
         if (col0 < nh) {
+            const int d0 = kOpUnit * u + kOpBlock * hb;           // wave-uniform
+            const int base = d0 * d_row_bytes + lane_out;
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int d = kOpUnit * u + kOpBlock * hb + 4 * lk + v;
-                if (d >= nv) continue;
-                const int off = static_cast<int>((d * jb.d_row + col0) * kSize);
-                float x = out[v];
-                if constexpr (!Bf)
-                    x = __fmul_rn(__fsub_rn(__fmul_rn(x, 64.0F), jb.conv.in_offset), jb.conv.in_scale);
-                store_sample<In>(ws, off, x, jb.conv);
-            }
+            for (int v = 0; v < 4; ++v)
+                if (d0 + 4 * lk + v < nv)
+                    store_sample<In>(ws, base + v * d_row_bytes, out[v], out_mul, out_add, out_max);
         }
     };
 
