@@ -97,16 +97,19 @@ __device__ __forceinline__ void split_u16(std::uint32_t d, std::uint32_t &p0, st
     p1 = __builtin_bit_cast(std::uint32_t, lv);
 }
 
-// An F32 value as three bfloat16 parts by truncation; exact (each remainder
-// has at most the bits the next part keeps).
-__device__ __forceinline__ void split_f32(float x, std::uint32_t &a, std::uint32_t &b, std::uint32_t &c) {
-    const std::uint32_t h = fbits(x) & 0xFFFF0000U;
-    const float r1 = __fsub_rn(x, bitsf(h));
-    const std::uint32_t m = fbits(r1) & 0xFFFF0000U;
-    const float r2 = __fsub_rn(r1, bitsf(m));
-    a = h >> 16;
-    b = m >> 16;
-    c = fbits(r2) >> 16;
+// Two F32 values as three packed pairs of bfloat16 parts (low half from
+// x0) by truncation: the high 16 bits, then those of the remainder, then the
+// remainder's remainder.  Exact: each remainder has at most the bits the
+// next part keeps.  One byte permute packs each pair's high halves.
+__device__ __forceinline__ void split_f32_pair(float x0, float x1, std::uint32_t &p0, std::uint32_t &p1,
+                                               std::uint32_t &p2) {
+    const std::uint32_t u0 = fbits(x0), u1 = fbits(x1);
+    p0 = __builtin_amdgcn_perm(u1, u0, 0x07060302U);
+    const float r0 = __fsub_rn(x0, bitsf(u0 & 0xFFFF0000U)), r1 = __fsub_rn(x1, bitsf(u1 & 0xFFFF0000U));
+    const std::uint32_t v0 = fbits(r0), v1 = fbits(r1);
+    p1 = __builtin_amdgcn_perm(v1, v0, 0x07060302U);
+    const float t0 = __fsub_rn(r0, bitsf(v0 & 0xFFFF0000U)), t1 = __fsub_rn(r1, bitsf(v1 & 0xFFFF0000U));
+    p2 = __builtin_amdgcn_perm(fbits(t1), fbits(t0), 0x07060302U);
 }
 
 // An F32 value as FP16 + FP16 remainder (round to nearest).
@@ -192,14 +195,13 @@ __device__ __forceinline__ void stage_chunk(const Chunk<In> &k, std::uint16_t *y
         u32x4 p2;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            std::uint32_t a0, b0, c0, a1, b1, c1;
+            std::uint32_t a, b, c;
             const float x0 = bitsf(q < 2 ? k.v[0][2 * q] : k.v[1][2 * q - 4]);
             const float x1 = bitsf(q < 2 ? k.v[0][2 * q + 1] : k.v[1][2 * q - 3]);
-            split_f32(x0, a0, b0, c0);
-            split_f32(x1, a1, b1, c1);
-            p0[q] = a0 | (a1 << 16);
-            p1[q] = b0 | (b1 << 16);
-            p2[q] = c0 | (c1 << 16);
+            split_f32_pair(x0, x1, a, b, c);
+            p0[q] = a;
+            p1[q] = b;
+            p2[q] = c;
         }
         *reinterpret_cast<u32x4 *>(ylds + 2 * plane + row * kYPitch + col) = p2;
     } else {
@@ -385,16 +387,22 @@ void operator2d(const OpJob *__restrict__ table, int count) {
         // Results (source rows r0 + r_lane + v, column 16 cb + lr) into the
         // ring, split into parts; r0 % kOpRing is a multiple of kOpSlab.
         const int rr = (first + kOpSlab * k) % kOpRing + r_lane;
-        std::uint32_t q[NP][4];
+        if constexpr (Bf) {
+            std::uint32_t a0, b0, c0, a1, b1, c1;
+            split_f32_pair(acc[0], acc[1], a0, b0, c0);
+            split_f32_pair(acc[2], acc[3], a1, b1, c1);
+            *reinterpret_cast<u32x2 *>(r_col + rr) = u32x2{a0, a1};
+            *reinterpret_cast<u32x2 *>(r_col + kRPlane + rr) = u32x2{b0, b1};
+            *reinterpret_cast<u32x2 *>(r_col + 2 * kRPlane + rr) = u32x2{c0, c1};
+        } else {
+            std::uint32_t q[NP][4];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            if constexpr (Bf) split_f32(acc[v], q[0][v], q[1][v], q[2][v]);
-            else split_half(acc[v], q[0][v], q[1][v]);
+            for (int v = 0; v < 4; ++v) split_half(acc[v], q[0][v], q[1][v]);
+#pragma unroll
+            for (int p = 0; p < NP; ++p)
+                *reinterpret_cast<u32x2 *>(r_col + p * kRPlane + rr) =
+                    u32x2{q[p][0] | (q[p][1] << 16), q[p][2] | (q[p][3] << 16)};
         }
-#pragma unroll
-        for (int p = 0; p < NP; ++p)
-            *reinterpret_cast<u32x2 *>(r_col + p * kRPlane + rr) =
-                u32x2{q[p][0] | (q[p][1] << 16), q[p][2] | (q[p][3] << 16)};
     };
     // Unit u: vertical block 2u + hb times the ring's column, and its outputs.
     auto unit = [&](int u) {
