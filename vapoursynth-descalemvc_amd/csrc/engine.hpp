@@ -7,6 +7,7 @@
 // images, executors) is this library's own design.
 #pragma once
 
+#include <atomic>
 #include <cstddef>
 #include <cstdint>
 #include <functional>
@@ -85,7 +86,24 @@ struct AxisPlan {
     // arrays.  Plans are immutable once built.
     std::uint64_t serial = 0;
 
+    // The result of the last full check, kept by the plan object itself and
+    // not by its copies (a copy starts unchecked); a plan is not changed
+    // after it has been used (reference engine.hpp:109-137 memoises the same
+    // way).
+    struct CheckedFlag {
+        mutable std::atomic<std::uint8_t> state{0};   // 0 unchecked, 1 valid, 2 invalid
+        CheckedFlag() noexcept = default;
+        CheckedFlag(const CheckedFlag &) noexcept {}
+        CheckedFlag &operator=(const CheckedFlag &) noexcept {
+            state.store(0, std::memory_order_relaxed);
+            return *this;
+        }
+    };
+    CheckedFlag checked;
+
     [[nodiscard]] bool valid() const noexcept;
+    // The full check behind valid(), every call.
+    [[nodiscard]] bool check_arrays() const noexcept;
     [[nodiscard]] bool requires_float64() const noexcept { return !ldlt_bands_f64.empty(); }
     [[nodiscard]] std::size_t storage_bytes() const noexcept;
 };

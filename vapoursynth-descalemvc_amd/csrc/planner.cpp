@@ -562,6 +562,14 @@ std::int32_t padding_index(std::int64_t index, std::int32_t size, BorderMode mod
 }
 
 bool AxisPlan::valid() const noexcept {
+    const std::uint8_t known = checked.state.load(std::memory_order_acquire);
+    if (known != 0) return known == 1;
+    const bool ok = check_arrays();
+    checked.state.store(ok ? 1 : 2, std::memory_order_release);
+    return ok;
+}
+
+bool AxisPlan::check_arrays() const noexcept {
     if (source_size <= 0 || destination_size <= 0 || support <= 0 || half_bandwidth < 0
         || half_bandwidth >= destination_size || !(active_length > 0.0)
         || !std::isfinite(active_length) || !std::isfinite(shift) || !std::isfinite(normal_rcond)
