@@ -229,9 +229,10 @@ int dsmvc_hip_device_count(int32_t *count);
  *   FUSED     AUTO, and every 2D call (F32, u8, u16 planes on the F32 route)
  *             takes the fused route wherever both plans' explicit operators
  *             fit it (dsmvc_hip_plan_operator): X = M_v Y M_h^T in one launch
- *             on MFMA with 16-bit split operands, no intermediate in memory.
- *             AUTO takes it the same way for u8/u16 planes unless
- *             DSMVC_HIP_FUSED=0. */
+ *             on MFMA with 16-bit split operands, no intermediate in memory
+ *             (F32 planes: FP16 parts, and an exact bfloat16 pass when a
+ *             sample is beyond FP16's range).  AUTO takes it the same way
+ *             unless DSMVC_HIP_FUSED=0. */
 int dsmvc_hip_executor_create(int32_t device, int32_t solve_mode, dsmvc_hip_executor **executor);
 void dsmvc_hip_executor_destroy(dsmvc_hip_executor *executor);
 /* Executor::prepare / seal (src/executor.cpp:313-349): uploads the plan image. */

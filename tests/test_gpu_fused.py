@@ -1,6 +1,8 @@
 """GPU parity of the fused 2D route (csrc/kernels_operator.hip): X = M_v Y M_h^T
 with the explicit operators M = G^-1 A^T of the two plans multiplied on MFMA
-in 16-bit split parts, no horizontal result in memory.
+in 16-bit split parts, no horizontal result in memory.  F32 planes run on
+FP16 parts and fall back to an exact bfloat16 pass for samples beyond FP16's
+range.
 
 Bars, as for every Float32 route and integer plane of this library:
   * F32 planes: max-abs <= f32_bar(e_ref) against the FP64 oracle (the oracle's
@@ -143,6 +145,22 @@ def test_integer_odd_strides_within_one_code(fused, oracle, bits):
         diff = int(np.abs(got[f] - want.astype(np.int32)).max())
         print(f"u{bits} frame {f}: max code diff {diff}", flush=True)
         assert diff <= 1
+
+
+def test_f32_values_beyond_fp16_take_the_exact_pass(fused, oracle):
+    """F32 planes on the fused route are computed with FP16 parts first; a
+    sample too large for that (here up to 3e5, against a bound of about
+    3e4) sets the call's flag and the bfloat16 pass recomputes the call.
+    Bar: f32_bar of the reference F32 path's own error on the same data."""
+    h, v = make_request(640, 427, 427.0, 0.0, BICUBIC), make_request(360, 240, 240.0, 0.0, BICUBIC)
+    rng = np.random.default_rng(45)
+    frames = (rng.uniform(0, 1, (2, 360, 640)) * 3.0e5).astype(np.float32)
+    got = fused.inverse_2d(_plan(h), _plan(v), torch.from_numpy(frames).to(DEV)).cpu().numpy()
+    _check_f32(oracle, h, v, frames, got, "f32 up to 3e5")
+    # The same call on in-range data afterwards: the flag was cleared.
+    small = (frames / 3.0e5).astype(np.float32)
+    got = fused.inverse_2d(_plan(h), _plan(v), torch.from_numpy(small).to(DEV)).cpu().numpy()
+    _check_f32(oracle, h, v, small, got, "f32 in range after")
 
 
 def test_plans_that_do_not_fit_fall_back(fused, oracle):

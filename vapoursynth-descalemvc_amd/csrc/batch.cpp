@@ -102,8 +102,8 @@ void WindowTables::add_check(const hip::CheckJob &job) {
     g.checks.push_back(j);
 }
 
-void WindowTables::add_op(const hip::OpJob &job, int steps, hip::Sample kind) {
-    Group &g = group_for(Kind::op, 0, steps, true, false, kind);
+void WindowTables::add_op(const hip::OpJob &job, int steps, hip::Sample kind, bool exact) {
+    Group &g = group_for(Kind::op, exact ? 1 : 0, steps, true, false, kind);
     hip::OpJob j = job;
     j.first_item = g.items;
     g.items += hip::operator_items(j);
@@ -120,7 +120,9 @@ int WindowTables::upload(hipStream_t stream) {
     // intermediate some horizontal group writes.
     std::stable_sort(groups_.begin(), groups_.end(), [](const Group &a, const Group &b) {
         auto rank = [](const Group &g) {
-            return g.type == Kind::check || g.type == Kind::op ? 0 : (g.rows ? 1 : 2);
+            if (g.type == Kind::check) return 0;
+            if (g.type == Kind::op) return g.band;   // first passes, then exact passes
+            return g.rows ? 2 : 3;
         };
         return rank(a) < rank(b);
     });
@@ -206,7 +208,7 @@ void WindowTables::launch(hipStream_t stream) {
             break;
         case Kind::op:
             e = hip::launch_operator_table(reinterpret_cast<const hip::OpJob *>(at), count, g.items,
-                                           g.width, g.kind, stream);
+                                           g.width, g.kind, g.band != 0, stream);
             break;
         }
         check(e, "table launch failed");

@@ -74,7 +74,9 @@ public:
     void add_check(const hip::CheckJob &job);
     // A fused 2D job (kernels_operator.hip) whose operator images were
     // packed at `steps` K steps; `kind`: its input and output samples.
-    void add_op(const hip::OpJob &job, int steps, hip::Sample kind);
+    // `exact`: an F32 job's second pass (bfloat16 images), launched after
+    // every first pass.
+    void add_op(const hip::OpJob &job, int steps, hip::Sample kind, bool exact = false);
     // Uploads the tables (stream-ordered when `stream` is given, else
     // synchronously) and returns the number of launches `launch` makes.
     int upload(hipStream_t stream);

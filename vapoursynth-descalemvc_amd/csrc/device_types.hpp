@@ -165,6 +165,12 @@ struct OpJob {
     // the kernel's result x = (M_v Y M_h^T) / 256 in input codes, i.e. the
     // conversion's ((256 x - in_offset) * in_scale) * out_scale + out_offset.
     float out_mul, out_add;
+    // F32 samples: the FP16 pass sets *flag when a sample is non-finite or
+    // above `range` in magnitude (so that the horizontal results stay within
+    // FP16's range); the exact pass runs only then.
+    float range;
+    std::int32_t pad_;
+    int *flag;
 };
 
 constexpr long operator_items(const OpJob &j) {

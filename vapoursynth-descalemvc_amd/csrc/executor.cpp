@@ -795,11 +795,13 @@ HipExecutor::Scratch &HipExecutor::enqueue_planes(const PlaneJob *planes, std::i
     WindowTables &tables = stream_tables(stream);
     tables.clear();
     const bool try_fused = fused_route(kind);
+    // The F32 fused jobs' range flag, next to the stream's Float64 flag.
+    int *fused_flag = kind == SampleKind::float32 ? scratch(stream, 0).flag + 1 : nullptr;
     for (std::size_t k = 0; k < n; ++k) {
         const AxisPlan &h = *planes[k].horizontal, &v = *planes[k].vertical;
         const bool f64 = float64_route(h) || float64_route(v);
         if (try_fused && !f64
-            && add_fused(h, v, kind, planes[k].batch, planes[k].conversion, tables, pins)) {
+            && add_fused(h, v, kind, planes[k].batch, planes[k].conversion, tables, pins, fused_flag)) {
             fused[k] = 1;
             continue;
         }
@@ -836,6 +838,8 @@ HipExecutor::Scratch &HipExecutor::enqueue_planes(const PlaneJob *planes, std::i
     if (total || any_fused) {
         Scratch &s = scratch(stream, total);
         last = &s;
+        if (any_fused && fused_flag)
+            check(hipMemsetAsync(fused_flag, 0, sizeof(int), stream), "flag reset failed");
         const Sample sk = sample_of(kind);
         bool armed = false;
         for (std::size_t k = 0; k < n; ++k) {
@@ -962,7 +966,7 @@ void HipExecutor::inverse_2d_batch(const AxisPlan &h, const AxisPlan &v, SampleK
 
 bool HipExecutor::add_fused(const AxisPlan &h, const AxisPlan &v, SampleKind kind, const PlaneBatch &b,
                             const IntegerConversion *conversion, WindowTables &tables,
-                            std::vector<std::shared_ptr<hip::DevicePlan>> &pins) {
+                            std::vector<std::shared_ptr<hip::DevicePlan>> &pins, int *flag) {
     const std::shared_ptr<hip::DevicePlan> ih = device_plan(h), iv = device_plan(v);
     const hip::OperatorPlan *ph = ih->operator_plan(), *pv = iv->operator_plan();
     if (!ph || !pv) return false;
@@ -975,9 +979,12 @@ bool HipExecutor::add_fused(const AxisPlan &h, const AxisPlan &v, SampleKind kin
         || span(b.output_row_stride, v.destination_size, h.destination_size) >= 0x7FFFFFF0U)
         return false;
     const int steps = std::max({ph->steps, pv->steps, 2});
-    const bool bf16 = kind == SampleKind::float32;
-    const hip::DevOperator *oh = ih->op(true, bf16, steps), *ov = iv->op(false, bf16, steps);
+    const bool f32 = kind == SampleKind::float32;
+    const hip::DevOperator *oh = ih->op(true, false, steps), *ov = iv->op(false, false, steps);
     if (!oh || !ov) return false;
+    const hip::DevOperator *eh = f32 ? ih->op(true, true, steps) : nullptr;
+    const hip::DevOperator *ev = f32 ? iv->op(false, true, steps) : nullptr;
+    if (f32 && (!eh || !ev || !flag)) return false;
     pins.push_back(ih);
     pins.push_back(iv);
     hip::OpJob j{};
@@ -1010,6 +1017,18 @@ bool HipExecutor::add_fused(const AxisPlan &h, const AxisPlan &v, SampleKind kin
         j.out_mul = static_cast<float>(256.0 * scale);
         j.out_add = static_cast<float>(conversion->output_offset - conversion->input_offset * scale);
     }
+    if (f32) {
+        // Horizontal results stay below FP16's largest value when every
+        // sample is at most this large in magnitude.
+        j.range = static_cast<float>(65504.0 / (1.01 * std::max(1.0, ph->max_row_l1)));
+        j.flag = flag;
+        hip::OpJob e = j;
+        e.h = *eh;
+        e.v = *ev;
+        tables.add_op(j, steps, sample_of(kind));
+        tables.add_op(e, steps, sample_of(kind), true);
+        return true;
+    }
     tables.add_op(j, steps, sample_of(kind));
     return true;
 }
@@ -1023,10 +1042,13 @@ HipExecutor::Scratch &HipExecutor::solve_2d(const AxisPlan &h, const AxisPlan &v
         std::vector<std::shared_ptr<hip::DevicePlan>> pins;
         WindowTables &tables = stream_tables(stream);
         tables.clear();
-        if (add_fused(h, v, kind, b, conversion, tables, pins)) {
+        Scratch &s0 = scratch(stream, 0);
+        int *flag = kind == SampleKind::float32 ? s0.flag + 1 : nullptr;
+        if (add_fused(h, v, kind, b, conversion, tables, pins, flag)) {
+            if (flag) check(hipMemsetAsync(flag, 0, sizeof(int), stream), "flag reset failed");
             tables.upload(stream);
             tables.launch(stream);
-            return scratch(stream, 0);
+            return s0;
         }
     }
     const std::size_t elem = f64 ? 8 : 4;

@@ -38,7 +38,7 @@ enum class SolveMode : std::uint8_t {
     refined = 4,     // as automatic, but plans that retain FP64 data take the F32 route
                      // plus refinement steps instead of the Float64 route
     fused = 5,       // as automatic, and 2D calls take the fused route wherever both plans
-                     // fit it (automatic does too for u8/u16 planes unless DSMVC_HIP_FUSED=0)
+                     // fit it (automatic does too unless DSMVC_HIP_FUSED=0)
 };
 
 // Modes whose solves choose among the parallel kernels by themselves.
@@ -308,19 +308,18 @@ private:
                       const IntegerConversion *conversion, hipStream_t stream);
     Scratch &enqueue_planes(const PlaneJob *planes, std::int32_t count, SampleKind kind,
                             hipStream_t stream);
-    // True when this executor's 2D calls of `kind` take the fused route where
-    // it fits: always in fused mode; in automatic mode for integer samples,
-    // where it outruns the two passes (F32 planes run its three-part
-    // bfloat16 scheme, slower than the two passes there).
-    bool fused_route(SampleKind kind) const {
-        return mode_ == SolveMode::fused
-               || (mode_ == SolveMode::automatic && fused_auto_ && kind != SampleKind::float32);
+    // True when this executor's 2D calls take the fused route where it fits:
+    // in fused mode, and in automatic mode unless DSMVC_HIP_FUSED=0.
+    bool fused_route(SampleKind) const {
+        return mode_ == SolveMode::fused || (mode_ == SolveMode::automatic && fused_auto_);
     }
     // Adds one plane batch to `tables` as a fused 2D job when both plans fit
     // the fused kernel (and pins their images); false otherwise.
+    // F32 planes add two jobs (the FP16 pass and the exact pass it may
+    // need) sharing `flag`, which the caller clears before the launches.
     bool add_fused(const AxisPlan &h, const AxisPlan &v, SampleKind kind, const PlaneBatch &b,
                    const IntegerConversion *conversion, WindowTables &tables,
-                   std::vector<std::shared_ptr<hip::DevicePlan>> &pins);
+                   std::vector<std::shared_ptr<hip::DevicePlan>> &pins, int *flag);
     void arm_flag(Scratch &s, hipStream_t stream);
     WindowTables &stream_tables(hipStream_t stream);
     void axis_pass(const AxisPlan &plan, bool rows, const void *input, hip::Sample in_kind,

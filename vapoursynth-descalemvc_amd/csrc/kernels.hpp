@@ -160,8 +160,10 @@ hipError_t launch_check_finite(const void *src, Sample in, long s_row, long s_fr
                                int cols, int frames, int *flag, hipStream_t s);
 
 // Fused 2D route (kernels_operator.hip): one launch over `count` jobs of one
-// sample kind whose operator images were packed at `steps` K steps (2..4).
+// sample kind whose operator images were packed at `steps` K steps (2..4)
+// (FP16 parts).  `exact`: the F32 jobs' second pass on bfloat16 images,
+// which returns at once unless the first pass set the jobs' flag.
 hipError_t launch_operator_table(const OpJob *table, int count, long items, int steps, Sample kind,
-                                 hipStream_t stream);
+                                 bool exact, hipStream_t stream);
 
 } // namespace dsmvc::hip

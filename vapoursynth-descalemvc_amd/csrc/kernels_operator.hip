@@ -21,8 +21,16 @@
 //     exactly as two FP16 parts, operators and the F32 ring values as FP16 +
 //     FP16 remainder; all four part products in the horizontal product, three
 //     in the vertical one;
-//   - F32 samples: every F32 value split exactly into three bfloat16 parts
-//     by truncation; operators as three rounded bfloat16 parts; six products.
+//   - F32 samples, first launch: FP16 + FP16 remainder like the integer
+//     route (22 bits), all four part products horizontally.  FP16 has a
+//     narrow range, so every staged sample is checked: a non-finite sample,
+//     or one large enough that the horizontal results could leave FP16's
+//     range (job.range, from the operator's largest row L1 norm), sets the
+//     call's flag;
+//   - F32 samples, second launch (Exact): returns at once unless the flag is
+//     set, else recomputes the call with every F32 value split exactly into
+//     three bfloat16 parts by truncation (F32's own range), operators as
+//     three rounded bfloat16 parts, six products.
 // MFMA products of 16-bit parts are exact in F32 and accumulate in F32.  The
 // integer route applies the input offset and scale to the result (the
 // operator maps constants to themselves), then the output conversion.
@@ -112,12 +120,14 @@ __device__ __forceinline__ void split_f32_pair(float x0, float x1, std::uint32_t
     p2 = __builtin_amdgcn_perm(fbits(t1), fbits(t0), 0x07060302U);
 }
 
-// An F32 value as FP16 + FP16 remainder (round to nearest).
-__device__ __forceinline__ void split_half(float x, std::uint32_t &a, std::uint32_t &b) {
-    const _Float16 h = static_cast<_Float16>(x);
-    const _Float16 r = static_cast<_Float16>(__fsub_rn(x, static_cast<float>(h)));
-    a = __builtin_bit_cast(std::uint16_t, h);
-    b = __builtin_bit_cast(std::uint16_t, r);
+// Two F32 values as FP16 + FP16 remainder pairs (round to nearest; packed,
+// low half from x0).
+__device__ __forceinline__ void split_half_pair(float x0, float x1, std::uint32_t &hi, std::uint32_t &lo) {
+    const h2 h = h2{static_cast<_Float16>(x0), static_cast<_Float16>(x1)};
+    const float r0 = __fsub_rn(x0, static_cast<float>(h[0])), r1 = __fsub_rn(x1, static_cast<float>(h[1]));
+    const h2 r = h2{static_cast<_Float16>(r0), static_cast<_Float16>(r1)};
+    hi = __builtin_bit_cast(std::uint32_t, h);
+    lo = __builtin_bit_cast(std::uint32_t, r);
 }
 
 // Staged samples of one chunk (8 consecutive source columns of one row).
@@ -186,12 +196,26 @@ __device__ __forceinline__ void fetch_chunk(Chunk<In> &k, __amdgpu_buffer_rsrc_t
 }
 
 // Writes one chunk's parts to the staged rows: part p at
-// ylds[p][row][col .. col + 8).
-template <class In>
-__device__ __forceinline__ void stage_chunk(const Chunk<In> &k, std::uint16_t *ylds, int row, int col) {
+// ylds[p][row][col .. col + 8).  F32 samples on the FP16 scheme: returns
+// true when a sample is non-finite or above `range` in magnitude.
+template <class In, bool Exact>
+__device__ __forceinline__ bool stage_chunk(const Chunk<In> &k, std::uint16_t *ylds, int row, int col,
+                                            float range) {
     constexpr int plane = kOpSlab * kYPitch;
     u32x4 p0, p1;
-    if constexpr (std::is_same_v<In, float>) {
+    bool bad = false;
+    if constexpr (std::is_same_v<In, float> && !Exact) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float x0 = bitsf(q < 2 ? k.v[0][2 * q] : k.v[1][2 * q - 4]);
+            const float x1 = bitsf(q < 2 ? k.v[0][2 * q + 1] : k.v[1][2 * q - 3]);
+            bad |= !(__builtin_fabsf(x0) <= range) || !(__builtin_fabsf(x1) <= range);
+            std::uint32_t a, b;
+            split_half_pair(x0, x1, a, b);
+            p0[q] = a;
+            p1[q] = b;
+        }
+    } else if constexpr (std::is_same_v<In, float>) {
         u32x4 p2;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -226,6 +250,7 @@ __device__ __forceinline__ void stage_chunk(const Chunk<In> &k, std::uint16_t *y
     }
     *reinterpret_cast<u32x4 *>(ylds + row * kYPitch + col) = p0;
     *reinterpret_cast<u32x4 *>(ylds + plane + row * kYPitch + col) = p1;
+    return bad;
 }
 
 // Workgroup b runs on XCD b % 8 (each XCD has its own L2), as the b / 8-th
@@ -268,11 +293,13 @@ template <class Out> __device__ __forceinline__ void store_sample(__amdgpu_buffe
     }
 }
 
-template <class In, int S>
+template <class In, int S, bool Exact>
 __global__ __launch_bounds__(kThreads)
-__attribute__((amdgpu_waves_per_eu(std::is_same_v<In, float> ? 2 : 4)))
+__attribute__((amdgpu_waves_per_eu(Exact ? 2 : 4)))
 void operator2d(const OpJob *__restrict__ table, int count) {
-    constexpr bool Bf = std::is_same_v<In, float>;
+    static_assert(!Exact || std::is_same_v<In, float>, "the exact scheme is for F32 samples");
+    constexpr bool Bf = Exact;                              // three bfloat16 parts
+    constexpr bool Half32 = std::is_same_v<In, float> && !Exact;   // F32 samples, FP16 scheme
     constexpr int NP = Bf ? 3 : 2;
     constexpr int kSize = static_cast<int>(sizeof(In));
     constexpr int kYPlane = kOpSlab * kYPitch;          // 16-bit values per staged part
@@ -285,6 +312,11 @@ void operator2d(const OpJob *__restrict__ table, int count) {
     if (job < 0) return;
     // The job's fields, read once (scalar registers).
     const OpJob jb = table[job];
+    // The exact pass runs only when the FP16 pass found a sample it could
+    // not take.
+    if constexpr (Exact)
+        if (uniform(__hip_atomic_load(jb.flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == 0) return;
+    bool bad = false;
     const long local = item - jb.first_item;
     const int tile = static_cast<int>(local % jb.tiles);
     const long rest = local / jb.tiles;
@@ -373,7 +405,7 @@ void operator2d(const OpJob *__restrict__ table, int count) {
         __syncthreads();   // every wavefront is done with the staged rows and the ring rows below
 #pragma unroll
         for (int i = 0; i < kChunks; ++i)
-            if (cact[i]) stage_chunk<In>(buf[i], ylds, crow[i], ccol[i]);
+            if (cact[i]) bad |= stage_chunk<In, Exact>(buf[i], ylds, crow[i], ccol[i], jb.range);
         if (k + 2 < slabs) fetch(buf, k + 2);
         __syncthreads();
         f32x4 acc = f32x4{0, 0, 0, 0};
@@ -395,13 +427,11 @@ void operator2d(const OpJob *__restrict__ table, int count) {
             *reinterpret_cast<u32x2 *>(r_col + kRPlane + rr) = u32x2{b0, b1};
             *reinterpret_cast<u32x2 *>(r_col + 2 * kRPlane + rr) = u32x2{c0, c1};
         } else {
-            std::uint32_t q[NP][4];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) split_half(acc[v], q[0][v], q[1][v]);
-#pragma unroll
-            for (int p = 0; p < NP; ++p)
-                *reinterpret_cast<u32x2 *>(r_col + p * kRPlane + rr) =
-                    u32x2{q[p][0] | (q[p][1] << 16), q[p][2] | (q[p][3] << 16)};
+            std::uint32_t h0, l0, h1, l1;
+            split_half_pair(acc[0], acc[1], h0, l0);
+            split_half_pair(acc[2], acc[3], h1, l1);
+            *reinterpret_cast<u32x2 *>(r_col + rr) = u32x2{h0, h1};
+            *reinterpret_cast<u32x2 *>(r_col + kRPlane + rr) = u32x2{l0, l1};
         }
     };
     // Unit u: vertical block 2u + hb times the ring's column, and its outputs.
@@ -452,37 +482,41 @@ void operator2d(const OpJob *__restrict__ table, int count) {
             drain(k + 1);
         }
     }
+    if constexpr (Half32)
+        if (bad) __hip_atomic_fetch_or(jb.flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 } // namespace
 } // namespace dev
 
-template <class In, int S>
+template <class In, int S, bool Exact>
 static hipError_t launch_op(const OpJob *table, int count, long items, hipStream_t stream) {
     const long blocks = dev::kXcds * ((items + dev::kXcds - 1) / dev::kXcds + count);
-    hipLaunchKernelGGL((dev::operator2d<In, S>), dim3(static_cast<unsigned>(blocks)), dim3(dev::kThreads), 0,
-                       stream, table, count);
+    hipLaunchKernelGGL((dev::operator2d<In, S, Exact>), dim3(static_cast<unsigned>(blocks)), dim3(dev::kThreads),
+                       0, stream, table, count);
     return hipGetLastError();
 }
 
-template <class In>
+template <class In, bool Exact>
 static hipError_t launch_op_steps(const OpJob *table, int count, long items, int steps, hipStream_t stream) {
     switch (steps) {
-    case 2: return launch_op<In, 2>(table, count, items, stream);
-    case 3: return launch_op<In, 3>(table, count, items, stream);
-    case 4: return launch_op<In, 4>(table, count, items, stream);
+    case 2: return launch_op<In, 2, Exact>(table, count, items, stream);
+    case 3: return launch_op<In, 3, Exact>(table, count, items, stream);
+    case 4: return launch_op<In, 4, Exact>(table, count, items, stream);
     default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t launch_operator_table(const OpJob *table, int count, long items, int steps, Sample kind,
-                                 hipStream_t stream) {
+                                 bool exact, hipStream_t stream) {
     if (count < 1 || items < 1) return hipSuccess;
     if (items + 8L * count > 0x7FFFFFF0L) return hipErrorInvalidValue;
+    if (exact) return kind == Sample::f32 ? launch_op_steps<float, true>(table, count, items, steps, stream)
+                                          : hipErrorInvalidValue;
     switch (kind) {
-    case Sample::f32: return launch_op_steps<float>(table, count, items, steps, stream);
-    case Sample::u8: return launch_op_steps<std::uint8_t>(table, count, items, steps, stream);
-    case Sample::u16: return launch_op_steps<std::uint16_t>(table, count, items, steps, stream);
+    case Sample::f32: return launch_op_steps<float, false>(table, count, items, steps, stream);
+    case Sample::u8: return launch_op_steps<std::uint8_t, false>(table, count, items, steps, stream);
+    case Sample::u16: return launch_op_steps<std::uint16_t, false>(table, count, items, steps, stream);
     default: return hipErrorInvalidValue;
     }
 }

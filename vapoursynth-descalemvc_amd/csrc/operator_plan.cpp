@@ -175,10 +175,13 @@ bool build_operator_plan(const AxisPlan &plan, OperatorPlan &out) {
         if (total > 0.0) out.dropped = std::max(out.dropped, dropped / total);
         out.lo[j] = cmin + static_cast<int>(a);
         out.count[j] = static_cast<int>(b - a);
+        double l1 = 0.0;
         for (std::size_t k = a; k < b; ++k) {
             out.values.push_back(row[k]);
             out.max_abs = std::max(out.max_abs, std::fabs(row[k]));
+            l1 += std::fabs(row[k]);
         }
+        out.max_row_l1 = std::max(out.max_row_l1, l1);
         out.start[j + 1] = static_cast<std::uint32_t>(out.values.size());
     }
 

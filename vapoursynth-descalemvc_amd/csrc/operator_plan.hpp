@@ -27,6 +27,7 @@ struct OperatorPlan {
     std::int32_t steps = 0;          // K steps of kOpStep source indices per block
     double dropped = 0.0;            // largest L1 share dropped from a row
     double max_abs = 0.0;            // largest |M(j, c)|
+    double max_row_l1 = 0.0;         // largest sum_c |M(j, c)| over the kept band
     std::vector<std::int32_t> lo;    // [n] first source index of row j's band
     std::vector<std::int32_t> count; // [n] band length
     std::vector<std::uint32_t> start;// [n + 1] row j's values at values[start[j] ..]
