@@ -103,7 +103,8 @@ void WindowTables::add_check(const hip::CheckJob &job) {
 }
 
 void WindowTables::add_op(const hip::OpJob &job, int steps, hip::Sample kind, bool exact) {
-    Group &g = group_for(Kind::op, exact ? 1 : 0, steps, true, false, kind);
+    // `block16` marks the launches whose jobs all allow wide source loads.
+    Group &g = group_for(Kind::op, exact ? 1 : 0, steps, true, job.vector_loads != 0, kind);
     hip::OpJob j = job;
     j.first_item = g.items;
     g.items += hip::operator_items(j);
@@ -208,7 +209,7 @@ void WindowTables::launch(hipStream_t stream) {
             break;
         case Kind::op:
             e = hip::launch_operator_table(reinterpret_cast<const hip::OpJob *>(at), count, g.items,
-                                           g.width, g.kind, g.band != 0, stream);
+                                           g.width, g.kind, g.band != 0, g.block16, stream);
             break;
         }
         check(e, "table launch failed");

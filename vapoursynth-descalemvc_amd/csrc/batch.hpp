@@ -92,7 +92,7 @@ private:
         Kind type = Kind::f32;
         int band = 0, width = 0;
         bool rows = false;
-        bool block16 = false;       // 16-row staging blocks (F32)
+        bool block16 = false;       // 16-row staging blocks (F32); wide source loads (fused jobs)
         hip::Sample kind = hip::Sample::f32;
         std::vector<hip::WinJob> jobs;
         std::vector<hip::Win64Job> jobs64;

@@ -139,6 +139,10 @@ constexpr int kOpTileBlocks = 4;
 constexpr int kOpTileCols = kOpBlock * kOpTileBlocks;
 constexpr int kOpUnit = 2 * kOpBlock;
 constexpr int kOpSlab = 32;
+// Most units of a vertical operator that complete with one slab (near the
+// plane's edges, and for near-identity plans, several units' K ranges start
+// in the same slab).
+constexpr int kOpMaxUnitsPerSlab = 4;
 constexpr int kOpStageCols = 208;
 constexpr int kOpRing = 192;
 

@@ -970,13 +970,16 @@ bool HipExecutor::add_fused(const AxisPlan &h, const AxisPlan &v, SampleKind kin
     const std::shared_ptr<hip::DevicePlan> ih = device_plan(h), iv = device_plan(v);
     const hip::OperatorPlan *ph = ih->operator_plan(), *pv = iv->operator_plan();
     if (!ph || !pv) return false;
-    // The kernel addresses one frame's plane with 32-bit byte offsets.
+    // The kernel addresses one frame's plane with 32-bit byte offsets; its
+    // stores run over whole units of rows (the rows past the plane are
+    // discarded by the buffer range check), so their offsets are bounded too.
     const std::size_t bytes = kind == SampleKind::float32 ? 4 : (kind == SampleKind::u16 ? 2 : 1);
     const auto span = [&](std::ptrdiff_t row, std::int32_t rows, std::int32_t cols) {
         return (static_cast<std::size_t>(rows - 1) * static_cast<std::size_t>(row) + cols) * bytes;
     };
+    const std::int32_t unit_rows = (v.destination_size + hip::kOpUnit - 1) / hip::kOpUnit * hip::kOpUnit;
     if (span(b.input_row_stride, v.source_size, h.source_size) >= 0x7FFFFFF0U
-        || span(b.output_row_stride, v.destination_size, h.destination_size) >= 0x7FFFFFF0U)
+        || span(b.output_row_stride, unit_rows, h.destination_size) >= 0x7FFFFFF0U)
         return false;
     const int steps = std::max({ph->steps, pv->steps, 2});
     const bool f32 = kind == SampleKind::float32;

@@ -163,7 +163,9 @@ hipError_t launch_check_finite(const void *src, Sample in, long s_row, long s_fr
 // sample kind whose operator images were packed at `steps` K steps (2..4)
 // (FP16 parts).  `exact`: the F32 jobs' second pass on bfloat16 images,
 // which returns at once unless the first pass set the jobs' flag.
+// `vector`: every job's source base and strides allow 16-byte loads (8 for
+// u8 samples), i.e. each has vector_loads set.
 hipError_t launch_operator_table(const OpJob *table, int count, long items, int steps, Sample kind,
-                                 bool exact, hipStream_t stream);
+                                 bool exact, bool vector, hipStream_t stream);
 
 } // namespace dsmvc::hip

@@ -136,22 +136,26 @@ template <> struct Chunk<std::uint16_t> { u32x4 v; };
 template <> struct Chunk<std::uint8_t> { u32x2 v; };
 template <> struct Chunk<float> { u32x4 v[2]; };
 
+template <class In> __device__ __forceinline__ std::uint32_t chunk_word(const Chunk<In> &k) {
+    if constexpr (std::is_same_v<In, float>) return k.v[0][0];
+    else return k.v[0];
+}
+
 // Byte offset that every buffer load treats as out of range (reads 0).
 constexpr int kOutside = 0x7FFFFFF0;
 
 // One chunk from the plane's buffer resource at byte offset `off` (kOutside
-// for a row past the plane).  `vector`: the offset is aligned for one wide
+// for a row past the plane).  `Vec`: the offset is aligned for one wide
 // load; otherwise one load per sample.  Columns at or past the plane's width
 // read whatever follows in memory (the row's padding or the next row): the
 // operator's entries there are zero, and an integer sample times zero is
-// zero; F32 samples there are cleared (`cols`), since padding may hold
-// non-finite bits.  Buffer loads never touch the LDS counter, so the chunk
-// stays in flight across the wavefront's LDS work.
-template <class In>
-__device__ __forceinline__ void fetch_chunk(Chunk<In> &k, __amdgpu_buffer_rsrc_t rs, int off, int c, int cols,
-                                            bool vector) {
+// zero; F32 samples there are cleared when staged (clear_past), since
+// padding may hold non-finite bits.  Buffer loads never touch the LDS
+// counter, so the chunk stays in flight across the wavefront's LDS work.
+template <class In, bool Vec>
+__device__ __forceinline__ void fetch_chunk(Chunk<In> &k, __amdgpu_buffer_rsrc_t rs, int off) {
     if constexpr (std::is_same_v<In, std::uint16_t>) {
-        if (vector) {
+        if constexpr (Vec) {
             k.v = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
         } else {
 #pragma unroll
@@ -162,7 +166,7 @@ __device__ __forceinline__ void fetch_chunk(Chunk<In> &k, __amdgpu_buffer_rsrc_t
             }
         }
     } else if constexpr (std::is_same_v<In, std::uint8_t>) {
-        if (vector) {
+        if constexpr (Vec) {
             k.v = __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, 0));
         } else {
 #pragma unroll
@@ -175,7 +179,7 @@ __device__ __forceinline__ void fetch_chunk(Chunk<In> &k, __amdgpu_buffer_rsrc_t
             }
         }
     } else {
-        if (vector) {
+        if constexpr (Vec) {
             k.v[0] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
             k.v[1] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rs, off + 16, 0, 0));
         } else {
@@ -185,12 +189,16 @@ __device__ __forceinline__ void fetch_chunk(Chunk<In> &k, __amdgpu_buffer_rsrc_t
                 k.v[1][q] = __builtin_amdgcn_raw_buffer_load_b32(rs, off + 16 + 4 * q, 0, 0);
             }
         }
-        if (c + 8 > cols) {
+    }
+}
+
+// F32 samples of a chunk at columns c.. that lie at or past `cols`, cleared.
+__device__ __forceinline__ void clear_past(Chunk<float> &k, int c, int cols) {
+    if (c + 8 > cols) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if (c + q >= cols) k.v[0][q] = 0U;
-                if (c + 4 + q >= cols) k.v[1][q] = 0U;
-            }
+        for (int q = 0; q < 4; ++q) {
+            if (c + q >= cols) k.v[0][q] = 0U;
+            if (c + 4 + q >= cols) k.v[1][q] = 0U;
         }
     }
 }
@@ -278,14 +286,17 @@ __device__ __forceinline__ int balanced_item(const OpJob *table, int count, long
 
 // Output sample from the result x: F32 as is; integer samples
 // clamp(x * mul + add, 0, max) rounded to nearest even, mul and add folding
-// the input and output conversions.
+// the input and output conversions.  A store at or past the resource's size
+// (a row past the plane) is discarded by the buffer's range check.
 template <class Out> __device__ __forceinline__ void store_sample(__amdgpu_buffer_rsrc_t rs, int off, float x,
                                                                 float mul, float add, float max) {
     if constexpr (std::is_same_v<Out, float>) {
         __builtin_amdgcn_raw_buffer_store_b32(fbits(x), rs, off, 0, 0);
     } else {
+        // v + 2^23 rounds v in [0, 2^23) to the nearest integer, ties to
+        // even, and leaves it in the low mantissa bits.
         const float v = __builtin_amdgcn_fmed3f(__builtin_fmaf(x, mul, add), 0.0F, max);
-        const auto q = static_cast<std::uint32_t>(__builtin_rintf(v));
+        const std::uint32_t q = fbits(v + 8388608.0F);
         if constexpr (std::is_same_v<Out, std::uint16_t>)
             __builtin_amdgcn_raw_buffer_store_b16(static_cast<std::uint16_t>(q), rs, off, 0, 0);
         else
@@ -293,7 +304,8 @@ template <class Out> __device__ __forceinline__ void store_sample(__amdgpu_buffe
     }
 }
 
-template <class In, int S, bool Exact>
+// Vec: every job of the launch allows 16-byte source loads (8 for u8).
+template <class In, int S, bool Exact, bool Vec>
 __global__ __launch_bounds__(kThreads)
 __attribute__((amdgpu_waves_per_eu(Exact ? 2 : 4)))
 void operator2d(const OpJob *__restrict__ table, int count) {
@@ -366,14 +378,14 @@ void operator2d(const OpJob *__restrict__ table, int count) {
     }
     const auto rs = plane_rsrc(static_cast<const In *>(jb.src) + frame * jb.s_frame,
                                (static_cast<long>(mv - 1) * s_row + mh) * kSize);
-    const bool vector = jb.vector_loads != 0;
     const int slab_bytes = static_cast<int>(kOpSlab * s_row * kSize);
+    // A slab at or past `slabs` is fetched with every load out of range.
     auto fetch = [&](Chunk<In> (&buf)[kChunks], int k) {
-        const int rows_left = mv - first - kOpSlab * k;   // slab rows inside the plane
+        const int rows_left = k < slabs ? mv - first - kOpSlab * k : 0;   // slab rows inside the plane
 #pragma unroll
         for (int i = 0; i < kChunks; ++i) {
             const int off = cact[i] && crow[i] < rows_left ? coff[i] + k * slab_bytes : kOutside;
-            fetch_chunk<In>(buf[i], rs, off, cs0 + ccol[i], mh, vector);
+            fetch_chunk<In, Vec>(buf[i], rs, off);
         }
     };
 
@@ -391,7 +403,7 @@ void operator2d(const OpJob *__restrict__ table, int count) {
                                (static_cast<long>(nv - 1) * jb.d_row + nh) * kSize);
     const GlobalPtr<u32x4> vdata = global(reinterpret_cast<const u32x4 *>(jb.v.data)) + l;
 
-    // Vertical block 2u + hb of unit u (loaded a unit ahead; it completes under the slabs).
+    // Vertical block 2u + hb of unit u.
     u32x4 av[S][NP];
     auto load_unit = [&](int u) {
 #pragma unroll
@@ -400,14 +412,8 @@ void operator2d(const OpJob *__restrict__ table, int count) {
             for (int p = 0; p < NP; ++p)
                 av[s][p] = vdata[((static_cast<long>(2 * u + hb) * S + s) * NP + p) * 64];
     };
-    // Slab k from its prefetched chunks, whose buffer then fetches slab k + 2.
-    auto slab = [&](Chunk<In> (&buf)[kChunks], int k) {
-        __syncthreads();   // every wavefront is done with the staged rows and the ring rows below
-#pragma unroll
-        for (int i = 0; i < kChunks; ++i)
-            if (cact[i]) bad |= stage_chunk<In, Exact>(buf[i], ylds, crow[i], ccol[i], jb.range);
-        if (k + 2 < slabs) fetch(buf, k + 2);
-        __syncthreads();
+    // Slab k x horizontal block cb, into the ring (split into parts).
+    auto horizontal = [&](int k) {
         f32x4 acc = f32x4{0, 0, 0, 0};
 #pragma unroll
         for (int s = 0; s < S; ++s) {
@@ -434,9 +440,10 @@ void operator2d(const OpJob *__restrict__ table, int count) {
             *reinterpret_cast<u32x2 *>(r_col + kRPlane + rr) = u32x2{l0, l1};
         }
     };
-    // Unit u: vertical block 2u + hb times the ring's column, and its outputs.
+    // Unit u: vertical block 2u + hb times the ring's column, and the lane's
+    // four outputs.  Rows at or past nv fall outside `ws` (the host keeps the
+    // offsets of a unit's rows below 2^31).
     auto unit = [&](int u) {
-        __syncthreads();   // the ring rows of this unit are written
         f32x4 out = f32x4{0, 0, 0, 0};
         const int k0 = kv[2 * u + hb] % kOpRing;
 #pragma unroll
@@ -450,38 +457,71 @@ void operator2d(const OpJob *__restrict__ table, int count) {
             out = products<Bf>(av[s], b, out);
         }
         if (col0 < nh) {
-            const int d0 = kOpUnit * u + kOpBlock * hb;           // wave-uniform
-            const int base = d0 * d_row_bytes + lane_out;
+            const int base = (kOpUnit * u + kOpBlock * hb) * d_row_bytes + lane_out;
 #pragma unroll
             for (int v = 0; v < 4; ++v)
-                if (d0 + 4 * lk + v < nv)
-                    store_sample<In>(ws, base + v * d_row_bytes, out[v], out_mul, out_add, out_max);
+                store_sample<In>(ws, base + v * d_row_bytes, out[v], out_mul, out_add, out_max);
         }
     };
-
-    Chunk<In> pa[kChunks], pb[kChunks];
-    fetch(pa, 0);
-    if (slabs > 1) fetch(pb, 1);
-    load_unit(u0);
+    // Whether the next unit's source rows all lie below `produced`.
     int u = u0;
-    // After slab k the ring holds source rows below first + 32 (k + 1);
-    // every unit whose rows are all there is finished before the next slab.
-    auto drain = [&](int k) {
-        const int produced = first + kOpSlab * (k + 1);
-        while (u < u1 && kv[2 * u + 1] + kOpStep * S <= produced) {
+    auto ready = [&](int produced) { return u < u1 && kv[2 * u + 1] + kOpStep * S <= produced; };
+    // The units whose source rows all lie below `produced`: at most
+    // kOpMaxUnitsPerSlab (the host checks the vertical operator).  The first
+    // one's operator blocks are loaded already; further ones (near the
+    // plane's edges, near-identity plans) load their own.  The loop is
+    // unrolled whole: a back edge would merge its pending loads and stores
+    // into the first unit's waits.  Returns whether any ran.
+    auto units = [&](int produced) {
+        if (!ready(produced)) return false;
+        unit(u);
+        ++u;
+#pragma unroll
+        for (int t = 1; t < kOpMaxUnitsPerSlab; ++t) {
+            if (!ready(produced)) break;
+            load_unit(u);
             unit(u);
             ++u;
-            if (u < u1) load_unit(u);
         }
+        return true;
     };
-    for (int k = 0; k < slabs; k += 2) {
-        slab(pa, k);
-        drain(k);
-        if (k + 1 < slabs) {
-            slab(pb, k + 1);
-            drain(k + 1);
+
+    // Step k: stage slab k (its loads were issued a step earlier), fetch
+    // slab k + 1, run the unit that the slabs below k completed (its operator
+    // blocks were loaded a step earlier), load the next unit's blocks, then
+    // multiply slab k into the ring.  The compiler waits for every
+    // outstanding vector-memory operation when stores are among them, so
+    // each step has one such wait, at the staging, and everything issued
+    // after it has a whole step to complete.  A unit run in step k reads ring
+    // rows that slab k may overwrite, so a barrier follows it.  Every chunk's
+    // first word goes into
+    // `sink`, which keeps the loads of inactive chunks issued (a fixed set of
+    // loads per step) and makes the staging's wait one point on every path.
+    Chunk<In> buf[kChunks];
+    std::uint32_t sink = 0;
+    load_unit(u0);
+    fetch(buf, 0);
+    for (int k = 0; k < slabs; ++k) {
+        __syncthreads();   // every wavefront is done with the staged rows of slab k - 1
+#pragma unroll
+        for (int i = 0; i < kChunks; ++i) sink ^= chunk_word(buf[i]);
+        asm volatile("" : "+v"(sink));
+#pragma unroll
+        for (int i = 0; i < kChunks; ++i) {
+            if constexpr (std::is_same_v<In, float>) clear_past(buf[i], cs0 + ccol[i], mh);
+            if (cact[i]) bad |= stage_chunk<In, Exact>(buf[i], ylds, crow[i], ccol[i], jb.range);
         }
+        fetch(buf, k + 1);
+        __syncthreads();   // slab k staged; the ring rows of slabs below k written
+        if (units(first + kOpSlab * k)) {
+            if (u < u1) load_unit(u);
+            __syncthreads();
+        }
+        horizontal(k);
     }
+    __syncthreads();
+    units(first + kOpSlab * slabs);
+    asm volatile("" ::"v"(sink));
     if constexpr (Half32)
         if (bad) __hip_atomic_fetch_or(jb.flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -489,36 +529,44 @@ void operator2d(const OpJob *__restrict__ table, int count) {
 } // namespace
 } // namespace dev
 
-template <class In, int S, bool Exact>
+template <class In, int S, bool Exact, bool Vec>
 static hipError_t launch_op(const OpJob *table, int count, long items, hipStream_t stream) {
     const long blocks = dev::kXcds * ((items + dev::kXcds - 1) / dev::kXcds + count);
-    hipLaunchKernelGGL((dev::operator2d<In, S, Exact>), dim3(static_cast<unsigned>(blocks)), dim3(dev::kThreads),
-                       0, stream, table, count);
+    hipLaunchKernelGGL((dev::operator2d<In, S, Exact, Vec>), dim3(static_cast<unsigned>(blocks)),
+                       dim3(dev::kThreads), 0, stream, table, count);
     return hipGetLastError();
 }
 
-template <class In, bool Exact>
+template <class In, bool Exact, bool Vec>
 static hipError_t launch_op_steps(const OpJob *table, int count, long items, int steps, hipStream_t stream) {
     switch (steps) {
-    case 2: return launch_op<In, 2, Exact>(table, count, items, stream);
-    case 3: return launch_op<In, 3, Exact>(table, count, items, stream);
-    case 4: return launch_op<In, 4, Exact>(table, count, items, stream);
+    case 2: return launch_op<In, 2, Exact, Vec>(table, count, items, stream);
+    case 3: return launch_op<In, 3, Exact, Vec>(table, count, items, stream);
+    case 4: return launch_op<In, 4, Exact, Vec>(table, count, items, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <bool Vec>
+static hipError_t launch_op_kind(const OpJob *table, int count, long items, int steps, Sample kind, bool exact,
+                                 hipStream_t stream) {
+    if (exact)
+        return kind == Sample::f32 ? launch_op_steps<float, true, Vec>(table, count, items, steps, stream)
+                                   : hipErrorInvalidValue;
+    switch (kind) {
+    case Sample::f32: return launch_op_steps<float, false, Vec>(table, count, items, steps, stream);
+    case Sample::u8: return launch_op_steps<std::uint8_t, false, Vec>(table, count, items, steps, stream);
+    case Sample::u16: return launch_op_steps<std::uint16_t, false, Vec>(table, count, items, steps, stream);
     default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t launch_operator_table(const OpJob *table, int count, long items, int steps, Sample kind,
-                                 bool exact, hipStream_t stream) {
+                                 bool exact, bool vector, hipStream_t stream) {
     if (count < 1 || items < 1) return hipSuccess;
     if (items + 8L * count > 0x7FFFFFF0L) return hipErrorInvalidValue;
-    if (exact) return kind == Sample::f32 ? launch_op_steps<float, true>(table, count, items, steps, stream)
-                                          : hipErrorInvalidValue;
-    switch (kind) {
-    case Sample::f32: return launch_op_steps<float, false>(table, count, items, steps, stream);
-    case Sample::u8: return launch_op_steps<std::uint8_t, false>(table, count, items, steps, stream);
-    case Sample::u16: return launch_op_steps<std::uint16_t, false>(table, count, items, steps, stream);
-    default: return hipErrorInvalidValue;
-    }
+    return vector ? launch_op_kind<true>(table, count, items, steps, kind, exact, stream)
+                  : launch_op_kind<false>(table, count, items, steps, kind, exact, stream);
 }
 
 } // namespace dsmvc::hip

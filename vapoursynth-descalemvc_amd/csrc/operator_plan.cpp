@@ -220,9 +220,18 @@ bool operator_fits(const OperatorPlan &op, bool rows, int steps) {
         for (int t = 0; t < op.blocks / kOpTileBlocks; ++t)
             if (op.kstart[4 * t + 3] - op.kstart[4 * t] + kOpStep * steps > kOpStageCols) return false;
     } else {
-        for (int u = 0; u < op.blocks / 2; ++u)
+        const int units = op.blocks / 2;
+        for (int u = 0; u < units; ++u)
             if (op.kstart[2 * u + 1] - op.kstart[2 * u] + kOpStep * steps + kOpSlab - 1 > kOpRing)
                 return false;
+        // A unit completes with the slab that holds its last block's K
+        // start; slabs are aligned to kOpSlab rows.  At most kOpMaxUnitsPerSlab
+        // of the units the kernel runs (ceil(n / kOpUnit)) may complete with
+        // one slab.
+        const int run = (op.n + kOpUnit - 1) / kOpUnit;
+        const auto slab_of = [&](int u) { return (op.kstart[2 * u + 1] + kOpSlab - 1) / kOpSlab; };
+        for (int u = 0; u + kOpMaxUnitsPerSlab < run; ++u)
+            if (slab_of(u + kOpMaxUnitsPerSlab) == slab_of(u)) return false;
     }
     return true;
 }
