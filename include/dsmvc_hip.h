@@ -509,7 +509,7 @@ typedef struct dsmvc_hip_batch_info {
     int64_t items;                   /* wavefront work items per run */
     int64_t intermediate_bytes;
     int32_t graph;                   /* 1: runs replay a captured HIP graph */
-    int32_t reserved;
+    int32_t fused_jobs;              /* of the table jobs, those on the fused 2D route */
 } dsmvc_hip_batch_info;
 
 typedef struct dsmvc_hip_batch dsmvc_hip_batch;

@@ -8,7 +8,11 @@ Bars:
     captured HIP graph;
   * jobs the table kernels cannot take (Float64 route) are solved one by one
     and equal the executor's own result for that job bit for bit.
+F32 jobs whose operators fit take the fused 2D route in the tables; the
+two-pass executor (fused route off) keeps the windowed tables covered.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -35,6 +39,23 @@ def executor():
     return dh.Executor(0, "auto")
 
 
+@pytest.fixture(scope="module")
+def two_pass_executor():
+    """Automatic mode with the fused 2D route off (DSMVC_HIP_FUSED=0 when the
+    executor is made): batch jobs take the windowed tables or the fallback."""
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    old = os.environ.get("DSMVC_HIP_FUSED")
+    os.environ["DSMVC_HIP_FUSED"] = "0"
+    try:
+        return dh.Executor(0, "auto")
+    finally:
+        if old is None:
+            del os.environ["DSMVC_HIP_FUSED"]
+        else:
+            os.environ["DSMVC_HIP_FUSED"] = old
+
+
 def _mitchell_jobs(heights, x):
     jobs = []
     for h in heights:
@@ -46,21 +67,29 @@ def _mitchell_jobs(heights, x):
     return jobs
 
 
-def test_batch_of_uneven_plans_matches_oracle(executor, oracle):
+@pytest.mark.parametrize("route", ["fused", "two-pass"])
+def test_batch_of_uneven_plans_matches_oracle(route, executor, two_pass_executor, oracle):
     rng = np.random.default_rng(4)
     frame = rng.uniform(0, 1, (1080, 1920)).astype(np.float32)
     x = torch.from_numpy(frame).to(DEV)
     heights = [500, 537, 600, 719, 720, 811, 900, 999, 1000]
     jobs = _mitchell_jobs(heights, x)
-    batch = dh.Batch(executor, [(hp, vp, x, out) for _, _, hp, vp, out in jobs])
+    ex = executor if route == "fused" else two_pass_executor
+    batch = dh.Batch(ex, [(hp, vp, x, out) for _, _, hp, vp, out in jobs])
     info = batch.info()
-    print(info, flush=True)
+    print(route, info, flush=True)
     assert info["jobs"] == len(heights) and info["table_jobs"] == len(heights)
-    # One launch per (direction, band class, tap width, staging block size)
-    # group: at most two block sizes per (direction, band class, tap width).
+    # Fused jobs: two launches (the FP16 pass and the exact pass); windowed
+    # jobs: one launch per (direction, band class, tap width, staging block
+    # size) group, at most two block sizes per (direction, band class, tap
+    # width).
     classes = {(rows, p.window_info()["band"], p.window_info()["width"])
                for _, _, hp, vp, _ in jobs for rows, p in ((True, hp), (False, vp))}
-    assert info["fallback_jobs"] == 0 and info["launches"] <= 2 * len(classes)
+    assert info["fallback_jobs"] == 0 and info["launches"] <= 2 + 2 * len(classes)
+    if route == "fused":
+        assert info["fused_jobs"] >= 1
+    else:
+        assert info["fused_jobs"] == 0
     batch.run()
     for (hr, vr, _, _, out), h in zip(jobs, heights):
         got = out.cpu().numpy()
@@ -118,7 +147,7 @@ def test_batch_solves_float64_jobs_one_by_one(executor):
     assert torch.equal(out64, expected)
 
 
-def test_batch_job_with_one_windowed_axis_runs_once_on_the_fallback(executor, oracle):
+def test_batch_job_with_one_windowed_axis_runs_once_on_the_fallback(two_pass_executor, oracle):
     """A job whose horizontal plan the windowed kernels take but whose
     vertical plan they do not (band 9) is counted as a fallback job, solved
     once by the executor, and equals the executor's own result; a windowed
@@ -134,6 +163,7 @@ def test_batch_job_with_one_windowed_axis_runs_once_on_the_fallback(executor, or
     fp = _plan(make_request(960, 700, kernel=BICUBIC))
     mixed = torch.full((360, 640), float("nan"), device=DEV)
     plain = torch.full((400, 700), float("nan"), device=DEV)
+    executor = two_pass_executor
     batch = dh.Batch(executor, [(hp, vp, x, mixed), (fp, gp, x, plain)])
     info = batch.info()
     assert info["fallback_jobs"] == 1 and info["table_jobs"] == 1

@@ -58,7 +58,16 @@ WindowTables::~WindowTables() {
     }
 }
 
-void WindowTables::clear() { groups_.clear(); }
+void WindowTables::clear() {
+    groups_.clear();
+    flags_ = nullptr;
+    flag_count_ = 0;
+}
+
+void WindowTables::clear_flags(int *flags, int count) {
+    flags_ = flags;
+    flag_count_ = flags ? count : 0;
+}
 
 WindowTables::Group &WindowTables::group_for(Kind type, int band, int width, bool rows,
                                              bool block16, hip::Sample kind) {
@@ -184,6 +193,7 @@ int WindowTables::upload(hipStream_t stream) {
 
 void WindowTables::launch(hipStream_t stream) {
     Slot &slot = slots_[current_];
+    if (flag_count_ > 0) check(hip::launch_fill_i32(flags_, flag_count_, 0, stream), "flag reset failed");
     for (const Group &g : groups_) {
         if (g.items == 0) continue;
         // Checks and horizontal groups come first (upload() sorts them).
@@ -249,6 +259,10 @@ HipBatch::HipBatch(HipExecutor &executor, std::vector<BatchJob> jobs, bool graph
     // Device images of the table jobs, held (pinned in the executor's plan
     // cache) for the batch's lifetime: the tables point into them.
     std::vector<std::shared_ptr<hip::DevicePlan>> ih(jobs_.size()), iv(jobs_.size());
+    // Jobs whose operators fit the fused 2D route (one kernel, no
+    // intermediate); their images are built with the others.
+    const bool fused = executor_.fused_enabled(SampleKind::float32);
+    std::vector<char> fit(jobs_.size(), 0);
     if (parallel) {
         // Device images and window tables of every job, built on host
         // threads (a scan brings hundreds of new plans).  Each task writes
@@ -263,6 +277,15 @@ HipBatch::HipBatch(HipExecutor &executor, std::vector<BatchJob> jobs, bool graph
                 try {
                     ih[k] = executor_.plan_image(h);
                     iv[k] = executor_.plan_image(v);
+                    if (fused) {
+                        const hip::OperatorPlan *oh = ih[k]->operator_plan(), *ov = iv[k]->operator_plan();
+                        if (oh && ov) {
+                            const int steps = std::max({oh->steps, ov->steps, 2});
+                            fit[k] = ih[k]->op(true, false, steps) && iv[k]->op(false, false, steps)
+                                     && ih[k]->op(true, true, steps) && iv[k]->op(false, true, steps);
+                        }
+                        if (fit[k]) continue;
+                    }
                     wh[k] = ih[k]->window(true);
                     wv[k] = iv[k]->window(false);
                 } catch (...) {
@@ -287,10 +310,28 @@ HipBatch::HipBatch(HipExecutor &executor, std::vector<BatchJob> jobs, bool graph
         for (const auto &e : failed)
             if (e) std::rethrow_exception(e);
     }
+    if (std::find(fit.begin(), fit.end(), 1) != fit.end())
+        check(hipMalloc(reinterpret_cast<void **>(&flags_), jobs_.size() * sizeof(int)),
+              "batch flag allocation failed");
     std::size_t total = 0;
     for (std::size_t k = 0; k < jobs_.size(); ++k) {
         const BatchJob &j = jobs_[k];
         const AxisPlan &h = *j.horizontal, &v = *j.vertical;
+        if (fit[k]) {
+            // Whole columns per workgroup: the table holds many jobs.
+            if (executor_.fused_job(h, v, SampleKind::float32,
+                                    PlaneBatch{j.input, j.input_row_stride, j.input_frame_stride, j.output,
+                                               j.output_row_stride, j.output_frame_stride, j.frames},
+                                    nullptr, tables_, images_, flags_ + k)) {
+                ++stats_.fused_jobs;
+                ++stats_.table_jobs;
+                continue;
+            }
+            fit[k] = 0;
+            // Not taken after all: the windowed tables, if they fit.
+            wh[k] = ih[k]->window(true);
+            wv[k] = iv[k]->window(false);
+        }
         if (!wh[k] || !wv[k]) {
             if (executor_.mode() == SolveMode::windowed)
                 throw std::invalid_argument("dsmvc: batch job cannot use the windowed HIP solve");
@@ -304,10 +345,12 @@ HipBatch::HipBatch(HipExecutor &executor, std::vector<BatchJob> jobs, bool graph
                           * static_cast<std::size_t>(j.frames) * sizeof(float));
     }
     if (total) check(hipMalloc(&intermediate_, total), "batch intermediate allocation failed");
+    if (flags_) tables_.clear_flags(flags_, static_cast<int>(jobs_.size()));
     for (std::size_t k = 0; k < jobs_.size(); ++k) {
         // Jobs with either pass outside the windowed kernels went to the
-        // fallback list above and are solved there only.
-        if (!wh[k] || !wv[k]) continue;
+        // fallback list above and are solved there only; fused jobs are in
+        // the tables already.
+        if (fit[k] || !wh[k] || !wv[k]) continue;
         const BatchJob &j = jobs_[k];
         const AxisPlan &h = *j.horizontal, &v = *j.vertical;
         const std::ptrdiff_t row = pitch_of(h.destination_size);
@@ -373,6 +416,7 @@ HipBatch::~HipBatch() {
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     if (intermediate_) (void)hipFree(intermediate_);
+    if (flags_) (void)hipFree(flags_);
 }
 
 void HipBatch::run(hipStream_t stream, bool async) {

@@ -46,6 +46,7 @@ struct BatchStats {
     std::int32_t jobs = 0;
     std::int32_t table_jobs = 0;       // solved by the table launches
     std::int32_t fallback_jobs = 0;    // solved one by one
+    std::int32_t fused_jobs = 0;       // of the table jobs, those on the fused 2D route
     std::int32_t launches = 0;         // table launches per run
     std::int64_t items = 0;            // wavefront items per run
     std::int64_t intermediate_bytes = 0;
@@ -77,6 +78,10 @@ public:
     // `exact`: an F32 job's second pass (bfloat16 images), launched after
     // every first pass.
     void add_op(const hip::OpJob &job, int steps, hip::Sample kind, bool exact = false);
+    // Device flags [flags, flags + count) that every launch() sets to zero
+    // (with a kernel) before its first table launch: the F32 fused jobs'
+    // range flags.  clear() forgets them.
+    void clear_flags(int *flags, int count);
     // Uploads the tables (stream-ordered when `stream` is given, else
     // synchronously) and returns the number of launches `launch` makes.
     int upload(hipStream_t stream);
@@ -108,6 +113,8 @@ private:
                    + checks.size() * sizeof(hip::CheckJob) + ops.size() * sizeof(hip::OpJob);
         }
     };
+    int *flags_ = nullptr;
+    int flag_count_ = 0;
     struct Slot {
         unsigned char *host = nullptr;     // pinned
         unsigned char *device = nullptr;
@@ -146,6 +153,7 @@ private:
     std::vector<std::shared_ptr<hip::DevicePlan>> images_;
     WindowTables tables_;
     void *intermediate_ = nullptr;
+    int *flags_ = nullptr;             // the fused jobs' range flags, one per job
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;
     BatchStats stats_{};

@@ -827,6 +827,7 @@ int dsmvc_hip_batch_info_get(const dsmvc_hip_batch *batch, dsmvc_hip_batch_info 
         info->items = s.items;
         info->intermediate_bytes = s.intermediate_bytes;
         info->graph = s.graph ? 1 : 0;
+        info->fused_jobs = s.fused_jobs;
     });
 }
 

@@ -838,8 +838,7 @@ HipExecutor::Scratch &HipExecutor::enqueue_planes(const PlaneJob *planes, std::i
     if (total || any_fused) {
         Scratch &s = scratch(stream, total);
         last = &s;
-        if (any_fused && fused_flag)
-            check(hipMemsetAsync(fused_flag, 0, sizeof(int), stream), "flag reset failed");
+        if (any_fused && fused_flag) tables.clear_flags(fused_flag, 1);
         const Sample sk = sample_of(kind);
         bool armed = false;
         for (std::size_t k = 0; k < n; ++k) {
@@ -964,9 +963,19 @@ void HipExecutor::inverse_2d_batch(const AxisPlan &h, const AxisPlan &v, SampleK
              [&](hipStream_t st) -> Scratch & { return solve_2d(h, v, kind, b, conversion, st); });
 }
 
-bool HipExecutor::add_fused(const AxisPlan &h, const AxisPlan &v, SampleKind kind, const PlaneBatch &b,
+bool HipExecutor::fused_job(const AxisPlan &h, const AxisPlan &v, SampleKind kind, const PlaneBatch &b,
                             const IntegerConversion *conversion, WindowTables &tables,
                             std::vector<std::shared_ptr<hip::DevicePlan>> &pins, int *flag) {
+    if (!fused_route(kind) || float64_route(h) || float64_route(v)) return false;
+    std::lock_guard lock(mutex_);
+    check(hipSetDevice(device_), "hipSetDevice failed");
+    return add_fused(h, v, kind, b, conversion, tables, pins, flag, true);
+}
+
+bool HipExecutor::add_fused(const AxisPlan &h, const AxisPlan &v, SampleKind kind, const PlaneBatch &b,
+                            const IntegerConversion *conversion, WindowTables &tables,
+                            std::vector<std::shared_ptr<hip::DevicePlan>> &pins, int *flag,
+                            bool whole_columns) {
     const std::shared_ptr<hip::DevicePlan> ih = device_plan(h), iv = device_plan(v);
     const hip::OperatorPlan *ph = ih->operator_plan(), *pv = iv->operator_plan();
     if (!ph || !pv) return false;
@@ -1006,7 +1015,8 @@ bool HipExecutor::add_fused(const AxisPlan &h, const AxisPlan &v, SampleKind kin
     // workgroups; each split repeats about one unit's source rows.
     const long per_column = static_cast<long>(j.frames) * j.tiles;
     int upi = j.units;
-    while (upi > 4 && per_column * ((j.units + upi - 1) / upi) < 4L * compute_units_) upi = (upi + 1) / 2;
+    while (!whole_columns && upi > 4 && per_column * ((j.units + upi - 1) / upi) < 4L * compute_units_)
+        upi = (upi + 1) / 2;
     j.units_per_item = upi;
     j.segments = (j.units + upi - 1) / upi;
     const std::size_t align = kind == SampleKind::u8 ? 8 : 16;
@@ -1048,7 +1058,7 @@ HipExecutor::Scratch &HipExecutor::solve_2d(const AxisPlan &h, const AxisPlan &v
         Scratch &s0 = scratch(stream, 0);
         int *flag = kind == SampleKind::float32 ? s0.flag + 1 : nullptr;
         if (add_fused(h, v, kind, b, conversion, tables, pins, flag)) {
-            if (flag) check(hipMemsetAsync(flag, 0, sizeof(int), stream), "flag reset failed");
+            if (flag) tables.clear_flags(flag, 1);
             tables.upload(stream);
             tables.launch(stream);
             return s0;

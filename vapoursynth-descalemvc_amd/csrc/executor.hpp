@@ -245,6 +245,16 @@ public:
     PlanCacheStats plan_cache_stats();
     void set_plan_cache_limit(std::size_t bytes);
 
+    // Whether 2D calls of `kind` take the fused route where it fits.
+    [[nodiscard]] bool fused_enabled(SampleKind kind) const { return fused_route(kind); }
+    // add_fused (below) under the executor's lock, for batches and served
+    // rounds, whose workgroups take whole columns (many jobs per launch).
+    // False when this executor does not take the fused route for `kind` or
+    // the plans do not fit it.
+    bool fused_job(const AxisPlan &h, const AxisPlan &v, SampleKind kind, const PlaneBatch &b,
+                   const IntegerConversion *conversion, WindowTables &tables,
+                   std::vector<std::shared_ptr<hip::DevicePlan>> &pins, int *flag);
+
 private:
     struct Scratch {
         void *buffer = nullptr;
@@ -314,12 +324,16 @@ private:
         return mode_ == SolveMode::fused || (mode_ == SolveMode::automatic && fused_auto_);
     }
     // Adds one plane batch to `tables` as a fused 2D job when both plans fit
-    // the fused kernel (and pins their images); false otherwise.
-    // F32 planes add two jobs (the FP16 pass and the exact pass it may
-    // need) sharing `flag`, which the caller clears before the launches.
+    // the fused kernel (and pins their images); false otherwise.  F32
+    // planes add two jobs (the FP16 pass and the exact pass it may need)
+    // sharing `flag`, which the tables clear before their launches
+    // (WindowTables::clear_flags).  `whole_columns`: a workgroup takes a
+    // whole column of the plane; otherwise columns are split until the job
+    // alone fills the device.
     bool add_fused(const AxisPlan &h, const AxisPlan &v, SampleKind kind, const PlaneBatch &b,
                    const IntegerConversion *conversion, WindowTables &tables,
-                   std::vector<std::shared_ptr<hip::DevicePlan>> &pins, int *flag);
+                   std::vector<std::shared_ptr<hip::DevicePlan>> &pins, int *flag,
+                   bool whole_columns = false);
     void arm_flag(Scratch &s, hipStream_t stream);
     WindowTables &stream_tables(hipStream_t stream);
     void axis_pass(const AxisPlan &plan, bool rows, const void *input, hip::Sample in_kind,

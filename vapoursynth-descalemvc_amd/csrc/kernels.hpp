@@ -168,4 +168,7 @@ hipError_t launch_check_finite(const void *src, Sample in, long s_row, long s_fr
 hipError_t launch_operator_table(const OpJob *table, int count, long items, int steps, Sample kind,
                                  bool exact, bool vector, hipStream_t stream);
 
+// p[0 .. n) = value, as a kernel (a graph may capture it; no memset node).
+hipError_t launch_fill_i32(int *p, int n, int value, hipStream_t stream);
+
 } // namespace dsmvc::hip

@@ -262,15 +262,31 @@ __device__ __forceinline__ bool stage_chunk(const Chunk<In> &k, std::uint16_t *y
 }
 
 // Workgroup b runs on XCD b % 8 (each XCD has its own L2), as the b / 8-th
-// workgroup there.  Every job's items are split into 8 contiguous shares and
-// share x goes to XCD x, one job after another: each XCD then holds the same
-// mix of long (luma) and short (chroma) items, and neighbouring tiles, whose
-// source columns overlap, share an L2.  The launch has 8 * (ceil(items / 8)
+// workgroup there.  With few jobs (the planes of a frame batch), every job's
+// items are split into 8 contiguous shares and share x goes to XCD x, one job
+// after another: each XCD then holds the same mix of long (luma) and short
+// (chroma) items, and neighbouring tiles, whose source columns overlap,
+// share an L2.  With many jobs (batches, served rounds) the launch's items
+// are split into 8 contiguous shares, and an item's job is found by binary
+// search over the jobs' first items.  The launch has 8 * (ceil(items / 8)
 // + jobs) workgroups, enough for the largest share sum; workgroups past
 // their XCD's shares return.  Returns the job, or -1.
 constexpr long kXcds = 8;
-__device__ __forceinline__ int balanced_item(const OpJob *table, int count, long b, long &item) {
+constexpr int kBalancedJobs = 16;
+__device__ __forceinline__ int find_item(const OpJob *table, int count, long items, long b, long &item) {
     const long x = b % kXcds, slot = b / kXcds;
+    if (count > kBalancedJobs) {
+        const long share = (items + kXcds - 1) / kXcds;
+        item = x * share + slot;
+        if (slot >= share || item >= items) return -1;
+        int lo = 0, hi = count - 1;   // the last job whose first item is at most `item`
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) / 2;
+            if (table[mid].first_item <= item) lo = mid;
+            else hi = mid - 1;
+        }
+        return lo;
+    }
     long before = 0;
     for (int j = 0; j < count; ++j) {
         const long n = operator_items(table[j]);
@@ -308,7 +324,7 @@ template <class Out> __device__ __forceinline__ void store_sample(__amdgpu_buffe
 template <class In, int S, bool Exact, bool Vec>
 __global__ __launch_bounds__(kThreads)
 __attribute__((amdgpu_waves_per_eu(Exact ? 2 : 4)))
-void operator2d(const OpJob *__restrict__ table, int count) {
+void operator2d(const OpJob *__restrict__ table, int count, long items) {
     static_assert(!Exact || std::is_same_v<In, float>, "the exact scheme is for F32 samples");
     constexpr bool Bf = Exact;                              // three bfloat16 parts
     constexpr bool Half32 = std::is_same_v<In, float> && !Exact;   // F32 samples, FP16 scheme
@@ -320,7 +336,7 @@ void operator2d(const OpJob *__restrict__ table, int count) {
     __shared__ __attribute__((aligned(16))) std::uint16_t ring[NP * kRPlane];
 
     long item = 0;
-    const int job = balanced_item(table, count, blockIdx.x, item);
+    const int job = find_item(table, count, items, blockIdx.x, item);
     if (job < 0) return;
     // The job's fields, read once (scalar registers).
     const OpJob jb = table[job];
@@ -526,14 +542,26 @@ void operator2d(const OpJob *__restrict__ table, int count) {
         if (bad) __hip_atomic_fetch_or(jb.flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+__global__ __launch_bounds__(256) void fill_i32(int *__restrict__ p, int n, int value) {
+    const int i = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    if (i < n) p[i] = value;
+}
+
 } // namespace
 } // namespace dev
+
+hipError_t launch_fill_i32(int *p, int n, int value, hipStream_t stream) {
+    if (n < 1) return hipSuccess;
+    hipLaunchKernelGGL(dev::fill_i32, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, stream, p, n,
+                       value);
+    return hipGetLastError();
+}
 
 template <class In, int S, bool Exact, bool Vec>
 static hipError_t launch_op(const OpJob *table, int count, long items, hipStream_t stream) {
     const long blocks = dev::kXcds * ((items + dev::kXcds - 1) / dev::kXcds + count);
     hipLaunchKernelGGL((dev::operator2d<In, S, Exact, Vec>), dim3(static_cast<unsigned>(blocks)),
-                       dim3(dev::kThreads), 0, stream, table, count);
+                       dim3(dev::kThreads), 0, stream, table, count, items);
     return hipGetLastError();
 }
 

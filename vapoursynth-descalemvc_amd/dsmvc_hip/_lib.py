@@ -140,7 +140,7 @@ class BatchInfoC(ctypes.Structure):
         ("items", ctypes.c_int64),
         ("intermediate_bytes", ctypes.c_int64),
         ("graph", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("fused_jobs", ctypes.c_int32),
     ]
 
 
