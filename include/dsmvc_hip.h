@@ -554,16 +554,30 @@ typedef struct dsmvc_hip_serve_request {
     ptrdiff_t dst_stride[3];
 } dsmvc_hip_serve_request;
 
+/* fused: when an F32 plane (of an executor that takes the fused 2D route)
+ * runs on the fused route instead of the windowed tables.
+ * DSMVC_HIP_SERVE_FUSED_WARM (0, the default): once the plan pair's
+ * operator images exist, which a builder thread prepares off the submit
+ * path after the pair's first request; DSMVC_HIP_SERVE_FUSED_ALWAYS: the
+ * images are built at submit (a new geometry's request waits for them), so
+ * the route never changes; DSMVC_HIP_SERVE_FUSED_OFF: never. */
+#define DSMVC_HIP_SERVE_FUSED_WARM 0
+#define DSMVC_HIP_SERVE_FUSED_ALWAYS 1
+#define DSMVC_HIP_SERVE_FUSED_OFF 2
+
 typedef struct dsmvc_hip_server_config {
     int64_t memory_budget;           /* device bytes for in-flight intermediates */
     int32_t slots;                   /* rounds in flight, 1..16 */
     int32_t max_round_requests;
+    int32_t fused;                   /* DSMVC_HIP_SERVE_FUSED_* */
+    int32_t reserved;
 } dsmvc_hip_server_config;
 
 typedef struct dsmvc_hip_server_stats {
     uint64_t submitted, completed, failed, rounds;
     uint64_t table_planes, fallback_planes, largest_round;
     int64_t memory_high_water;
+    uint64_t fused_planes;           /* of the table planes, those on the fused 2D route */
 } dsmvc_hip_server_stats;
 
 typedef struct dsmvc_hip_server dsmvc_hip_server;

@@ -2,7 +2,10 @@
 
 Bars:
   * every request's result is bit-identical alone (one request in flight)
-    and in a mix submitted concurrently from several threads;
+    and in a mix submitted concurrently from several threads, with F32 planes
+    on a fixed route (fused "always", and "off": the windowed tables); in
+    the default "warm" policy a plane moves to the fused route once its
+    operator images exist, and both routes meet the oracle bars below;
   * every plane of every request, chroma included, against the oracle:
     F32-route planes within f32_bar(e_ref) of the FP64 oracle, Float64-route
     planes within 1 ULP of the oracle's Float64 route (requests derived by
@@ -67,9 +70,10 @@ def _submit(server, req):
     return rid, outputs
 
 
-def test_results_same_alone_and_in_the_mix(executor):
+@pytest.mark.parametrize("fused", ["always", "off"])
+def test_results_same_alone_and_in_the_mix(fused, executor):
     reqs = _requests(11, 24)
-    server = dh.Server(executor, memory_budget=512 << 20, slots=3)
+    server = dh.Server(executor, memory_budget=512 << 20, slots=3, fused=fused)
     alone = []
     for req in reqs:
         rid, outs = _submit(server, req)
@@ -98,9 +102,10 @@ def test_results_same_alone_and_in_the_mix(executor):
     server.close()
 
 
-def test_served_results_match_oracle_and_float64_route(executor, oracle):
+@pytest.mark.parametrize("fused", ["warm", "always"])
+def test_served_results_match_oracle_and_float64_route(fused, executor, oracle):
     reqs = _requests(12, 12)
-    server = dh.Server(executor, memory_budget=512 << 20, slots=2)
+    server = dh.Server(executor, memory_budget=512 << 20, slots=2, fused=fused)
     submitted = [_submit(server, r) for r in reqs]
     routes = set()
     for (rid, outs), req in zip(submitted, reqs):
@@ -137,8 +142,9 @@ def test_served_results_match_oracle_and_float64_route(executor, oracle):
 def test_admission_waits_for_memory(executor):
     reqs = [r for r in _requests(13, 16) if r["mode"] != 2]
     big = max(4 * ((w + 63) // 64 * 64) * h for r in reqs for (w, h, dw, _) in r["planes"])
-    # Room for about two requests at a time.
-    server = dh.Server(executor, memory_budget=3 * big, slots=4)
+    # Room for about two requests at a time (windowed tables: every plane
+    # needs its intermediate).
+    server = dh.Server(executor, memory_budget=3 * big, slots=4, fused="off")
     submitted = [_submit(server, r) for r in reqs]
     for rid, outs in submitted:
         server.wait(rid)

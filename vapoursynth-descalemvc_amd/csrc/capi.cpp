@@ -841,6 +841,11 @@ int dsmvc_hip_server_create(dsmvc_hip_executor *executor, const dsmvc_hip_server
         c.memory_budget = static_cast<std::size_t>(config->memory_budget);
         c.slots = config->slots;
         c.max_round_requests = config->max_round_requests;
+        require(config->fused >= DSMVC_HIP_SERVE_FUSED_WARM && config->fused <= DSMVC_HIP_SERVE_FUSED_OFF,
+                "dsmvc: invalid server arguments");
+        c.fused = config->fused == DSMVC_HIP_SERVE_FUSED_ALWAYS ? dsmvc::ServeFused::always
+                  : config->fused == DSMVC_HIP_SERVE_FUSED_OFF ? dsmvc::ServeFused::off
+                                                               : dsmvc::ServeFused::warm;
         auto handle = std::make_unique<dsmvc_hip_server>();
         handle->server = std::make_unique<dsmvc::HipServer>(exec_of(executor), c);
         *server = handle.release();
@@ -922,6 +927,7 @@ int dsmvc_hip_server_stats_get(const dsmvc_hip_server *server, dsmvc_hip_server_
         stats->fallback_planes = s.fallback_planes;
         stats->largest_round = s.largest_round;
         stats->memory_high_water = static_cast<int64_t>(s.memory_high_water);
+        stats->fused_planes = s.fused_planes;
     });
 }
 

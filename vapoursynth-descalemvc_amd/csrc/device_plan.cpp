@@ -741,6 +741,19 @@ const OperatorPlan *DevicePlan::operator_plan() {
     return operator_plan_locked();
 }
 
+int DevicePlan::built_op_steps() {
+    std::unique_lock lock(mutex_, std::try_to_lock);
+    if (!lock.owns_lock() || !op_built_) return -1;
+    return op_plan_ ? op_plan_->steps : 0;
+}
+
+const DevOperator *DevicePlan::built_op(bool rows, bool bf16, int steps) {
+    std::unique_lock lock(mutex_, std::try_to_lock);
+    if (!lock.owns_lock() || !op_plan_ || !operator_fits(*op_plan_, rows, steps)) return nullptr;
+    const auto it = op_images_.find((bf16 ? 8 : 0) + steps);
+    return it != op_images_.end() && it->second.ok ? &it->second.dev : nullptr;
+}
+
 const DevOperator *DevicePlan::op(bool rows, bool bf16, int steps) {
     std::lock_guard lock(mutex_);
     const OperatorPlan *host = operator_plan_locked();

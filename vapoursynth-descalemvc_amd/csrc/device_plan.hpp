@@ -131,6 +131,14 @@ public:
     // or bfloat16 parts; nullptr when the plan cannot take the route in the
     // given direction at that step count.  Built lazily, cached.
     const DevOperator *op(bool rows, bool bf16, int steps);
+    // The explicit operator's K steps if operator_plan() was built already
+    // (0: no direction can take the route); -1 if it was not built yet, or
+    // another thread holds the plan.  Never builds and never waits.
+    int built_op_steps();
+    // op(rows, bf16, steps) only if an earlier call built it; nullptr
+    // otherwise, or while another thread holds the plan.  Never builds and
+    // never waits.
+    const DevOperator *built_op(bool rows, bool bf16, int steps);
     [[nodiscard]] std::size_t device_bytes() const { return bytes_; }
     // Device bytes of the image and of every table built for it so far.
     [[nodiscard]] std::size_t total_bytes() const { return total_bytes_.load(std::memory_order_relaxed); }

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <array>
 #include <future>
+#include <set>
 #include <stdexcept>
 #include <system_error>
 
@@ -47,8 +48,10 @@ struct HipServer::Slot {
     bool float64 = false;    // the round ran Float64 table jobs (per-request flags)
     bool busy = false;
     // One Float64 failure flag per request of a round: set by its table
-    // jobs, copied back with the round, read when it completes.
-    int *flags = nullptr;         // device
+    // jobs, copied back with the round, read when it completes.  Then one
+    // range flag per request for its fused F32 planes (the FP16 pass sets
+    // it, the exact pass reads it).
+    int *flags = nullptr;         // device, 2 x max_round_requests
     int *flags_host = nullptr;    // pinned
 };
 
@@ -67,15 +70,28 @@ HipServer::HipServer(HipExecutor &executor, const ServerConfig &config)
         check(hipEventCreateWithFlags(&slot->done, hipEventDisableTiming), "event creation failed");
         slot->tables = std::make_unique<WindowTables>(executor_.device());
         const std::size_t flag_bytes = static_cast<std::size_t>(config_.max_round_requests) * sizeof(int);
-        check(hipMalloc(reinterpret_cast<void **>(&slot->flags), flag_bytes), "flag allocation failed");
+        check(hipMalloc(reinterpret_cast<void **>(&slot->flags), 2 * flag_bytes), "flag allocation failed");
         check(hipHostMalloc(reinterpret_cast<void **>(&slot->flags_host), flag_bytes),
               "flag allocation failed");
         slots_.push_back(std::move(slot));
     }
     worker_ = std::thread([this] { loop(); });
+    if (config_.fused == ServeFused::warm && executor_.fused_enabled(SampleKind::float32)) {
+        try {
+            builder_ = std::thread([this] { builder(); });
+        } catch (const std::system_error &) {
+            // No builder: planes keep the windowed route.
+        }
+    }
 }
 
 HipServer::~HipServer() {
+    {
+        std::lock_guard lock(build_mutex_);
+        build_stop_ = true;
+    }
+    build_cv_.notify_all();
+    if (builder_.joinable()) builder_.join();
     {
         std::lock_guard lock(mutex_);
         stop_ = true;
@@ -98,6 +114,45 @@ HipServer::~HipServer() {
         if (slot->stream) (void)hipStreamDestroy(slot->stream);
     }
     if (arena_) (void)hipFree(arena_);
+}
+
+void HipServer::request_operators(const std::shared_ptr<hip::DevicePlan> &h,
+                                  const std::shared_ptr<hip::DevicePlan> &v) {
+    std::lock_guard lock(build_mutex_);
+    if (!builder_.joinable() || build_stop_ || build_queue_.size() >= 1024) return;
+    if (!build_pending_.insert({h.get(), v.get()}).second) return;
+    build_queue_.emplace_back(h, v);
+    build_cv_.notify_one();
+}
+
+// Builds the operator images (FP16 and bfloat16 parts, at the pair's common
+// step count) of queued plan pairs; a build that fails leaves the pair on
+// the windowed route.
+void HipServer::builder() {
+    for (;;) {
+        std::pair<std::shared_ptr<hip::DevicePlan>, std::shared_ptr<hip::DevicePlan>> job;
+        {
+            std::unique_lock lock(build_mutex_);
+            build_cv_.wait(lock, [&] { return build_stop_ || !build_queue_.empty(); });
+            if (build_stop_) return;
+            job = std::move(build_queue_.front());
+            build_queue_.pop_front();
+        }
+        try {
+            check(hipSetDevice(executor_.device()), "hipSetDevice failed");
+            const hip::OperatorPlan *oh = job.first->operator_plan(), *ov = job.second->operator_plan();
+            if (oh && ov) {
+                const int steps = std::max({oh->steps, ov->steps, 2});
+                if (job.first->op(true, false, steps) && job.second->op(false, false, steps)) {
+                    (void)job.first->op(true, true, steps);
+                    (void)job.second->op(false, true, steps);
+                }
+            }
+        } catch (...) {
+        }
+        std::lock_guard lock(build_mutex_);
+        build_pending_.erase({job.first.get(), job.second.get()});
+    }
 }
 
 double HipServer::now() const {
@@ -234,7 +289,28 @@ std::uint64_t HipServer::submit(const ServeRequest &r) {
         }
         plane.table = plane.wh && plane.wv;
         plane.table64 = plane.wh64 && plane.wv64;
-        if (plane.table || plane.table64)
+        // The fused 2D route once the pair's operator images exist (built
+        // off this path by the builder thread, or here); no intermediate then.
+        if (!f64_pair && config_.fused != ServeFused::off && executor_.fused_enabled(SampleKind::float32)) {
+            if (config_.fused == ServeFused::always) {
+                const hip::OperatorPlan *oh = plane.ih->operator_plan(), *ov = plane.iv->operator_plan();
+                if (oh && ov) {
+                    const int steps = std::max({oh->steps, ov->steps, 2});
+                    if (plane.ih->op(true, false, steps) && plane.iv->op(false, false, steps)) {
+                        (void)plane.ih->op(true, true, steps);
+                        (void)plane.iv->op(false, true, steps);
+                    }
+                }
+            }
+            const int a = plane.ih->built_op_steps(), b = plane.iv->built_op_steps();
+            if (a > 0 && b > 0) {
+                const int steps = std::max({a, b, 2});
+                plane.fused = plane.ih->built_op(true, false, steps) && plane.iv->built_op(false, false, steps)
+                              && plane.ih->built_op(true, true, steps) && plane.iv->built_op(false, true, steps);
+            }
+            if (!plane.fused && a != 0 && b != 0) request_operators(plane.ih, plane.iv);
+        }
+        if (!plane.fused && (plane.table || plane.table64))
             plane.inter_bytes = align256(static_cast<std::size_t>(pitch_of(dw)) * sh
                                          * (plane.table64 ? sizeof(double) : sizeof(float)));
         entry->bytes += plane.inter_bytes;
@@ -342,16 +418,34 @@ bool HipServer::admit_round(Slot &slot, std::unique_lock<std::mutex> &lock) {
     lock.unlock();
     bool fallback = false, float64_round = false;
     std::string failure;
-    std::uint64_t table_planes = 0, fallback_planes = 0;
+    std::uint64_t table_planes = 0, fallback_planes = 0, fused_planes = 0;
     try {
         check(hipSetDevice(executor_.device()), "hipSetDevice failed");
         WindowTables &tables = *slot.tables;
         tables.clear();
         bool float64 = false;
+        int *fused_flags = slot.flags + config_.max_round_requests;
+        bool any_fused = false;
+        // Fused planes the executor did not take after all: solved with the
+        // other executor-route planes after the tables.
+        std::set<const Plane *> untaken;
+        std::vector<std::shared_ptr<hip::DevicePlan>> pins;   // the planes hold their images
         for (std::size_t r = 0; r < round.size(); ++r) {
             const auto &e = round[r];
             std::size_t at = e->offset;
             for (const Plane &p : e->planes) {
+                if (p.fused) {
+                    if (executor_.fused_job(*p.h, *p.v, SampleKind::float32,
+                                            PlaneBatch{p.src, p.src_stride, 0, p.dst, p.dst_stride, 0, 1},
+                                            nullptr, tables, pins, fused_flags + r)) {
+                        any_fused = true;
+                        ++table_planes;
+                        ++fused_planes;
+                    } else {
+                        untaken.insert(&p);
+                    }
+                    continue;
+                }
                 if (!p.table && !p.table64) continue;
                 const AxisPlan &h = *p.h, &v = *p.v;
                 const std::ptrdiff_t row = pitch_of(h.destination_size);
@@ -425,6 +519,7 @@ bool HipServer::admit_round(Slot &slot, std::unique_lock<std::mutex> &lock) {
         if (float64)
             check(hipMemsetAsync(slot.flags, 0, round.size() * sizeof(int), slot.stream),
                   "flag reset failed");
+        if (any_fused) tables.clear_flags(fused_flags, static_cast<int>(round.size()));
         tables.upload(slot.stream);
         tables.launch(slot.stream);
         if (float64)
@@ -433,7 +528,7 @@ bool HipServer::admit_round(Slot &slot, std::unique_lock<std::mutex> &lock) {
                   "flag readback failed");
         for (const auto &e : round)
             for (const Plane &p : e->planes) {
-                if (p.table || p.table64) continue;
+                if (p.fused ? untaken.count(&p) == 0 : (p.table || p.table64)) continue;
                 executor_.inverse_2d_batch(*p.h, *p.v, SampleKind::float32,
                                            PlaneBatch{p.src, p.src_stride, 0, p.dst, p.dst_stride, 0, 1},
                                            nullptr, slot.stream, true);
@@ -449,6 +544,7 @@ bool HipServer::admit_round(Slot &slot, std::unique_lock<std::mutex> &lock) {
     slot.fallback = fallback;
     slot.float64 = float64_round;
     stats_.table_planes += table_planes;
+    stats_.fused_planes += fused_planes;
     stats_.fallback_planes += fallback_planes;
     if (!failure.empty()) {
         // Nothing of this round can be trusted: wait for whatever was queued,
@@ -488,7 +584,7 @@ void HipServer::finish_round(Slot &slot, std::unique_lock<std::mutex> &lock, std
         arena_free(e->offset, e->bytes);
         bool uses_fallback = false, uses_table64 = false;
         for (const Plane &p : e->planes) {
-            uses_fallback |= !p.table && !p.table64;
+            uses_fallback |= !p.fused && !p.table && !p.table64;
             uses_table64 |= p.table64;
         }
         // The request's own Float64 flag (copied back with the round).

@@ -31,6 +31,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <set>
 #include <string>
 #include <thread>
 #include <unordered_map>
@@ -65,10 +66,16 @@ struct ServeRequest {
     std::ptrdiff_t dst_stride[3] = {};
 };
 
+// When an F32 plane takes the fused 2D route (for an executor that takes
+// it): once its plan pair's operator images exist (built off the submit path
+// after the pair's first request), always (built at submit), or never.
+enum class ServeFused : std::uint8_t { warm, always, off };
+
 struct ServerConfig {
     std::size_t memory_budget = std::size_t{1} << 30;   // device bytes for in-flight intermediates
     std::int32_t slots = 4;                             // rounds in flight (one stream each)
     std::int32_t max_round_requests = 256;
+    ServeFused fused = ServeFused::warm;
 };
 
 struct ServerStats {
@@ -77,6 +84,7 @@ struct ServerStats {
     std::uint64_t table_planes = 0, fallback_planes = 0;
     std::uint64_t largest_round = 0;
     std::size_t memory_high_water = 0;
+    std::uint64_t fused_planes = 0;    // of the table planes, those on the fused 2D route
 };
 
 enum class RequestState : std::int32_t { pending = 0, done = 1, failed = -1 };
@@ -110,6 +118,7 @@ private:
         std::ptrdiff_t dst_stride = 0;
         bool table = false;          // windowed F32 table launch
         bool table64 = false;        // windowed Float64 table launch (else executor route)
+        bool fused = false;          // fused 2D table launch (operator images built)
         std::size_t inter_bytes = 0;
         // The plans' device images, held so that the executor's plan cache
         // keeps them (and their window tables) until the request is gone.
@@ -153,6 +162,17 @@ private:
     bool stop_ = false;
     ServerStats stats_{};
     std::thread worker_;
+
+    // Operator images of F32 plan pairs, built off the submit path by one
+    // builder thread; a plane takes the fused route once they exist.
+    void builder();
+    void request_operators(const std::shared_ptr<hip::DevicePlan> &h, const std::shared_ptr<hip::DevicePlan> &v);
+    std::mutex build_mutex_;
+    std::condition_variable build_cv_;
+    std::deque<std::pair<std::shared_ptr<hip::DevicePlan>, std::shared_ptr<hip::DevicePlan>>> build_queue_;
+    std::set<std::pair<const hip::DevicePlan *, const hip::DevicePlan *>> build_pending_;
+    bool build_stop_ = false;
+    std::thread builder_;
 };
 
 } // namespace dsmvc

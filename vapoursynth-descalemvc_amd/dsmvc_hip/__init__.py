@@ -653,10 +653,17 @@ class Server:
     tensors that must stay alive until the request finishes (the Server keeps
     references until `wait`)."""
 
+    FUSED = {"warm": 0, "always": 1, "off": 2}
+
     def __init__(self, executor: "Executor", memory_budget: int = 1 << 30, slots: int = 4,
-                 max_round_requests: int = 256):
+                 max_round_requests: int = 256, fused: str = "warm"):
+        """`fused`: when F32 planes take the fused 2D route -- "warm" once
+        their plan pair's operator images exist (built off the submit path),
+        "always" (built at submit; the route never changes), "off"."""
         self._executor = executor
-        cfg = _lib.ServerConfigC(memory_budget, slots, max_round_requests)
+        if fused not in self.FUSED:
+            raise ValueError(f"dsmvc: fused must be one of {sorted(self.FUSED)}")
+        cfg = _lib.ServerConfigC(memory_budget, slots, max_round_requests, self.FUSED[fused], 0)
         handle = ctypes.c_void_p()
         check(load().dsmvc_hip_server_create(executor._handle, ctypes.byref(cfg),
                                              ctypes.byref(handle)))

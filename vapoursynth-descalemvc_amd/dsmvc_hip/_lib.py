@@ -173,6 +173,8 @@ class ServerConfigC(ctypes.Structure):
         ("memory_budget", ctypes.c_int64),
         ("slots", ctypes.c_int32),
         ("max_round_requests", ctypes.c_int32),
+        ("fused", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -186,6 +188,7 @@ class ServerStatsC(ctypes.Structure):
         ("fallback_planes", ctypes.c_uint64),
         ("largest_round", ctypes.c_uint64),
         ("memory_high_water", ctypes.c_int64),
+        ("fused_planes", ctypes.c_uint64),
     ]
 
 
