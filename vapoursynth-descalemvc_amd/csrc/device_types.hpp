@@ -173,7 +173,10 @@ struct OpJob {
     // above `range` in magnitude (so that the horizontal results stay within
     // FP16's range); the exact pass runs only then.
     float range;
-    std::int32_t pad_;
+    // A unit run in a slab step may read ring rows the step's slab
+    // overwrites (some unit spans more than kOpRing - 2 kOpSlab + 1 rows,
+    // its last block's K range included): a barrier separates them.
+    std::int32_t unit_barrier;
     int *flag;
 };
 

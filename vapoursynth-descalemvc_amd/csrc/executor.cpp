@@ -795,13 +795,20 @@ HipExecutor::Scratch &HipExecutor::enqueue_planes(const PlaneJob *planes, std::i
     WindowTables &tables = stream_tables(stream);
     tables.clear();
     const bool try_fused = fused_route(kind);
+    // Workgroup columns of the whole call, so that a plane's columns are
+    // split only when the call as a whole leaves the device short.
+    long columns = 0;
+    for (std::size_t k = 0; k < n; ++k)
+        columns += static_cast<long>(planes[k].batch.frames)
+                   * ((planes[k].horizontal->destination_size + hip::kOpTileCols - 1) / hip::kOpTileCols);
     // The F32 fused jobs' range flag, next to the stream's Float64 flag.
     int *fused_flag = kind == SampleKind::float32 ? scratch(stream, 0).flag + 1 : nullptr;
     for (std::size_t k = 0; k < n; ++k) {
         const AxisPlan &h = *planes[k].horizontal, &v = *planes[k].vertical;
         const bool f64 = float64_route(h) || float64_route(v);
         if (try_fused && !f64
-            && add_fused(h, v, kind, planes[k].batch, planes[k].conversion, tables, pins, fused_flag)) {
+            && add_fused(h, v, kind, planes[k].batch, planes[k].conversion, tables, pins, fused_flag, false,
+                         columns)) {
             fused[k] = 1;
             continue;
         }
@@ -975,7 +982,7 @@ bool HipExecutor::fused_job(const AxisPlan &h, const AxisPlan &v, SampleKind kin
 bool HipExecutor::add_fused(const AxisPlan &h, const AxisPlan &v, SampleKind kind, const PlaneBatch &b,
                             const IntegerConversion *conversion, WindowTables &tables,
                             std::vector<std::shared_ptr<hip::DevicePlan>> &pins, int *flag,
-                            bool whole_columns) {
+                            bool whole_columns, long launch_columns) {
     const std::shared_ptr<hip::DevicePlan> ih = device_plan(h), iv = device_plan(v);
     const hip::OperatorPlan *ph = ih->operator_plan(), *pv = iv->operator_plan();
     if (!ph || !pv) return false;
@@ -1012,13 +1019,21 @@ bool HipExecutor::add_fused(const AxisPlan &h, const AxisPlan &v, SampleKind kin
     j.tiles = (h.destination_size + hip::kOpTileCols - 1) / hip::kOpTileCols;
     j.units = (v.destination_size + hip::kOpUnit - 1) / hip::kOpUnit;
     // Whole columns per workgroup unless that leaves the device too few
-    // workgroups; each split repeats about one unit's source rows.
-    const long per_column = static_cast<long>(j.frames) * j.tiles;
+    // workgroups (counting every column of the launch); each split repeats
+    // about one unit's source rows.
+    const long per_column = std::max(static_cast<long>(j.frames) * j.tiles, launch_columns);
     int upi = j.units;
     while (!whole_columns && upi > 4 && per_column * ((j.units + upi - 1) / upi) < 4L * compute_units_)
         upi = (upi + 1) / 2;
     j.units_per_item = upi;
     j.segments = (j.units + upi - 1) / upi;
+    // A barrier after a step's units when some unit spans more than
+    // kOpRing - 2 kOpSlab + 1 ring rows (its two blocks' K ranges at
+    // `steps`): only then can the step's slab overwrite rows it reads.
+    int span = 0;
+    for (int u = 0; u < j.units; ++u)
+        span = std::max(span, pv->kstart[2 * u + 1] - pv->kstart[2 * u] + hip::kOpStep * steps);
+    j.unit_barrier = span > hip::kOpRing - 2 * hip::kOpSlab + 1 ? 1 : 0;
     const std::size_t align = kind == SampleKind::u8 ? 8 : 16;
     const bool aligned = reinterpret_cast<std::uintptr_t>(b.input) % align == 0
                          && (static_cast<std::size_t>(b.input_row_stride) * bytes) % align == 0

@@ -328,12 +328,13 @@ private:
     // planes add two jobs (the FP16 pass and the exact pass it may need)
     // sharing `flag`, which the tables clear before their launches
     // (WindowTables::clear_flags).  `whole_columns`: a workgroup takes a
-    // whole column of the plane; otherwise columns are split until the job
-    // alone fills the device.
+    // whole column of the plane; otherwise columns are split until the
+    // launch fills the device, `launch_columns` counting the workgroup
+    // columns of every plane the launch takes.
     bool add_fused(const AxisPlan &h, const AxisPlan &v, SampleKind kind, const PlaneBatch &b,
                    const IntegerConversion *conversion, WindowTables &tables,
                    std::vector<std::shared_ptr<hip::DevicePlan>> &pins, int *flag,
-                   bool whole_columns = false);
+                   bool whole_columns = false, long launch_columns = 0);
     void arm_flag(Scratch &s, hipStream_t stream);
     WindowTables &stream_tables(hipStream_t stream);
     void axis_pass(const AxisPlan &plan, bool rows, const void *input, hip::Sample in_kind,

@@ -508,9 +508,9 @@ void operator2d(const OpJob *__restrict__ table, int count, long items) {
     // multiply slab k into the ring.  The compiler waits for every
     // outstanding vector-memory operation when stores are among them, so
     // each step has one such wait, at the staging, and everything issued
-    // after it has a whole step to complete.  A unit run in step k reads ring
-    // rows that slab k may overwrite, so a barrier follows it.  Every chunk's
-    // first word goes into
+    // after it has a whole step to complete.  A unit run in step k may read
+    // ring rows that slab k overwrites (jobs whose units span many rows), and
+    // then a barrier follows it.  Every chunk's first word goes into
     // `sink`, which keeps the loads of inactive chunks issued (a fixed set of
     // loads per step) and makes the staging's wait one point on every path.
     Chunk<In> buf[kChunks];
@@ -531,7 +531,7 @@ void operator2d(const OpJob *__restrict__ table, int count, long items) {
         __syncthreads();   // slab k staged; the ring rows of slabs below k written
         if (units(first + kOpSlab * k)) {
             if (u < u1) load_unit(u);
-            __syncthreads();
+            if (jb.unit_barrier) __syncthreads();
         }
         horizontal(k);
     }
