@@ -46,6 +46,7 @@ using u32x4 = std::uint32_t __attribute__((ext_vector_type(4)));
 using u32x2 = std::uint32_t __attribute__((ext_vector_type(2)));
 using h8 = _Float16 __attribute__((ext_vector_type(8)));
 using h2 = _Float16 __attribute__((ext_vector_type(2)));
+using f2 = float __attribute__((ext_vector_type(2)));
 using bf8 = __bf16 __attribute__((ext_vector_type(8)));
 using f32x4 = float __attribute__((ext_vector_type(4)));
 
@@ -121,11 +122,12 @@ __device__ __forceinline__ void split_f32_pair(float x0, float x1, std::uint32_t
 }
 
 // Two F32 values as FP16 + FP16 remainder pairs (round to nearest; packed,
-// low half from x0).
+// low half from x0): one packed conversion per part, the high part widened
+// back from its packed form.
 __device__ __forceinline__ void split_half_pair(float x0, float x1, std::uint32_t &hi, std::uint32_t &lo) {
-    const h2 h = h2{static_cast<_Float16>(x0), static_cast<_Float16>(x1)};
-    const float r0 = __fsub_rn(x0, static_cast<float>(h[0])), r1 = __fsub_rn(x1, static_cast<float>(h[1]));
-    const h2 r = h2{static_cast<_Float16>(r0), static_cast<_Float16>(r1)};
+    const h2 h = __builtin_convertvector(f2{x0, x1}, h2);
+    const f2 w = __builtin_convertvector(h, f2);
+    const h2 r = __builtin_convertvector(f2{__fsub_rn(x0, w[0]), __fsub_rn(x1, w[1])}, h2);
     hi = __builtin_bit_cast(std::uint32_t, h);
     lo = __builtin_bit_cast(std::uint32_t, r);
 }
@@ -461,11 +463,13 @@ void operator2d(const OpJob *__restrict__ table, int count, long items) {
     // offsets of a unit's rows below 2^31).
     auto unit = [&](int u) {
         f32x4 out = f32x4{0, 0, 0, 0};
-        const int k0 = kv[2 * u + hb] % kOpRing;
+        const int k0 = kv[2 * u + hb];
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-            int rr = k0 + kOpStep * s + 8 * lk;
-            rr = rr >= kOpRing ? rr - kOpRing : rr;
+            // Wave-uniform ring row of the step's K range, then the lane's
+            // 8-row group, which wraps at most once.
+            const int rs = uniform((k0 + kOpStep * s) % kOpRing);
+            int rr = rs + 8 * lk;
             rr = rr >= kOpRing ? rr - kOpRing : rr;
             u32x4 b[NP];
 #pragma unroll
