@@ -1030,10 +1030,10 @@ bool HipExecutor::add_fused(const AxisPlan &h, const AxisPlan &v, SampleKind kin
     // A barrier after a step's units when some unit spans more than
     // kOpRing - 2 kOpSlab + 1 ring rows (its two blocks' K ranges at
     // `steps`): only then can the step's slab overwrite rows it reads.
-    int span = 0;
+    int unit_span = 0;
     for (int u = 0; u < j.units; ++u)
-        span = std::max(span, pv->kstart[2 * u + 1] - pv->kstart[2 * u] + hip::kOpStep * steps);
-    j.unit_barrier = span > hip::kOpRing - 2 * hip::kOpSlab + 1 ? 1 : 0;
+        unit_span = std::max(unit_span, pv->kstart[2 * u + 1] - pv->kstart[2 * u] + hip::kOpStep * steps);
+    j.unit_barrier = unit_span > hip::kOpRing - 2 * hip::kOpSlab + 1 ? 1 : 0;
     const std::size_t align = kind == SampleKind::u8 ? 8 : 16;
     const bool aligned = reinterpret_cast<std::uintptr_t>(b.input) % align == 0
                          && (static_cast<std::size_t>(b.input_row_stride) * bytes) % align == 0

@@ -34,6 +34,7 @@
 // MFMA products of 16-bit parts are exact in F32 and accumulate in F32.  The
 // integer route applies the input offset and scale to the result (the
 // operator maps constants to themselves), then the output conversion.
+#include <algorithm>
 #include <type_traits>
 
 #include "kernels_common.hpp"
@@ -322,11 +323,10 @@ template <class Out> __device__ __forceinline__ void store_sample(__amdgpu_buffe
     }
 }
 
-// Vec: every job of the launch allows 16-byte source loads (8 for u8).
+// Workgroup slot b of a launch (b % 8 is its XCD).  Vec: every job of the
+// launch allows 16-byte source loads (8 for u8).
 template <class In, int S, bool Exact, bool Vec>
-__global__ __launch_bounds__(kThreads)
-__attribute__((amdgpu_waves_per_eu(Exact ? 2 : 4)))
-void operator2d(const OpJob *__restrict__ table, int count, long items) {
+__device__ __forceinline__ void operator2d_item(const OpJob *__restrict__ table, int count, long items, long b) {
     static_assert(!Exact || std::is_same_v<In, float>, "the exact scheme is for F32 samples");
     constexpr bool Bf = Exact;                              // three bfloat16 parts
     constexpr bool Half32 = std::is_same_v<In, float> && !Exact;   // F32 samples, FP16 scheme
@@ -338,7 +338,7 @@ void operator2d(const OpJob *__restrict__ table, int count, long items) {
     __shared__ __attribute__((aligned(16))) std::uint16_t ring[NP * kRPlane];
 
     long item = 0;
-    const int job = find_item(table, count, items, blockIdx.x, item);
+    const int job = find_item(table, count, items, b, item);
     if (job < 0) return;
     // The job's fields, read once (scalar registers).
     const OpJob jb = table[job];
@@ -546,6 +546,26 @@ void operator2d(const OpJob *__restrict__ table, int count, long items) {
         if (bad) __hip_atomic_fetch_or(jb.flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// The first pass takes one workgroup slot per workgroup.  The exact pass,
+// which usually returns at once (its job's flag is clear), runs on a fixed
+// grid of kExactGrid workgroups that walk the slots (a multiple of 8, so a
+// workgroup stays on its XCD): a mostly empty launch costs a few hundred
+// workgroups instead of one per slot.
+constexpr long kExactGrid = 256;
+template <class In, int S, bool Exact, bool Vec>
+__global__ __launch_bounds__(kThreads)
+__attribute__((amdgpu_waves_per_eu(Exact ? 2 : 4)))
+void operator2d(const OpJob *__restrict__ table, int count, long items, long slots) {
+    if constexpr (Exact) {
+        for (long b = blockIdx.x; b < slots; b += gridDim.x) {
+            operator2d_item<In, S, Exact, Vec>(table, count, items, b);
+            __syncthreads();   // the next slot reuses the LDS
+        }
+    } else {
+        operator2d_item<In, S, Exact, Vec>(table, count, items, blockIdx.x);
+    }
+}
+
 __global__ __launch_bounds__(256) void fill_i32(int *__restrict__ p, int n, int value) {
     const int i = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
     if (i < n) p[i] = value;
@@ -563,9 +583,10 @@ hipError_t launch_fill_i32(int *p, int n, int value, hipStream_t stream) {
 
 template <class In, int S, bool Exact, bool Vec>
 static hipError_t launch_op(const OpJob *table, int count, long items, hipStream_t stream) {
-    const long blocks = dev::kXcds * ((items + dev::kXcds - 1) / dev::kXcds + count);
+    const long slots = dev::kXcds * ((items + dev::kXcds - 1) / dev::kXcds + count);
+    const long blocks = Exact ? std::min(slots, dev::kExactGrid) : slots;
     hipLaunchKernelGGL((dev::operator2d<In, S, Exact, Vec>), dim3(static_cast<unsigned>(blocks)),
-                       dim3(dev::kThreads), 0, stream, table, count, items);
+                       dim3(dev::kThreads), 0, stream, table, count, items, slots);
     return hipGetLastError();
 }
 
