@@ -217,10 +217,15 @@ void WindowTables::launch(hipStream_t stream) {
             e = hip::launch_window64_table(reinterpret_cast<const hip::Win64Job *>(at), count,
                                            g.items, g.band, g.width, g.rows, g.kind, stream);
             break;
-        case Kind::op:
+        case Kind::op: {
+            // The exact pass reads one flag first when all its jobs share it.
+            const int *shared = g.ops.empty() ? nullptr : g.ops.front().flag;
+            for (const hip::OpJob &j : g.ops)
+                if (j.flag != shared) shared = nullptr;
             e = hip::launch_operator_table(reinterpret_cast<const hip::OpJob *>(at), count, g.items,
-                                           g.width, g.kind, g.band != 0, g.block16, stream);
+                                           g.width, g.kind, g.band != 0, g.block16, shared, stream);
             break;
+        }
         }
         check(e, "table launch failed");
     }

@@ -164,9 +164,10 @@ hipError_t launch_check_finite(const void *src, Sample in, long s_row, long s_fr
 // (FP16 parts).  `exact`: the F32 jobs' second pass on bfloat16 images,
 // which returns at once unless the first pass set the jobs' flag.
 // `vector`: every job's source base and strides allow 16-byte loads (8 for
-// u8 samples), i.e. each has vector_loads set.
+// u8 samples), i.e. each has vector_loads set.  `shared_flag` (exact pass):
+// the flag every job of the table uses, or nullptr when they differ.
 hipError_t launch_operator_table(const OpJob *table, int count, long items, int steps, Sample kind,
-                                 bool exact, bool vector, hipStream_t stream);
+                                 bool exact, bool vector, const int *shared_flag, hipStream_t stream);
 
 // p[0 .. n) = value, as a kernel (a graph may capture it; no memset node).
 hipError_t launch_fill_i32(int *p, int n, int value, hipStream_t stream);

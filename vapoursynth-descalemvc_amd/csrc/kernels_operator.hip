@@ -550,13 +550,17 @@ __device__ __forceinline__ void operator2d_item(const OpJob *__restrict__ table,
 // which usually returns at once (its job's flag is clear), runs on a fixed
 // grid of kExactGrid workgroups that walk the slots (a multiple of 8, so a
 // workgroup stays on its XCD): a mostly empty launch costs a few hundred
-// workgroups instead of one per slot.
+// workgroups instead of one per slot.  `shared_flag`: the one flag every
+// job of the launch uses (nullptr when they differ), read before any job
+// lookup.
 constexpr long kExactGrid = 256;
 template <class In, int S, bool Exact, bool Vec>
 __global__ __launch_bounds__(kThreads)
 __attribute__((amdgpu_waves_per_eu(Exact ? 2 : 4)))
-void operator2d(const OpJob *__restrict__ table, int count, long items, long slots) {
+void operator2d(const OpJob *__restrict__ table, int count, long items, long slots, const int *shared_flag) {
     if constexpr (Exact) {
+        if (shared_flag && uniform(__hip_atomic_load(shared_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == 0)
+            return;
         for (long b = blockIdx.x; b < slots; b += gridDim.x) {
             operator2d_item<In, S, Exact, Vec>(table, count, items, b);
             __syncthreads();   // the next slot reuses the LDS
@@ -582,44 +586,47 @@ hipError_t launch_fill_i32(int *p, int n, int value, hipStream_t stream) {
 }
 
 template <class In, int S, bool Exact, bool Vec>
-static hipError_t launch_op(const OpJob *table, int count, long items, hipStream_t stream) {
+static hipError_t launch_op(const OpJob *table, int count, long items, const int *shared_flag,
+                            hipStream_t stream) {
     const long slots = dev::kXcds * ((items + dev::kXcds - 1) / dev::kXcds + count);
     const long blocks = Exact ? std::min(slots, dev::kExactGrid) : slots;
     hipLaunchKernelGGL((dev::operator2d<In, S, Exact, Vec>), dim3(static_cast<unsigned>(blocks)),
-                       dim3(dev::kThreads), 0, stream, table, count, items, slots);
+                       dim3(dev::kThreads), 0, stream, table, count, items, slots, shared_flag);
     return hipGetLastError();
 }
 
 template <class In, bool Exact, bool Vec>
-static hipError_t launch_op_steps(const OpJob *table, int count, long items, int steps, hipStream_t stream) {
+static hipError_t launch_op_steps(const OpJob *table, int count, long items, int steps, const int *shared_flag,
+                                  hipStream_t stream) {
     switch (steps) {
-    case 2: return launch_op<In, 2, Exact, Vec>(table, count, items, stream);
-    case 3: return launch_op<In, 3, Exact, Vec>(table, count, items, stream);
-    case 4: return launch_op<In, 4, Exact, Vec>(table, count, items, stream);
+    case 2: return launch_op<In, 2, Exact, Vec>(table, count, items, shared_flag, stream);
+    case 3: return launch_op<In, 3, Exact, Vec>(table, count, items, shared_flag, stream);
+    case 4: return launch_op<In, 4, Exact, Vec>(table, count, items, shared_flag, stream);
     default: return hipErrorInvalidValue;
     }
 }
 
 template <bool Vec>
 static hipError_t launch_op_kind(const OpJob *table, int count, long items, int steps, Sample kind, bool exact,
-                                 hipStream_t stream) {
+                                 const int *shared_flag, hipStream_t stream) {
     if (exact)
-        return kind == Sample::f32 ? launch_op_steps<float, true, Vec>(table, count, items, steps, stream)
-                                   : hipErrorInvalidValue;
+        return kind == Sample::f32
+                   ? launch_op_steps<float, true, Vec>(table, count, items, steps, shared_flag, stream)
+                   : hipErrorInvalidValue;
     switch (kind) {
-    case Sample::f32: return launch_op_steps<float, false, Vec>(table, count, items, steps, stream);
-    case Sample::u8: return launch_op_steps<std::uint8_t, false, Vec>(table, count, items, steps, stream);
-    case Sample::u16: return launch_op_steps<std::uint16_t, false, Vec>(table, count, items, steps, stream);
+    case Sample::f32: return launch_op_steps<float, false, Vec>(table, count, items, steps, nullptr, stream);
+    case Sample::u8: return launch_op_steps<std::uint8_t, false, Vec>(table, count, items, steps, nullptr, stream);
+    case Sample::u16: return launch_op_steps<std::uint16_t, false, Vec>(table, count, items, steps, nullptr, stream);
     default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t launch_operator_table(const OpJob *table, int count, long items, int steps, Sample kind,
-                                 bool exact, bool vector, hipStream_t stream) {
+                                 bool exact, bool vector, const int *shared_flag, hipStream_t stream) {
     if (count < 1 || items < 1) return hipSuccess;
     if (items + 8L * count > 0x7FFFFFF0L) return hipErrorInvalidValue;
-    return vector ? launch_op_kind<true>(table, count, items, steps, kind, exact, stream)
-                  : launch_op_kind<false>(table, count, items, steps, kind, exact, stream);
+    return vector ? launch_op_kind<true>(table, count, items, steps, kind, exact, shared_flag, stream)
+                  : launch_op_kind<false>(table, count, items, steps, kind, exact, shared_flag, stream);
 }
 
 } // namespace dsmvc::hip
